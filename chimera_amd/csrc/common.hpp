@@ -1,0 +1,109 @@
+// common.hpp -- runtime plumbing shared by every translation unit of
+// libchimera_hip: status codes, the library stream, error text, staging of
+// f2py-layout arguments that may live in host or device memory, and a few
+// complex-arithmetic helpers on double2 / float2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace chimera {
+
+enum : int32_t {
+    ST_OK = 0,
+    ST_EINVAL = -1,   // bad argument (shape, size, null pointer)
+    ST_EHIP = -2,     // HIP runtime error
+    ST_ENOMEM = -3,   // device allocation failed
+    ST_EUNSUP = -4,   // valid in the reference, not supported here (message says which)
+};
+
+// Per-thread last-error text (read through chimera_last_error()).
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+int32_t fail(int32_t code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int32_t hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+hipStream_t stream();
+void set_stream(hipStream_t s);
+
+#define CH_HIP(expr)                                                                  \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) return ::chimera::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define CH_LAUNCH_CHECK(name)                                                             \
+    do {                                                                                  \
+        hipError_t _e = hipGetLastError();                                                \
+        if (_e != hipSuccess) return ::chimera::hip_fail(_e, name, __FILE__, __LINE__);  \
+    } while (0)
+
+#define CH_TRY(expr)                         \
+    do {                                     \
+        int32_t _s = (expr);                 \
+        if (_s != ::chimera::ST_OK) return _s; \
+    } while (0)
+
+// True when p points into device (or managed) memory that kernels may read.
+bool is_device_ptr(const void* p);
+
+// Staging of entry-point arguments.  Pointers to device memory pass through
+// untouched; host pointers are copied into temporary device buffers on the
+// library stream and, for written arguments, copied back by finish(), which
+// then synchronises the stream.  A call whose arguments are all device
+// pointers stays asynchronous.
+class Stage {
+  public:
+    Stage() = default;
+    Stage(const Stage&) = delete;
+    Stage& operator=(const Stage&) = delete;
+    ~Stage();
+
+    template <class T>
+    T* in(const T* p, size_t n) { return static_cast<T*>(put(const_cast<T*>(p), n * sizeof(T), true, false)); }
+    template <class T>
+    T* inout(T* p, size_t n) { return static_cast<T*>(put(p, n * sizeof(T), true, true)); }
+    template <class T>
+    T* out(T* p, size_t n) { return static_cast<T*>(put(p, n * sizeof(T), false, true)); }
+
+    int32_t status() const { return st_; }
+    bool staged() const { return any_host_; }
+    int32_t finish();
+
+  private:
+    void* put(void* p, size_t bytes, bool copy_in, bool copy_out);
+    struct Buf { void* host; void* dev; size_t bytes; bool copy_out; };
+    std::vector<Buf> bufs_;
+    int32_t st_ = ST_OK;
+    bool any_host_ = false;
+};
+
+// Scratch device memory owned by the library, grown on demand and reused
+// across calls on the same stream (the stream orders every use).
+void* scratch(int slot, size_t bytes);
+constexpr int SCRATCH_SLOTS = 8;
+
+// Device copy of a small host table (job lists, twiddles), cached by content.
+const void* device_table(const void* host, size_t bytes);
+
+inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace chimera
+
+// ---------------------------------------------------------------- complex helpers
+__host__ __device__ inline double2 cmk(double re, double im) { return make_double2(re, im); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 crs(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__host__ __device__ inline double2 cmuli(double2 a) { return make_double2(-a.y, a.x); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline double2 cneg(double2 a) { return make_double2(-a.x, -a.y); }

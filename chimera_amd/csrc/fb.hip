@@ -1,0 +1,717 @@
+// fb.hip -- the Fourier-Bessel side of the step: the forward and inverse FB
+// transforms of fb_io.f90 (discrete Hankel transform along r + FFT along x),
+// the absorbing-layer filter, and the FB-space curl / grad / div / grad-div of
+// fb_math.f90.
+//
+// Every dense contraction (Hankel matrices In/Out, the mode-coupling matrices
+// DpS2S/DmS2S) goes through ONE job-driven kernel, k_fb_gemm: a job is an
+// output mode of one operator, made of up to two K-segments
+//   P_s(x,k) = sum_k' A_s(x,k') D_s(k',k),   A_s = c0*T(a0) + c1*T(a1)
+// (T = identity, or the reflected conjugate -conj(v(-kx)) that stands for mode
+// -1 at m = 0) accumulated into one or two accumulators, and up to three
+// outputs, each a complex combination of the accumulators plus the diagonal
+// i*kx term of the operator.  All jobs of one operator run in one launch
+// (grid.z = job), and no two jobs write the same output.
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "fb_kernels.hpp"
+#include "fft.hpp"
+#include "../../include/chimera_hip.h"
+
+namespace chimera {
+
+// ================================================================== FFT
+
+const double2* fft_twiddles(int N) {
+    static std::mutex mu;
+    static std::map<int, const double2*> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(N);
+    if (it != cache.end()) return it->second;
+    std::vector<double2> h(N > 0 ? N : 1);
+    for (int k = 0; k < N; k++) {
+        // exact argument reduction: angle = -2 pi k/N, symmetric in k and N-k
+        const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)N;
+        h[k] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    const double2* d = static_cast<const double2*>(device_table(h.data(), h.size() * sizeof(double2)));
+    cache[N] = d;
+    return d;
+}
+
+template <int EPT>
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgs a) {
+    constexpr int TOT = EPT * FFT_THREADS;
+    extern __shared__ double2 sbuf[];
+    const int N = a.N;
+    const int lpb = TOT / N;
+    const int64_t line0 = (int64_t)blockIdx.x * lpb;
+    // ---- load (with the per-x pre-multiplier)
+    for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+        const int64_t line = line0 + i / N;
+        const int x = i % N;
+        double2 v = cmk(0.0, 0.0);
+        if (line < a.nlines) {
+            const int64_t r = line % a.n_inner, q = line / a.n_inner;
+            v = a.in[r * a.in_r + q * a.in_q + x];
+            if (a.pre) v = cmul(v, a.pre[x]);
+        }
+        sbuf[i] = v;
+    }
+    __syncthreads();
+    fft_lds<EPT>(sbuf, N, a.log2N, a.sign, a.tw);
+    if (a.filtr) {
+        // fb_io.f90:263: multiply the first nxfilt points, transform back
+        for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+            const int x = i % N;
+            if (x < a.nxfilt) sbuf[i] = crs(a.filtr[x], sbuf[i]);
+        }
+        __syncthreads();
+        fft_lds<EPT>(sbuf, N, a.log2N, -a.sign, a.tw);
+    }
+    // ---- store (with post-multiplier, real scale table, mode normalisation)
+    for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+        const int64_t line = line0 + i / N;
+        if (line >= a.nlines) continue;
+        const int x = i % N;
+        const int64_t r = line % a.n_inner, q = line / a.n_inner;
+        double2 v = sbuf[i];
+        if (a.post) v = cmul(v, a.post[x]);
+        if (a.scale) v = crs(a.scale[x + (int64_t)N * r], v);
+        int m = 0;
+        if (a.ghost) {
+            m = (int)(q % a.nmodes);
+            if (a.ghost == 2) v = crs(m == 0 ? 0.5 / M_PI : 1.0 / M_PI, v);
+        }
+        double2* o = a.out + r * a.out_r + q * a.out_q + x;
+        *o = v;
+        if (a.ghost && r == 0) {
+            // ghost radial row below row 1: zero (fb_io.f90:199) or +-row 1 (grid_deps.f90:248-265)
+            o[-a.out_r] = a.ghost == 1 ? cmk(0.0, 0.0) : (m == 0 ? v : cneg(v));
+        }
+    }
+}
+
+int32_t launch_fft(FftArgs a) {
+    if (a.nlines <= 0) return ST_OK;
+    const int N = a.N;
+    if (N < 1 || (N & (N - 1)) != 0 || N > 8192)
+        return fail(ST_EUNSUP, "FFT length %d: this build transforms powers of two up to 8192 along x", N);
+    int l2 = 0;
+    while ((1 << l2) < N) l2++;
+    a.log2N = l2;
+    a.tw = fft_twiddles(N);
+    if (!a.tw) return ST_ENOMEM;
+    int ept = N <= 2048 ? 8 : N / FFT_THREADS;
+    const int tot = ept * FFT_THREADS;
+    const int lpb = tot / N;
+    const unsigned grid = (unsigned)((a.nlines + lpb - 1) / lpb);
+    const size_t shmem = (size_t)tot * sizeof(double2);
+    static bool attr_done = false;
+    if (!attr_done) {
+        // lines of 4096 and 8192 points use 64 and 128 KiB of LDS per workgroup
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 32768));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<16>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+        attr_done = true;
+    }
+    switch (ept) {
+        case 8:
+            hipLaunchKernelGGL(k_fft_lines<8>, grid, FFT_THREADS, shmem, stream(), a);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_fft_lines<16>, grid, FFT_THREADS, shmem, stream(), a);
+            break;
+        case 32:
+            hipLaunchKernelGGL(k_fft_lines<32>, grid, FFT_THREADS, shmem, stream(), a);
+            break;
+        default:
+            return fail(ST_EUNSUP, "FFT length %d", N);
+    }
+    CH_LAUNCH_CHECK("k_fft_lines");
+    return ST_OK;
+}
+
+// Per-x phase exp(s*i*kx*leftX) (fb_io.f90:40, :120, :202), on the device.
+__global__ void k_phase(double2* __restrict__ ph, const double* __restrict__ kx, double leftX, double s, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sn, cs;
+    sincos(leftX * kx[i], &sn, &cs);
+    ph[i] = cmk(cs, s * sn);
+}
+
+// 1/(N*exp(i*kx*leftX)) as the reference forms it (fb_io.f90:247).
+__global__ void k_phase_inv(double2* __restrict__ ph, const double* __restrict__ kx, double leftX, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sn, cs;
+    sincos(leftX * kx[i], &sn, &cs);
+    const double a = (double)n * cs, b = (double)n * sn;
+    const double d = a * a + b * b;
+    ph[i] = cmk(a / d, -b / d);
+}
+
+int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n) {
+    hipLaunchKernelGGL(k_phase, blocks_for(n, 256), 256, 0, stream(), ph, kx, leftX, s, n);
+    CH_LAUNCH_CHECK("k_phase");
+    return ST_OK;
+}
+
+// ================================================================== FB GEMM
+
+constexpr int GB_X = 64, GB_N = 64, GB_K = 16, GB_THREADS = 256;
+
+__device__ __forceinline__ double2 seg_load(const FbSeg& s, int64_t x, int64_t kk, int nx, int64_t lda) {
+    int64_t xr = x;
+    if (s.refl) xr = (x == 0) ? 0 : nx - x;
+    const int64_t off = xr + lda * kk;
+    double2 v0 = s.a0[off];
+    if (s.refl) v0 = cneg(cconj(v0));
+    double2 v = cmul(s.c0, v0);
+    if (s.a1) {
+        double2 v1 = s.a1[off];
+        if (s.refl) v1 = cneg(cconj(v1));
+        v = cadd(v, cmul(s.c1, v1));
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict__ jobs, FbGemmArgs g) {
+    const FbJob& J = jobs[blockIdx.z];
+    const int n0 = blockIdx.y * GB_N;
+    if (n0 >= J.Ncol) return;
+    const int x0 = blockIdx.x * GB_X;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+
+    __shared__ double2 As[GB_K][GB_X];
+    __shared__ double Ds[GB_K][GB_N + 1];
+
+    double2 acc[2][4][4];
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[s][i][j] = cmk(0.0, 0.0);
+
+    for (int si = 0; si < J.nseg; si++) {
+        const FbSeg& S = J.seg[si];
+        const int K = J.K;
+        for (int k0 = 0; k0 < K; k0 += GB_K) {
+#pragma unroll
+            for (int i = 0; i < (GB_K * GB_X) / GB_THREADS; i++) {
+                const int e = tid + GB_THREADS * i;
+                const int xl = e % GB_X, kk = e / GB_X;
+                const int64_t xg = x0 + xl;
+                double2 v = cmk(0.0, 0.0);
+                if (xg < g.nx && k0 + kk < K) v = seg_load(S, xg, k0 + kk, g.nx, g.lda);
+                As[kk][xl] = v;
+            }
+#pragma unroll
+            for (int i = 0; i < (GB_K * GB_N) / GB_THREADS; i++) {
+                const int e = tid + GB_THREADS * i;
+                const int kk = e % GB_K, nl = e / GB_K;
+                double v = 0.0;
+                if (k0 + kk < K && n0 + nl < J.Ncol) v = S.D[(int64_t)(k0 + kk) + S.ldd * (int64_t)(n0 + nl)];
+                Ds[kk][nl] = v;
+            }
+            __syncthreads();
+            if (S.acc == 0) {
+#pragma unroll
+                for (int kk = 0; kk < GB_K; kk++) {
+                    double2 a[4];
+                    double d[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) d[j] = Ds[kk][ty + 16 * j];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            acc[0][i][j].x += a[i].x * d[j];
+                            acc[0][i][j].y += a[i].y * d[j];
+                        }
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < GB_K; kk++) {
+                    double2 a[4];
+                    double d[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) d[j] = Ds[kk][ty + 16 * j];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            acc[1][i][j].x += a[i].x * d[j];
+                            acc[1][i][j].y += a[i].y * d[j];
+                        }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // ---- epilogue
+    for (int o = 0; o < J.nout; o++) {
+        const FbOut& O = J.out[o];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t xg = x0 + tx + 16 * i;
+            if (xg >= g.nx) continue;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t ng = n0 + ty + 16 * j;
+                if (ng >= J.Ncol) continue;
+                double2 v = cadd(cmul(O.alpha0, acc[0][i][j]), cmul(O.alpha1, acc[1][i][j]));
+                const int64_t off = xg + g.ldo * ng;
+                if (O.diag) v = cadd(v, crs(g.kx[xg], cmul(O.gamma, O.diag[off])));
+                if (O.scale) v = crs(O.scale[off], v);
+                double2* dst = O.p + off;
+                if (O.poiss) {
+                    // maxwell_solvers.f90:153-158: J += (v + (g_np1 - g_n)*dt_inv)*w2_inv
+                    const double2 d = crs(g.dt_inv, csub(O.poiss_np1[off], O.poiss_n[off]));
+                    *dst = cadd(*dst, crs(O.poiss_w2[off], cadd(v, d)));
+                } else {
+                    *dst = J.beta ? cadd(*dst, v) : v;
+                }
+            }
+        }
+    }
+}
+
+int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g) {
+    if (jobs.empty()) return ST_OK;
+    int maxN = 0;
+    for (const auto& j : jobs) maxN = j.Ncol > maxN ? j.Ncol : maxN;
+    const FbJob* dj = static_cast<const FbJob*>(device_table(jobs.data(), jobs.size() * sizeof(FbJob)));
+    if (!dj) return ST_ENOMEM;
+    dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + GB_N - 1) / GB_N), (unsigned)jobs.size());
+    hipLaunchKernelGGL(k_fb_gemm, grid, GB_THREADS, 0, stream(), dj, g);
+    CH_LAUNCH_CHECK("k_fb_gemm");
+    return ST_OK;
+}
+
+// ================================================================== job builders
+
+static FbSeg seg(const double2* a0, double2 c0, const double2* a1, double2 c1, const double* D, int64_t ldd, int refl,
+                 int acc) {
+    FbSeg s;
+    s.a0 = a0;
+    s.a1 = a1;
+    s.c0 = c0;
+    s.c1 = c1;
+    s.D = D;
+    s.ldd = ldd;
+    s.refl = refl;
+    s.acc = acc;
+    return s;
+}
+
+static FbOut outp(double2* p, double2 al0, double2 al1, const double2* diag = nullptr, double2 gamma = {0, 0}) {
+    FbOut o;
+    memset(&o, 0, sizeof o);
+    o.p = p;
+    o.alpha0 = al0;
+    o.alpha1 = al1;
+    o.diag = diag;
+    o.gamma = gamma;
+    return o;
+}
+
+static FbJob job0() {
+    FbJob j;
+    memset(&j, 0, sizeof j);
+    return j;
+}
+
+static const double2 ONE = {1, 0}, MONE = {-1, 0}, I1 = {0, 1}, MI = {0, -1}, ZERO = {0, 0};
+
+void build_dht_jobs(std::vector<FbJob>& jobs, const double2* in, int64_t in_m, int64_t in_l, const double* D,
+                    int64_t ldd, int64_t d_m, int K, double2* out, int64_t out_m, int64_t out_l, int Ncol,
+                    int nmodes, int ncomp) {
+    for (int l = 0; l < ncomp; l++)
+        for (int m = 0; m < nmodes; m++) {
+            FbJob j = job0();
+            j.seg[0] = seg(in + m * in_m + l * in_l, ONE, nullptr, ZERO, D + m * d_m, ldd, 0, 0);
+            j.nseg = 1;
+            j.out[0] = outp(out + m * out_m + l * out_l, ONE, ZERO);
+            j.nout = 1;
+            j.K = K;
+            j.Ncol = Ncol;
+            jobs.push_back(j);
+        }
+}
+
+// Views of an FB-space array a(nkx, ncol, 0:nmod-1, ncomp): column base of (mode, comp).
+struct FbView {
+    const double2* p;
+    int64_t nkx, ncol, nmod;
+    const double2* at(int m, int c) const { return p + nkx * ncol * (m + nmod * c); }
+};
+struct FbViewW {
+    double2* p;
+    int64_t nkx, ncol, nmod;
+    double2* at(int m, int c) const { return p + nkx * ncol * (m + nmod * c); }
+};
+
+void build_rot_jobs(std::vector<FbJob>& jobs, FbViewW loc, FbView v, const double* Dp, const double* Dm, int nkr,
+                    int nkr_loc, int nkO, const double* scale_tab) {
+    auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
+    for (int m = 0; m <= nkO; m++) {
+        // loc_x = -D+ (i v_y - v_z)[m+1] - D- (i v_y + v_z)[m-1]      fb_math.f90:47-53, :63-69, :76-82
+        FbJob a = job0();
+        if (m < nkO) a.seg[a.nseg++] = seg(v.at(m + 1, 1), I1, v.at(m + 1, 2), MONE, D(Dp, m), nkr, 0, 0);
+        if (m > 0) a.seg[a.nseg++] = seg(v.at(m - 1, 1), I1, v.at(m - 1, 2), ONE, D(Dm, m), nkr, 0, 0);
+        else if (nkO >= 1) a.seg[a.nseg++] = seg(v.at(1, 1), I1, v.at(1, 2), ONE, D(Dm, 0), nkr, 1, 0);
+        a.out[0] = outp(loc.at(m, 0), MONE, ZERO);
+        a.nout = 1;
+        a.K = nkr;
+        a.Ncol = nkr_loc;
+        // loc_y = -i kx v_z + i W + i V ; loc_z = i kx v_y - W + V     fb_math.f90:42-45, :55-60, :70-75
+        FbJob b = job0();
+        if (m < nkO) b.seg[b.nseg++] = seg(v.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 0);
+        if (m > 0) b.seg[b.nseg++] = seg(v.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 1);
+        else if (nkO >= 1) b.seg[b.nseg++] = seg(v.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 1);
+        b.out[0] = outp(loc.at(m, 1), I1, I1, v.at(m, 2), MI);
+        b.out[1] = outp(loc.at(m, 2), MONE, ONE, v.at(m, 1), I1);
+        b.nout = 2;
+        b.K = nkr;
+        b.Ncol = nkr_loc;
+        if (scale_tab) {
+            const double* sc = scale_tab + (int64_t)v.nkx * nkr_loc * m;
+            a.out[0].scale = sc;
+            b.out[0].scale = sc;
+            b.out[1].scale = sc;
+        }
+        jobs.push_back(a);
+        jobs.push_back(b);
+    }
+}
+
+void build_grad_jobs(std::vector<FbJob>& jobs, FbViewW loc, FbView s, const double* Dp, const double* Dm, int nkr,
+                     int nkr_loc, int nkO, int nmod_in) {
+    (void)nmod_in;
+    auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
+    for (int m = 0; m <= nkO; m++) {
+        // fb_math.f90:119-144: loc_x = i kx s[m]; loc_y = -V + W; loc_z = i V + i W
+        FbJob j = job0();
+        if (m > 0) j.seg[j.nseg++] = seg(s.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 0);
+        else if (nkO >= 1) j.seg[j.nseg++] = seg(s.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 0);
+        if (m < nkO) j.seg[j.nseg++] = seg(s.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 1);
+        j.out[0] = outp(loc.at(m, 0), ZERO, ZERO, s.at(m, 0), I1);
+        j.out[1] = outp(loc.at(m, 1), MONE, ONE);
+        j.out[2] = outp(loc.at(m, 2), I1, I1);
+        j.nout = 3;
+        j.K = nkr;
+        j.Ncol = nkr_loc;
+        jobs.push_back(j);
+    }
+}
+
+// div of v (modes 0..nkO) into out modes 0..nmod_out-1; D+ exists for m < nkO.
+void build_div_jobs(std::vector<FbJob>& jobs, FbViewW out, FbView v, const double* Dp, const double* Dm, int nkr,
+                    int nkr_loc, int nkO, int nmod_out) {
+    auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
+    for (int m = 0; m < nmod_out; m++) {
+        // fb_math.f90:173-194
+        FbJob j = job0();
+        if (m > 0) j.seg[j.nseg++] = seg(v.at(m - 1, 2), I1, v.at(m - 1, 1), MONE, D(Dm, m), nkr, 0, 0);
+        else if (nkO >= 1) j.seg[j.nseg++] = seg(v.at(1, 2), I1, v.at(1, 1), MONE, D(Dm, 0), nkr, 1, 0);
+        if (m < nkO) j.seg[j.nseg++] = seg(v.at(m + 1, 2), I1, v.at(m + 1, 1), ONE, D(Dp, m), nkr, 0, 0);
+        if (m <= nkO) j.out[0] = outp(out.at(m, 0), ONE, ZERO, v.at(m, 0), I1);
+        else j.out[0] = outp(out.at(m, 0), ONE, ZERO);
+        j.nout = 1;
+        j.K = nkr;
+        j.Ncol = nkr_loc;
+        jobs.push_back(j);
+    }
+}
+
+// Second half of grad-div: gradient of s (modes 0..nkO+1) into v (modes 0..nkO).
+void build_graddiv2_jobs(std::vector<FbJob>& jobs, FbViewW v, FbView s, const double* Dp, const double* Dm, int nkr,
+                         int nkr_loc, int nkO) {
+    auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
+    for (int m = 0; m <= nkO; m++) {
+        // fb_math.f90:266-288
+        FbJob j = job0();
+        if (m > 0) j.seg[j.nseg++] = seg(s.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 0);
+        else if (nkO >= 1) j.seg[j.nseg++] = seg(s.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 0);
+        j.seg[j.nseg++] = seg(s.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 1);
+        j.out[0] = outp(v.at(m, 0), ZERO, ZERO, s.at(m, 0), I1);
+        j.out[1] = outp(v.at(m, 1), MONE, ONE);
+        j.out[2] = outp(v.at(m, 2), I1, I1);
+        j.nout = 3;
+        j.K = nkr_loc;
+        j.Ncol = nkr_loc;
+        jobs.push_back(j);
+    }
+}
+
+// ================================================================== composite device-side operations
+
+int32_t fb_forward(double2* out, const double2* in, double leftX, const double* kx_dev, const double* In_dev, int nkx,
+                   int nr, int nmodes, int nkr, int ncomp, const double* scale_tab) {
+    const size_t tmp_elems = (size_t)nkx * nkr * nmodes * ncomp;
+    double2* tmp = static_cast<double2*>(scratch(0, tmp_elems * sizeof(double2)));
+    double2* ph = static_cast<double2*>(scratch(1, (size_t)nkx * sizeof(double2)));
+    if (!tmp || !ph) return ST_ENOMEM;
+    std::vector<FbJob> jobs;
+    // vec(nkx, 0:nr, nmodes, ncomp): row 1 (first real node) is the DHT's r = 1
+    build_dht_jobs(jobs, in + nkx, (int64_t)nkx * (nr + 1), (int64_t)nkx * (nr + 1) * nmodes, In_dev, nr,
+                   (int64_t)nr * nkr, nr, tmp, (int64_t)nkx * nkr, (int64_t)nkx * nkr * nmodes, nkr, nmodes, ncomp);
+    FbGemmArgs g;
+    memset(&g, 0, sizeof g);
+    g.nx = nkx;
+    g.lda = nkx;
+    g.ldo = nkx;
+    g.kx = kx_dev;
+    CH_TRY(launch_fb_gemm(jobs, g));
+    CH_TRY(make_phase(ph, kx_dev, leftX, -1.0, nkx));
+    FftArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = tmp;
+    a.out = out;
+    a.nlines = (int64_t)nkr * nmodes * ncomp;
+    a.n_inner = nkr * nmodes;
+    a.in_r = nkx;
+    a.in_q = (int64_t)nkx * nkr * nmodes;
+    a.out_r = nkx;
+    a.out_q = (int64_t)nkx * nkr * nmodes;
+    a.post = ph;
+    a.scale = scale_tab;
+    a.sign = -1;
+    a.N = nkx;
+    return launch_fft(a);
+}
+
+int32_t fb_inverse(double2* eb, const double2* e_fb, const double2* b_fb, int64_t b_comp_stride, double leftX,
+                   const double* kx_dev, const double* Out_dev, int nkx, int nr, int nmodes, int nkr, int ghost) {
+    const size_t tmp_elems = (size_t)nkx * nr * nmodes * 6;
+    double2* tmp = static_cast<double2*>(scratch(0, tmp_elems * sizeof(double2)));
+    double2* ph = static_cast<double2*>(scratch(1, (size_t)nkx * sizeof(double2)));
+    if (!tmp || !ph) return ST_ENOMEM;
+    std::vector<FbJob> jobs;
+    const int64_t fbc = (int64_t)nkx * nkr * nmodes;  // e_fb component stride
+    build_dht_jobs(jobs, e_fb, (int64_t)nkx * nkr, fbc, Out_dev, nkr, (int64_t)nkr * nr, nkr, tmp, (int64_t)nkx * nr,
+                   (int64_t)nkx * nr * nmodes, nr, nmodes, 3);
+    build_dht_jobs(jobs, b_fb, (int64_t)nkx * nkr, b_comp_stride, Out_dev, nkr, (int64_t)nkr * nr, nkr,
+                   tmp + (size_t)nkx * nr * nmodes * 3, (int64_t)nkx * nr, (int64_t)nkx * nr * nmodes, nr, nmodes, 3);
+    FbGemmArgs g;
+    memset(&g, 0, sizeof g);
+    g.nx = nkx;
+    g.lda = nkx;
+    g.ldo = nkx;
+    g.kx = kx_dev;
+    CH_TRY(launch_fb_gemm(jobs, g));
+    CH_TRY(make_phase(ph, kx_dev, leftX, +1.0, nkx));
+    FftArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = tmp;
+    a.out = eb + nkx;  // row 1
+    a.nlines = (int64_t)nr * nmodes * 6;
+    a.n_inner = nr;
+    a.in_r = nkx;
+    a.in_q = (int64_t)nkx * nr;
+    a.out_r = nkx;
+    a.out_q = (int64_t)nkx * (nr + 1);
+    a.pre = ph;
+    a.sign = +1;
+    a.ghost = ghost;
+    a.nmodes = nmodes;
+    a.N = nkx;
+    return launch_fft(a);
+}
+
+int32_t fb_filter(double2* vec, double leftX, const double* kx_dev, const double* filtr_dev, int nxfilt, int nkx,
+                  int nkr, int nmodes, int ncomp) {
+    double2* ph = static_cast<double2*>(scratch(1, (size_t)nkx * sizeof(double2)));
+    double2* phi = static_cast<double2*>(scratch(2, (size_t)nkx * sizeof(double2)));
+    if (!ph || !phi) return ST_ENOMEM;
+    CH_TRY(make_phase(ph, kx_dev, leftX, +1.0, nkx));
+    hipLaunchKernelGGL(k_phase_inv, blocks_for(nkx, 256), 256, 0, stream(), phi, kx_dev, leftX, nkx);
+    CH_LAUNCH_CHECK("k_phase_inv");
+    FftArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = vec;
+    a.out = vec;
+    a.nlines = (int64_t)nkr * nmodes * ncomp;
+    a.n_inner = 1;
+    a.in_r = 0;
+    a.in_q = nkx;
+    a.out_r = 0;
+    a.out_q = nkx;
+    a.pre = ph;
+    a.post = phi;
+    a.sign = +1;
+    a.filtr = filtr_dev;
+    a.nxfilt = nxfilt < nkx ? nxfilt : nkx;
+    a.N = nkx;
+    return launch_fft(a);
+}
+
+int32_t fb_op(int op, double2* out, const double2* in, const double* Dp, const double* Dm, const double* kx_dev,
+              int nkx, int nkr, int nkO, int nkr_loc, const double* scale_tab) {
+    std::vector<FbJob> jobs;
+    FbGemmArgs g;
+    memset(&g, 0, sizeof g);
+    g.nx = nkx;
+    g.lda = nkx;
+    g.ldo = nkx;
+    g.kx = kx_dev;
+    const int nmod = nkO + 1;
+    switch (op) {
+        case FB_ROT:
+            build_rot_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
+                           nkO, scale_tab);
+            return launch_fb_gemm(jobs, g);
+        case FB_GRAD:
+            build_grad_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
+                            nkO, nmod);
+            return launch_fb_gemm(jobs, g);
+        case FB_DIV:
+            build_div_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
+                           nkO, nmod);
+            return launch_fb_gemm(jobs, g);
+        case FB_GRADDIV: {
+            if (nkr_loc != nkr) return fail(ST_EUNSUP, "fb_graddiv needs nkr_loc == nkr (got %d, %d)", nkr_loc, nkr);
+            double2* s = static_cast<double2*>(scratch(3, (size_t)nkx * nkr_loc * (nkO + 2) * sizeof(double2)));
+            if (!s) return ST_ENOMEM;
+            build_div_jobs(jobs, FbViewW{s, nkx, nkr_loc, nkO + 2}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
+                           nkO, nkO + 2);
+            CH_TRY(launch_fb_gemm(jobs, g));
+            jobs.clear();
+            build_graddiv2_jobs(jobs, FbViewW{out, nkx, nkr, nmod}, FbView{s, nkx, nkr_loc, nkO + 2}, Dp, Dm, nkr,
+                                nkr_loc, nkO);
+            return launch_fb_gemm(jobs, g);
+        }
+        default:
+            return fail(ST_EINVAL, "unknown FB operator %d", op);
+    }
+}
+
+}  // namespace chimera
+
+using namespace chimera;
+
+static int32_t pow2_check(int nkx) {
+    if (nkx < 1 || (nkx & (nkx - 1)) != 0 || nkx > 8192)
+        return fail(ST_EUNSUP, "nkx = %d: the x transform needs a power of two <= 8192", nkx);
+    return ST_OK;
+}
+
+extern "C" {
+
+int32_t chimera_fb_vec_in(double* vec_fb, const double* vec, double leftX, const double* kx, const double* In,
+                          int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr) {
+    if (nkx < 1 || nr < 1 || nkO < 1 || nkr < 1) return fail(ST_EINVAL, "fb_vec_in: bad sizes");
+    CH_TRY(pow2_check(nkx));
+    Stage st;
+    double2* o = reinterpret_cast<double2*>(st.out(vec_fb, 2 * (size_t)nkx * nkr * nkO * 3));
+    const double2* v = reinterpret_cast<const double2*>(st.in(vec, 2 * (size_t)nkx * (nr + 1) * nkO * 3));
+    const double* k = st.in(kx, nkx);
+    const double* M = st.in(In, (size_t)nr * nkr * nkO);
+    CH_TRY(st.status());
+    CH_TRY(fb_forward(o, v, leftX, k, M, nkx, nr, nkO, nkr, 3, nullptr));
+    return st.finish();
+}
+
+int32_t chimera_fb_scl_in(double* scl_fb, const double* scl, double leftX, const double* kx, const double* In,
+                          int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr) {
+    if (nkx < 1 || nr < 1 || nkO < 1 || nkr < 1) return fail(ST_EINVAL, "fb_scl_in: bad sizes");
+    CH_TRY(pow2_check(nkx));
+    Stage st;
+    double2* o = reinterpret_cast<double2*>(st.out(scl_fb, 2 * (size_t)nkx * nkr * nkO));
+    const double2* v = reinterpret_cast<const double2*>(st.in(scl, 2 * (size_t)nkx * (nr + 1) * nkO));
+    const double* k = st.in(kx, nkx);
+    const double* M = st.in(In, (size_t)nr * nkr * nkO);
+    CH_TRY(st.status());
+    CH_TRY(fb_forward(o, v, leftX, k, M, nkx, nr, nkO, nkr, 1, nullptr));
+    return st.finish();
+}
+
+int32_t chimera_fb_eb_out(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
+                          const double* Out, int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr) {
+    if (nkx < 1 || nr < 1 || nkO < 1 || nkr < 1) return fail(ST_EINVAL, "fb_eb_out: bad sizes");
+    CH_TRY(pow2_check(nkx));
+    Stage st;
+    double2* eb = reinterpret_cast<double2*>(st.out(eb_spc, 2 * (size_t)nkx * (nr + 1) * nkO * 6));
+    const double2* e = reinterpret_cast<const double2*>(st.in(e_fb, 2 * (size_t)nkx * nkr * nkO * 6));
+    const double2* b = reinterpret_cast<const double2*>(st.in(b_fb, 2 * (size_t)nkx * nkr * nkO * 3));
+    const double* k = st.in(kx, nkx);
+    const double* M = st.in(Out, (size_t)nkr * nr * nkO);
+    CH_TRY(st.status());
+    CH_TRY(fb_inverse(eb, e, b, (int64_t)nkx * nkr * nkO, leftX, k, M, nkx, nr, nkO, nkr, 1));
+    return st.finish();
+}
+
+int32_t chimera_fb_filtr(double* vec, double leftX, const double* kx, const double* filtr, int32_t modefilt,
+                         int32_t nkx, int32_t nkr, int32_t nkO, int32_t nxfilt) {
+    if (modefilt != 0)
+        return fail(ST_EUNSUP, "fb_filtr: modefilt=%d; only 0 ('left') is defined (the reference's other modes index "
+                               "the profile out of its bounds)", modefilt);
+    if (nkx < 1 || nkr < 1 || nkO < 1 || nxfilt < 0) return fail(ST_EINVAL, "fb_filtr: bad sizes");
+    CH_TRY(pow2_check(nkx));
+    Stage st;
+    double2* v = reinterpret_cast<double2*>(st.inout(vec, 2 * (size_t)nkx * nkr * nkO * 3));
+    const double* k = st.in(kx, nkx);
+    const double* f = st.in(filtr, (size_t)(nxfilt > 0 ? nxfilt : 0));
+    CH_TRY(st.status());
+    CH_TRY(fb_filter(v, leftX, k, nxfilt > 0 ? f : nullptr, nxfilt, nkx, nkr, nkO, 3));
+    return st.finish();
+}
+
+static int32_t fb_op_entry(int op, double* out, const double* in, const double* DpS2S, const double* DmS2S,
+                           const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc, int in_comp,
+                           int out_comp) {
+    if (nkx < 1 || nkr < 1 || nkO < 0 || nkr_loc < 1) return fail(ST_EINVAL, "FB operator: bad sizes");
+    if (nkr_loc > nkr) return fail(ST_EINVAL, "FB operator: nkr_loc=%d exceeds nkr=%d", nkr_loc, nkr);
+    Stage st;
+    const size_t nin = (size_t)nkx * nkr * (nkO + 1) * in_comp;
+    const size_t nout = (size_t)nkx * nkr_loc * (nkO + 1) * out_comp;
+    double2* o;
+    const double2* i;
+    if (op == FB_GRADDIV) {
+        o = reinterpret_cast<double2*>(st.inout(out, 2 * nin));
+        i = o;
+    } else {
+        o = reinterpret_cast<double2*>(st.out(out, 2 * nout));
+        i = reinterpret_cast<const double2*>(st.in(in, 2 * nin));
+    }
+    const size_t nd = (size_t)nkr * nkr_loc * (nkO + 2);
+    const double* Dp = st.in(DpS2S, nd);
+    const double* Dm = st.in(DmS2S, nd);
+    const double* k = st.in(kx, nkx);
+    CH_TRY(st.status());
+    CH_TRY(fb_op(op, o, i, Dp, Dm, k, nkx, nkr, nkO, nkr_loc, nullptr));
+    return st.finish();
+}
+
+int32_t chimera_fb_rot(double* vec_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                       const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc) {
+    return fb_op_entry(FB_ROT, vec_fb_loc, vec_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 3, 3);
+}
+
+int32_t chimera_fb_grad(double* vec_fb_loc, const double* scl_fb, const double* DpS2S, const double* DmS2S,
+                        const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc) {
+    return fb_op_entry(FB_GRAD, vec_fb_loc, scl_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 1, 3);
+}
+
+int32_t chimera_fb_div(double* scl_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                       const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc) {
+    return fb_op_entry(FB_DIV, scl_fb_loc, vec_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 3, 1);
+}
+
+int32_t chimera_fb_graddiv(double* vec_fb, const double* DpS2S, const double* DmS2S, const double* kx, int32_t nkx,
+                           int32_t nkr, int32_t nkO, int32_t nkr_loc) {
+    return fb_op_entry(FB_GRADDIV, vec_fb, vec_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 3, 3);
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+// runtime.hip -- library state: stream, error text, staging buffers, scratch
+// memory and the cached device copies of small host tables.
+#include <map>
+#include <mutex>
+
+#include "common.hpp"
+#include "../../include/chimera_hip.h"
+
+namespace chimera {
+
+namespace {
+thread_local std::string g_err;
+hipStream_t g_stream = nullptr;
+std::mutex g_mu;
+
+struct ScratchSlot { void* p = nullptr; size_t bytes = 0; };
+ScratchSlot g_scratch[SCRATCH_SLOTS];
+
+std::map<std::string, void*> g_tables;
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int32_t fail(int32_t code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int32_t hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    return e == hipErrorOutOfMemory ? ST_ENOMEM : ST_EHIP;
+}
+
+hipStream_t stream() { return g_stream; }
+void set_stream(hipStream_t s) { g_stream = s; }
+
+bool is_device_ptr(const void* p) {
+    if (p == nullptr) return false;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // unregistered host memory reports an error: clear it
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+void* Stage::put(void* p, size_t bytes, bool copy_in, bool copy_out) {
+    if (st_ != ST_OK) return nullptr;
+    if (bytes == 0) return p;
+    if (p == nullptr) {
+        st_ = fail(ST_EINVAL, "null array argument");
+        return nullptr;
+    }
+    if (is_device_ptr(p)) return p;
+    any_host_ = true;
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) {
+        st_ = hip_fail(e, "hipMalloc (staging)", __FILE__, __LINE__);
+        return nullptr;
+    }
+    bufs_.push_back({p, d, bytes, copy_out});
+    if (copy_in) {
+        e = hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, g_stream);
+        if (e != hipSuccess) {
+            st_ = hip_fail(e, "hipMemcpyAsync H2D (staging)", __FILE__, __LINE__);
+            return nullptr;
+        }
+    }
+    return d;
+}
+
+int32_t Stage::finish() {
+    if (st_ != ST_OK) return st_;
+    if (!any_host_) return ST_OK;
+    for (auto& b : bufs_)
+        if (b.copy_out) CH_HIP(hipMemcpyAsync(b.host, b.dev, b.bytes, hipMemcpyDeviceToHost, g_stream));
+    CH_HIP(hipStreamSynchronize(g_stream));
+    for (auto& b : bufs_) {
+        (void)hipFree(b.dev);
+        b.dev = nullptr;
+    }
+    bufs_.clear();
+    return ST_OK;
+}
+
+Stage::~Stage() {
+    if (!bufs_.empty()) {
+        (void)hipStreamSynchronize(g_stream);
+        for (auto& b : bufs_)
+            if (b.dev) (void)hipFree(b.dev);
+    }
+}
+
+void* scratch(int slot, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ScratchSlot& s = g_scratch[slot];
+    if (s.bytes >= bytes && s.p) return s.p;
+    if (s.p) {
+        (void)hipStreamSynchronize(g_stream);
+        (void)hipFree(s.p);
+        s.p = nullptr;
+        s.bytes = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    if (hipMalloc(&s.p, want) != hipSuccess) {
+        s.p = nullptr;
+        set_error("hipMalloc of %zu scratch bytes failed", want);
+        return nullptr;
+    }
+    s.bytes = want;
+    return s.p;
+}
+
+const void* device_table(const void* host, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::string key(static_cast<const char*>(host), bytes);
+    auto it = g_tables.find(key);
+    if (it != g_tables.end()) return it->second;
+    if (g_tables.size() > 4096) {  // bound the cache: tables of a stale run are dropped
+        (void)hipStreamSynchronize(g_stream);
+        for (auto& kv : g_tables) (void)hipFree(kv.second);
+        g_tables.clear();
+    }
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) {
+        set_error("hipMalloc of a %zu-byte table failed", bytes);
+        return nullptr;
+    }
+    if (hipMemcpy(d, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        set_error("upload of a %zu-byte table failed", bytes);
+        return nullptr;
+    }
+    g_tables.emplace(std::move(key), d);
+    return d;
+}
+
+}  // namespace chimera
+
+using namespace chimera;
+
+extern "C" {
+
+const char* chimera_version(void) { return "chimera_hip 0.1 (gfx950)"; }
+
+const char* chimera_last_error(void) { return g_err.c_str(); }
+
+int32_t chimera_set_stream(void* hip_stream) {
+    set_stream(static_cast<hipStream_t>(hip_stream));
+    return ST_OK;
+}
+
+void* chimera_get_stream(void) { return static_cast<void*>(stream()); }
+
+int32_t chimera_synchronize(void) {
+    CH_HIP(hipStreamSynchronize(stream()));
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+/*
+ * chimera_hip.h -- C ABI of libchimera_hip, the MI355X (gfx950) implementation
+ * of CHIMERA's particle-in-cell step.
+ *
+ * Drop-in tier: one entry point per routine of the reference's f2py module
+ * `fimera` on the PIC-step path.  The argument list of each is the Fortran
+ * subroutine's dummy-argument list, in the same order and with the same
+ * meaning (including the dimension arguments f2py hides, with the meaning the
+ * Fortran declaration gives them: e.g. `nx` is the last index of a 0:nx array,
+ * and `nkO` is the MAXIMUM mode in grid_deps/fb_math routines but the NUMBER
+ * of modes in fb_io/maxwell_solvers routines).  Scalars are passed by value.
+ * Arrays are Fortran-ordered; complex(kind=8) arrays are interleaved (re, im)
+ * doubles passed as `double*`.
+ *
+ * Array pointers may point to host or to device memory, argument by argument:
+ * device pointers are used in place (the call is then asynchronous on the
+ * library stream), host pointers are staged through device buffers and the
+ * call returns when their results are back in host memory.  Scalar outputs
+ * (num2stay, GoOut, indPart, IndInChnk) are host pointers.
+ *
+ * Every entry point returns 0 on success or a negative status
+ * (CHIMERA_EINVAL, CHIMERA_EHIP, CHIMERA_ENOMEM, CHIMERA_EUNSUP) and never
+ * aborts; chimera_last_error() describes the last failure of the calling thread.
+ *
+ * Reference file:line citations are relative to the reference repository
+ * (hightower8083/chimera).
+ */
+#ifndef CHIMERA_HIP_H
+#define CHIMERA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHIMERA_OK 0
+#define CHIMERA_EINVAL (-1)
+#define CHIMERA_EHIP (-2)
+#define CHIMERA_ENOMEM (-3)
+#define CHIMERA_EUNSUP (-4)
+
+/* ---------------------------------------------------------------- runtime */
+const char* chimera_version(void);
+const char* chimera_last_error(void);
+/* Stream (hipStream_t) every later launch goes to; NULL = the null stream. */
+int32_t chimera_set_stream(void* hip_stream);
+void* chimera_get_stream(void);
+int32_t chimera_synchronize(void);
+
+/* ---------------------------------------------------------------- particle_tools.f90 */
+
+/* Relativistic Boris push.  momenta(3,np) in/out, Fld(6,np).
+ * Replaces push_velocs, particle_tools.f90:18-56. */
+int32_t chimera_push_velocs(double* momenta, const double* Fld, double dt, int32_t np);
+
+/* Leap-frog position push; coord_cntr = half-step positions.
+ * Replaces push_coords, particle_tools.f90:58-82. */
+int32_t chimera_push_coords(double* coord, const double* momenta, double* coord_cntr, double dt, int32_t np);
+
+/* FixedCell lattice.  coord(4,np) out, *indPart = particles written;
+ * RandPackO(nx,nr); PackO complex(PPC).  Replaces genparts, particle_tools.f90:84-128. */
+int32_t chimera_genparts(double* coord, int32_t* indPart, const double* Xgrid, const double* Rgrid,
+                         const double* RandPackO, const double* PackX, const double* PackR, const double* PackO,
+                         int32_t np, int32_t nx, int32_t nr, int32_t PPC);
+
+/* Domain filter: zero-based indices of the particles inside lims, in order.
+ * Replaces sortpartsout, particle_tools.f90:130-153. */
+int32_t chimera_sortpartsout(int32_t* indx2stay, int32_t* num2stay, const double* coord, const double* lims,
+                             int32_t np);
+
+/* Chunk id per particle (-2 outside lims), chunk offsets IndInChnk(0:nchnk)
+ * and the count of particles outside.  Xgrid(0:nx).
+ * Replaces chunk_coords_boundaries, particle_tools.f90:155-208. */
+int32_t chimera_chunk_coords_boundaries(int8_t* chunked_indx, int32_t* IndInChnk, int32_t* GoOut, const double* coord,
+                                        const double* lims, const double* Xgrid, int32_t nchnk, int32_t np,
+                                        int32_t nx);
+
+/* dat(:,1:np) = dat(:,idx+1) for dat(3,np0) / dat(np0); idx zero-based int64.
+ * Replaces align_data_vec / align_data_scl, particle_tools.f90:270-324. */
+int32_t chimera_align_data_vec(double* dat, const int64_t* chunked_indx, int32_t np, int32_t np0);
+int32_t chimera_align_data_scl(double* dat, const int64_t* chunked_indx, int32_t np, int32_t np0);
+
+/* ---------------------------------------------------------------- grid_deps.f90 / grid_deps_chnk.f90 */
+
+/* Current deposition onto curr(0:nx,0:nr,0:nkO,3) (accumulated), then the
+ * ghost-row fold.  Replaces dep_curr, grid_deps.f90:18-87. */
+int32_t chimera_dep_curr(const double* coord, const double* momenta, const double* wghts, double* curr, double leftX,
+                         const double* Rgrid, double dx_inv, double dr_inv, int32_t np, int32_t nx, int32_t nr,
+                         int32_t nkO);
+
+/* Charge deposition onto dens(0:nx,0:nr,0:nkO).  Replaces dep_dens, grid_deps.f90:89-147. */
+int32_t chimera_dep_dens(const double* coord, const double* wghts, double* dens, double leftX, const double* Rgrid,
+                         double dx_inv, double dr_inv, int32_t np, int32_t nx, int32_t nr, int32_t nkO);
+
+/* Chunked variants: deposit particles IndInChunk(0)..IndInChunk(nchnk)-1 with
+ * the unchunked semantics (no guard-cell race, no dropped edge deposits).
+ * Replace dep_curr_chnk / dep_dens_chnk, grid_deps_chnk.f90:18-130, :132-234. */
+int32_t chimera_dep_curr_chnk(const double* coord, const double* momenta, const double* wghts, double* curr,
+                              const int32_t* IndInChunk, int32_t guards, double leftX, const double* Rgrid,
+                              double dx_inv, double dr_inv, int32_t np, int32_t nx, int32_t nr, int32_t nkO,
+                              int32_t nchnk);
+int32_t chimera_dep_dens_chnk(const double* coord, const double* wghts, double* dens, const int32_t* IndInChunk,
+                              int32_t guards, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
+                              int32_t np, int32_t nx, int32_t nr, int32_t nkO, int32_t nchnk);
+
+/* Field gather: Fld_tot(6,np) += fields of Fld(0:nx,0:nr,0:nkO,6) at the
+ * particles.  Replaces proj_fld, grid_deps.f90:149-217. */
+int32_t chimera_proj_fld(const double* coord, const double* wghts, const double* Fld, double* Fld_tot, double leftX,
+                         const double* Rgrid, double dx_inv, double dr_inv, int32_t np, int32_t nx, int32_t nr,
+                         int32_t nkO);
+
+/* Mode normalisation and ghost row of eb_spc(0:nx,0:nr,nkO,6) (nkO = number
+ * of modes).  Replaces eb_correction, grid_deps.f90:219-266. */
+int32_t chimera_eb_correction(double* eb_spc, int32_t nx, int32_t nr, int32_t nkO);
+
+/* ---------------------------------------------------------------- fb_io.f90 (nkO = number of modes) */
+
+/* Forward FB transform: DHT with In(nr,nkr,nkO) over rows 1..nr of
+ * vec(nkx,0:nr,nkO,3), FFT along x, times exp(-i kx leftX) -> vec_fb(nkx,nkr,nkO,3).
+ * Replaces fb_vec_in, fb_io.f90:18-59; fb_scl_in (one component), fb_io.f90:61-98. */
+int32_t chimera_fb_vec_in(double* vec_fb, const double* vec, double leftX, const double* kx, const double* In,
+                          int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr);
+int32_t chimera_fb_scl_in(double* scl_fb, const double* scl, double leftX, const double* kx, const double* In,
+                          int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr);
+
+/* Inverse FB transform of E (components 1..3 of e_fb(nkx,nkr,nkO,6)) and
+ * b_fb(nkx,nkr,nkO,3) into eb_spc(nkx,0:nr,nkO,6) (ghost row zero).
+ * Replaces fb_eb_out, fb_io.f90:182-228. */
+int32_t chimera_fb_eb_out(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
+                          const double* Out, int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr);
+
+/* Absorbing layer on vec(nkx,nkr,nkO,3); modefilt 0 ('left') only.
+ * Replaces fb_filtr, fb_io.f90:230-308. */
+int32_t chimera_fb_filtr(double* vec, double leftX, const double* kx, const double* filtr, int32_t modefilt,
+                         int32_t nkx, int32_t nkr, int32_t nkO, int32_t nxfilt);
+
+/* ---------------------------------------------------------------- fb_math.f90 (nkO = max mode) */
+
+/* Curl: vec_fb(nkx,nkr,0:nkO,3) -> vec_fb_loc(nkx,nkr_loc,0:nkO,3); DpS2S,
+ * DmS2S(nkr,nkr_loc,0:nkO+1).  Replaces fb_rot, fb_math.f90:18-94. */
+int32_t chimera_fb_rot(double* vec_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                       const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc);
+/* Gradient of scl_fb(nkx,nkr,0:nkO).  Replaces fb_grad, fb_math.f90:96-149. */
+int32_t chimera_fb_grad(double* vec_fb_loc, const double* scl_fb, const double* DpS2S, const double* DmS2S,
+                        const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc);
+/* Divergence.  Replaces fb_div, fb_math.f90:151-199. */
+int32_t chimera_fb_div(double* scl_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                       const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc);
+/* Gradient of divergence, in place (needs nkr_loc == nkr).
+ * Replaces fb_graddiv, fb_math.f90:201-293. */
+int32_t chimera_fb_graddiv(double* vec_fb, const double* DpS2S, const double* DmS2S, const double* kx, int32_t nkx,
+                           int32_t nkr, int32_t nkO, int32_t nkr_loc);
+
+/* ---------------------------------------------------------------- maxwell_solvers.f90 (nkO = number of modes) */
+
+/* PSATD step with space charge; C1, C2 real (nkx,nkr,nkO,5).
+ * Replaces maxwell_push_with_spchrg, maxwell_solvers.f90:18-60. */
+int32_t chimera_maxwell_push_with_spchrg(double* EG_fb, const double* j_fb, const double* grad_rho_n_fb,
+                                         const double* grad_rho_np1_fb, const double* C1, const double* C2,
+                                         int32_t nkx, int32_t nkr, int32_t nkO);
+/* PSATD step without space charge; C1, C2 real (nkx,nkr,nkO,3) (the Fortran
+ * declares them complex and the driver passes real tables that f2py converts).
+ * Replaces maxwell_push_wo_spchrg, maxwell_solvers.f90:62-96. */
+int32_t chimera_maxwell_push_wo_spchrg(double* EG_fb, const double* j_fb, const double* C1, const double* C2,
+                                       int32_t nkx, int32_t nkr, int32_t nkO);
+/* Static initial push; C1, C2 complex (nkx,nkr,nkO,2).
+ * Replaces maxwell_init_push, maxwell_solvers.f90:98-129. */
+int32_t chimera_maxwell_init_push(double* EG_fb, const double* j_fb, const double* grad_rho_n_fb, const double* C1,
+                                  const double* C2, int32_t nkx, int32_t nkr, int32_t nkO);
+/* j += (grad div j + (grad rho_np1 - grad rho_n) dt_inv) w2_inv.
+ * Replaces poiss_corr, maxwell_solvers.f90:131-164. */
+int32_t chimera_poiss_corr(double* j_fb, const double* grad_div_j_fb, const double* grad_rho_n_fb,
+                           const double* grad_rho_np1_fb, double dt_inv, const double* w2_inv, int32_t nkx,
+                           int32_t nkr, int32_t nkO);
+/* Elementwise helpers.  Replace omp_mult_vec/scl, omp_add_vec/scl, maxwell_solvers.f90:228-318. */
+int32_t chimera_omp_mult_vec(double* vec_fb, const double* A, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_omp_mult_scl(double* scl_fb, const double* A, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_omp_add_vec(double* vec_fb, const double* A, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_omp_add_scl(double* scl_fb, const double* A, int32_t nkx, int32_t nkr, int32_t nkO);
+
+/* ---------------------------------------------------------------- devices.f90 */
+
+/* Analytic planar undulator, params = (a0, lambda, X0, Lx); Fld(6,np) += field.
+ * Replaces Undul_Analytic, devices.f90:162-203. */
+int32_t chimera_undul_analytic(const double* coord, double* Fld, double t, const double* params, int32_t np);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHIMERA_HIP_H */

@@ -1,0 +1,261 @@
+"""Pins the oracle (oracle/cpu_ref.c) before anything is compared with it.
+
+* Against the reference itself: the reference's own Fortran for every routine
+  that builds without FFTW (oracle/_ref, compiled from the reference sources by
+  oracle/Makefile) -- results must be bit-identical, since both sides are plain
+  IEEE double arithmetic in the same order.
+* For fb_io.f90 (FFTW3 absent, so not buildable): against numpy.fft and dense
+  linear algebra, and against closed forms (a single Bessel mode maps to one FB
+  coefficient; a pure phase shift; the absorbing profile on a constant line).
+* Invariants that a dropped term, wrong sign or transposed index would break:
+  deposition conserves charge, gather is the adjoint of deposition, the FB
+  operators satisfy div(rot) = 0 and rot(grad) = 0 with the reference's own
+  operator construction, Boris conserves |u| without E and rotates by 2 atan|t|.
+"""
+import numpy as np
+import pytest
+
+from tests._cases import C, F, Grid, particles, rel_l2, rel_max, rng
+
+
+# ------------------------------------------------------------------ oracle == reference Fortran (bit-exact)
+
+def test_push_matches_reference(port, reference):
+    r = rng(3)
+    u, f = F(r, 3, 2000), F(r, 6, 2000)
+    assert np.array_equal(port.push_velocs(u.copy(order="F"), f, 0.37), reference.push_velocs(u.copy(order="F"), f, 0.37))
+    x, xc = F(r, 3, 2000), F(r, 3, 2000)
+    a = port.push_coords(x.copy(order="F"), u, xc.copy(order="F"), 0.05)
+    b = reference.push_coords(x.copy(order="F"), u, xc.copy(order="F"), 0.05)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("nko", [0, 1, 3])
+def test_deposit_gather_match_reference(port, reference, nko):
+    g = Grid(nx=40, nkr=12, nko=nko)
+    coord, mom, w = particles(g, 3000, seed=10 + nko)
+    z = np.zeros((g.nx, g.nkr + 1, nko + 1, 3), complex, order="F")
+    args = (g.leftX, g.Rgrid, 1 / g.dx, 1 / g.dr)
+    assert np.array_equal(port.dep_curr(coord, mom, w, z.copy(order="F"), *args),
+                          reference.dep_curr(coord, mom, w, z.copy(order="F"), *args))
+    zs = np.zeros((g.nx, g.nkr + 1, nko + 1), complex, order="F")
+    assert np.array_equal(port.dep_dens(coord, w, zs.copy(order="F"), *args),
+                          reference.dep_dens(coord, w, zs.copy(order="F"), *args))
+    EB = C(rng(5), g.nx, g.nkr + 1, nko + 1, 6)
+    ft = F(rng(6), 6, coord.shape[1])
+    assert np.array_equal(port.proj_fld(coord, w, EB, ft.copy(order="F"), *args),
+                          reference.proj_fld(coord, w, EB, ft.copy(order="F"), *args))
+    assert np.array_equal(port.eb_correction(EB.copy(order="F")), reference.eb_correction(EB.copy(order="F")))
+
+
+@pytest.mark.parametrize("nko", [1, 2, 3])
+def test_fb_math_matches_reference(port, reference, nko):
+    r = rng(20 + nko)
+    nkx, nkr = 32, 10
+    kx = 2 * np.pi * np.fft.fftfreq(nkx, 0.1)
+    Dp, Dm = F(r, nkr, nkr, nko + 2), F(r, nkr, nkr, nko + 2)
+    v, s = C(r, nkx, nkr, nko + 1, 3), C(r, nkx, nkr, nko + 1)
+    z3 = np.zeros((nkx, nkr, nko + 1, 3), complex, order="F")
+    z1 = np.zeros((nkx, nkr, nko + 1), complex, order="F")
+    for name, out, inp in (("fb_rot", z3, v), ("fb_grad", z3, s), ("fb_div", z1, v)):
+        a = getattr(port, name)(out.copy(order="F"), inp, Dp, Dm, kx)
+        b = getattr(reference, name)(out.copy(order="F"), inp, Dp, Dm, kx)
+        assert np.array_equal(a, b), name
+    assert np.array_equal(port.fb_graddiv(v.copy(order="F"), Dp, Dm, kx),
+                          reference.fb_graddiv(v.copy(order="F"), Dp, Dm, kx))
+
+
+def test_maxwell_match_reference(port, reference):
+    r = rng(30)
+    sh = (24, 10, 3)
+    EG = C(r, *sh, 6)
+    j, gn, gp, gd = C(r, *sh, 3), C(r, *sh, 3), C(r, *sh, 3), C(r, *sh, 3)
+    C1, C2 = F(r, *sh, 5), F(r, *sh, 5)
+    assert np.array_equal(port.maxwell_push_with_spchrg(EG.copy(order="F"), j, gn, gp, C1, C2),
+                          reference.maxwell_push_with_spchrg(EG.copy(order="F"), j, gn, gp, C1, C2))
+    assert np.array_equal(port.maxwell_push_wo_spchrg(EG.copy(order="F"), j, C1[..., :3], C2[..., :3]),
+                          reference.maxwell_push_wo_spchrg(EG.copy(order="F"), j, C1[..., :3], C2[..., :3]))
+    CC1, CC2 = C(r, *sh, 2), C(r, *sh, 2)
+    assert np.array_equal(port.maxwell_init_push(EG.copy(order="F"), j, gn, CC1, CC2),
+                          reference.maxwell_init_push(EG.copy(order="F"), j, gn, CC1, CC2))
+    w2 = F(r, *sh)
+    assert np.array_equal(port.poiss_corr(j.copy(order="F"), gd, gn, gp, 20.0, w2),
+                          reference.poiss_corr(j.copy(order="F"), gd, gn, gp, 20.0, w2))
+    for name, A in (("omp_mult_vec", w2), ("omp_add_vec", gn)):
+        assert np.array_equal(getattr(port, name)(j.copy(order="F"), A), getattr(reference, name)(j.copy(order="F"), A))
+    s = C(r, *sh)
+    assert np.array_equal(port.omp_mult_scl(s.copy(order="F"), w2), reference.omp_mult_scl(s.copy(order="F"), w2))
+    assert np.array_equal(port.omp_add_scl(s.copy(order="F"), s), reference.omp_add_scl(s.copy(order="F"), s))
+
+
+def test_particle_tools_match_reference(port, reference):
+    g = Grid(nx=30, nkr=10, nko=1)
+    coord, mom, w = particles(g, 2500, seed=40)
+    p = np.array([1.7, 0.9, -2.0, 3.5])
+    ft = F(rng(41), 6, coord.shape[1])
+    assert np.array_equal(port.undul_analytic(coord, ft.copy(order="F"), 0.0, p),
+                          reference.undul_analytic(coord, ft.copy(order="F"), 0.0, p))
+    Xg = g.leftX + g.dx * np.arange(g.nx)
+    RP = rng(42).random((g.nx, g.nkr + 1))
+    pX, pR, pO = np.mgrid[1:2:2j, 1:2:2j, 1:4:4j]
+    PX, PR = (pX.ravel() - 0.5) / 2, (pR.ravel() - 0.5) / 2
+    PO = np.exp(2j * np.pi * (pO.ravel() - 1) / 4)
+    co = np.zeros((4, g.nx * (g.nkr + 1) * 16), order="F")
+    a = port.genparts(co.copy(order="F"), Xg, g.Rgrid, RP, PX, PR, PO)
+    b = reference.genparts(co.copy(order="F"), Xg, g.Rgrid, RP, PX, PR, PO)
+    assert a[1] == b[1] == (g.nx - 1) * g.nkr * 16
+    assert np.array_equal(a[0], b[0])
+    lims = np.array([g.leftX + 0.5, g.leftX + 2.5, 0.0, 4.0])
+    a, b = port.sortpartsout(coord, lims), reference.sortpartsout(coord, lims)
+    assert a[1] == b[1] and np.array_equal(a[0], b[0])
+    Xg0 = g.leftX + g.dx * np.arange(g.nx + 1)
+    a, b = port.chunk_coords_boundaries(coord, lims, Xg0, 4), reference.chunk_coords_boundaries(coord, lims, Xg0, 4)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    idx = rng(43).permutation(coord.shape[1])[:1700]
+    assert np.array_equal(port.align_data_vec(coord.copy(order="F"), idx), reference.align_data_vec(coord.copy(order="F"), idx))
+    assert np.array_equal(port.align_data_scl(w.copy(), idx), reference.align_data_scl(w.copy(), idx))
+
+
+# ------------------------------------------------------------------ fb_io.f90: numpy and closed forms
+
+def _np_fb_in(vec, leftX, kx, In):
+    # DHT over rows 1..nr, numpy FFT along x, phase exp(-i kx leftX)  (fb_io.f90:42-53)
+    A = np.einsum("xrml,rkm->xkml", vec[:, 1:], In)
+    return np.fft.fft(A, axis=0) * np.exp(-1j * kx * leftX)[:, None, None, None]
+
+
+def test_fb_in_matches_numpy(port):
+    r = rng(50)
+    nkx, nr, nkr, M = 64, 12, 11, 3
+    vec = C(r, nkx, nr + 1, M, 3)
+    In = F(r, nr, nkr, M)
+    kx = 2 * np.pi * np.fft.fftfreq(nkx, 0.07)
+    out = port.fb_vec_in(np.zeros((nkx, nkr, M, 3), complex, order="F"), vec, -1.3, kx, In)
+    assert rel_l2(out, _np_fb_in(vec, -1.3, kx, In)) < 1e-14
+    outs = port.fb_scl_in(np.zeros((nkx, nkr, M), complex, order="F"), vec[..., 0].copy(order="F"), -1.3, kx, In)
+    assert rel_l2(outs, _np_fb_in(vec[..., :1], -1.3, kx, In)[..., 0]) < 1e-14
+
+
+def test_fb_in_non_power_of_two_line(port):
+    r = rng(51)
+    nkx, nr, nkr, M = 24, 5, 5, 2
+    vec = C(r, nkx, nr + 1, M, 3)
+    In = F(r, nr, nkr, M)
+    kx = 2 * np.pi * np.fft.fftfreq(nkx, 0.2)
+    out = port.fb_vec_in(np.zeros((nkx, nkr, M, 3), complex, order="F"), vec, 0.4, kx, In)
+    assert rel_l2(out, _np_fb_in(vec, 0.4, kx, In)) < 1e-13
+
+
+def test_fb_eb_out_matches_numpy(port):
+    r = rng(52)
+    nkx, nr, nkr, M = 32, 9, 9, 2
+    e, b = C(r, nkx, nkr, M, 6), C(r, nkx, nkr, M, 3)
+    Out = F(r, nkr, nr, M)
+    kx = 2 * np.pi * np.fft.fftfreq(nkx, 0.1)
+    eb = port.fb_eb_out(np.ones((nkx, nr + 1, M, 6), complex, order="F"), e, b, 0.77, kx, Out)
+    ph = np.exp(1j * kx * 0.77)[:, None, None, None]
+    ref_e = np.fft.ifft(np.einsum("xkml,krm->xrml", e[..., :3], Out) * ph, axis=0) * nkx
+    ref_b = np.fft.ifft(np.einsum("xkml,krm->xrml", b, Out) * ph, axis=0) * nkx
+    assert rel_l2(eb[:, 1:, :, :3], ref_e) < 1e-14
+    assert rel_l2(eb[:, 1:, :, 3:], ref_b) < 1e-14
+    assert not np.any(eb[:, 0])                      # ghost row zeroed (fb_io.f90:199)
+
+
+def test_fb_single_bessel_mode(port):
+    """J_m(kr_j r) exp(i kx_q x) on the grid maps to exactly one FB coefficient
+    when In = inv(Out) (solvers.py:742-743): the transform is diagonal."""
+    from scipy.special import jn, jn_zeros
+    nkx, nkr, M, dx, dr = 32, 16, 2, 0.1, 0.3
+    lengthR = dr * (nkr + 0.5)
+    Rg = dr * (np.arange(nkr + 1) - 0.5)
+    kx = 2 * np.pi * np.fft.fftfreq(nkx, dx)
+    In = np.zeros((nkr, nkr, M), order="F")
+    for m in range(M):
+        kr = jn_zeros(m, nkr) / lengthR
+        Out = jn(m, Rg[1:, None] * kr[None, :])        # (r, k)
+        In[:, :, m] = np.linalg.inv(Out).T                 # swapped axes (solvers.py:152-153): In(r, k)
+    q, j, m = 5, 3, 1
+    x = np.arange(nkx) * dx
+    kr = jn_zeros(m, nkr) / lengthR
+    vec = np.zeros((nkx, nkr + 1, M), complex, order="F")
+    vec[:, 1:, m] = np.exp(1j * kx[q] * x)[:, None] * jn(m, Rg[1:] * kr[j])[None, :]
+    out = port.fb_scl_in(np.zeros((nkx, nkr, M), complex, order="F"), vec, 0.0, kx, In)
+    expect = np.zeros_like(out)
+    expect[q, j, m] = nkx
+    assert np.abs(out - expect).max() < 1e-9 * nkx
+
+
+def test_fb_filtr_closed_forms(port):
+    nkx, nkr, M = 64, 4, 2
+    kx = 2 * np.pi * np.fft.fftfreq(nkx, 0.1)
+    filt = np.linspace(0.0, 1.0, 20)
+    # leftX = 0, a line that is constant in x: the filtered line is the profile
+    v = np.zeros((nkx, nkr, M, 3), complex, order="F")
+    v[0] = 3.0 * nkx          # FFT of a constant 3 (unnormalised)
+    out = port.fb_filtr(v.copy(order="F"), 0.0, kx, filt, 0)
+    line = np.fft.ifft(out[:, 0, 0, 0])
+    prof = np.ones(nkx) * 3.0
+    prof[:20] *= filt
+    assert np.abs(line - prof).max() < 1e-12
+    # with a profile of ones the filter is the identity for any leftX
+    r = rng(53)
+    w = C(r, nkx, nkr, M, 3)
+    out = port.fb_filtr(w.copy(order="F"), 1.234, kx, np.ones(30), 0)
+    assert rel_max(out, w) < 1e-13
+    with pytest.raises(ValueError):
+        port.fb_filtr(w.copy(order="F"), 0.0, kx, filt, 1)
+
+
+# ------------------------------------------------------------------ invariants of the particle kernels
+
+def test_deposit_conserves_charge_and_gather_is_adjoint(port):
+    g = Grid(nx=48, nkr=12, nko=2)
+    coord, mom, w = particles(g, 3000, seed=60)
+    args = (g.leftX, g.Rgrid, 1 / g.dx, 1 / g.dr)
+    dens = port.dep_dens(coord, w, np.zeros((g.nx, g.nkr + 1, 3), complex, order="F"), *args)
+    # m = 0: the four bilinear weights sum to 1; the ghost fold moves row 0 into row 1 with a minus sign,
+    # so total charge minus twice the ghost-row share is conserved: sum = sum(w) - 2*sum(ghost part)
+    rp = np.hypot(coord[1], coord[2])
+    fr = (rp - g.Rgrid[0]) / g.dr
+    ir = np.floor(fr).astype(int)
+    s1 = (rp - g.Rgrid[ir]) / g.dr
+    ghost = np.where(ir == 0, (1 - s1) * w, 0.0)
+    assert abs(dens[:, :, 0].sum().real - (w.sum() - 2 * ghost.sum())) < 1e-11 * w.sum()
+    assert abs(dens[:, :, 0].sum().imag) < 1e-12 * w.sum()
+    # adjointness: with phases e^{+im phi} in the gather and e^{-im phi} in the deposit,
+    # sum_p w_p Fgathered_p = Re sum_nodes F * conj(rho).  Rows 0 and 1 of F are zeroed so
+    # that the ghost fold (which only touches those rows) drops out.
+    r = rng(61)
+    Fld = C(r, g.nx, g.nkr + 1, 3, 6)
+    Fld[:, :2] = 0.0
+    ft = port.proj_fld(coord, w, Fld, np.zeros((6, coord.shape[1]), order="F"), *args)
+    for l in (0, 4):
+        lhs = (ft[l] * w).sum()
+        rhs = np.real((Fld[..., l] * np.conj(dens)).sum())
+        assert abs(lhs - rhs) < 1e-12 * np.abs(ft[l] * w).sum()
+
+
+def test_boris_closed_forms(port):
+    r = rng(70)
+    n = 500
+    u = F(r, 3, n)
+    # E = 0: |u| conserved
+    fld = np.zeros((6, n), order="F")
+    fld[3:] = F(r, 3, n)
+    out = port.push_velocs(u.copy(order="F"), fld, 0.2)
+    assert np.allclose(np.linalg.norm(out, axis=0), np.linalg.norm(u, axis=0), rtol=1e-14, atol=0)
+    # uniform B along x: rotation angle 2*atan(|t|), t = dt/2 * B / gamma
+    B = 1.3
+    fld = np.zeros((6, n), order="F")
+    fld[3] = B
+    out = port.push_velocs(u.copy(order="F"), fld, 0.2)
+    gam = np.sqrt(1 + (u ** 2).sum(0))
+    ang = 2 * np.arctan(0.1 * B / gam)
+    a0 = np.arctan2(u[2], u[1])
+    a1 = np.arctan2(out[2], out[1])
+    assert np.allclose(np.angle(np.exp(1j * (a0 - a1))), ang, atol=1e-12)
+    # pure E kick: du = dt*E
+    fld = np.zeros((6, n), order="F")
+    fld[:3] = F(r, 3, n)
+    out = port.push_velocs(u.copy(order="F"), fld, 0.2)
+    assert np.allclose(out - u, 0.2 * fld[:3], rtol=0, atol=1e-14)
