@@ -1,0 +1,231 @@
+"""ChimeraRun: the time loop of the PIC step, driving resident Solver/Specie
+objects through libchimera_hip.  Same constructor dictionary and methods as the
+reference's ChimeraRun (chimera_main.py:23-318) and the same call order inside
+make_halfstep / make_step (chimera_main.py:61-92); the per-particle work of
+make_field + proj_fld + make_device + push_velocs runs as one fused gather-push
+per species, and push_coords precedes the deposits exactly as in the reference.
+
+SimComps may carry ``'Features': ('PreDepositBackground',)``: the still
+background is then deposited before the initial half step, so that the static
+initial field of a neutral plasma starts from the neutral charge density (the
+reference deposits it only on moving-frame steps, chimera_main.py:277-284, and
+its make_halfstep therefore sees electrons without ions).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .device import call
+
+
+class ChimeraRun:
+    def __init__(self, SimComps):
+        self.Particles = SimComps.get("Particles", ())
+        self.Solvers = SimComps.get("Solvers", ())
+        self.MovingFrames = SimComps["MovingFrames"] if "MovingFrames" in SimComps else ({},)
+        self.Features = SimComps.get("Features", ())
+        self.init_Moving_Frames()
+        self.init_damp_profile()
+        if "PreDepositBackground" in self.Features:
+            for solver in self.Solvers:
+                if "StillAsBackground" in solver.Args["Features"]:
+                    self.dep_bg(solver)
+        self.make_halfstep()
+
+    def init_Moving_Frames(self):
+        for wind in self.MovingFrames:                                     # chimera_main.py:41-52
+            wind.setdefault("Features", ())
+            wind.setdefault("TimeActive", (0.0, np.inf))
+            wind.setdefault("Velocity", 1.0)
+            wind.setdefault("TimeStep", 1.0)
+            wind.setdefault("Steps", 1)
+            f = 0.5 if "Staged" in wind["Features"] else 1.0
+            wind["shiftX"] = f * wind["Velocity"] * wind["TimeStep"] * wind["Steps"]
+
+    def init_damp_profile(self):
+        for solver in self.Solvers:
+            for wind in self.MovingFrames:
+                if "AbsorbLayer" in wind:
+                    solver.Args["damp_profile"] = solver.get_damp_profile(wind["AbsorbLayer"])
+
+    # ------------------------------------------------------------------ the step
+    def make_halfstep(self):
+        for species in self.Particles:                                     # chimera_main.py:61-80
+            a = self.Solvers[0].Args if len(self.Solvers) > 0 else self.Particles[0].Args
+            SimDom = np.asfortranarray([a["leftX"], a["rightX"], 0, a["upperR"] ** 2])
+            if SimDom[0] != SimDom[1]:
+                species.chunk_and_damp(SimDom=SimDom)
+        self.project_current()
+        self.project_density()
+        for solver in self.Solvers:
+            for species in self.Particles:
+                solver.maxwell_solver_stat(species.Args["MomentaMeans"][0])
+        self.project_fields()
+        self.gather_push(0, dt_frac=0.5)
+
+    def make_step(self, istep):
+        self.frame_act(istep)                                              # chimera_main.py:82-92
+        for species in self.Particles:
+            species.push_coords()
+        self.chunk_particles(istep)
+        self.project_current()
+        self.frame_act(istep, "stage2")
+        self.project_density()
+        self.update_fields()
+        self.project_fields()
+        self.gather_push(istep)
+
+    # ------------------------------------------------------------------ sources
+    def project_current(self):
+        for solver in self.Solvers:
+            if "NoCurrent" in solver.Args["Features"]:
+                solver.Data["J"].zero_()
+                solver.Data["J_fb"].zero_()
+                continue
+            self.dep_curr(solver)
+            solver.fb_curr_in()
+
+    def project_density(self):
+        for solver in self.Solvers:
+            if "SpaceCharge" not in solver.Args["Features"]:
+                continue
+            D = solver.Data
+            D["gradRho_fb_prv"], D["gradRho_fb_nxt"] = D["gradRho_fb_nxt"], D["gradRho_fb_prv"]
+            self.dep_dens(solver)
+            solver.fb_dens_in()
+            solver.FBGradDens()
+
+    def dep_curr(self, solver):
+        """J = sum of species deposits, unfolded (the fold is applied by fb_curr_in)."""
+        J = solver.Data["J"]
+        J.zero_()
+        g = solver.geom_args()
+        for species in self.Particles:
+            if species.np == 0 or "Still" in species.Args["Features"]:
+                continue
+            call("chimera_soa_dep_curr", species.ptr("coords_halfstep"), species.ptr("momenta"),
+                 species.ptr("weights"), species.ld, species.np, J.ptr, *g)
+
+    def dep_dens(self, solver, component="coords"):
+        R = solver.Data["Rho"]
+        if "StillAsBackground" in solver.Args["Features"]:
+            R.copy_(solver.Data["BckGrndRho"])
+        else:
+            R.zero_()
+        g = solver.geom_args()
+        for species in self.Particles:
+            if species.np == 0 or "Still" in species.Args["Features"]:
+                continue
+            call("chimera_soa_dep_dens", species.ptr(component), species.ptr("weights"), species.ld, species.np,
+                 R.ptr, *g)
+
+    def dep_bg(self, solver):
+        B = solver.Data["BckGrndRho"]
+        B.zero_()
+        g = solver.geom_args()
+        for species in self.Particles:
+            if species.np == 0 or "Still" not in species.Args["Features"]:
+                continue
+            call("chimera_soa_dep_dens", species.ptr("coords"), species.ptr("weights"), species.ld, species.np,
+                 B.ptr, *g)
+
+    # ------------------------------------------------------------------ fields
+    def update_fields(self):
+        for solver in self.Solvers:                                        # chimera_main.py:116-128
+            solver.poiss_corr()
+            solver.maxwell_solver()
+
+    def project_fields(self):
+        for solver in self.Solvers:                                        # chimera_main.py:130-151
+            if "NoPush" in solver.Args["Features"]:
+                solver.Data["EB"].zero_()
+                continue
+            solver.G2B_FBRot()
+            solver.fb_fld_out()
+
+    def gather_push(self, istep, dt_frac=1.0):
+        """make_field + proj_fld (each solver) + make_device + push_velocs, fused
+        per species (chimera_main.py:130-151, :91-92; species.py:246-298)."""
+        if len(self.Solvers) > 1:
+            raise NotImplementedError("one field solver per run is supported")
+        for species in self.Particles:
+            if species.np == 0 or "Still" in species.Args["Features"]:
+                continue
+            A = species.Args
+            und = species.undulator_params()
+            push_dt = float(A["push_fact"] * dt_frac * A["TimeStep"])
+            if self.Solvers and "NoPush" not in self.Solvers[0].Args["Features"]:
+                solver = self.Solvers[0]
+                EB, g = solver.Data["EB"].ptr, solver.geom_args()
+            else:
+                EB, g = 0, (0.0, 0, 1.0, 1.0, 0.0, -1.0, 1, 1, 0)   # rmax < 0: nothing gathered
+            call("chimera_soa_gather_push", species.ptr("coords"), species.ptr("momenta"), species.ptr("weights"),
+                 species.ld, species.np, EB, *g, push_dt, und.ctypes.data if und is not None else None)
+
+    # ------------------------------------------------------------------ moving frames (chimera_main.py:250-318)
+    def damp_fields(self, wind):
+        if "AbsorbLayer" not in wind or wind["AbsorbLayer"] <= 0:
+            return
+        for solver in self.Solvers:
+            solver.damp_field()
+
+    def damp_plasma(self, wind):
+        if "AbsorbLayer" not in wind:
+            return
+        for species in self.Particles:
+            species.chunk_and_damp(wind=wind)
+
+    def add_plasma(self, wind):
+        if "AddPlasma" not in wind:
+            return
+        xs = int(wind["shiftX"] / wind["TimeStep"]) + 1
+        if "IonsOnTop" in wind["Features"]:
+            s1, s2 = self.Particles[:2]
+            parts = s1.gen_parts(Xsteps=xs, ProfileFunc=wind["AddPlasma"])
+            s1.add_particles(*parts)
+            parts[-1] = parts[-1] * -1
+            s2.add_particles(*parts)
+        else:
+            for species in self.Particles:
+                species.add_particles(*species.gen_parts(Xsteps=xs, ProfileFunc=wind["AddPlasma"]))
+
+    def postframe_corr(self, wind):
+        # chimera_main.py:277-284: the condition is always true in the reference
+        for solver in self.Solvers:
+            if "SpaceCharge" in solver.Args["Features"]:
+                if "StillAsBackground" in solver.Args["Features"]:
+                    self.dep_bg(solver)
+                # the reference also re-deposits Rho here; project_density overwrites it before any use
+
+    def move_frame(self, wind):
+        for comp in tuple(self.Solvers) + tuple(self.Particles):
+            comp.Args["Xgrid"] = comp.Args["Xgrid"] + wind["shiftX"]
+            comp.Args["leftX"] = comp.Args["Xgrid"][0]
+            comp.Args["rightX"] = comp.Args["Xgrid"][-1]
+
+    def frame_act(self, istep, act="stage1"):
+        for wind in self.MovingFrames:
+            if istep < wind["TimeActive"][0] or istep > wind["TimeActive"][1]:
+                continue
+            if np.mod(istep - wind["TimeActive"][0], wind["Steps"]) != 0:
+                continue
+            if act == "stage1":
+                self.damp_fields(wind)
+                self.move_frame(wind)
+                self.add_plasma(wind)
+                self.damp_plasma(wind)
+                self.postframe_corr(wind)
+            elif act == "stage2" and "Staged" in wind["Features"]:
+                self.move_frame(wind)
+
+    def chunk_particles(self, istep=0):
+        for species in self.Particles:                                     # chimera_main.py:306-318
+            if "Xchunked" not in species.Args or "NoSorting" in species.Args["Features"]:
+                continue
+            if np.mod(istep, species.Args["Xchunked"][1] + 1) != 0:
+                continue
+            if species.np == 0:
+                continue
+            a = self.Solvers[0].Args if len(self.Solvers) > 0 else self.Particles[0].Args
+            SimDom = np.asfortranarray([a["leftX"], a["rightX"], 0, a["Rgrid"].max() ** 2])
+            species.chunk_and_damp(SimDom=SimDom, position="cntr")

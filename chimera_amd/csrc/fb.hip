@@ -171,10 +171,12 @@ __device__ __forceinline__ double2 seg_load(const FbSeg& s, int64_t x, int64_t k
     if (s.refl) xr = (x == 0) ? 0 : nx - x;
     const int64_t off = xr + lda * kk;
     double2 v0 = s.a0[off];
+    if (s.fold && kk == 0) v0 = csub(v0, s.a0[off - lda]);
     if (s.refl) v0 = cneg(cconj(v0));
     double2 v = cmul(s.c0, v0);
     if (s.a1) {
         double2 v1 = s.a1[off];
+        if (s.fold && kk == 0) v1 = csub(v1, s.a1[off - lda]);
         if (s.refl) v1 = cneg(cconj(v1));
         v = cadd(v, cmul(s.c1, v1));
     }
@@ -312,6 +314,7 @@ static FbSeg seg(const double2* a0, double2 c0, const double2* a1, double2 c1, c
     s.ldd = ldd;
     s.refl = refl;
     s.acc = acc;
+    s.fold = 0;
     return s;
 }
 
@@ -336,11 +339,12 @@ static const double2 ONE = {1, 0}, MONE = {-1, 0}, I1 = {0, 1}, MI = {0, -1}, ZE
 
 void build_dht_jobs(std::vector<FbJob>& jobs, const double2* in, int64_t in_m, int64_t in_l, const double* D,
                     int64_t ldd, int64_t d_m, int K, double2* out, int64_t out_m, int64_t out_l, int Ncol,
-                    int nmodes, int ncomp) {
+                    int nmodes, int ncomp, int fold = 0) {
     for (int l = 0; l < ncomp; l++)
         for (int m = 0; m < nmodes; m++) {
             FbJob j = job0();
             j.seg[0] = seg(in + m * in_m + l * in_l, ONE, nullptr, ZERO, D + m * d_m, ldd, 0, 0);
+            j.seg[0].fold = fold;
             j.nseg = 1;
             j.out[0] = outp(out + m * out_m + l * out_l, ONE, ZERO);
             j.nout = 1;
@@ -436,8 +440,12 @@ void build_div_jobs(std::vector<FbJob>& jobs, FbViewW out, FbView v, const doubl
 }
 
 // Second half of grad-div: gradient of s (modes 0..nkO+1) into v (modes 0..nkO).
+// With poiss != null the outputs take the Poisson-correction epilogue
+// v += (graddiv + (gp - gn)*dt_inv)*w2_inv (maxwell_solvers.f90:153-158).
+struct PoissArgs { const double2* gn; const double2* gp; const double* w2; };
+
 void build_graddiv2_jobs(std::vector<FbJob>& jobs, FbViewW v, FbView s, const double* Dp, const double* Dm, int nkr,
-                         int nkr_loc, int nkO) {
+                         int nkr_loc, int nkO, const PoissArgs* poiss = nullptr) {
     auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
     for (int m = 0; m <= nkO; m++) {
         // fb_math.f90:266-288
@@ -451,6 +459,16 @@ void build_graddiv2_jobs(std::vector<FbJob>& jobs, FbViewW v, FbView s, const do
         j.nout = 3;
         j.K = nkr_loc;
         j.Ncol = nkr_loc;
+        if (poiss) {
+            const int64_t fc = v.nkx * v.ncol * v.nmod;   // component stride
+            const int64_t fm = v.nkx * v.ncol;            // mode stride
+            for (int c = 0; c < 3; c++) {
+                j.out[c].poiss = 1;
+                j.out[c].poiss_n = poiss->gn + m * fm + c * fc;
+                j.out[c].poiss_np1 = poiss->gp + m * fm + c * fc;
+                j.out[c].poiss_w2 = poiss->w2 + m * fm;
+            }
+        }
         jobs.push_back(j);
     }
 }
@@ -458,7 +476,7 @@ void build_graddiv2_jobs(std::vector<FbJob>& jobs, FbViewW v, FbView s, const do
 // ================================================================== composite device-side operations
 
 int32_t fb_forward(double2* out, const double2* in, double leftX, const double* kx_dev, const double* In_dev, int nkx,
-                   int nr, int nmodes, int nkr, int ncomp, const double* scale_tab) {
+                   int nr, int nmodes, int nkr, int ncomp, const double* scale_tab, int fold) {
     const size_t tmp_elems = (size_t)nkx * nkr * nmodes * ncomp;
     double2* tmp = static_cast<double2*>(scratch(0, tmp_elems * sizeof(double2)));
     double2* ph = static_cast<double2*>(scratch(1, (size_t)nkx * sizeof(double2)));
@@ -466,7 +484,7 @@ int32_t fb_forward(double2* out, const double2* in, double leftX, const double* 
     std::vector<FbJob> jobs;
     // vec(nkx, 0:nr, nmodes, ncomp): row 1 (first real node) is the DHT's r = 1
     build_dht_jobs(jobs, in + nkx, (int64_t)nkx * (nr + 1), (int64_t)nkx * (nr + 1) * nmodes, In_dev, nr,
-                   (int64_t)nr * nkr, nr, tmp, (int64_t)nkx * nkr, (int64_t)nkx * nkr * nmodes, nkr, nmodes, ncomp);
+                   (int64_t)nr * nkr, nr, tmp, (int64_t)nkx * nkr, (int64_t)nkx * nkr * nmodes, nkr, nmodes, ncomp, fold);
     FbGemmArgs g;
     memset(&g, 0, sizeof g);
     g.nx = nkx;
@@ -597,6 +615,29 @@ int32_t fb_op(int op, double2* out, const double2* in, const double* Dp, const d
     }
 }
 
+// j += (graddiv j + (gp - gn)*dt_inv)*w2_inv, one Poisson-correction iteration
+// (solvers.py:317-326 with maxwell_solvers.f90:131-164), graddiv fused.
+int32_t fb_graddiv_poisson(double2* j, const double2* gn, const double2* gp, double dt_inv, const double* w2,
+                           const double* Dp, const double* Dm, const double* kx_dev, int nkx, int nkr, int nkO) {
+    std::vector<FbJob> jobs;
+    FbGemmArgs g;
+    memset(&g, 0, sizeof g);
+    g.nx = nkx;
+    g.lda = nkx;
+    g.ldo = nkx;
+    g.kx = kx_dev;
+    g.dt_inv = dt_inv;
+    const int nmod = nkO + 1;
+    double2* s = static_cast<double2*>(scratch(3, (size_t)nkx * nkr * (nkO + 2) * sizeof(double2)));
+    if (!s) return ST_ENOMEM;
+    build_div_jobs(jobs, FbViewW{s, nkx, nkr, nkO + 2}, FbView{j, nkx, nkr, nmod}, Dp, Dm, nkr, nkr, nkO, nkO + 2);
+    CH_TRY(launch_fb_gemm(jobs, g));
+    jobs.clear();
+    PoissArgs pa{gn, gp, w2};
+    build_graddiv2_jobs(jobs, FbViewW{j, nkx, nkr, nmod}, FbView{s, nkx, nkr, nkO + 2}, Dp, Dm, nkr, nkr, nkO, &pa);
+    return launch_fb_gemm(jobs, g);
+}
+
 }  // namespace chimera
 
 using namespace chimera;
@@ -619,7 +660,7 @@ int32_t chimera_fb_vec_in(double* vec_fb, const double* vec, double leftX, const
     const double* k = st.in(kx, nkx);
     const double* M = st.in(In, (size_t)nr * nkr * nkO);
     CH_TRY(st.status());
-    CH_TRY(fb_forward(o, v, leftX, k, M, nkx, nr, nkO, nkr, 3, nullptr));
+    CH_TRY(fb_forward(o, v, leftX, k, M, nkx, nr, nkO, nkr, 3, nullptr, 0));
     return st.finish();
 }
 
@@ -633,7 +674,7 @@ int32_t chimera_fb_scl_in(double* scl_fb, const double* scl, double leftX, const
     const double* k = st.in(kx, nkx);
     const double* M = st.in(In, (size_t)nr * nkr * nkO);
     CH_TRY(st.status());
-    CH_TRY(fb_forward(o, v, leftX, k, M, nkx, nr, nkO, nkr, 1, nullptr));
+    CH_TRY(fb_forward(o, v, leftX, k, M, nkx, nr, nkO, nkr, 1, nullptr, 0));
     return st.finish();
 }
 
@@ -712,6 +753,73 @@ int32_t chimera_fb_div(double* scl_fb_loc, const double* vec_fb, const double* D
 int32_t chimera_fb_graddiv(double* vec_fb, const double* DpS2S, const double* DmS2S, const double* kx, int32_t nkx,
                            int32_t nkr, int32_t nkO, int32_t nkr_loc) {
     return fb_op_entry(FB_GRADDIV, vec_fb, vec_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 3, 3);
+}
+
+
+/* ---------------------------------------------------------------- resident tier (device pointers) */
+
+int32_t chimera_fb_forward(double* out, const double* in, double leftX, const double* kx, const double* In,
+                           const double* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
+                           int32_t fold) {
+    if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ncomp < 1) return fail(ST_EINVAL, "fb_forward: bad sizes");
+    CH_TRY(pow2_check(nkx));
+    if (!is_device_ptr(out) || !is_device_ptr(in) || !is_device_ptr(kx) || !is_device_ptr(In) ||
+        (scale && !is_device_ptr(scale)))
+        return fail(ST_EINVAL, "fb_forward: resident entry points take device pointers");
+    return fb_forward(reinterpret_cast<double2*>(out), reinterpret_cast<const double2*>(in), leftX, kx, In, nkx, nr,
+                      nmodes, nkr, ncomp, scale, fold);
+}
+
+int32_t chimera_fb_inverse(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
+                           const double* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost) {
+    if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 2)
+        return fail(ST_EINVAL, "fb_inverse: bad sizes or ghost mode");
+    CH_TRY(pow2_check(nkx));
+    if (!is_device_ptr(eb_spc) || !is_device_ptr(e_fb) || !is_device_ptr(b_fb) || !is_device_ptr(kx) ||
+        !is_device_ptr(Out))
+        return fail(ST_EINVAL, "fb_inverse: resident entry points take device pointers");
+    return fb_inverse(reinterpret_cast<double2*>(eb_spc), reinterpret_cast<const double2*>(e_fb),
+                      reinterpret_cast<const double2*>(b_fb), (int64_t)nkx * nkr * nmodes, leftX, kx, Out, nkx, nr,
+                      nmodes, nkr, ghost);
+}
+
+int32_t chimera_fb_rot_scaled(double* vec_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                              const double* kx, const double* scale, int32_t nkx, int32_t nkr, int32_t nkO) {
+    if (nkx < 1 || nkr < 1 || nkO < 0) return fail(ST_EINVAL, "fb_rot_scaled: bad sizes");
+    if (!is_device_ptr(vec_fb_loc) || !is_device_ptr(vec_fb) || !is_device_ptr(scale))
+        return fail(ST_EINVAL, "fb_rot_scaled: resident entry points take device pointers");
+    return fb_op(FB_ROT, reinterpret_cast<double2*>(vec_fb_loc), reinterpret_cast<const double2*>(vec_fb), DpS2S,
+                 DmS2S, kx, nkx, nkr, nkO, nkr, scale);
+}
+
+int32_t chimera_fb_graddiv_poisson(double* j_fb, const double* grad_rho_n_fb, const double* grad_rho_np1_fb,
+                                   double dt_inv, const double* w2_inv, const double* DpS2S, const double* DmS2S,
+                                   const double* kx, int32_t nkx, int32_t nkr, int32_t nkO) {
+    if (nkx < 1 || nkr < 1 || nkO < 0) return fail(ST_EINVAL, "fb_graddiv_poisson: bad sizes");
+    if (!is_device_ptr(j_fb) || !is_device_ptr(grad_rho_n_fb) || !is_device_ptr(grad_rho_np1_fb) ||
+        !is_device_ptr(w2_inv))
+        return fail(ST_EINVAL, "fb_graddiv_poisson: resident entry points take device pointers");
+    return fb_graddiv_poisson(reinterpret_cast<double2*>(j_fb), reinterpret_cast<const double2*>(grad_rho_n_fb),
+                              reinterpret_cast<const double2*>(grad_rho_np1_fb), dt_inv, w2_inv, DpS2S, DmS2S, kx,
+                              nkx, nkr, nkO);
+}
+
+int32_t chimera_fb_op(int32_t op, double* out, const double* in, const double* DpS2S, const double* DmS2S,
+                      const double* kx, int32_t nkx, int32_t nkr, int32_t nkO) {
+    if (nkx < 1 || nkr < 1 || nkO < 0 || op < 0 || op > 3) return fail(ST_EINVAL, "fb_op: bad arguments");
+    if (!is_device_ptr(out) || !is_device_ptr(in)) return fail(ST_EINVAL, "fb_op: resident entry points take device pointers");
+    return fb_op(op, reinterpret_cast<double2*>(out), reinterpret_cast<const double2*>(in), DpS2S, DmS2S, kx, nkx, nkr,
+                 nkO, nkr, nullptr);
+}
+
+int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const double* filtr, int32_t nxfilt,
+                          int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp) {
+    if (nkx < 1 || nkr < 1 || nmodes < 1 || ncomp < 1 || nxfilt < 0) return fail(ST_EINVAL, "fb_filter: bad sizes");
+    CH_TRY(pow2_check(nkx));
+    if (!is_device_ptr(vec) || !is_device_ptr(kx) || (nxfilt > 0 && !is_device_ptr(filtr)))
+        return fail(ST_EINVAL, "fb_filter: resident entry points take device pointers");
+    return fb_filter(reinterpret_cast<double2*>(vec), leftX, kx, nxfilt > 0 ? filtr : nullptr, nxfilt, nkx, nkr,
+                     nmodes, ncomp);
 }
 
 }  // extern "C"

@@ -36,6 +36,7 @@ struct FbSeg {
     int64_t ldd;
     int refl;
     int acc;
+    int fold;   // element k' = 0 minus the row before it (deposition ghost fold, grid_deps.f90:80-85)
 };
 
 struct FbOut {
@@ -70,9 +71,10 @@ int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n)
 int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g);
 
 // Forward FB transform (fb_io.f90:18-98): in(nkx,0:nr,nmodes,ncomp) ->
-// out(nkx,nkr,nmodes,ncomp), optionally times scale_tab(nkx,nkr,nmodes).
+// out(nkx,nkr,nmodes,ncomp), optionally times scale_tab(nkx,nkr,nmodes); with
+// fold, row 1 is read as row 1 - row 0 (the deposition ghost fold).
 int32_t fb_forward(double2* out, const double2* in, double leftX, const double* kx_dev, const double* In_dev, int nkx,
-                   int nr, int nmodes, int nkr, int ncomp, const double* scale_tab);
+                   int nr, int nmodes, int nkr, int ncomp, const double* scale_tab, int fold);
 // Inverse FB transform of E (first 3 components of e_fb) and B into
 // eb(nkx,0:nr,nmodes,6) (fb_io.f90:182-228); ghost as in FftArgs.
 int32_t fb_inverse(double2* eb, const double2* e_fb, const double2* b_fb, int64_t b_comp_stride, double leftX,

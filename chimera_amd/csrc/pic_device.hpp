@@ -126,4 +126,12 @@ __device__ __forceinline__ void undul_field(const UndulParams& up, double x, dou
     f[3] += ampl * cos(up.ku * (x - up.X0)) * sinh(up.ku * y);
 }
 
+// Launchers of the particle kernels (particles.hip).
+int32_t launch_push_coords(double* x, PStride sx, const double* u, PStride su, double* xc, PStride sc, double dt,
+                           int np);
+int32_t launch_dep_curr(const double* x, PStride sx, const double* u, PStride su, const double* w, double2* curr,
+                        const GridGeom& g, int ip0, int ip1);
+int32_t launch_dep_dens(const double* x, PStride sx, const double* w, double2* dens, const GridGeom& g, int ip0,
+                        int ip1);
+
 }  // namespace chimera

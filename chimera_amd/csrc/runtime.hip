@@ -149,11 +149,85 @@ const void* device_table(const void* host, size_t bytes) {
     return d;
 }
 
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t* __restrict__ p, uint8_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_axpb(double* __restrict__ p, int64_t n, double a, double b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = a * p[i] + b;
+}
+
+// Byte fill by a kernel (not hipMemsetAsync), so that a captured graph replays it correctly.
+int32_t fill_bytes(void* p, int value, size_t bytes) {
+    if (bytes == 0) return ST_OK;
+    const uint8_t v8 = (uint8_t)value;
+    if ((reinterpret_cast<uintptr_t>(p) & 3) == 0 && (bytes & 3) == 0) {
+        const uint32_t v = v8 * 0x01010101u;
+        const int64_t n = (int64_t)(bytes / 4);
+        hipLaunchKernelGGL(k_fill_u32, blocks_for(n, 256), 256, 0, g_stream, static_cast<uint32_t*>(p), v, n);
+    } else {
+        hipLaunchKernelGGL(k_fill_u8, blocks_for((int64_t)bytes, 256), 256, 0, g_stream, static_cast<uint8_t*>(p), v8,
+                           (int64_t)bytes);
+    }
+    CH_LAUNCH_CHECK("k_fill");
+    return ST_OK;
+}
+
 }  // namespace chimera
 
 using namespace chimera;
 
 extern "C" {
+
+int32_t chimera_malloc(void** ptr, int64_t bytes) {
+    if (ptr == nullptr || bytes < 0) return fail(ST_EINVAL, "chimera_malloc: bad arguments");
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 256;
+    CH_HIP(hipMalloc(ptr, (size_t)bytes));
+    CH_TRY(fill_bytes(*ptr, 0, (size_t)bytes));
+    return ST_OK;
+}
+
+int32_t chimera_free(void* ptr) {
+    if (ptr == nullptr) return ST_OK;
+    CH_HIP(hipStreamSynchronize(g_stream));
+    CH_HIP(hipFree(ptr));
+    return ST_OK;
+}
+
+int32_t chimera_memcpy(void* dst, const void* src, int64_t bytes) {
+    if (bytes < 0 || (bytes > 0 && (dst == nullptr || src == nullptr))) return fail(ST_EINVAL, "chimera_memcpy: bad arguments");
+    if (bytes == 0) return ST_OK;
+    const bool dd = is_device_ptr(dst), sd = is_device_ptr(src);
+    if (dd && sd) {
+        CH_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, g_stream));
+        return ST_OK;
+    }
+    CH_HIP(hipStreamSynchronize(g_stream));
+    CH_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDefault));
+    return ST_OK;
+}
+
+int32_t chimera_memset(void* ptr, int32_t value, int64_t bytes) {
+    if (bytes < 0 || (bytes > 0 && ptr == nullptr)) return fail(ST_EINVAL, "chimera_memset: bad arguments");
+    return fill_bytes(ptr, value, (size_t)bytes);
+}
+
+int32_t chimera_axpb(double* p, int64_t n, double a, double b) {
+    if (n < 0 || (n > 0 && p == nullptr)) return fail(ST_EINVAL, "chimera_axpb: bad arguments");
+    if (n == 0) return ST_OK;
+    hipLaunchKernelGGL(k_axpb, blocks_for(n, 256), 256, 0, g_stream, p, n, a, b);
+    CH_LAUNCH_CHECK("k_axpb");
+    return ST_OK;
+}
+
 
 const char* chimera_version(void) { return "chimera_hip 0.1 (gfx950)"; }
 

@@ -1,0 +1,211 @@
+"""Specie: a particle species with its particles resident in device memory.
+
+Same constructor dictionary, attributes and methods as the reference's Specie
+(species.py:24-398).  Particles are kept structure-of-arrays on the device:
+``coords``, ``coords_halfstep``, ``momenta`` as (3, capacity) and ``weights`` as
+(capacity,), of which the first ``np`` columns are live.  gen_parts returns host
+arrays exactly as the reference does (the random draws are NumPy's legacy
+generator in the reference's order, the FixedCell lattice is built by
+libchimera_hip); add_particles uploads them.
+"""
+from __future__ import annotations
+
+import ctypes
+import inspect
+
+import numpy as np
+
+from . import fimera as F
+from .device import DeviceArray, call, copy_elems
+
+
+class Specie:
+    def __init__(self, specie_in):
+        self.Args = specie_in
+        A = self.Args
+        A.setdefault("Features", ())
+        A.setdefault("Charge", -1.0)
+        A.setdefault("Mass", 1.0)
+        A.setdefault("Density", 0.0)
+        A.setdefault("MomentaMeans", (0.0, 0.0, 0.0))
+        A.setdefault("MomentaSpreads", (0.0, 0.0, 0.0))
+        A.setdefault("Grid", (0.0, 0.0, 0.0, 1.0, 1.0))
+        self.Devices = A["Devices"] if "Devices" in A else ()
+
+        A["leftX"], A["rightX"], A["lengthR"], A["dx"], A["dr"] = A["Grid"]
+        A["push_fact"] = 2 * np.pi * A["Charge"] / A["Mass"]             # species.py:64
+        if "Xchunked" in A:
+            A["nthrds"] = int(A["Xchunked"][0])
+            A["Nx"] = 2 * A["nthrds"] * int(np.round(0.5 / A["dx"] / A["nthrds"] * (A["rightX"] - A["leftX"])))
+        else:
+            A["Nx"] = 2 * int(np.round(0.5 / A["dx"] * (A["rightX"] - A["leftX"])))
+        if A["Nx"] > 0:
+            A["Xgrid"] = A["rightX"] - A["dx"] * np.arange(A["Nx"])[::-1]
+        else:
+            A["Xgrid"] = np.array([0.0])
+        A["leftX"] = A["Xgrid"][0]
+        A["Nr"] = int(np.round(A["lengthR"] / A["dr"]))
+        A["Rgrid"] = A["dr"] * (np.arange(A["Nr"]) - 0.5) if A["Nr"] > 0 else np.array([0.0])
+        A["lowerR"] = (A["Rgrid"] * (A["Rgrid"] >= 0)).min()
+        A["upperR"] = A["Rgrid"].max()
+
+        if "FixedCell" in A:                                                 # species.py:95-107
+            A["Num_p"] = int(np.prod(A["FixedCell"]))
+            fc = A["FixedCell"]
+            packX, packR, packO = np.mgrid[1:fc[0]:fc[0] * 1j, 1:fc[1]:fc[1] * 1j, 1:fc[2]:fc[2] * 1j]
+            packX = np.asfortranarray((packX.ravel() - 0.5) / fc[0])
+            packR = np.asfortranarray((packR.ravel() - 0.5) / fc[1])
+            packO = np.asfortranarray(np.exp(2.0j * np.pi * (packO.ravel() - 1) / fc[2]))
+            A["Pax"] = (packX, packR, packO)
+        elif "RandCell" in A:
+            raise NotImplementedError("RandCell generation is outside the PIC-step path (FixedCell is supported)")
+        else:
+            A["Num_p"] = 0
+        A["wght0"] = (A["Charge"] * A["Density"] * A["dr"] * A["dx"] * 2 * np.pi / A["Num_p"]) if A["Num_p"] else 0.0
+        A["NpSlice"] = A["Nx"] * A["Num_p"]
+
+        self.np = 0
+        self.cap = 0
+        self.Data = {}
+        self._alloc(1024)
+
+    # ------------------------------------------------------------------ storage
+    # coords, coords_halfstep, momenta: component c of particle i at c*cap + i
+    # (a Fortran (cap, 3) array); weights: (cap,).
+    def _alloc(self, cap):
+        old, oldcap = (self.Data, self.cap) if self.cap else (None, 0)
+        D = {k: DeviceArray((cap, 3), np.float64) for k in ("coords", "coords_halfstep", "momenta")}
+        D["weights"] = DeviceArray((cap,), np.float64)
+        if old is not None and self.np:
+            for k in ("coords", "coords_halfstep", "momenta"):
+                for c in range(3):
+                    copy_elems(D[k], c * cap, old[k], c * oldcap, self.np)
+            copy_elems(D["weights"], 0, old["weights"], 0, self.np)
+        self.Data = D
+        self.cap = cap
+
+    @property
+    def ld(self):
+        return self.cap
+
+    def ptr(self, key):
+        return self.Data[key].ptr
+
+    def host(self, key):
+        """Live part of a particle array as a host array in the reference's layout."""
+        a = self.Data[key].numpy()
+        if key == "weights":
+            return a[:self.np].copy()
+        return np.asfortranarray(a[:self.np].T)
+
+    # ------------------------------------------------------------------ generation (species.py:132-244)
+    def gen_parts(self, Domain=None, Xsteps=None, ProfileFunc=None):
+        A = self.Args
+        Xgrid, Rgrid = A["Xgrid"], A["Rgrid"]
+        if Domain is not None:
+            p_left, p_right, p_low, p_up = Domain
+            if A["leftX"] > p_right or A["rightX"] < p_left or p_up < A["lowerR"] or p_low > A["upperR"]:
+                return [np.zeros((3, 0), order="F"), np.zeros((3, 0), order="F"), np.zeros(0)]
+            ixb, ixe = (Xgrid < p_left).sum() - 1, (Xgrid < p_right).sum() + 1
+            Xgrid = Xgrid[ixb:ixe]
+            irb = 0 if p_low <= Rgrid.min() else (Rgrid < p_low).sum() - 1
+            ire = Rgrid.shape[0] if p_up >= Rgrid.max() else (Rgrid < p_up).sum() + 1
+            Rgrid = Rgrid[irb:ire]
+        elif Xsteps is not None:
+            Xgrid = Xgrid[-2 * Xsteps:-Xsteps]
+        if "FixedCell" not in A:
+            return [np.zeros((3, 0), order="F"), np.zeros((3, 0), order="F"), np.zeros(0)]
+        coords = np.zeros((4, Xgrid.shape[0] * Rgrid.shape[0] * A["Num_p"]), order="F")
+        RandPackO = np.random.rand(Xgrid.shape[0], Rgrid.shape[0]).astype("d", order="F")
+        coords, n = F.genparts(coords, Xgrid, Rgrid, RandPackO, *A["Pax"])
+        coords = coords[:, :n]
+        if ProfileFunc is None:
+            coords[-1] *= A["wght0"]
+        else:
+            nargs = len(inspect.signature(ProfileFunc).parameters)
+            if nargs == 1:
+                coords[-1] *= A["wght0"] * ProfileFunc(coords[0])
+            elif nargs == 2:
+                coords[-1] *= A["wght0"] * ProfileFunc(coords[0], np.sqrt((coords[1:3] ** 2).sum(0)))
+            elif nargs == 3:
+                coords[-1] *= A["wght0"] * ProfileFunc(*coords[0:3])
+            else:
+                raise ValueError("can't understand the ProfileFunc")
+        if "FlatSpectrum" in A["Features"]:
+            rand_mom = 2 * np.random.rand(3, n) - 1.0
+        else:
+            rand_mom = np.random.randn(3, n)
+        mm, ms = A["MomentaMeans"], A["MomentaSpreads"]
+        momenta = np.vstack([mm[c] + ms[c] * rand_mom[c] for c in range(3)]).astype("d", order="F")
+        weights = coords[-1].astype("d", order="F")
+        return [np.asfortranarray(coords[0:3]), momenta, weights]
+
+    def add_particles(self, coords, momenta, weights):
+        """Append host particles (species.py:217-244); the half-step copy starts at coords."""
+        n = coords.shape[-1]
+        if n == 0:
+            return
+        if self.np + n > self.cap:
+            self._alloc(int((self.np + n) * 1.25) + 1024)
+        D, cap, o = self.Data, self.cap, self.np
+        for c in range(3):
+            D["coords"].write(c * cap + o, coords[c])
+            D["coords_halfstep"].write(c * cap + o, coords[c])
+            D["momenta"].write(c * cap + o, momenta[c])
+        D["weights"].write(o, weights)
+        self.np += n
+
+    def denoise(self, WaveNums2Kill):
+        """Mirror particles in anti-phase (species.py:322-349)."""
+        for k_supp in WaveNums2Kill:
+            n = self.np
+            if 2 * n > self.cap:
+                self._alloc(2 * n + 1024)
+            D, cap = self.Data, self.cap
+            for key in ("coords", "coords_halfstep", "momenta"):
+                for c in range(3):
+                    copy_elems(D[key], c * cap + n, D[key], c * cap, n)
+            for key in ("coords", "coords_halfstep"):
+                call("chimera_axpb", D[key].offset_ptr(n), n, 1.0, 0.5 / k_supp)
+            copy_elems(D["weights"], n, D["weights"], 0, n)
+            call("chimera_axpb", D["weights"].ptr, 2 * n, 0.5, 0.0)
+            self.np = 2 * n
+
+    # ------------------------------------------------------------------ absorbing boundary (species.py:351-398)
+    def chunk_and_damp(self, wind=None, SimDom=None, position="stag"):
+        """Drop the particles outside SimDom, keeping the order of the others.
+        The reference tests coords_halfstep only on the chunked path with
+        position='cntr' (species.py:378-386); so does this."""
+        if self.np == 0:
+            return
+        A = self.Args
+        if wind is None:
+            wind = {"Features": (), "AbsorbLayer": 0.0}
+        if SimDom is None:
+            SimDom = np.array([A["leftX"] + wind["AbsorbLayer"] * A["dx"], A["rightX"], 0.0, A["upperR"] ** 2])
+        half = ("Xchunked" in A and "NoSorting" not in wind["Features"] and position == "cntr")
+        lims = np.ascontiguousarray(SimDom, dtype=np.float64)
+        nout = ctypes.c_int32(0)
+        call("chimera_soa_compact", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"),
+             self.ptr("weights"), self.ld, self.np, lims.ctypes.data, 1 if half else 0, ctypes.addressof(nout))
+        self.np = int(nout.value)
+
+    # ------------------------------------------------------------------ pushes
+    def push_coords(self, dt_frac=1.0):
+        if self.np == 0 or "Still" in self.Args["Features"]:
+            return
+        call("chimera_soa_push_coords", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"), self.ld,
+             self.np, float(dt_frac * self.Args["TimeStep"]))
+
+    def undulator_params(self):
+        """The analytic-undulator device of this species (params on the host), or None."""
+        if not self.Devices:
+            return None
+        if len(self.Devices) > 1:
+            raise NotImplementedError("one analytic undulator per species is supported")
+        dev = self.Devices[0]
+        fn = dev[0]
+        name = fn if isinstance(fn, str) else getattr(fn, "__name__", "")
+        if name.lower() != "undul_analytic":
+            raise NotImplementedError(f"device {name!r}: only the analytic undulator (devices.f90:162-203) is fused")
+        return np.ascontiguousarray(np.asarray(dev[1], dtype=np.float64)[:4])

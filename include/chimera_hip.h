@@ -48,6 +48,18 @@ int32_t chimera_set_stream(void* hip_stream);
 void* chimera_get_stream(void);
 int32_t chimera_synchronize(void);
 
+/* Device memory for resident simulations (no counterpart in the reference,
+ * whose arrays are NumPy-owned).  chimera_malloc returns zero-filled device
+ * memory; chimera_memcpy copies between any two of host/device memory (device
+ * to device asynchronously on the library stream, otherwise after draining
+ * it); chimera_memset fills bytes with a kernel (safe inside captured graphs);
+ * chimera_axpb sets p[i] = a*p[i] + b for n doubles. */
+int32_t chimera_malloc(void** ptr, int64_t bytes);
+int32_t chimera_free(void* ptr);
+int32_t chimera_memcpy(void* dst, const void* src, int64_t bytes);
+int32_t chimera_memset(void* ptr, int32_t value, int64_t bytes);
+int32_t chimera_axpb(double* p, int64_t n, double a, double b);
+
 /* ---------------------------------------------------------------- particle_tools.f90 */
 
 /* Relativistic Boris push.  momenta(3,np) in/out, Fld(6,np).
@@ -184,6 +196,82 @@ int32_t chimera_omp_add_scl(double* scl_fb, const double* A, int32_t nkx, int32_
 /* Analytic planar undulator, params = (a0, lambda, X0, Lx); Fld(6,np) += field.
  * Replaces Undul_Analytic, devices.f90:162-203. */
 int32_t chimera_undul_analytic(const double* coord, double* Fld, double t, const double* params, int32_t np);
+
+/* ================================================================ resident tier
+ *
+ * Entry points for simulations whose state stays in device memory (the
+ * chimera_amd.Solver / Specie / ChimeraRun driver).  Every array pointer is
+ * device memory and the calls are asynchronous on the library stream, except
+ * where a host output is named.  Particles are structure-of-arrays: component
+ * c of particle i at coord[c*ld + i].  Grid arrays keep the reference layout.
+ * These fuse steps that the reference runs as separate f2py calls; each cites
+ * the calls it stands for.
+ */
+
+/* Leap-frog push, SoA.  = push_coords (particle_tools.f90:58-82). */
+int32_t chimera_soa_push_coords(double* coord, double* coord_half, const double* momenta, int64_t ld, int32_t np,
+                                double dt);
+
+/* Deposition WITHOUT the ghost fold (chimera_fb_forward applies it with fold=1).
+ * r0 = Rgrid(0), rmax = Rgrid(nr).  = dep_curr / dep_dens particle loops
+ * (grid_deps.f90:34-78, :104-142). */
+int32_t chimera_soa_dep_curr(const double* coord, const double* momenta, const double* wghts, int64_t ld, int32_t np,
+                             double* curr, double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0,
+                             double rmax, int32_t nx, int32_t nr, int32_t nkO);
+int32_t chimera_soa_dep_dens(const double* coord, const double* wghts, int64_t ld, int32_t np, double* dens,
+                             double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
+                             int32_t nx, int32_t nr, int32_t nkO);
+
+/* Gather of Fld(0:nx,0:nr,0:nkO,6) at the particles, plus the analytic
+ * undulator when undul_params (HOST double[4]) is not NULL, then the Boris push
+ * with dt = push_dt.  = make_field + proj_fld + undul_analytic + push_velocs
+ * (species.py:246-298, grid_deps.f90:149-217, devices.f90:162-203,
+ * particle_tools.f90:18-56). */
+int32_t chimera_soa_gather_push(const double* coord, double* momenta, const double* wghts, int64_t ld, int32_t np,
+                                const double* Fld, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
+                                double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
+                                const double* undul_params);
+
+/* Stable removal of the particles outside lims (HOST double[4]: x range,
+ * r^2 range), tested on coord (test_half=0) or coord_half (1); *np_out (HOST)
+ * receives the new count.  = sortpartsout + align_data_* (species.py:385-398). */
+int32_t chimera_soa_compact(double* coord, double* coord_half, double* momenta, double* wghts, int64_t ld, int32_t np,
+                            const double* lims, int32_t test_half, int32_t* np_out);
+
+/* Forward FB transform of ncomp components, fb_vec_in/fb_scl_in with nkO =
+ * nmodes, times scale(nkx,nkr,nmodes) (DepFact) when not NULL, and with fold=1
+ * the deposition ghost fold applied to row 1 as it is read.
+ * = dep fold + fb_vec_in + omp_mult_vec (grid_deps.f90:80-85, fb_io.f90:18-98,
+ * solvers.py:417-435). */
+int32_t chimera_fb_forward(double* out, const double* in, double leftX, const double* kx, const double* In,
+                           const double* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
+                           int32_t fold);
+
+/* Inverse FB transform into eb_spc(nkx,0:nr,nmodes,6); ghost=1: ghost row zero
+ * (fb_eb_out alone), ghost=2: eb_correction applied as well.
+ * = fb_eb_out + eb_correction (fb_io.f90:182-228, grid_deps.f90:219-266). */
+int32_t chimera_fb_inverse(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
+                           const double* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost);
+
+/* B = rot(G) * scale (PoissFact).  = fb_rot + omp_mult_vec (solvers.py:536-553). */
+int32_t chimera_fb_rot_scaled(double* vec_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                              const double* kx, const double* scale, int32_t nkx, int32_t nkr, int32_t nkO);
+
+/* One Poisson-correction iteration, j += (graddiv j + (gp - gn) dt_inv) w2_inv.
+ * = fb_graddiv + poiss_corr (solvers.py:317-326). */
+int32_t chimera_fb_graddiv_poisson(double* j_fb, const double* grad_rho_n_fb, const double* grad_rho_np1_fb,
+                                   double dt_inv, const double* w2_inv, const double* DpS2S, const double* DmS2S,
+                                   const double* kx, int32_t nkx, int32_t nkr, int32_t nkO);
+
+/* op = 0 rot, 1 grad, 2 div, 3 graddiv (in may equal out only for 3), nkr_loc = nkr.
+ * = fb_rot / fb_grad / fb_div / fb_graddiv (fb_math.f90). */
+int32_t chimera_fb_op(int32_t op, double* out, const double* in, const double* DpS2S, const double* DmS2S,
+                      const double* kx, int32_t nkx, int32_t nkr, int32_t nkO);
+
+/* Absorbing layer on ncomp components of vec(nkx,nkr,nmodes,ncomp).
+ * = fb_filtr with modefilt 0 (fb_io.f90:230-270). */
+int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const double* filtr, int32_t nxfilt,
+                          int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp);
 
 #ifdef __cplusplus
 }
