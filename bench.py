@@ -1,0 +1,186 @@
+"""Benchmark of the PIC step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lpa_cfg1]
+                    [--no-cpu-baseline] [--cpu-steps 1]
+
+A step is one ChimeraRun.make_step of the workload (sort/absorb, deposition,
+FB transforms, Poisson correction, PSATD, inverse transforms, gather + push,
+moving-window actions on their steps), all state resident in HBM.  W untimed
+steps, then K timed steps between two device synchronisations.  Prints ONE
+JSON line on stdout; progress goes to stderr.
+
+value = pushed-particle steps per second over the timed steps (electrons; the
+still ions are not pushed).  Per-kernel-family times come from HIP events that
+libchimera_hip records on its own stream around every launch.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "particle-steps/s + FB-transform ms/step, FP64 vs mixed; p50/p99 scan latency"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+FAMILIES = ["push_coords", "dep_curr", "dep_dens", "gather_push", "dht", "fft", "fb_ops", "psatd", "sort", "other"]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def prof_read(lib, fam):
+    import ctypes
+    t, n = ctypes.c_double(0.0), ctypes.c_int64(0)
+    lib.chimera_prof_read(fam, ctypes.addressof(t), ctypes.addressof(n))
+    return t.value, n.value
+
+
+def algorithmic(fam, P, Gr, Gk, Nx, Nkr, M, K):
+    """(amount, unit, bound) per timed region of a family, from the per-unit
+    figures: particles P (per launch), real-space nodes Gr = Nx*(Nkr+1)*M,
+    spectral elements Gk = Nx*Nkr*M (complex double = 16 B)."""
+    if fam == "gather_push":      # x,y,z + u read, u written, w read; the 6-component field read once
+        return P * 80 + 6 * Gr * 16, "B", "hbm"
+    if fam == "dep_curr":         # half-step x,y,z + u + w read; 3-component J read+written once
+        return P * 56 + 3 * Gr * 32, "B", "hbm"
+    if fam == "dep_dens":         # x,y,z + w read; rho read+written once
+        return P * 32 + Gr * 32, "B", "hbm"
+    if fam == "push_coords":      # x,u read; x, x_half written
+        return P * 96, "B", "hbm"
+    if fam == "psatd":            # E,G,J,grad rho_n, grad rho_n+1 read, E,G written, 10 table doubles
+        return Gk * (21 * 16 + 80), "B", "hbm"
+    return None
+
+
+def run_gpu(args):
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    from chimera_amd._lib import lib
+
+    spec = W.WORKLOADS[args.workload]()
+    log(f"building {args.workload}")
+    t0 = time.time()
+    run = W.build(G, spec, seed=args.seed)
+    L = lib()
+    L.chimera_synchronize()
+    s, e = run.Solvers[0], run.Particles[0]
+    A = s.Args
+    Nx, Nkr, M = A["Nx"], A["Nkr"], A["Mtot"]
+    log(f"built in {time.time() - t0:.1f}s: Nx={Nx} Nkr={Nkr} modes={M} electrons={e.np} ions={run.Particles[1].np}")
+    for i in range(1, args.warmup + 1):
+        run.make_step(i)
+    L.chimera_synchronize()
+    log(f"warmup {args.warmup} steps done")
+    L.chimera_prof_reset()
+    L.chimera_prof_enable(1)
+    psteps = 0
+    L.chimera_synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.warmup + 1, args.warmup + args.steps + 1):
+        run.make_step(i)
+        psteps += e.np
+    L.chimera_synchronize()
+    wall = time.perf_counter() - t0
+    L.chimera_prof_enable(0)
+    fams = {}
+    for f, name in enumerate(FAMILIES):
+        ms, n = prof_read(L, f)
+        fams[name] = (ms, n)
+    log(f"timed {args.steps} steps in {wall:.3f}s")
+    for name, (ms, n) in fams.items():
+        if n:
+            log(f"  {name:12s} {ms / args.steps:9.3f} ms/step  ({n} launches)")
+    return dict(run=run, wall=wall, psteps=psteps, fams=fams, Nx=Nx, Nkr=Nkr, M=M, P=e.np,
+                Gr=Nx * A["Nr"] * M, Gk=Nx * Nkr * M)
+
+
+def roofline(res, K):
+    ranked = sorted(((ms, name) for name, (ms, n) in res["fams"].items() if n), reverse=True)
+    for ms, name in ranked:
+        alg = algorithmic(name, res["P"], res["Gr"], res["Gk"], res["Nx"], res["Nkr"], res["M"], K)
+        if alg is None:
+            continue
+        amount, unit, bound = alg
+        n = res["fams"][name][1]
+        avg_s = ms / n / 1e3
+        launches_per_step = n / K
+        per_launch = amount / launches_per_step if bound == "hbm" and name == "psatd" else amount
+        achieved = per_launch / avg_s / 1e9
+        return {"kernel": name, "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_s * 1e3, 4), "algorithmic_bytes_per_launch": int(per_launch),
+                "share_of_step": round(ms / sum(v[0] for v in res["fams"].values()), 3)}
+    return None
+
+
+def cpu_baseline(args, P_gpu):
+    from chimera_amd import workloads as W
+    from oracle import driver as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    spec = W.WORKLOADS[args.workload]()
+    log(f"CPU baseline: building the oracle at {args.workload} ({threads} OpenMP threads)")
+    t0 = time.time()
+    run = W.build(O, spec, seed=args.seed)
+    log(f"CPU baseline: built + half step in {time.time() - t0:.1f}s")
+    P = run.Particles[0].Data["coords"].shape[1]
+    t0 = time.perf_counter()
+    for i in range(1, args.cpu_steps + 1):
+        run.make_step(i)
+    dt = time.perf_counter() - t0
+    log(f"CPU baseline: {args.cpu_steps} step(s) in {dt:.1f}s")
+    return {"value": round(P * args.cpu_steps / dt, 1), "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} make_step of {args.workload} ({P} electrons) after the initial half step; "
+                      f"oracle/cpu_ref.c (FP64, naive DHT/operator loops, serial deposition as in grid_deps.f90) "
+                      f"driven by oracle/driver.py, OMP_NUM_THREADS={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="lpa_cfg1")
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the PIC step of one simulation runs on one MI355X and this build "
+              f"has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    res = run_gpu(args)
+    K = args.steps
+    value = res["psteps"] / res["wall"]
+    fb_ms = (res["fams"]["dht"][0] + res["fams"]["fft"][0]) / K
+    ops_ms = res["fams"]["fb_ops"][0] / K
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": round(res["wall"] / K * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded FixedCell 2x2x4 plasma (n=0.005 n_c) + Gaussian laser, random draws from NumPy",
+        "config": {"workload": args.workload, "Nx": res["Nx"], "Nkr": res["Nkr"], "modes": res["M"],
+                   "electrons": res["P"], "precision": "fp64", "moving_window": True},
+        "fb_transform_ms_per_step": round(fb_ms, 3),
+        "spectral_operator_ms_per_step": round(ops_ms, 3),
+        "kernel_ms_per_step": {k: round(v[0] / K, 3) for k, v in res["fams"].items() if v[1]},
+        "roofline": roofline(res, K),
+    }
+    del res["run"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args, res["P"])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

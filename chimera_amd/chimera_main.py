@@ -5,6 +5,11 @@ make_halfstep / make_step (chimera_main.py:61-92); the per-particle work of
 make_field + proj_fld + make_device + push_velocs runs as one fused gather-push
 per species, and push_coords precedes the deposits exactly as in the reference.
 
+SimComps may carry ``'SortEvery': n``: every n steps the pushed species are
+sorted by cell on the device (a performance option; the reference sorts only
+by x-chunk, chimera_main.py:306-318, and the physics does not depend on the
+particle order beyond the order of floating-point sums).
+
 SimComps may carry ``'Features': ('PreDepositBackground',)``: the still
 background is then deposited before the initial half step, so that the static
 initial field of a neutral plasma starts from the neutral charge density (the
@@ -24,6 +29,7 @@ class ChimeraRun:
         self.Solvers = SimComps.get("Solvers", ())
         self.MovingFrames = SimComps["MovingFrames"] if "MovingFrames" in SimComps else ({},)
         self.Features = SimComps.get("Features", ())
+        self.SortEvery = int(SimComps.get("SortEvery", 0))
         self.init_Moving_Frames()
         self.init_damp_profile()
         if "PreDepositBackground" in self.Features:
@@ -65,15 +71,78 @@ class ChimeraRun:
 
     def make_step(self, istep):
         self.frame_act(istep)                                              # chimera_main.py:82-92
-        for species in self.Particles:
-            species.push_coords()
-        self.chunk_particles(istep)
-        self.project_current()
-        self.frame_act(istep, "stage2")
-        self.project_density()
+        if self.SortEvery and np.mod(istep, self.SortEvery) == 0 and self.Solvers:
+            for species in self.Particles:
+                if "Still" not in species.Args["Features"]:
+                    species.sort_cells(self.Solvers[0])
+        if self._fusable(istep):
+            # push_coords + dep_curr + dep_dens in one pass per species; the
+            # transforms then run in the reference's order around stage 2
+            self.push_and_deposit(istep)
+            for solver in self.Solvers:
+                if "NoCurrent" in solver.Args["Features"]:
+                    solver.Data["J_fb"].zero_()
+                else:
+                    solver.fb_curr_in()
+            self.frame_act(istep, "stage2")
+            for solver in self.Solvers:
+                if "SpaceCharge" in solver.Args["Features"]:
+                    D = solver.Data
+                    D["gradRho_fb_prv"], D["gradRho_fb_nxt"] = D["gradRho_fb_nxt"], D["gradRho_fb_prv"]
+                    solver.fb_dens_in()
+                    solver.FBGradDens()
+        else:
+            for species in self.Particles:
+                species.push_coords()
+            self.chunk_particles(istep)
+            self.project_current()
+            self.frame_act(istep, "stage2")
+            self.project_density()
         self.update_fields()
         self.project_fields()
         self.gather_push(istep)
+
+    def _fusable(self, istep):
+        if len(self.Solvers) != 1:
+            return False
+        for sp in self.Particles:
+            if "Xchunked" in sp.Args and "NoSorting" not in sp.Args["Features"] and \
+                    np.mod(istep, sp.Args["Xchunked"][1] + 1) == 0:
+                return False
+        return True
+
+    def _stage2_leftX(self, istep, leftX, Xgrid):
+        """leftX after the stage-2 moves of this step, formed as move_frame forms it."""
+        for wind in self.MovingFrames:
+            if istep < wind["TimeActive"][0] or istep > wind["TimeActive"][1]:
+                continue
+            if np.mod(istep - wind["TimeActive"][0], wind["Steps"]) != 0:
+                continue
+            if "Staged" in wind["Features"]:
+                Xgrid = Xgrid + wind["shiftX"]
+                leftX = Xgrid[0]
+        return leftX
+
+    def push_and_deposit(self, istep):
+        solver = self.Solvers[0]
+        A, D = solver.Args, solver.Data
+        curr = None if "NoCurrent" in A["Features"] else D["J"]
+        dens = D["Rho"] if "SpaceCharge" in A["Features"] else None
+        if curr is not None:
+            curr.zero_()
+        if dens is not None:
+            if "StillAsBackground" in A["Features"]:
+                dens.copy_(D["BckGrndRho"])
+            else:
+                dens.zero_()
+        g = solver.geom_args()
+        leftX_rho = self._stage2_leftX(istep, A["leftX"], A["Xgrid"])
+        for sp in self.Particles:
+            if sp.np == 0 or "Still" in sp.Args["Features"]:
+                continue
+            call("chimera_soa_push_deposit", sp.ptr("coords"), sp.ptr("coords_halfstep"), sp.ptr("momenta"),
+                 sp.ptr("weights"), sp.ld, sp.np, float(sp.Args["TimeStep"]), g[0], float(leftX_rho), *g[1:],
+                 curr.ptr if curr is not None else None, dens.ptr if dens is not None else None)
 
     # ------------------------------------------------------------------ sources
     def project_current(self):
@@ -103,8 +172,9 @@ class ChimeraRun:
         for species in self.Particles:
             if species.np == 0 or "Still" in species.Args["Features"]:
                 continue
-            call("chimera_soa_dep_curr", species.ptr("coords_halfstep"), species.ptr("momenta"),
-                 species.ptr("weights"), species.ld, species.np, J.ptr, *g)
+            call("chimera_soa_push_deposit", species.ptr("coords"), species.ptr("coords_halfstep"),
+                 species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
+                 J.ptr, None)
 
     def dep_dens(self, solver, component="coords"):
         R = solver.Data["Rho"]
@@ -116,8 +186,9 @@ class ChimeraRun:
         for species in self.Particles:
             if species.np == 0 or "Still" in species.Args["Features"]:
                 continue
-            call("chimera_soa_dep_dens", species.ptr(component), species.ptr("weights"), species.ld, species.np,
-                 R.ptr, *g)
+            call("chimera_soa_push_deposit", species.ptr(component), species.ptr("coords_halfstep"),
+                 species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
+                 None, R.ptr)
 
     def dep_bg(self, solver):
         B = solver.Data["BckGrndRho"]
@@ -126,8 +197,9 @@ class ChimeraRun:
         for species in self.Particles:
             if species.np == 0 or "Still" not in species.Args["Features"]:
                 continue
-            call("chimera_soa_dep_dens", species.ptr("coords"), species.ptr("weights"), species.ld, species.np,
-                 B.ptr, *g)
+            call("chimera_soa_push_deposit", species.ptr("coords"), species.ptr("coords_halfstep"),
+                 species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
+                 None, B.ptr)
 
     # ------------------------------------------------------------------ fields
     def update_fields(self):

@@ -87,12 +87,39 @@ class Stage {
 // Scratch device memory owned by the library, grown on demand and reused
 // across calls on the same stream (the stream orders every use).
 void* scratch(int slot, size_t bytes);
-constexpr int SCRATCH_SLOTS = 8;
+constexpr int SCRATCH_SLOTS = 16;
 
 // Device copy of a small host table (job lists, twiddles), cached by content.
 const void* device_table(const void* host, size_t bytes);
 
 inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// Kernel families timed by the optional in-library profiler.
+enum KernelFamily : int {
+    KF_PUSH_COORDS = 0,  // leap-frog position push
+    KF_DEP_CURR = 1,     // current deposition
+    KF_DEP_DENS = 2,     // charge deposition
+    KF_GATHER_PUSH = 3,  // field gather + device + Boris (and the drop-in gather)
+    KF_DHT = 4,          // Hankel-transform GEMMs of the FB transforms
+    KF_FFT = 5,          // x FFTs of the FB transforms and the absorbing layer
+    KF_FBOPS = 6,        // FB-space operator GEMMs (rot, grad, div, grad-div)
+    KF_PSATD = 7,        // PSATD and elementwise field updates
+    KF_SORT = 8,         // compaction, scans, sorting
+    KF_OTHER = 9,
+    KF_COUNT = 10
+};
+
+// While profiling is on, a KTimer brackets the launches in its scope with
+// HIP events recorded on the library stream.
+struct KTimer {
+    explicit KTimer(int fam);
+    ~KTimer();
+    KTimer(const KTimer&) = delete;
+    KTimer& operator=(const KTimer&) = delete;
+    int fam;
+    bool on;
+    hipEvent_t a;
+};
 
 }  // namespace chimera
 

@@ -119,6 +119,7 @@ int32_t launch_fft(FftArgs a) {
         CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
         attr_done = true;
     }
+    KTimer kt(KF_FFT);
     switch (ept) {
         case 8:
             hipLaunchKernelGGL(k_fft_lines<8>, grid, FFT_THREADS, shmem, stream(), a);
@@ -164,7 +165,15 @@ int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n)
 
 // ================================================================== FB GEMM
 
-constexpr int GB_X = 64, GB_N = 64, GB_K = 16, GB_THREADS = 256;
+// Block tile: GB_X complex x (2*GB_X real rows, re/im interleaved as in memory)
+// by GB_N output columns; K staged GB_K at a time through LDS.  Four waves in a
+// 2 x 2 arrangement, each owning 64 real rows x 32 columns as 4 x 2 tiles of
+// v_mfma_f64_16x16x4f64.  Operand layout of that instruction: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+i][l%16], i = 0..3, so
+// D elements i = 0,1 (and 2,3) of a lane are the real and imaginary parts of
+// one complex output.
+constexpr int GB_X = 64, GB_ROWS = 2 * GB_X, GB_N = 64, GB_K = 16, GB_THREADS = 256;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double2 seg_load(const FbSeg& s, int64_t x, int64_t kk, int nx, int64_t lda) {
     int64_t xr = x;
@@ -183,23 +192,41 @@ __device__ __forceinline__ double2 seg_load(const FbSeg& s, int64_t x, int64_t k
     return v;
 }
 
+__device__ __forceinline__ void mfma_stage(dbl4 (&acc)[4][2], const double (*As)[GB_ROWS + 4],
+                                           const double (*Ds)[GB_N + 4], int lane, int wr, int wc) {
+#pragma unroll
+    for (int kq = 0; kq < GB_K / 4; kq++) {
+        const int kk = kq * 4 + (lane >> 4);
+        double a[4], b[2];
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = As[kk][wr * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < 2; j++) b[j] = Ds[kk][wc * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+}
+
 __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict__ jobs, FbGemmArgs g) {
     const FbJob& J = jobs[blockIdx.z];
     const int n0 = blockIdx.y * GB_N;
     if (n0 >= J.Ncol) return;
     const int x0 = blockIdx.x * GB_X;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
 
-    __shared__ double2 As[GB_K][GB_X];
-    __shared__ double Ds[GB_K][GB_N + 1];
+    __shared__ double As[GB_K][GB_ROWS + 4];
+    __shared__ double Ds[GB_K][GB_N + 4];
 
-    double2 acc[2][4][4];
+    dbl4 acc0[4][2], acc1[4][2];
 #pragma unroll
-    for (int s = 0; s < 2; s++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[s][i][j] = cmk(0.0, 0.0);
+        for (int j = 0; j < 2; j++) {
+            acc0[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+            acc1[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
 
     for (int si = 0; si < J.nseg; si++) {
         const FbSeg& S = J.seg[si];
@@ -212,7 +239,8 @@ __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict_
                 const int64_t xg = x0 + xl;
                 double2 v = cmk(0.0, 0.0);
                 if (xg < g.nx && k0 + kk < K) v = seg_load(S, xg, k0 + kk, g.nx, g.lda);
-                As[kk][xl] = v;
+                As[kk][2 * xl] = v.x;
+                As[kk][2 * xl + 1] = v.y;
             }
 #pragma unroll
             for (int i = 0; i < (GB_K * GB_N) / GB_THREADS; i++) {
@@ -223,74 +251,46 @@ __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict_
                 Ds[kk][nl] = v;
             }
             __syncthreads();
-            if (S.acc == 0) {
-#pragma unroll
-                for (int kk = 0; kk < GB_K; kk++) {
-                    double2 a[4];
-                    double d[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) a[i] = As[kk][tx + 16 * i];
-#pragma unroll
-                    for (int j = 0; j < 4; j++) d[j] = Ds[kk][ty + 16 * j];
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-#pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            acc[0][i][j].x += a[i].x * d[j];
-                            acc[0][i][j].y += a[i].y * d[j];
-                        }
-                }
-            } else {
-#pragma unroll
-                for (int kk = 0; kk < GB_K; kk++) {
-                    double2 a[4];
-                    double d[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) a[i] = As[kk][tx + 16 * i];
-#pragma unroll
-                    for (int j = 0; j < 4; j++) d[j] = Ds[kk][ty + 16 * j];
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-#pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            acc[1][i][j].x += a[i].x * d[j];
-                            acc[1][i][j].y += a[i].y * d[j];
-                        }
-                }
-            }
+            if (S.acc == 0) mfma_stage(acc0, As, Ds, lane, wr, wc);
+            else mfma_stage(acc1, As, Ds, lane, wr, wc);
             __syncthreads();
         }
     }
-    // ---- epilogue
+    // ---- epilogue: lane holds complex outputs at x = x0 + (wr*64 + i*16 + 4*(lane/16))/2 + h, h = 0,1
     for (int o = 0; o < J.nout; o++) {
         const FbOut& O = J.out[o];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int64_t xg = x0 + tx + 16 * i;
-            if (xg >= g.nx) continue;
+        for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int64_t ng = n0 + ty + 16 * j;
+            for (int j = 0; j < 2; j++) {
+                const int64_t ng = n0 + wc * 32 + j * 16 + (lane & 15);
                 if (ng >= J.Ncol) continue;
-                double2 v = cadd(cmul(O.alpha0, acc[0][i][j]), cmul(O.alpha1, acc[1][i][j]));
-                const int64_t off = xg + g.ldo * ng;
-                if (O.diag) v = cadd(v, crs(g.kx[xg], cmul(O.gamma, O.diag[off])));
-                if (O.scale) v = crs(O.scale[off], v);
-                double2* dst = O.p + off;
-                if (O.poiss) {
-                    // maxwell_solvers.f90:153-158: J += (v + (g_np1 - g_n)*dt_inv)*w2_inv
-                    const double2 d = crs(g.dt_inv, csub(O.poiss_np1[off], O.poiss_n[off]));
-                    *dst = cadd(*dst, crs(O.poiss_w2[off], cadd(v, d)));
-                } else {
-                    *dst = J.beta ? cadd(*dst, v) : v;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int64_t xg = x0 + (wr * 64 + i * 16 + 4 * (lane >> 4)) / 2 + h;
+                    if (xg >= g.nx) continue;
+                    const double2 p0 = cmk(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]);
+                    const double2 p1 = cmk(acc1[i][j][2 * h], acc1[i][j][2 * h + 1]);
+                    double2 v = cadd(cmul(O.alpha0, p0), cmul(O.alpha1, p1));
+                    const int64_t off = xg + g.ldo * ng;
+                    if (O.diag) v = cadd(v, crs(g.kx[xg], cmul(O.gamma, O.diag[off])));
+                    if (O.scale) v = crs(O.scale[off], v);
+                    double2* dst = O.p + off;
+                    if (O.poiss) {
+                        // maxwell_solvers.f90:153-158: J += (v + (g_np1 - g_n)*dt_inv)*w2_inv
+                        const double2 d = crs(g.dt_inv, csub(O.poiss_np1[off], O.poiss_n[off]));
+                        *dst = cadd(*dst, crs(O.poiss_w2[off], cadd(v, d)));
+                    } else {
+                        *dst = J.beta ? cadd(*dst, v) : v;
+                    }
                 }
             }
-        }
     }
 }
 
-int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g) {
+int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family) {
     if (jobs.empty()) return ST_OK;
+    KTimer kt(family);
     int maxN = 0;
     for (const auto& j : jobs) maxN = j.Ncol > maxN ? j.Ncol : maxN;
     const FbJob* dj = static_cast<const FbJob*>(device_table(jobs.data(), jobs.size() * sizeof(FbJob)));
@@ -491,7 +491,7 @@ int32_t fb_forward(double2* out, const double2* in, double leftX, const double* 
     g.lda = nkx;
     g.ldo = nkx;
     g.kx = kx_dev;
-    CH_TRY(launch_fb_gemm(jobs, g));
+    CH_TRY(launch_fb_gemm(jobs, g, KF_DHT));
     CH_TRY(make_phase(ph, kx_dev, leftX, -1.0, nkx));
     FftArgs a;
     memset(&a, 0, sizeof a);
@@ -528,7 +528,7 @@ int32_t fb_inverse(double2* eb, const double2* e_fb, const double2* b_fb, int64_
     g.lda = nkx;
     g.ldo = nkx;
     g.kx = kx_dev;
-    CH_TRY(launch_fb_gemm(jobs, g));
+    CH_TRY(launch_fb_gemm(jobs, g, KF_DHT));
     CH_TRY(make_phase(ph, kx_dev, leftX, +1.0, nkx));
     FftArgs a;
     memset(&a, 0, sizeof a);
@@ -589,26 +589,26 @@ int32_t fb_op(int op, double2* out, const double2* in, const double* Dp, const d
         case FB_ROT:
             build_rot_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
                            nkO, scale_tab);
-            return launch_fb_gemm(jobs, g);
+            return launch_fb_gemm(jobs, g, KF_FBOPS);
         case FB_GRAD:
             build_grad_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
                             nkO, nmod);
-            return launch_fb_gemm(jobs, g);
+            return launch_fb_gemm(jobs, g, KF_FBOPS);
         case FB_DIV:
             build_div_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
                            nkO, nmod);
-            return launch_fb_gemm(jobs, g);
+            return launch_fb_gemm(jobs, g, KF_FBOPS);
         case FB_GRADDIV: {
             if (nkr_loc != nkr) return fail(ST_EUNSUP, "fb_graddiv needs nkr_loc == nkr (got %d, %d)", nkr_loc, nkr);
             double2* s = static_cast<double2*>(scratch(3, (size_t)nkx * nkr_loc * (nkO + 2) * sizeof(double2)));
             if (!s) return ST_ENOMEM;
             build_div_jobs(jobs, FbViewW{s, nkx, nkr_loc, nkO + 2}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
                            nkO, nkO + 2);
-            CH_TRY(launch_fb_gemm(jobs, g));
+            CH_TRY(launch_fb_gemm(jobs, g, KF_FBOPS));
             jobs.clear();
             build_graddiv2_jobs(jobs, FbViewW{out, nkx, nkr, nmod}, FbView{s, nkx, nkr_loc, nkO + 2}, Dp, Dm, nkr,
                                 nkr_loc, nkO);
-            return launch_fb_gemm(jobs, g);
+            return launch_fb_gemm(jobs, g, KF_FBOPS);
         }
         default:
             return fail(ST_EINVAL, "unknown FB operator %d", op);
@@ -631,11 +631,11 @@ int32_t fb_graddiv_poisson(double2* j, const double2* gn, const double2* gp, dou
     double2* s = static_cast<double2*>(scratch(3, (size_t)nkx * nkr * (nkO + 2) * sizeof(double2)));
     if (!s) return ST_ENOMEM;
     build_div_jobs(jobs, FbViewW{s, nkx, nkr, nkO + 2}, FbView{j, nkx, nkr, nmod}, Dp, Dm, nkr, nkr, nkO, nkO + 2);
-    CH_TRY(launch_fb_gemm(jobs, g));
+    CH_TRY(launch_fb_gemm(jobs, g, KF_FBOPS));
     jobs.clear();
     PoissArgs pa{gn, gp, w2};
     build_graddiv2_jobs(jobs, FbViewW{j, nkx, nkr, nmod}, FbView{s, nkx, nkr, nkO + 2}, Dp, Dm, nkr, nkr, nkO, &pa);
-    return launch_fb_gemm(jobs, g);
+    return launch_fb_gemm(jobs, g, KF_FBOPS);
 }
 
 }  // namespace chimera

@@ -68,7 +68,7 @@ enum { FB_ROT = 0, FB_GRAD = 1, FB_DIV = 2, FB_GRADDIV = 3 };
 
 int32_t launch_fft(FftArgs a);
 int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n);
-int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g);
+int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family);
 
 // Forward FB transform (fb_io.f90:18-98): in(nkx,0:nr,nmodes,ncomp) ->
 // out(nkx,nkr,nmodes,ncomp), optionally times scale_tab(nkx,nkr,nmodes); with

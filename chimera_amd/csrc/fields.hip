@@ -85,6 +85,7 @@ __global__ void k_add(double2* __restrict__ v, const double2* __restrict__ A, in
 int32_t psatd_spchrg(double2* EG, const double2* J, const double2* Gn, const double2* Gp, const double* C1,
                      const double* C2, int64_t n) {
     if (n <= 0) return ST_OK;
+    KTimer kt(KF_PSATD);
     hipLaunchKernelGGL(k_psatd_spchrg, blocks_for(n, 256), 256, 0, stream(), EG, J, Gn, Gp, C1, C2, n);
     CH_LAUNCH_CHECK("k_psatd_spchrg");
     return ST_OK;
@@ -92,6 +93,7 @@ int32_t psatd_spchrg(double2* EG, const double2* J, const double2* Gn, const dou
 
 int32_t psatd_wo(double2* EG, const double2* J, const double* C1, const double* C2, int64_t n) {
     if (n <= 0) return ST_OK;
+    KTimer kt(KF_PSATD);
     hipLaunchKernelGGL(k_psatd_wo, blocks_for(n, 256), 256, 0, stream(), EG, J, C1, C2, n);
     CH_LAUNCH_CHECK("k_psatd_wo");
     return ST_OK;
@@ -99,6 +101,7 @@ int32_t psatd_wo(double2* EG, const double2* J, const double* C1, const double* 
 
 int32_t init_push(double2* EG, const double2* J, const double2* Gn, const double2* C1, const double2* C2, int64_t n) {
     if (n <= 0) return ST_OK;
+    KTimer kt(KF_PSATD);
     hipLaunchKernelGGL(k_init_push, blocks_for(n, 256), 256, 0, stream(), EG, J, Gn, C1, C2, n);
     CH_LAUNCH_CHECK("k_init_push");
     return ST_OK;
@@ -107,6 +110,7 @@ int32_t init_push(double2* EG, const double2* J, const double2* Gn, const double
 int32_t poiss_corr(double2* J, const double2* GD, const double2* Gn, const double2* Gp, double dt_inv,
                    const double* w2, int64_t n) {
     if (n <= 0) return ST_OK;
+    KTimer kt(KF_PSATD);
     hipLaunchKernelGGL(k_poiss_corr, blocks_for(n, 256), 256, 0, stream(), J, GD, Gn, Gp, dt_inv, w2, n);
     CH_LAUNCH_CHECK("k_poiss_corr");
     return ST_OK;

@@ -203,6 +203,7 @@ __global__ void k_genparts(double* __restrict__ coord, const double* __restrict_
 // Launch wrappers shared with the resident entry points (pic_resident.hip).
 int32_t launch_push_coords(double* x, PStride sx, const double* u, PStride su, double* xc, PStride sc, double dt,
                            int np) {
+    KTimer kt(KF_PUSH_COORDS);
     hipLaunchKernelGGL(k_push_coords, blocks_for(np, 256), 256, 0, stream(), x, sx, u, su, xc, sc, dt, np);
     CH_LAUNCH_CHECK("k_push_coords");
     return ST_OK;
@@ -211,6 +212,7 @@ int32_t launch_push_coords(double* x, PStride sx, const double* u, PStride su, d
 int32_t launch_dep_curr(const double* x, PStride sx, const double* u, PStride su, const double* w, double2* curr,
                         const GridGeom& g, int ip0, int ip1) {
     if (ip1 <= ip0) return ST_OK;
+    KTimer kt(KF_DEP_CURR);
     hipLaunchKernelGGL(k_dep_curr, blocks_for(ip1 - ip0, 256), 256, 0, stream(), x, sx, u, su, w, curr, g, ip0, ip1);
     CH_LAUNCH_CHECK("k_dep_curr");
     return ST_OK;
@@ -219,6 +221,7 @@ int32_t launch_dep_curr(const double* x, PStride sx, const double* u, PStride su
 int32_t launch_dep_dens(const double* x, PStride sx, const double* w, double2* dens, const GridGeom& g, int ip0,
                         int ip1) {
     if (ip1 <= ip0) return ST_OK;
+    KTimer kt(KF_DEP_DENS);
     hipLaunchKernelGGL(k_dep_dens, blocks_for(ip1 - ip0, 256), 256, 0, stream(), x, sx, w, dens, g, ip0, ip1);
     CH_LAUNCH_CHECK("k_dep_dens");
     return ST_OK;

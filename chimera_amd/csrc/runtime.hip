@@ -2,6 +2,7 @@
 // memory and the cached device copies of small host tables.
 #include <map>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "../../include/chimera_hip.h"
@@ -17,7 +18,39 @@ struct ScratchSlot { void* p = nullptr; size_t bytes = 0; };
 ScratchSlot g_scratch[SCRATCH_SLOTS];
 
 std::map<std::string, void*> g_tables;
+
+bool g_prof = false;
+struct TimerRec { hipEvent_t a, b; int fam; };
+std::vector<TimerRec> g_recs;
+std::vector<hipEvent_t> g_free_events;
+
+hipEvent_t event_get() {
+    if (!g_free_events.empty()) {
+        hipEvent_t e = g_free_events.back();
+        g_free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
 }  // namespace
+
+KTimer::KTimer(int f) : fam(f), on(g_prof), a(nullptr) {
+    if (on) {
+        a = event_get();
+        if (a) (void)hipEventRecord(a, g_stream);
+        else on = false;
+    }
+}
+
+KTimer::~KTimer() {
+    if (!on) return;
+    hipEvent_t b = event_get();
+    if (!b) return;
+    (void)hipEventRecord(b, g_stream);
+    g_recs.push_back({a, b, fam});
+}
 
 void set_error(const char* fmt, ...) {
     char buf[1024];
@@ -218,6 +251,39 @@ int32_t chimera_memcpy(void* dst, const void* src, int64_t bytes) {
 int32_t chimera_memset(void* ptr, int32_t value, int64_t bytes) {
     if (bytes < 0 || (bytes > 0 && ptr == nullptr)) return fail(ST_EINVAL, "chimera_memset: bad arguments");
     return fill_bytes(ptr, value, (size_t)bytes);
+}
+
+int32_t chimera_prof_enable(int32_t on) {
+    g_prof = on != 0;
+    return ST_OK;
+}
+
+int32_t chimera_prof_reset(void) {
+    CH_HIP(hipStreamSynchronize(g_stream));
+    for (auto& r : g_recs) {
+        g_free_events.push_back(r.a);
+        g_free_events.push_back(r.b);
+    }
+    g_recs.clear();
+    return ST_OK;
+}
+
+int32_t chimera_prof_read(int32_t family, double* total_ms, int64_t* count) {
+    if (total_ms == nullptr || count == nullptr || family < 0 || family >= KF_COUNT)
+        return fail(ST_EINVAL, "chimera_prof_read: bad arguments");
+    CH_HIP(hipStreamSynchronize(g_stream));
+    double t = 0.0;
+    int64_t n = 0;
+    for (auto& r : g_recs) {
+        if (r.fam != family) continue;
+        float ms = 0.f;
+        CH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        t += ms;
+        n++;
+    }
+    *total_ms = t;
+    *count = n;
+    return ST_OK;
 }
 
 int32_t chimera_axpb(double* p, int64_t n, double a, double b) {

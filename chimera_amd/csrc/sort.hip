@@ -154,6 +154,118 @@ __global__ void k_gather_rows(const double* __restrict__ dat, const int64_t* __r
     for (int c = 0; c < ncomp; c++) tmp[(int64_t)ncomp * ip + c] = dat[ncomp * j + c];
 }
 
+// ------------------------------------------------------------------ stable LSD radix sort of integer keys
+//
+// Keys are sorted RS_BITS bits at a time.  Every pass: (1) each block of
+// RS_TILE consecutive items counts its digits, (2) an exclusive scan over the
+// (digit, block) counts in digit-major order gives every block its output
+// offset per digit, (3) each block writes its items to their offsets in input
+// order.  Ranks inside a wave come from a ballot-based match of equal digits
+// (the lanes of one digit and their popcount), ranks across the block's waves
+// from per-wave digit counts in LDS, so the sort is stable and deterministic.
+constexpr int RS_BITS = 8, RS_BINS = 1 << RS_BITS, RS_T = 256, RS_ITEMS = 8, RS_TILE = RS_T * RS_ITEMS;
+
+__device__ __forceinline__ uint64_t match_digit(int d, bool valid) {
+    // lanes with the same (valid, digit) as this lane
+    uint64_t m = __ballot(1);
+    const uint64_t v = __ballot(valid);
+    m &= valid ? v : ~v;
+#pragma unroll
+    for (int b = 0; b < RS_BITS; b++) {
+        const bool bit = (d >> b) & 1;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                   int32_t* __restrict__ hist, int64_t nblocks) {
+    __shared__ int32_t h[RS_BINS];
+    for (int i = threadIdx.x; i < RS_BINS; i += RS_T) h[i] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int it = 0; it < RS_ITEMS; it++) {
+        const int64_t i = base + (int64_t)it * RS_T + threadIdx.x;
+        const bool valid = i < n;
+        const int d = valid ? (int)((keys[i] >> shift) & (RS_BINS - 1)) : 0;
+        const uint64_t m = match_digit(d, valid);
+        const int lead = __ffsll((unsigned long long)m) - 1;
+        if (valid && lane_id() == lead) atomicAdd(&h[d], __popcll(m));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < RS_BINS; i += RS_T) hist[(int64_t)i * nblocks + blockIdx.x] = h[i];
+}
+
+__global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t* __restrict__ keys, const int32_t* __restrict__ idx,
+                                                      int64_t n, int shift, const int64_t* __restrict__ offs,
+                                                      int64_t nblocks, uint32_t* __restrict__ keys_out,
+                                                      int32_t* __restrict__ idx_out) {
+    __shared__ int32_t running[RS_BINS];
+    __shared__ int32_t wcount[RS_T / 64][RS_BINS];
+    for (int i = threadIdx.x; i < RS_BINS; i += RS_T) running[i] = 0;
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    for (int it = 0; it < RS_ITEMS; it++) {
+        for (int i = threadIdx.x; i < (RS_T / 64) * RS_BINS; i += RS_T) (&wcount[0][0])[i] = 0;
+        __syncthreads();
+        const int64_t i = base + (int64_t)it * RS_T + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys[i] : 0u;
+        const int d = (int)((key >> shift) & (RS_BINS - 1));
+        const uint64_t m = match_digit(d, valid);
+        const int rank = __popcll(m & lt);
+        if (valid && rank == 0) wcount[wave][d] = __popcll(m);
+        __syncthreads();
+        if (valid) {
+            int pos = running[d] + rank;
+            for (int w = 0; w < wave; w++) pos += wcount[w][d];
+            const int64_t o = offs[(int64_t)d * nblocks + blockIdx.x] + pos;
+            keys_out[o] = key;
+            idx_out[o] = idx ? idx[i] : (int32_t)i;
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < RS_BINS; b += RS_T) {
+            int add = 0;
+            for (int w = 0; w < RS_T / 64; w++) add += wcount[w][b];
+            running[b] += add;
+        }
+        __syncthreads();
+    }
+}
+
+int32_t radix_sort_keys(uint32_t* keys, int32_t* idx, uint32_t* keys_tmp, int32_t* idx_tmp, int64_t n, int key_bits,
+                        int scratch_hist, int scratch_scan) {
+    if (n <= 0) return ST_OK;
+    const int64_t nblocks = (n + RS_TILE - 1) / RS_TILE;
+    int32_t* hist = static_cast<int32_t*>(scratch(scratch_hist, (size_t)RS_BINS * nblocks * sizeof(int32_t)));
+    int64_t* offs = static_cast<int64_t*>(scratch(scratch_scan, ((size_t)RS_BINS * nblocks + 8) * sizeof(int64_t)));
+    if (!hist || !offs) return ST_ENOMEM;
+    uint32_t* kbuf[2] = {keys, keys_tmp};
+    int32_t* ibuf[2] = {idx, idx_tmp};
+    int passes = (key_bits + RS_BITS - 1) / RS_BITS;
+    if (passes < 1) passes = 1;
+    for (int p = 0; p < passes; p++) {
+        const int src = p & 1, dst = src ^ 1;
+        const int shift = p * RS_BITS;
+        hipLaunchKernelGGL(k_rs_hist, (unsigned)nblocks, RS_T, 0, stream(), kbuf[src], n, shift, hist, nblocks);
+        CH_LAUNCH_CHECK("k_rs_hist");
+        CH_TRY(exclusive_scan(hist, offs, (int64_t)RS_BINS * nblocks, nullptr, scratch_scan + 1));
+        // pass 0 reads the identity permutation
+        hipLaunchKernelGGL(k_rs_scatter, (unsigned)nblocks, RS_T, 0, stream(), kbuf[src], p == 0 ? nullptr : ibuf[src],
+                           n, shift, offs, nblocks, kbuf[dst], ibuf[dst]);
+        CH_LAUNCH_CHECK("k_rs_scatter");
+    }
+    if (passes & 1) {  // the result is in the work buffers
+        CH_HIP(hipMemcpyAsync(keys, keys_tmp, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream()));
+        CH_HIP(hipMemcpyAsync(idx, idx_tmp, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToDevice, stream()));
+    }
+    return ST_OK;
+}
+
 }  // namespace chimera
 
 using namespace chimera;

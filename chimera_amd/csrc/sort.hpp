@@ -12,4 +12,9 @@ namespace chimera {
 // scratch_slot for the tile sums.
 int32_t exclusive_scan(const int32_t* in, int64_t* out, int64_t n, int64_t* total_dev, int scratch_slot);
 
+// Stable LSD radix sort of n keys of key_bits bits; idx receives the source
+// index of every sorted key.  keys_tmp/idx_tmp are n-element work buffers.
+int32_t radix_sort_keys(uint32_t* keys, int32_t* idx, uint32_t* keys_tmp, int32_t* idx_tmp, int64_t n, int key_bits,
+                        int scratch_hist, int scratch_scan);
+
 }  // namespace chimera

@@ -190,6 +190,28 @@ class Specie:
              self.ptr("weights"), self.ld, self.np, lims.ctypes.data, 1 if half else 0, ctypes.addressof(nout))
         self.np = int(nout.value)
 
+    def sort_cells(self, solver, SimDom=None, position="stag"):
+        """Stable counting sort of the particles by cell of `solver`'s grid,
+        dropping those outside SimDom when it is given (libchimera_hip
+        chimera_soa_sort_cells).  Keeps the physics and changes the order, so
+        that deposition and gather touch neighbouring nodes."""
+        if self.np == 0:
+            return
+        spare = getattr(self, "_spare", None)
+        if spare is None or spare["weights"].shape[0] != self.cap:
+            spare = {k: DeviceArray((self.cap, 3), np.float64) for k in ("coords", "coords_halfstep", "momenta")}
+            spare["weights"] = DeviceArray((self.cap,), np.float64)
+        g = solver.geom_args()
+        lims = np.ascontiguousarray(SimDom, dtype=np.float64) if SimDom is not None else None
+        nout = ctypes.c_int32(0)
+        call("chimera_soa_sort_cells", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"),
+             self.ptr("weights"), self.ld, self.np, spare["coords"].ptr, spare["coords_halfstep"].ptr,
+             spare["momenta"].ptr, spare["weights"].ptr, self.cap, g[0], g[2], g[3], g[4], g[6], g[7],
+             lims.ctypes.data if lims is not None else None, 1 if position == "cntr" else 0, ctypes.addressof(nout),
+             None)
+        self._spare, self.Data = self.Data, spare
+        self.np = int(nout.value)
+
     # ------------------------------------------------------------------ pushes
     def push_coords(self, dt_frac=1.0):
         if self.np == 0 or "Still" in self.Args["Features"]:

@@ -65,8 +65,9 @@ def lpa_small(Nx=128, Nkr=32, nko=1, moving=False, steps=10):
 WORKLOADS = {"lpa_cfg0": lpa_cfg0, "lpa_cfg1": lpa_cfg1, "lpa_small": lpa_small}
 
 
-def build(mod, spec, seed=12345):
-    """Instantiate a workload with mod.Solver / mod.Specie / mod.ChimeraRun."""
+def build(mod, spec, seed=12345, **run_options):
+    """Instantiate a workload with mod.Solver / mod.Specie / mod.ChimeraRun;
+    run_options are extra ChimeraRun keys (e.g. SortEvery)."""
     spec = copy.deepcopy(spec)
     solver = mod.Solver(spec["solver"])
     solver.add_gauss_beam(spec["laser"])
@@ -75,6 +76,8 @@ def build(mod, spec, seed=12345):
     np.random.seed(seed)
     electrons.add_particles(*electrons.gen_parts())
     ions.add_particles(*ions.gen_parts())
-    run = mod.ChimeraRun({"Solvers": (solver,), "Particles": (electrons, ions), "MovingFrames": spec["frames"],
-                          "Features": spec["features"]})
+    comps = {"Solvers": (solver,), "Particles": (electrons, ions), "MovingFrames": spec["frames"],
+             "Features": spec["features"]}
+    comps.update(run_options)
+    run = mod.ChimeraRun(comps)
     return run

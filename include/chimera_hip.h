@@ -60,6 +60,16 @@ int32_t chimera_memcpy(void* dst, const void* src, int64_t bytes);
 int32_t chimera_memset(void* ptr, int32_t value, int64_t bytes);
 int32_t chimera_axpb(double* p, int64_t n, double a, double b);
 
+/* In-library kernel timing (no counterpart in the reference).  While enabled,
+ * the launches of each kernel family are bracketed by HIP events on the
+ * library stream; chimera_prof_read sums the elapsed times of one family.
+ * Families: 0 position push, 1 current deposition, 2 charge deposition,
+ * 3 gather + push, 4 Hankel-transform GEMMs, 5 x FFTs, 6 FB-operator GEMMs,
+ * 7 PSATD / elementwise field updates, 8 compaction / sort, 9 other. */
+int32_t chimera_prof_enable(int32_t on);
+int32_t chimera_prof_reset(void);
+int32_t chimera_prof_read(int32_t family, double* total_ms, int64_t* count);
+
 /* ---------------------------------------------------------------- particle_tools.f90 */
 
 /* Relativistic Boris push.  momenta(3,np) in/out, Fld(6,np).
@@ -222,6 +232,18 @@ int32_t chimera_soa_dep_dens(const double* coord, const double* wghts, int64_t l
                              double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
                              int32_t nx, int32_t nr, int32_t nkO);
 
+/* Fused position push and deposition: with dt != 0 the positions are pushed
+ * first (coord -> coord at n+1, coord_half = midpoint), otherwise coord_half is
+ * read as given; then J (curr, if not NULL) is deposited from coord_half and
+ * momenta with leftX_J, and rho (dens, if not NULL) from coord with leftX_rho.
+ * Deposits are accumulated per block in LDS tiles and added to the grid
+ * unfolded.  = push_coords + dep_curr + dep_dens (particle_tools.f90:58-82,
+ * grid_deps.f90:18-147), as make_step runs them (chimera_main.py:84-88). */
+int32_t chimera_soa_push_deposit(double* coord, double* coord_half, const double* momenta, const double* wghts,
+                                 int64_t ld, int32_t np, double dt, double leftX_J, double leftX_rho,
+                                 const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax, int32_t nx,
+                                 int32_t nr, int32_t nkO, double* curr, double* dens);
+
 /* Gather of Fld(0:nx,0:nr,0:nkO,6) at the particles, plus the analytic
  * undulator when undul_params (HOST double[4]) is not NULL, then the Boris push
  * with dt = push_dt.  = make_field + proj_fld + undul_analytic + push_velocs
@@ -237,6 +259,21 @@ int32_t chimera_soa_gather_push(const double* coord, double* momenta, const doub
  * receives the new count.  = sortpartsout + align_data_* (species.py:385-398). */
 int32_t chimera_soa_compact(double* coord, double* coord_half, double* momenta, double* wghts, int64_t ld, int32_t np,
                             const double* lims, int32_t test_half, int32_t* np_out);
+
+/* Stable counting sort of the particles by cell, absorbing those outside lims
+ * (HOST double[4], tested on coord or, with test_half, coord_half; NULL keeps
+ * all).  Cell key = ir*nx + ix from the expressions of grid_deps.f90:46-47 with
+ * nx, nr the CELL counts, kept particles outside the grid after all cells.
+ * The sorted, compacted particles are written to the second set of arrays
+ * (leading dimension ld2); *np_out (HOST) receives their number and perm
+ * (device, np entries, may be NULL) the source index of each.
+ * = chunk_coords_boundaries + argsort + align_data_* (species.py:378-398),
+ * with cells as the chunks and a stable order. */
+int32_t chimera_soa_sort_cells(const double* coord, const double* coord_half, const double* momenta,
+                               const double* wghts, int64_t ld, int32_t np, double* coord2, double* coord_half2,
+                               double* momenta2, double* wghts2, int64_t ld2, double leftX, double dx_inv,
+                               double dr_inv, double r0, int32_t nx, int32_t nr, const double* lims, int32_t test_half,
+                               int32_t* np_out, int32_t* perm);
 
 /* Forward FB transform of ncomp components, fb_vec_in/fb_scl_in with nkO =
  * nmodes, times scale(nkx,nkr,nmodes) (DepFact) when not NULL, and with fold=1
