@@ -49,8 +49,8 @@ def algorithmic(fam, P, Gr, Gk, Nx, Nkr, M, K):
     spectral elements Gk = Nx*Nkr*M (complex double = 16 B)."""
     if fam == "gather_push":      # x,y,z + u read, u written, w read; the 6-component field read once
         return P * 80 + 6 * Gr * 16, "B", "hbm"
-    if fam == "dep_curr":         # half-step x,y,z + u + w read; 3-component J read+written once
-        return P * 56 + 3 * Gr * 32, "B", "hbm"
+    if fam == "dep_curr":         # fused push + J + rho: x, u, w read, x and x_half written;
+        return P * 104 + 4 * Gr * 32, "B", "hbm"   # J (3 components) and rho read+written once
     if fam == "dep_dens":         # x,y,z + w read; rho read+written once
         return P * 32 + Gr * 32, "B", "hbm"
     if fam == "push_coords":      # x,u read; x, x_half written
@@ -98,7 +98,7 @@ def run_gpu(args):
     for name, (ms, n) in fams.items():
         if n:
             log(f"  {name:12s} {ms / args.steps:9.3f} ms/step  ({n} launches)")
-    return dict(run=run, wall=wall, psteps=psteps, fams=fams, Nx=Nx, Nkr=Nkr, M=M, P=e.np,
+    return dict(run=run, wall=wall, psteps=psteps, fams=fams, Nx=Nx, Nkr=Nkr, M=M, P=e.np, workload=args.workload,
                 Gr=Nx * A["Nr"] * M, Gk=Nx * Nkr * M)
 
 
@@ -115,7 +115,7 @@ def roofline(res, K):
         per_launch = amount / launches_per_step if bound == "hbm" and name == "psatd" else amount
         achieved = per_launch / avg_s / 1e9
         return {"kernel": name, "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(name, res["workload"]),
                 "avg_launch_ms": round(avg_s * 1e3, 4), "algorithmic_bytes_per_launch": int(per_launch),
                 "share_of_step": round(ms / sum(v[0] for v in res["fams"].values()), 3)}
     return None
@@ -143,6 +143,78 @@ def cpu_baseline(args, P_gpu):
                       f"driven by oracle/driver.py, OMP_NUM_THREADS={threads}"}
 
 
+def copy_calibration(gib):
+    """One device-to-device copy of a known size: the FETCH_SIZE / WRITE_SIZE it
+    reads under rocprofv3 --pmc calibrate the counters of the PIC kernels."""
+    from chimera_amd.device import DeviceArray, call
+    n = int(gib * (1 << 30)) // 8
+    a, b = DeviceArray((n,), np.float64), DeviceArray((n,), np.float64)
+    call("chimera_synchronize")
+    call("chimera_memcpy", b.ptr, a.ptr, n * 8)
+    call("chimera_synchronize")
+    print(json.dumps({"copy_bytes": n * 8}), flush=True)
+
+
+def traffic_for(kernel, workload):
+    """HBM bytes per launch of a kernel family, measured with rocprofv3 --pmc and
+    calibrated against a plain copy (profiles/traffic.json), if recorded."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    e = rec.get(workload, {}).get(kernel)
+    return int(e["bytes_per_launch"]) if e else None
+
+
+TRAFFIC_KERNELS = {"dep_curr": "k_push_deposit", "gather_push": "k_gather_push"}
+
+
+def summarize_traffic(root, workload, steps, out_dir):
+    """Turn four rocprofv3 --pmc runs into HBM bytes per launch.
+
+    root holds pmc_copy_fetch/, pmc_copy_write/ (python bench.py --copy-calibration 1
+    under --pmc FETCH_SIZE, then WRITE_SIZE) and pmc_step_fetch/, pmc_step_write/
+    (python bench.py --steps S --warmup 0 --no-cpu-baseline, same counters).  Each
+    counter is scaled by (known copy bytes) / (what it read for the copy); a kernel's
+    figure is the mean over its last S launches (the launches of the S steps after
+    the initial half step)."""
+    import csv
+
+    def rows(tag):
+        d = os.path.join(root, f"pmc_{tag}")
+        f = [x for x in os.listdir(d) if x.endswith("counter_collection.csv")][0]
+        return list(csv.DictReader(open(os.path.join(d, f))))
+
+    copy_bytes = 1 << 30
+    cf = [float(r["Counter_Value"]) for r in rows("copy_fetch") if "copyBuffer" in r["Kernel_Name"]][0]
+    cw = [float(r["Counter_Value"]) for r in rows("copy_write") if "copyBuffer" in r["Kernel_Name"]][0]
+    f_fetch, f_write = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
+    fetch, write = rows("step_fetch"), rows("step_write")
+    rec = {"_calibration": {"copy_bytes": copy_bytes, "fetch_KiB": cf, "write_KiB": cw,
+                            "fetch_factor": round(f_fetch, 4), "write_factor": round(f_write, 4)}}
+    lines = []
+    for fam, kname in TRAFFIC_KERNELS.items():
+        fv = [float(r["Counter_Value"]) for r in fetch if kname in r["Kernel_Name"]]
+        wv = [float(r["Counter_Value"]) for r in write if kname in r["Kernel_Name"]]
+        if not fv or not wv:
+            continue
+        n = min(len(fv), steps)            # one launch per step for these kernels
+        fb = np.mean(fv[-n:]) * 1024 * f_fetch
+        wb = np.mean(wv[-n:]) * 1024 * f_write
+        rec[fam] = {"kernel": kname, "launches": n, "fetch_bytes": int(fb), "write_bytes": int(wb),
+                    "bytes_per_launch": int(fb + wb)}
+        lines.append(f"| {fam} | `{kname}` | {n} | {fb / 1e9:.3f} | {wb / 1e9:.3f} | {(fb + wb) / 1e9:.3f} |")
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, "traffic.json")
+    allrec = {}
+    if os.path.exists(path):
+        allrec = json.load(open(path))
+    allrec[workload] = rec
+    json.dump(allrec, open(path, "w"), indent=1)
+    return rec, lines
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,7 +224,20 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only copy GIB gibibytes device to device once (reference kernel for the HBM counters)")
+    ap.add_argument("--summarize-traffic", metavar="DIR",
+                    help="write profiles/traffic.json from the four rocprofv3 --pmc runs under DIR")
     args = ap.parse_args()
+    if args.summarize_traffic:
+        rec, lines = summarize_traffic(args.summarize_traffic, args.workload, args.steps,
+                                       os.path.join(HERE, "profiles"))
+        print(json.dumps(rec, indent=1))
+        print("\n".join(lines))
+        return
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the PIC step of one simulation runs on one MI355X and this build "
               f"has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)

@@ -97,7 +97,7 @@ inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + thre
 // Kernel families timed by the optional in-library profiler.
 enum KernelFamily : int {
     KF_PUSH_COORDS = 0,  // leap-frog position push
-    KF_DEP_CURR = 1,     // current deposition
+    KF_DEP_CURR = 1,     // current deposition (with the fused position push and charge deposition)
     KF_DEP_DENS = 2,     // charge deposition
     KF_GATHER_PUSH = 3,  // field gather + device + Boris (and the drop-in gather)
     KF_DHT = 4,          // Hankel-transform GEMMs of the FB transforms

@@ -165,13 +165,15 @@ int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n)
 
 // ================================================================== FB GEMM
 
-// Block tile: GB_X complex x (2*GB_X real rows, re/im interleaved as in memory)
-// by GB_N output columns; K staged GB_K at a time through LDS.  Four waves in a
-// 2 x 2 arrangement, each owning 64 real rows x 32 columns as 4 x 2 tiles of
-// v_mfma_f64_16x16x4f64.  Operand layout of that instruction: lane l supplies
-// A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+i][l%16], i = 0..3, so
-// D elements i = 0,1 (and 2,3) of a lane are the real and imaginary parts of
-// one complex output.
+// Block tile: GB_X complex x (2*GB_X real rows) by GB_N output columns; K
+// staged GB_K at a time through LDS.  Four waves in a 2 x 2 arrangement, each
+// owning 64 real rows x 32 columns as 4 x 2 tiles of v_mfma_f64_16x16x4f64.
+// Lane layout of that instruction on gfx950 (checked on the device): lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4i][l%16],
+// i = 0..3.  Each 16-row MFMA tile covers 8 complex x; complex c (0..7) puts
+// its real part in row (c%4) + 8*(c/4) and its imaginary part 4 rows below, so
+// that a lane's items (0,1) and (2,3) are the (re, im) of x = c and x = c + 4
+// with c = l/16.
 constexpr int GB_X = 64, GB_ROWS = 2 * GB_X, GB_N = 64, GB_K = 16, GB_THREADS = 256;
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -209,54 +211,79 @@ __device__ __forceinline__ void mfma_stage(dbl4 (&acc)[4][2], const double (*As)
     }
 }
 
+template <int W>
+__device__ __forceinline__ void mfma_stage2(dbl4 (&acc)[4][W], const double (*As)[GB_ROWS + 4],
+                                            const double (*Ds)[GB_N + 4], int lane, int wr, int wc) {
+    if constexpr (W == 2) mfma_stage(acc, As, Ds, lane, wr, wc);
+}
+
+template <int NACC>
 __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict__ jobs, FbGemmArgs g) {
     const FbJob& J = jobs[blockIdx.z];
+    const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
     const int n0 = blockIdx.y * GB_N;
-    if (n0 >= J.Ncol) return;
+    if (n0 >= Ncol) return;
     const int x0 = blockIdx.x * GB_X;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
 
     __shared__ double As[GB_K][GB_ROWS + 4];
     __shared__ double Ds[GB_K][GB_N + 4];
 
-    dbl4 acc0[4][2], acc1[4][2];
+    dbl4 acc0[4][2], acc1[4][NACC == 2 ? 2 : 1];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             acc0[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
-            acc1[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+            if (NACC == 2) acc1[i][j % (NACC == 2 ? 2 : 1)] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
 
-    for (int si = 0; si < J.nseg; si++) {
-        const FbSeg& S = J.seg[si];
-        const int K = J.K;
-        for (int k0 = 0; k0 < K; k0 += GB_K) {
+    // Software pipeline over the stages of all segments: the global loads of
+    // stage t+1 are in flight (in registers) while the MFMAs of stage t run.
+    constexpr int NA = (GB_K * GB_X) / GB_THREADS, ND = (GB_K * GB_N) / GB_THREADS;
+    const int stages_per_seg = (K + GB_K - 1) / GB_K;
+    const int nstages = stages_per_seg * nseg;
+    double2 ra[NA];
+    double rd[ND];
+    auto fetch = [&](int t) {
+        const FbSeg& S = J.seg[t / stages_per_seg];
+        const int k0 = (t % stages_per_seg) * GB_K;
 #pragma unroll
-            for (int i = 0; i < (GB_K * GB_X) / GB_THREADS; i++) {
-                const int e = tid + GB_THREADS * i;
-                const int xl = e % GB_X, kk = e / GB_X;
-                const int64_t xg = x0 + xl;
-                double2 v = cmk(0.0, 0.0);
-                if (xg < g.nx && k0 + kk < K) v = seg_load(S, xg, k0 + kk, g.nx, g.lda);
-                As[kk][2 * xl] = v.x;
-                As[kk][2 * xl + 1] = v.y;
-            }
-#pragma unroll
-            for (int i = 0; i < (GB_K * GB_N) / GB_THREADS; i++) {
-                const int e = tid + GB_THREADS * i;
-                const int kk = e % GB_K, nl = e / GB_K;
-                double v = 0.0;
-                if (k0 + kk < K && n0 + nl < J.Ncol) v = S.D[(int64_t)(k0 + kk) + S.ldd * (int64_t)(n0 + nl)];
-                Ds[kk][nl] = v;
-            }
-            __syncthreads();
-            if (S.acc == 0) mfma_stage(acc0, As, Ds, lane, wr, wc);
-            else mfma_stage(acc1, As, Ds, lane, wr, wc);
-            __syncthreads();
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + GB_THREADS * i;
+            const int xl = e % GB_X, kk = e / GB_X;
+            const int64_t xg = x0 + xl;
+            ra[i] = (xg < g.nx && k0 + kk < K) ? seg_load(S, xg, k0 + kk, g.nx, g.lda) : cmk(0.0, 0.0);
         }
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            const int e = tid + GB_THREADS * i;
+            const int kk = e % GB_K, nl = e / GB_K;
+            rd[i] = (k0 + kk < K && n0 + nl < Ncol) ? S.D[(int64_t)(k0 + kk) + S.ldd * (int64_t)(n0 + nl)] : 0.0;
+        }
+    };
+    if (nstages > 0) fetch(0);
+    for (int t = 0; t < nstages; t++) {
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + GB_THREADS * i;
+            const int xl = e % GB_X, kk = e / GB_X;
+            const int c = xl & 7, slot = (xl >> 3) * 16 + (c & 3) + 8 * (c >> 2);
+            As[kk][slot] = ra[i].x;
+            As[kk][slot + 4] = ra[i].y;
+        }
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            const int e = tid + GB_THREADS * i;
+            Ds[e % GB_K][e / GB_K] = rd[i];
+        }
+        __syncthreads();
+        if (t + 1 < nstages) fetch(t + 1);
+        if (NACC == 1 || J.seg[t / stages_per_seg].acc == 0) mfma_stage(acc0, As, Ds, lane, wr, wc);
+        else mfma_stage2(acc1, As, Ds, lane, wr, wc);
+        __syncthreads();
     }
-    // ---- epilogue: lane holds complex outputs at x = x0 + (wr*64 + i*16 + 4*(lane/16))/2 + h, h = 0,1
+    // ---- epilogue: lane holds complex outputs at x = x0 + wr*32 + i*8 + lane/16 + 4h, h = 0,1
     for (int o = 0; o < J.nout; o++) {
         const FbOut& O = J.out[o];
 #pragma unroll
@@ -264,14 +291,17 @@ __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict_
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int64_t ng = n0 + wc * 32 + j * 16 + (lane & 15);
-                if (ng >= J.Ncol) continue;
+                if (ng >= Ncol) continue;
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    const int64_t xg = x0 + (wr * 64 + i * 16 + 4 * (lane >> 4)) / 2 + h;
+                    const int64_t xg = x0 + wr * 32 + i * 8 + (lane >> 4) + 4 * h;
                     if (xg >= g.nx) continue;
                     const double2 p0 = cmk(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]);
-                    const double2 p1 = cmk(acc1[i][j][2 * h], acc1[i][j][2 * h + 1]);
-                    double2 v = cadd(cmul(O.alpha0, p0), cmul(O.alpha1, p1));
+                    double2 v = cmul(O.alpha0, p0);
+                    if (NACC == 2) {
+                        const int jj = j % (NACC == 2 ? 2 : 1);
+                        v = cadd(v, cmul(O.alpha1, cmk(acc1[i][jj][2 * h], acc1[i][jj][2 * h + 1])));
+                    }
                     const int64_t off = xg + g.ldo * ng;
                     if (O.diag) v = cadd(v, crs(g.kx[xg], cmul(O.gamma, O.diag[off])));
                     if (O.scale) v = crs(O.scale[off], v);
@@ -291,13 +321,25 @@ __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict_
 int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family) {
     if (jobs.empty()) return ST_OK;
     KTimer kt(family);
-    int maxN = 0;
-    for (const auto& j : jobs) maxN = j.Ncol > maxN ? j.Ncol : maxN;
-    const FbJob* dj = static_cast<const FbJob*>(device_table(jobs.data(), jobs.size() * sizeof(FbJob)));
-    if (!dj) return ST_ENOMEM;
-    dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + GB_N - 1) / GB_N), (unsigned)jobs.size());
-    hipLaunchKernelGGL(k_fb_gemm, grid, GB_THREADS, 0, stream(), dj, g);
-    CH_LAUNCH_CHECK("k_fb_gemm");
+    // jobs whose segments all feed accumulator 0 run in the one-accumulator kernel
+    std::vector<FbJob> one, two;
+    for (const auto& j : jobs) {
+        bool uses1 = false;
+        for (int s = 0; s < j.nseg; s++) uses1 |= j.seg[s].acc != 0;
+        (uses1 ? two : one).push_back(j);
+    }
+    for (int pass = 0; pass < 2; pass++) {
+        const std::vector<FbJob>& js = pass == 0 ? one : two;
+        if (js.empty()) continue;
+        int maxN = 0;
+        for (const auto& j : js) maxN = j.Ncol > maxN ? j.Ncol : maxN;
+        const FbJob* dj = static_cast<const FbJob*>(device_table(js.data(), js.size() * sizeof(FbJob)));
+        if (!dj) return ST_ENOMEM;
+        dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + GB_N - 1) / GB_N), (unsigned)js.size());
+        if (pass == 0) hipLaunchKernelGGL((k_fb_gemm<1>), grid, GB_THREADS, 0, stream(), dj, g);
+        else hipLaunchKernelGGL((k_fb_gemm<2>), grid, GB_THREADS, 0, stream(), dj, g);
+        CH_LAUNCH_CHECK("k_fb_gemm");
+    }
     return ST_OK;
 }
 

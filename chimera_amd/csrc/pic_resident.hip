@@ -18,6 +18,7 @@
 //                              (particle_tools.f90:130-153 + the permutation
 //                              gathers of species.py:385-398)
 #include <climits>
+#include <cstdlib>
 
 #include "common.hpp"
 #include "pic_device.hpp"
@@ -69,8 +70,7 @@ __global__ void k_gather_push(const double* __restrict__ coord, PStride sx, doub
 // per node and component.  A box too large for the LDS budget makes that block
 // deposit straight to the grid with global atomics.  Deposits are unfolded; the
 // forward FB transform applies the ghost fold.
-constexpr int DEP_THREADS = 256, DEP_PPT = 4, DEP_CHUNK = DEP_THREADS * DEP_PPT;
-constexpr int DEP_LDS_BYTES = 48 * 1024;
+
 
 struct DepArgs {
     double* x;          // positions (3, ld), updated when push
@@ -109,7 +109,9 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+template <int DEP_THREADS, int DEP_PPT, int DEP_LDS_BYTES>
 __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
+    constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
     extern __shared__ double tile[];
     __shared__ int red[8][DEP_THREADS / 64];
     __shared__ int box[8];
@@ -461,14 +463,32 @@ int32_t chimera_soa_push_deposit(double* coord, double* coord_half, const double
     a.gR = resident_geom(leftX_rho, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
     a.J = reinterpret_cast<double2*>(curr);
     a.R = reinterpret_cast<double2*>(dens);
-    static bool attr = false;
-    if (!attr) {
-        CH_HIP(hipFuncSetAttribute((const void*)k_push_deposit, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   DEP_LDS_BYTES));
-        attr = true;
+    // block shape: threads x consecutive particles per thread, LDS tile budget;
+    // CHIMERA_DEP_SHAPE selects one for sweeps
+    static int shape = -1;
+    if (shape < 0) {
+        const char* e = getenv("CHIMERA_DEP_SHAPE");
+        shape = e ? atoi(e) : 4;
+        if (shape < 0 || shape > 4) shape = 4;
     }
     KTimer kt(a.J ? KF_DEP_CURR : KF_DEP_DENS);
-    hipLaunchKernelGGL(k_push_deposit, blocks_for(np, DEP_CHUNK), DEP_THREADS, DEP_LDS_BYTES, stream(), a);
+    switch (shape) {
+        case 0:
+            hipLaunchKernelGGL((k_push_deposit<256, 4, 48 * 1024>), blocks_for(np, 1024), 256, 48 * 1024, stream(), a);
+            break;
+        case 1:
+            hipLaunchKernelGGL((k_push_deposit<64, 16, 48 * 1024>), blocks_for(np, 1024), 64, 48 * 1024, stream(), a);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_push_deposit<64, 8, 24 * 1024>), blocks_for(np, 512), 64, 24 * 1024, stream(), a);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_push_deposit<128, 4, 24 * 1024>), blocks_for(np, 512), 128, 24 * 1024, stream(), a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_push_deposit<64, 4, 12 * 1024>), blocks_for(np, 256), 64, 12 * 1024, stream(), a);
+            break;
+    }
     CH_LAUNCH_CHECK("k_push_deposit");
     return ST_OK;
 }

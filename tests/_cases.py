@@ -44,7 +44,8 @@ def particles(g: Grid, n=4000, seed=1, margin=0.02, zero_weights=True):
     coord = np.asfortranarray(np.vstack([x, rr * np.sin(th), rr * np.cos(th)]))
     coord[1:, 0] = 0.0                       # on the axis: phase 0 (grid_deps.f90:55-59)
     mom = F(r, 3, n)
-    mom[:, 1] = 0.0                          # zero momentum: skipped by dep_curr (:42)
+    if n > 1:
+        mom[:, 1] = 0.0                      # zero momentum: skipped by dep_curr (:42)
     w = r.uniform(0.1, 1.0, n)
     if zero_weights:
         w[::37] = 0.0                        # skipped everywhere (:36, :106, :171)
