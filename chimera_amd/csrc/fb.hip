@@ -165,16 +165,16 @@ int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n)
 
 // ================================================================== FB GEMM
 
-// Block tile: GB_X complex x (2*GB_X real rows) by GB_N output columns; K
+// Block tile: GB_X complex x (2*GB_X real rows) by 32*WN output columns; K
 // staged GB_K at a time through LDS.  Four waves in a 2 x 2 arrangement, each
-// owning 64 real rows x 32 columns as 4 x 2 tiles of v_mfma_f64_16x16x4f64.
+// owning 64 real rows x 16*WN columns as 4 x WN tiles of v_mfma_f64_16x16x4f64.
 // Lane layout of that instruction on gfx950 (checked on the device): lane l
 // supplies A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4i][l%16],
 // i = 0..3.  Each 16-row MFMA tile covers 8 complex x; complex c (0..7) puts
 // its real part in row (c%4) + 8*(c/4) and its imaginary part 4 rows below, so
 // that a lane's items (0,1) and (2,3) are the (re, im) of x = c and x = c + 4
 // with c = l/16.
-constexpr int GB_X = 64, GB_ROWS = 2 * GB_X, GB_N = 64, GB_K = 16, GB_THREADS = 256;
+constexpr int GB_X = 64, GB_ROWS = 2 * GB_X, GB_K = 16, GB_THREADS = 256;
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double2 seg_load(const FbSeg& s, int64_t x, int64_t kk, int nx, int64_t lda) {
@@ -194,53 +194,56 @@ __device__ __forceinline__ double2 seg_load(const FbSeg& s, int64_t x, int64_t k
     return v;
 }
 
-__device__ __forceinline__ void mfma_stage(dbl4 (&acc)[4][2], const double (*As)[GB_ROWS + 4],
-                                           const double (*Ds)[GB_N + 4], int lane, int wr, int wc) {
+// One K-stage (GB_K = 16) of a wave's 64 x (16*WN) tile: 4 x WN MFMA tiles.
+template <int WN>
+__device__ __forceinline__ void mfma_stage(dbl4 (&acc)[4][WN], const double* __restrict__ As,
+                                           const double* __restrict__ Ds, int lane, int wr, int wc) {
+    constexpr int LDA = GB_ROWS + 4, LDD = 32 * WN + 4;
 #pragma unroll
     for (int kq = 0; kq < GB_K / 4; kq++) {
         const int kk = kq * 4 + (lane >> 4);
-        double a[4], b[2];
+        double a[4], b[WN];
 #pragma unroll
-        for (int i = 0; i < 4; i++) a[i] = As[kk][wr * 64 + i * 16 + (lane & 15)];
+        for (int i = 0; i < 4; i++) a[i] = As[kk * LDA + wr * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-        for (int j = 0; j < 2; j++) b[j] = Ds[kk][wc * 32 + j * 16 + (lane & 15)];
+        for (int j = 0; j < WN; j++) b[j] = Ds[kk * LDD + wc * 16 * WN + j * 16 + (lane & 15)];
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < WN; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
 }
 
-template <int W>
-__device__ __forceinline__ void mfma_stage2(dbl4 (&acc)[4][W], const double (*As)[GB_ROWS + 4],
-                                            const double (*Ds)[GB_N + 4], int lane, int wr, int wc) {
-    if constexpr (W == 2) mfma_stage(acc, As, Ds, lane, wr, wc);
-}
+// LDS bytes of one k_fb_gemm<NACC, WN> block: A and D stages, double-buffered.
+constexpr size_t gb_lds_bytes(int WN) { return 2 * (size_t)GB_K * ((GB_ROWS + 4) + (32 * WN + 4)) * sizeof(double); }
 
-template <int NACC>
-__global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict__ jobs, FbGemmArgs g) {
+// Block tile: 64 complex x (128 real rows) x 32*WN columns; 4 waves as 2 x 2,
+// each 64 rows x 16*WN columns.  NACC = 2 keeps a second accumulator set for
+// segments with acc = 1.  The K-stages of all segments form one pipeline: the
+// global loads of stage t+1 go to registers while the MFMAs of stage t run,
+// then land in the other LDS buffer, with one barrier per stage.
+template <int NACC, int WN>
+__global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restrict__ jobs, FbGemmArgs g) {
+    constexpr int BN = 32 * WN, LDA = GB_ROWS + 4, LDD = BN + 4;
+    constexpr int A_STAGE = GB_K * LDA, D_STAGE = GB_K * LDD;
     const FbJob& J = jobs[blockIdx.z];
     const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
-    const int n0 = blockIdx.y * GB_N;
+    const int n0 = blockIdx.y * BN;
     if (n0 >= Ncol) return;
     const int x0 = blockIdx.x * GB_X;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
 
-    __shared__ double As[GB_K][GB_ROWS + 4];
-    __shared__ double Ds[GB_K][GB_N + 4];
+    extern __shared__ double gb_smem[];
+    double* const As = gb_smem;                  // [2][GB_K][LDA]
+    double* const Ds = gb_smem + 2 * A_STAGE;    // [2][GB_K][LDD]
 
-    dbl4 acc0[4][2], acc1[4][NACC == 2 ? 2 : 1];
+    dbl4 acc0[4][WN], acc1[4][NACC == 2 ? WN : 1];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            acc0[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
-            if (NACC == 2) acc1[i][j % (NACC == 2 ? 2 : 1)] = dbl4{0.0, 0.0, 0.0, 0.0};
-        }
+        for (int j = 0; j < WN; j++) acc0[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-    // Software pipeline over the stages of all segments: the global loads of
-    // stage t+1 are in flight (in registers) while the MFMAs of stage t run.
-    constexpr int NA = (GB_K * GB_X) / GB_THREADS, ND = (GB_K * GB_N) / GB_THREADS;
+    constexpr int NA = (GB_K * GB_X) / GB_THREADS, ND = (GB_K * BN) / GB_THREADS;
     const int stages_per_seg = (K + GB_K - 1) / GB_K;
     const int nstages = stages_per_seg * nseg;
     double2 ra[NA];
@@ -262,26 +265,48 @@ __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict_
             rd[i] = (k0 + kk < K && n0 + nl < Ncol) ? S.D[(int64_t)(k0 + kk) + S.ldd * (int64_t)(n0 + nl)] : 0.0;
         }
     };
-    if (nstages > 0) fetch(0);
-    for (int t = 0; t < nstages; t++) {
+    auto stash = [&](int buf) {
+        double* A = As + buf * A_STAGE;
+        double* D = Ds + buf * D_STAGE;
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
             const int xl = e % GB_X, kk = e / GB_X;
             const int c = xl & 7, slot = (xl >> 3) * 16 + (c & 3) + 8 * (c >> 2);
-            As[kk][slot] = ra[i].x;
-            As[kk][slot + 4] = ra[i].y;
+            A[kk * LDA + slot] = ra[i].x;
+            A[kk * LDA + slot + 4] = ra[i].y;
         }
 #pragma unroll
         for (int i = 0; i < ND; i++) {
             const int e = tid + GB_THREADS * i;
-            Ds[e % GB_K][e / GB_K] = rd[i];
+            D[(e % GB_K) * LDD + e / GB_K] = rd[i];
         }
-        __syncthreads();
-        if (t + 1 < nstages) fetch(t + 1);
-        if (NACC == 1 || J.seg[t / stages_per_seg].acc == 0) mfma_stage(acc0, As, Ds, lane, wr, wc);
-        else mfma_stage2(acc1, As, Ds, lane, wr, wc);
-        __syncthreads();
+    };
+    // stages [tb, te) into one accumulator set; the prefetch runs across the
+    // boundary between the two segments of a two-accumulator job
+    auto pipe = [&](dbl4 (&acc)[4][WN], int tb, int te) {
+        for (int t = tb; t < te; t++) {
+            if (t + 1 < nstages) fetch(t + 1);
+            mfma_stage<WN>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
+            if (t + 1 < nstages) stash((t + 1) & 1);
+            __syncthreads();
+        }
+    };
+    if (nstages > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    if constexpr (NACC == 2) {
+        // launch_fb_gemm hands this kernel jobs whose segment s feeds accumulator s
+        pipe(acc0, 0, stages_per_seg);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < WN; j++) acc1[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+        pipe(acc1, stages_per_seg, nstages);
+    } else {
+        pipe(acc0, 0, nstages);
     }
     // ---- epilogue: lane holds complex outputs at x = x0 + wr*32 + i*8 + lane/16 + 4h, h = 0,1
     for (int o = 0; o < J.nout; o++) {
@@ -289,8 +314,8 @@ __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict_
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int64_t ng = n0 + wc * 32 + j * 16 + (lane & 15);
+            for (int j = 0; j < WN; j++) {
+                const int64_t ng = n0 + wc * 16 * WN + j * 16 + (lane & 15);
                 if (ng >= Ncol) continue;
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
@@ -298,10 +323,7 @@ __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict_
                     if (xg >= g.nx) continue;
                     const double2 p0 = cmk(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]);
                     double2 v = cmul(O.alpha0, p0);
-                    if (NACC == 2) {
-                        const int jj = j % (NACC == 2 ? 2 : 1);
-                        v = cadd(v, cmul(O.alpha1, cmk(acc1[i][jj][2 * h], acc1[i][jj][2 * h + 1])));
-                    }
+                    if constexpr (NACC == 2) v = cadd(v, cmul(O.alpha1, cmk(acc1[i][j][2 * h], acc1[i][j][2 * h + 1])));
                     const int64_t off = xg + g.ldo * ng;
                     if (O.diag) v = cadd(v, crs(g.kx[xg], cmul(O.gamma, O.diag[off])));
                     if (O.scale) v = crs(O.scale[off], v);
@@ -318,15 +340,43 @@ __global__ void __launch_bounds__(GB_THREADS) k_fb_gemm(const FbJob* __restrict_
     }
 }
 
+template <int NACC, int WN>
+static int32_t launch_gemm_variant(const FbJob* dj, size_t njobs, int maxN, const FbGemmArgs& g) {
+    constexpr int BN = 32 * WN;
+    static bool attr = false;
+    if (!attr) {
+        CH_HIP(hipFuncSetAttribute((const void*)k_fb_gemm<NACC, WN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)gb_lds_bytes(WN)));
+        attr = true;
+    }
+    dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
+    hipLaunchKernelGGL((k_fb_gemm<NACC, WN>), grid, GB_THREADS, gb_lds_bytes(WN), stream(), dj, g);
+    CH_LAUNCH_CHECK("k_fb_gemm");
+    return ST_OK;
+}
+
 int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family) {
     if (jobs.empty()) return ST_OK;
     KTimer kt(family);
-    // jobs whose segments all feed accumulator 0 run in the one-accumulator kernel
+    // Jobs whose segments all feed one accumulator run in the one-accumulator
+    // kernel (with the output coefficients of accumulator 1 moved to 0 where
+    // needed); the others are ordered so that segment s feeds accumulator s.
     std::vector<FbJob> one, two;
-    for (const auto& j : jobs) {
-        bool uses1 = false;
-        for (int s = 0; s < j.nseg; s++) uses1 |= j.seg[s].acc != 0;
-        (uses1 ? two : one).push_back(j);
+    for (FbJob j : jobs) {
+        bool any0 = false, any1 = false;
+        for (int s = 0; s < j.nseg; s++) (j.seg[s].acc ? any1 : any0) = true;
+        if (any1 && !any0) {
+            for (int s = 0; s < j.nseg; s++) j.seg[s].acc = 0;
+            for (int o = 0; o < j.nout; o++) j.out[o].alpha0 = j.out[o].alpha1;
+            any1 = false;
+        }
+        if (!any1) {
+            one.push_back(j);
+            continue;
+        }
+        if (j.nseg != 2) return fail(ST_EINVAL, "fb gemm job with %d segments on two accumulators", j.nseg);
+        if (j.seg[0].acc == 1) std::swap(j.seg[0], j.seg[1]);
+        two.push_back(j);
     }
     for (int pass = 0; pass < 2; pass++) {
         const std::vector<FbJob>& js = pass == 0 ? one : two;
@@ -335,10 +385,14 @@ int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int 
         for (const auto& j : js) maxN = j.Ncol > maxN ? j.Ncol : maxN;
         const FbJob* dj = static_cast<const FbJob*>(device_table(js.data(), js.size() * sizeof(FbJob)));
         if (!dj) return ST_ENOMEM;
-        dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + GB_N - 1) / GB_N), (unsigned)js.size());
-        if (pass == 0) hipLaunchKernelGGL((k_fb_gemm<1>), grid, GB_THREADS, 0, stream(), dj, g);
-        else hipLaunchKernelGGL((k_fb_gemm<2>), grid, GB_THREADS, 0, stream(), dj, g);
-        CH_LAUNCH_CHECK("k_fb_gemm");
+        // 128-column tiles when they still give every CU two blocks, else 64-column tiles
+        const int64_t xb = (g.nx + GB_X - 1) / GB_X;
+        const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
+        int32_t st;
+        if (pass == 0) st = wide ? launch_gemm_variant<1, 4>(dj, js.size(), maxN, g)
+                                 : launch_gemm_variant<1, 2>(dj, js.size(), maxN, g);
+        else st = launch_gemm_variant<2, 2>(dj, js.size(), maxN, g);
+        CH_TRY(st);
     }
     return ST_OK;
 }

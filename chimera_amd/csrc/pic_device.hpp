@@ -103,6 +103,15 @@ __device__ __forceinline__ void boris(double u[3], const double f[6], double dt_
 }
 
 // Analytic planar undulator (devices.f90:162-203).
+// Workgroups of a 1-D launch go to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8), and each XCD has its own L2.  xcd_block renumbers them so that XCD x
+// takes one contiguous range of the logical block indices: with particles in
+// cell order that is one region of the grid per L2.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 struct UndulParams { double a0, lambda, X0, Lx, ku; };
 
 inline UndulParams make_undul(const double p[4]) {

@@ -30,7 +30,7 @@ namespace chimera {
 __global__ void k_gather_push(const double* __restrict__ coord, PStride sx, double* __restrict__ u, PStride su,
                               const double* __restrict__ w, const double2* __restrict__ Fld, GridGeom g,
                               double push_dt, UndulParams up, int n_undul, int np) {
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ip = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (ip >= np) return;
     const double xp = coord[sx.at(ip, 0)], yp = coord[sx.at(ip, 1)], zp = coord[sx.at(ip, 2)];
     double f[6] = {0, 0, 0, 0, 0, 0};
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
     __shared__ int red[8][DEP_THREADS / 64];
     __shared__ int box[8];
     const int tid = threadIdx.x;
-    const int64_t p0 = (int64_t)blockIdx.x * DEP_CHUNK;
+    const int64_t p0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * DEP_CHUNK;
     const int nmod = a.gJ.nkO + 1;
     const int64_t ld = a.ld;
     const bool doJ = a.J != nullptr, doR = a.R != nullptr;

@@ -183,11 +183,17 @@ void ref_align_data(double *dat, const int64_t *idx, int32_t ncomp, int32_t n, i
  * grid_deps.f90:46-66.  sgn = -1 for deposition ((y - i z)/r), +1 for gather. */
 typedef struct { int64_t ix, ir; double S0[2][2]; cplx phase[16]; } pgeom;
 
-static inline void ref_geom(pgeom *g, double xp, double yp, double zp, double rp, double leftX,
-                            const double *Rgrid, double dx_inv, double dr_inv, int nkO, double sgn)
+/* Cell, linear weights and azimuthal phases of one particle (grid_deps.f90:46-59).
+ * Returns 0 when the cell lies outside the arrays: the reference indexes
+ * without an x bound there (grid_deps.f90:46-47) and writes or reads outside
+ * its array; this restatement skips such particles, as the GPU kernels do. */
+static inline int ref_geom(pgeom *g, double xp, double yp, double zp, double rp, double leftX,
+                           const double *Rgrid, double dx_inv, double dr_inv, int nx, int nr, int nkO, double sgn)
 {
-    g->ix = (int64_t)floor((xp - leftX) * dx_inv);
-    g->ir = (int64_t)floor((rp - Rgrid[0]) * dr_inv);
+    const double fx = (xp - leftX) * dx_inv, fr = (rp - Rgrid[0]) * dr_inv;
+    if (!(fx >= 0.0 && fx < (double)nx) || !(fr >= 0.0 && fr < (double)nr)) return 0;
+    g->ix = (int64_t)floor(fx);
+    g->ir = (int64_t)floor(fr);
     g->S0[1][0] = (xp - leftX) * dx_inv - (double)g->ix;
     g->S0[0][0] = 1.0 - g->S0[1][0];
     g->S0[1][1] = (rp - Rgrid[g->ir]) * dr_inv;
@@ -197,6 +203,7 @@ static inline void ref_geom(pgeom *g, double xp, double yp, double zp, double rp
     else ph = cmk(0.0, 0.0);
     g->phase[0] = cmk(1.0, 0.0);
     for (int iO = 1; iO <= nkO; iO++) g->phase[iO] = cmul(g->phase[iO - 1], ph);
+    return 1;
 }
 
 /* Current deposition, serial.  grid_deps.f90:18-87.
@@ -217,7 +224,7 @@ void ref_dep_curr(const double *coord, const double *momenta, const double *wght
         const double gp = sqrt(1 + v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         for (int c = 0; c < 3; c++) v[c] = v[c] * wp / gp;
         pgeom g;
-        ref_geom(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nkO, -1.0);
+        if (!ref_geom(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nx, nr, nkO, -1.0)) continue;
         double cp[2][2];
         for (int b = 0; b < 2; b++)
             for (int a = 0; a < 2; a++) cp[a][b] = g.S0[a][0] * g.S0[b][1];
@@ -254,7 +261,7 @@ void ref_dep_dens(const double *coord, const double *wghts, double *dens_, doubl
         const double rp = sqrt(yp * yp + zp * zp);
         if (rp >= Rgrid[nr]) continue;
         pgeom g;
-        ref_geom(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nkO, -1.0);
+        if (!ref_geom(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nx, nr, nkO, -1.0)) continue;
         double dp[2][2];
         for (int b = 0; b < 2; b++)
             for (int a = 0; a < 2; a++) dp[a][b] = g.S0[a][0] * g.S0[b][1] * wp;
@@ -289,7 +296,7 @@ void ref_proj_fld(const double *coord, const double *wghts, const double *Fld_, 
         const double rp = sqrt(yp * yp + zp * zp);
         if (rp >= Rgrid[nr]) continue;
         pgeom g;
-        ref_geom(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nkO, +1.0);
+        if (!ref_geom(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nx, nr, nkO, +1.0)) continue;
         cplx pc[16][2][2];
         for (int iO = 0; iO <= nkO; iO++)
             for (int b = 0; b < 2; b++)

@@ -241,3 +241,29 @@ def test_maxwell(gpu, port):
     s = C(r, *sh)
     assert rel_max(gpu.omp_mult_scl(s.copy(order="F"), w2), port.omp_mult_scl(s.copy(order="F"), w2)) < TOL_ELEM
     assert rel_max(gpu.omp_add_scl(s.copy(order="F"), s), port.omp_add_scl(s.copy(order="F"), s)) < TOL_ELEM
+
+
+def test_particles_outside_the_grid(gpu, port):
+    """Particles before leftX, on or past the last x node and at Rgrid[nr]
+    are skipped by deposition and gather on both sides."""
+    say("case: dep_curr/dep_dens/proj_fld with particles outside the grid")
+    g = Grid(nx=40, nkr=10, nko=1)
+    coord, mom, w = particles(g, 400, seed=64)
+    xmax = g.leftX + (g.nx - 1) * g.dx
+    out = np.array([[g.leftX - 0.3 * g.dx, 0.2, 0.1], [xmax, 0.2, 0.1], [xmax + 0.7 * g.dx, 0.1, 0.3],
+                    [g.leftX + 5 * g.dx, 0.0, g.Rgrid[-1]]]).T
+    coord = np.asfortranarray(np.concatenate([coord, out], axis=1))
+    mom = np.asfortranarray(np.concatenate([mom, np.ones((3, 4))], axis=1))
+    w = np.concatenate([w, np.ones(4)])
+    args = (g.leftX, g.Rgrid, 1 / g.dx, 1 / g.dr)
+    z = np.zeros((g.nx, g.nkr + 1, 2, 3), complex, order="F")
+    assert rel_l2(gpu.dep_curr(coord, mom, w, z.copy(order="F"), *args),
+                  port.dep_curr(coord, mom, w, z.copy(order="F"), *args)) < TOL_SUM
+    zr = np.zeros((g.nx, g.nkr + 1, 2), complex, order="F")
+    assert rel_l2(gpu.dep_dens(coord, w, zr.copy(order="F"), *args),
+                  port.dep_dens(coord, w, zr.copy(order="F"), *args)) < TOL_SUM
+    Fld = C(rng(65), g.nx, g.nkr + 1, 2, 6)
+    z6 = np.zeros((6, coord.shape[1]), order="F")
+    a = gpu.proj_fld(coord, w, Fld, z6.copy(order="F"), *args)
+    b = port.proj_fld(coord, w, Fld, z6.copy(order="F"), *args)
+    assert np.all(a[:, -4:] == 0.0) and rel_max(a, b) < TOL_ELEM

@@ -259,3 +259,32 @@ def test_boris_closed_forms(port):
     fld[:3] = F(r, 3, n)
     out = port.push_velocs(u.copy(order="F"), fld, 0.2)
     assert np.allclose(out - u, 0.2 * fld[:3], rtol=0, atol=1e-14)
+
+
+def test_particles_outside_the_grid_are_skipped(port):
+    """A particle whose cell lies outside the arrays (x before leftX, x at or
+    past the last node, r at or past Rgrid[nr]) deposits nothing and gathers
+    zero fields; the reference has no x bound there (grid_deps.f90:46-47)."""
+    g = Grid(nx=40, nkr=10, nko=1)
+    coord, mom, w = particles(g, 400, seed=62)
+    xmax = g.leftX + (g.nx - 1) * g.dx
+    out = np.asfortranarray(np.array([[g.leftX - 0.3 * g.dx, 0.2, 0.1], [xmax, 0.2, 0.1],
+                                      [xmax + 0.7 * g.dx, 0.1, 0.3], [g.leftX + 5 * g.dx, 0.0, g.Rgrid[-1]]]).T)
+    mo = np.asfortranarray(np.ones((3, 4)))
+    wo = np.ones(4)
+    args = (g.leftX, g.Rgrid, 1 / g.dx, 1 / g.dr)
+    mix = lambda a, b: np.asfortranarray(np.concatenate([a, b], axis=-1))
+    z = lambda *s: np.zeros(s, complex, order="F")
+    for fn, extra, shape in (("dep_curr", (mom, mo), (g.nx, g.nkr + 1, 2, 3)), ("dep_dens", None, (g.nx, g.nkr + 1, 2))):
+        f = getattr(port, fn)
+        if extra:
+            a = f(coord, mom, w, z(*shape), *args)
+            b = f(mix(coord, out), mix(mom, mo), mix(w, wo), z(*shape), *args)
+        else:
+            a = f(coord, w, z(*shape), *args)
+            b = f(mix(coord, out), mix(w, wo), z(*shape), *args)
+        assert np.array_equal(a, b), fn
+    r = rng(63)
+    Fld = C(r, g.nx, g.nkr + 1, 2, 6)
+    ft = port.proj_fld(mix(coord, out), mix(w, wo), Fld, np.zeros((6, coord.shape[1] + 4), order="F"), *args)
+    assert np.all(ft[:, -4:] == 0.0) and np.any(ft[:, :-4] != 0.0)
