@@ -134,3 +134,23 @@ __host__ __device__ inline double2 crs(double s, double2 a) { return make_double
 __host__ __device__ inline double2 cmuli(double2 a) { return make_double2(-a.y, a.x); }
 __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 __host__ __device__ inline double2 cneg(double2 a) { return make_double2(-a.x, -a.y); }
+
+// ---------------------------------------------------------------- global-memory access helpers
+// A pointer read from a table in memory (rather than passed as a kernel
+// argument) is not known to point to global memory, and the compiler emits
+// FLAT loads for it.  FLAT accesses count against lgkmcnt as well as vmcnt, so
+// the wait for an LDS read would also wait for them; these helpers access such
+// pointers as global memory explicitly.
+#define CH_GLOBAL __attribute__((address_space(1)))
+typedef double ch_vd2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 gld2(const double2* p, int64_t i) {
+    const ch_vd2 v = ((const CH_GLOBAL ch_vd2*)p)[i];
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ double gld1(const double* p, int64_t i) { return ((const CH_GLOBAL double*)p)[i]; }
+__device__ __forceinline__ void gst2(double2* p, int64_t i, double2 v) {
+    ch_vd2 w;
+    w.x = v.x;
+    w.y = v.y;
+    ((CH_GLOBAL ch_vd2*)p)[i] = w;
+}

@@ -175,22 +175,26 @@ int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n)
 // that a lane's items (0,1) and (2,3) are the (re, im) of x = c and x = c + 4
 // with c = l/16.
 constexpr int GB_X = 64, GB_ROWS = 2 * GB_X, GB_K = 16, GB_THREADS = 256;
+// LDS row padding in doubles: rows 16 doubles (128 B) apart modulo 256 B put the
+// two rows that a half-wave reads in one ds_read_b64 on disjoint bank halves.
+constexpr int GB_PAD = 16;
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ double2 seg_load(const FbSeg& s, int64_t x, int64_t kk, int nx, int64_t lda) {
-    int64_t xr = x;
-    if (s.refl) xr = (x == 0) ? 0 : nx - x;
-    const int64_t off = xr + lda * kk;
-    double2 v0 = s.a0[off];
-    if (s.fold && kk == 0) v0 = csub(v0, s.a0[off - lda]);
-    if (s.refl) v0 = cneg(cconj(v0));
-    double2 v = cmul(s.c0, v0);
-    if (s.a1) {
-        double2 v1 = s.a1[off];
-        if (s.fold && kk == 0) v1 = csub(v1, s.a1[off - lda]);
-        if (s.refl) v1 = cneg(cconj(v1));
-        v = cadd(v, cmul(s.c1, v1));
+// Element (x, k') of segment s reads a0/a1 at seg_off; seg_finish turns the raw
+// loads into c0*T(a0) + c1*T(a1).  Loads and arithmetic are split so that the
+// loads of the next K-stage stay in flight while the MFMAs run.
+__device__ __forceinline__ int64_t seg_off(const FbSeg& s, int64_t x, int64_t kk, int nx, int64_t lda) {
+    const int64_t xr = s.refl ? (x == 0 ? 0 : nx - x) : x;
+    return xr + lda * kk;
+}
+
+__device__ __forceinline__ double2 seg_finish(const FbSeg& s, double2 v0, double2 v1) {
+    if (s.refl) {
+        v0 = cneg(cconj(v0));
+        v1 = cneg(cconj(v1));
     }
+    double2 v = cmul(s.c0, v0);
+    if (s.a1) v = cadd(v, cmul(s.c1, v1));
     return v;
 }
 
@@ -198,7 +202,7 @@ __device__ __forceinline__ double2 seg_load(const FbSeg& s, int64_t x, int64_t k
 template <int WN>
 __device__ __forceinline__ void mfma_stage(dbl4 (&acc)[4][WN], const double* __restrict__ As,
                                            const double* __restrict__ Ds, int lane, int wr, int wc) {
-    constexpr int LDA = GB_ROWS + 4, LDD = 32 * WN + 4;
+    constexpr int LDA = GB_ROWS + GB_PAD, LDD = 32 * WN + GB_PAD;
 #pragma unroll
     for (int kq = 0; kq < GB_K / 4; kq++) {
         const int kk = kq * 4 + (lane >> 4);
@@ -215,16 +219,17 @@ __device__ __forceinline__ void mfma_stage(dbl4 (&acc)[4][WN], const double* __r
 }
 
 // LDS bytes of one k_fb_gemm<NACC, WN> block: A and D stages, double-buffered.
-constexpr size_t gb_lds_bytes(int WN) { return 2 * (size_t)GB_K * ((GB_ROWS + 4) + (32 * WN + 4)) * sizeof(double); }
+constexpr size_t gb_lds_bytes(int WN) { return 2 * (size_t)GB_K * ((GB_ROWS + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(double); }
 
 // Block tile: 64 complex x (128 real rows) x 32*WN columns; 4 waves as 2 x 2,
-// each 64 rows x 16*WN columns.  NACC = 2 keeps a second accumulator set for
-// segments with acc = 1.  The K-stages of all segments form one pipeline: the
-// global loads of stage t+1 go to registers while the MFMAs of stage t run,
-// then land in the other LDS buffer, with one barrier per stage.
+// each 64 rows x 16*WN columns.  NACC = 2 keeps a second accumulator set,
+// fed by segment 1 (launch_fb_gemm orders the segments so).  The K-stages of
+// all segments form one pipeline: the global loads of stage t+1 go to
+// registers while the MFMAs of stage t run, then land in the other LDS
+// buffer, with one barrier per stage.
 template <int NACC, int WN>
 __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restrict__ jobs, FbGemmArgs g) {
-    constexpr int BN = 32 * WN, LDA = GB_ROWS + 4, LDD = BN + 4;
+    constexpr int BN = 32 * WN, LDA = GB_ROWS + GB_PAD, LDD = BN + GB_PAD;
     constexpr int A_STAGE = GB_K * LDA, D_STAGE = GB_K * LDD;
     const FbJob& J = jobs[blockIdx.z];
     const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
@@ -246,26 +251,43 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     constexpr int NA = (GB_K * GB_X) / GB_THREADS, ND = (GB_K * BN) / GB_THREADS;
     const int stages_per_seg = (K + GB_K - 1) / GB_K;
     const int nstages = stages_per_seg * nseg;
-    double2 ra[NA];
+    const FbSeg S0 = J.seg[0];
+    const FbSeg S1 = J.seg[nseg > 1 ? 1 : 0];
+    double2 ra[NA], rb[NA];
     double rd[ND];
-    auto fetch = [&](int t) {
-        const FbSeg& S = J.seg[t / stages_per_seg];
-        const int k0 = (t % stages_per_seg) * GB_K;
+    auto fetch_seg = [&](const FbSeg& S, int k0) __attribute__((always_inline)) {
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
             const int xl = e % GB_X, kk = e / GB_X;
             const int64_t xg = x0 + xl;
-            ra[i] = (xg < g.nx && k0 + kk < K) ? seg_load(S, xg, k0 + kk, g.nx, g.lda) : cmk(0.0, 0.0);
+            const bool ok = xg < g.nx && k0 + kk < K;
+            const int64_t off = seg_off(S, xg, k0 + kk, g.nx, g.lda);
+            ra[i] = ok ? gld2(S.a0, off) : cmk(0.0, 0.0);
+        }
+        if (S.a1) {
+#pragma unroll
+            for (int i = 0; i < NA; i++) {
+                const int e = tid + GB_THREADS * i;
+                const int xl = e % GB_X, kk = e / GB_X;
+                const int64_t xg = x0 + xl;
+                const bool ok = xg < g.nx && k0 + kk < K;
+                rb[i] = ok ? gld2(S.a1, seg_off(S, xg, k0 + kk, g.nx, g.lda)) : cmk(0.0, 0.0);
+            }
         }
 #pragma unroll
         for (int i = 0; i < ND; i++) {
             const int e = tid + GB_THREADS * i;
             const int kk = e % GB_K, nl = e / GB_K;
-            rd[i] = (k0 + kk < K && n0 + nl < Ncol) ? S.D[(int64_t)(k0 + kk) + S.ldd * (int64_t)(n0 + nl)] : 0.0;
+            rd[i] = (k0 + kk < K && n0 + nl < Ncol) ? gld1(S.D, (int64_t)(k0 + kk) + S.ldd * (int64_t)(n0 + nl)) : 0.0;
         }
     };
-    auto stash = [&](int buf) {
+    auto fetch = [&](int t) __attribute__((always_inline)) {
+        const int k0 = (t % stages_per_seg) * GB_K;
+        if (t < stages_per_seg) fetch_seg(S0, k0);
+        else fetch_seg(S1, k0);
+    };
+    auto stash_seg = [&](const FbSeg& S, int buf) __attribute__((always_inline)) {
         double* A = As + buf * A_STAGE;
         double* D = Ds + buf * D_STAGE;
 #pragma unroll
@@ -273,8 +295,9 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
             const int e = tid + GB_THREADS * i;
             const int xl = e % GB_X, kk = e / GB_X;
             const int c = xl & 7, slot = (xl >> 3) * 16 + (c & 3) + 8 * (c >> 2);
-            A[kk * LDA + slot] = ra[i].x;
-            A[kk * LDA + slot + 4] = ra[i].y;
+            const double2 v = seg_finish(S, ra[i], rb[i]);
+            A[kk * LDA + slot] = v.x;
+            A[kk * LDA + slot + 4] = v.y;
         }
 #pragma unroll
         for (int i = 0; i < ND; i++) {
@@ -282,23 +305,26 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
             D[(e % GB_K) * LDD + e / GB_K] = rd[i];
         }
     };
-    // stages [tb, te) into one accumulator set; the prefetch runs across the
+    auto stash = [&](int buf, int t) __attribute__((always_inline)) {
+        if (t < stages_per_seg) stash_seg(S0, buf);
+        else stash_seg(S1, buf);
+    };
+    // K-stages [tb, te) into one accumulator set; the prefetch runs across the
     // boundary between the two segments of a two-accumulator job
-    auto pipe = [&](dbl4 (&acc)[4][WN], int tb, int te) {
+    auto pipe = [&](dbl4 (&acc)[4][WN], int tb, int te) __attribute__((always_inline)) {
         for (int t = tb; t < te; t++) {
             if (t + 1 < nstages) fetch(t + 1);
             mfma_stage<WN>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
-            if (t + 1 < nstages) stash((t + 1) & 1);
+            if (t + 1 < nstages) stash((t + 1) & 1, t + 1);
             __syncthreads();
         }
     };
     if (nstages > 0) {
         fetch(0);
-        stash(0);
+        stash(0, 0);
     }
     __syncthreads();
     if constexpr (NACC == 2) {
-        // launch_fb_gemm hands this kernel jobs whose segment s feeds accumulator s
         pipe(acc0, 0, stages_per_seg);
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -321,19 +347,17 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
                 for (int h = 0; h < 2; h++) {
                     const int64_t xg = x0 + wr * 32 + i * 8 + (lane >> 4) + 4 * h;
                     if (xg >= g.nx) continue;
-                    const double2 p0 = cmk(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]);
-                    double2 v = cmul(O.alpha0, p0);
+                    double2 v = cmul(O.alpha0, cmk(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]));
                     if constexpr (NACC == 2) v = cadd(v, cmul(O.alpha1, cmk(acc1[i][j][2 * h], acc1[i][j][2 * h + 1])));
                     const int64_t off = xg + g.ldo * ng;
-                    if (O.diag) v = cadd(v, crs(g.kx[xg], cmul(O.gamma, O.diag[off])));
-                    if (O.scale) v = crs(O.scale[off], v);
-                    double2* dst = O.p + off;
+                    if (O.diag) v = cadd(v, crs(g.kx[xg], cmul(O.gamma, gld2(O.diag, off))));
+                    if (O.scale) v = crs(gld1(O.scale, off), v);
                     if (O.poiss) {
                         // maxwell_solvers.f90:153-158: J += (v + (g_np1 - g_n)*dt_inv)*w2_inv
-                        const double2 d = crs(g.dt_inv, csub(O.poiss_np1[off], O.poiss_n[off]));
-                        *dst = cadd(*dst, crs(O.poiss_w2[off], cadd(v, d)));
+                        const double2 d = crs(g.dt_inv, csub(gld2(O.poiss_np1, off), gld2(O.poiss_n, off)));
+                        gst2(O.p, off, cadd(gld2(O.p, off), crs(gld1(O.poiss_w2, off), cadd(v, d))));
                     } else {
-                        *dst = J.beta ? cadd(*dst, v) : v;
+                        gst2(O.p, off, J.beta ? cadd(gld2(O.p, off), v) : v);
                     }
                 }
             }
@@ -410,7 +434,6 @@ static FbSeg seg(const double2* a0, double2 c0, const double2* a1, double2 c1, c
     s.ldd = ldd;
     s.refl = refl;
     s.acc = acc;
-    s.fold = 0;
     return s;
 }
 
@@ -435,12 +458,11 @@ static const double2 ONE = {1, 0}, MONE = {-1, 0}, I1 = {0, 1}, MI = {0, -1}, ZE
 
 void build_dht_jobs(std::vector<FbJob>& jobs, const double2* in, int64_t in_m, int64_t in_l, const double* D,
                     int64_t ldd, int64_t d_m, int K, double2* out, int64_t out_m, int64_t out_l, int Ncol,
-                    int nmodes, int ncomp, int fold = 0) {
+                    int nmodes, int ncomp) {
     for (int l = 0; l < ncomp; l++)
         for (int m = 0; m < nmodes; m++) {
             FbJob j = job0();
             j.seg[0] = seg(in + m * in_m + l * in_l, ONE, nullptr, ZERO, D + m * d_m, ldd, 0, 0);
-            j.seg[0].fold = fold;
             j.nseg = 1;
             j.out[0] = outp(out + m * out_m + l * out_l, ONE, ZERO);
             j.nout = 1;
@@ -571,16 +593,17 @@ void build_graddiv2_jobs(std::vector<FbJob>& jobs, FbViewW v, FbView s, const do
 
 // ================================================================== composite device-side operations
 
-int32_t fb_forward(double2* out, const double2* in, double leftX, const double* kx_dev, const double* In_dev, int nkx,
+int32_t fb_forward(double2* out, double2* in, double leftX, const double* kx_dev, const double* In_dev, int nkx,
                    int nr, int nmodes, int nkr, int ncomp, const double* scale_tab, int fold) {
     const size_t tmp_elems = (size_t)nkx * nkr * nmodes * ncomp;
     double2* tmp = static_cast<double2*>(scratch(0, tmp_elems * sizeof(double2)));
     double2* ph = static_cast<double2*>(scratch(1, (size_t)nkx * sizeof(double2)));
     if (!tmp || !ph) return ST_ENOMEM;
+    if (fold) CH_TRY(ghost_fold(in, nkx, nr + 1, (int64_t)nmodes * ncomp));
     std::vector<FbJob> jobs;
     // vec(nkx, 0:nr, nmodes, ncomp): row 1 (first real node) is the DHT's r = 1
     build_dht_jobs(jobs, in + nkx, (int64_t)nkx * (nr + 1), (int64_t)nkx * (nr + 1) * nmodes, In_dev, nr,
-                   (int64_t)nr * nkr, nr, tmp, (int64_t)nkx * nkr, (int64_t)nkx * nkr * nmodes, nkr, nmodes, ncomp, fold);
+                   (int64_t)nr * nkr, nr, tmp, (int64_t)nkx * nkr, (int64_t)nkx * nkr * nmodes, nkr, nmodes, ncomp);
     FbGemmArgs g;
     memset(&g, 0, sizeof g);
     g.nx = nkx;
@@ -756,7 +779,7 @@ int32_t chimera_fb_vec_in(double* vec_fb, const double* vec, double leftX, const
     const double* k = st.in(kx, nkx);
     const double* M = st.in(In, (size_t)nr * nkr * nkO);
     CH_TRY(st.status());
-    CH_TRY(fb_forward(o, v, leftX, k, M, nkx, nr, nkO, nkr, 3, nullptr, 0));
+    CH_TRY(fb_forward(o, const_cast<double2*>(v), leftX, k, M, nkx, nr, nkO, nkr, 3, nullptr, 0));  // fold 0: read only
     return st.finish();
 }
 
@@ -770,7 +793,7 @@ int32_t chimera_fb_scl_in(double* scl_fb, const double* scl, double leftX, const
     const double* k = st.in(kx, nkx);
     const double* M = st.in(In, (size_t)nr * nkr * nkO);
     CH_TRY(st.status());
-    CH_TRY(fb_forward(o, v, leftX, k, M, nkx, nr, nkO, nkr, 1, nullptr, 0));
+    CH_TRY(fb_forward(o, const_cast<double2*>(v), leftX, k, M, nkx, nr, nkO, nkr, 1, nullptr, 0));  // fold 0: read only
     return st.finish();
 }
 
@@ -854,7 +877,7 @@ int32_t chimera_fb_graddiv(double* vec_fb, const double* DpS2S, const double* Dm
 
 /* ---------------------------------------------------------------- resident tier (device pointers) */
 
-int32_t chimera_fb_forward(double* out, const double* in, double leftX, const double* kx, const double* In,
+int32_t chimera_fb_forward(double* out, double* in, double leftX, const double* kx, const double* In,
                            const double* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
                            int32_t fold) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ncomp < 1) return fail(ST_EINVAL, "fb_forward: bad sizes");
@@ -862,7 +885,7 @@ int32_t chimera_fb_forward(double* out, const double* in, double leftX, const do
     if (!is_device_ptr(out) || !is_device_ptr(in) || !is_device_ptr(kx) || !is_device_ptr(In) ||
         (scale && !is_device_ptr(scale)))
         return fail(ST_EINVAL, "fb_forward: resident entry points take device pointers");
-    return fb_forward(reinterpret_cast<double2*>(out), reinterpret_cast<const double2*>(in), leftX, kx, In, nkx, nr,
+    return fb_forward(reinterpret_cast<double2*>(out), reinterpret_cast<double2*>(in), leftX, kx, In, nkx, nr,
                       nmodes, nkr, ncomp, scale, fold);
 }
 

@@ -36,7 +36,6 @@ struct FbSeg {
     int64_t ldd;
     int refl;
     int acc;
-    int fold;   // element k' = 0 minus the row before it (deposition ghost fold, grid_deps.f90:80-85)
 };
 
 struct FbOut {
@@ -70,10 +69,14 @@ int32_t launch_fft(FftArgs a);
 int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n);
 int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family);
 
+// Deposition ghost fold in place (grid_deps.f90:80-85): row 1 -= row 0, row 0
+// = 0 on each of nlines (nx1, nr1) planes.  Implemented in particles.hip.
+int32_t ghost_fold(double2* a, int nx1, int nr1, int64_t nlines);
+
 // Forward FB transform (fb_io.f90:18-98): in(nkx,0:nr,nmodes,ncomp) ->
 // out(nkx,nkr,nmodes,ncomp), optionally times scale_tab(nkx,nkr,nmodes); with
-// fold, row 1 is read as row 1 - row 0 (the deposition ghost fold).
-int32_t fb_forward(double2* out, const double2* in, double leftX, const double* kx_dev, const double* In_dev, int nkx,
+// fold, the deposition ghost fold is first applied to in, in place.
+int32_t fb_forward(double2* out, double2* in, double leftX, const double* kx_dev, const double* In_dev, int nkx,
                    int nr, int nmodes, int nkr, int ncomp, const double* scale_tab, int fold);
 // Inverse FB transform of E (first 3 components of e_fb) and B into
 // eb(nkx,0:nr,nmodes,6) (fb_io.f90:182-228); ghost as in FftArgs.

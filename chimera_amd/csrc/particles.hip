@@ -8,6 +8,7 @@
 // layout (strides ncomp, 1) and the structure-of-arrays layout of resident
 // simulations (strides 1, ld).
 #include "common.hpp"
+#include "fb_kernels.hpp"
 #include "pic_device.hpp"
 #include "../../include/chimera_hip.h"
 
@@ -132,6 +133,14 @@ __global__ void k_ghost_fold(double2* __restrict__ a, int nx1, int nr1, int nlin
     double2* r0 = a + ix + q * (int64_t)nx1 * nr1;
     r0[nx1] = csub(r0[nx1], r0[0]);
     r0[0] = cmk(0.0, 0.0);
+}
+
+int32_t ghost_fold(double2* a, int nx1, int nr1, int64_t nlines) {
+    if (nlines <= 0) return ST_OK;
+    hipLaunchKernelGGL(k_ghost_fold, blocks_for((int64_t)nx1 * nlines, 256), 256, 0, stream(), a, nx1, nr1,
+                       (int)nlines);
+    CH_LAUNCH_CHECK("k_ghost_fold");
+    return ST_OK;
 }
 
 // Mode normalisation and ghost row of the gathered field.  grid_deps.f90:219-266.

@@ -276,11 +276,12 @@ int32_t chimera_soa_sort_cells(const double* coord, const double* coord_half, co
                                int32_t* np_out, int32_t* perm);
 
 /* Forward FB transform of ncomp components, fb_vec_in/fb_scl_in with nkO =
- * nmodes, times scale(nkx,nkr,nmodes) (DepFact) when not NULL, and with fold=1
- * the deposition ghost fold applied to row 1 as it is read.
+ * nmodes, times scale(nkx,nkr,nmodes) (DepFact) when not NULL.  With fold=1
+ * the deposition ghost fold is first applied to `in` in place (row 1 -= row 0,
+ * row 0 = 0), which leaves `in` as the reference's dep_curr/dep_dens return it.
  * = dep fold + fb_vec_in + omp_mult_vec (grid_deps.f90:80-85, fb_io.f90:18-98,
  * solvers.py:417-435). */
-int32_t chimera_fb_forward(double* out, const double* in, double leftX, const double* kx, const double* In,
+int32_t chimera_fb_forward(double* out, double* in, double leftX, const double* kx, const double* In,
                            const double* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
                            int32_t fold);
 
