@@ -87,7 +87,10 @@ class Stage {
 // Scratch device memory owned by the library, grown on demand and reused
 // across calls on the same stream (the stream orders every use).
 void* scratch(int slot, size_t bytes);
-constexpr int SCRATCH_SLOTS = 16;
+// Byte fill of device memory by a kernel on the library stream (no hipMemsetAsync,
+// so that a captured graph replays it correctly).
+int32_t fill_bytes(void* p, int value, size_t bytes);
+constexpr int SCRATCH_SLOTS = 24;
 
 // Device copy of a small host table (job lists, twiddles), cached by content.
 const void* device_table(const void* host, size_t bytes);

@@ -18,7 +18,6 @@
 //                              (particle_tools.f90:130-153 + the permutation
 //                              gathers of species.py:385-398)
 #include <climits>
-#include <cstdlib>
 
 #include "common.hpp"
 #include "pic_device.hpp"
@@ -27,6 +26,9 @@
 
 namespace chimera {
 
+// NMOD = number of azimuthal modes, a template parameter so that every mode
+// index is a constant and the per-particle phases stay in registers.
+template <int NMOD>
 __global__ void k_gather_push(const double* __restrict__ coord, PStride sx, double* __restrict__ u, PStride su,
                               const double* __restrict__ w, const double2* __restrict__ Fld, GridGeom g,
                               double push_dt, UndulParams up, int n_undul, int np) {
@@ -36,11 +38,12 @@ __global__ void k_gather_push(const double* __restrict__ coord, PStride sx, doub
     double f[6] = {0, 0, 0, 0, 0, 0};
     PartGeom pg;
     if (w[ip] != 0.0 && part_geom(pg, xp, yp, zp, g, +1.0)) {
-        const int64_t sr = g.nx + 1, sm = sr * (g.nr + 1), sl = sm * (g.nkO + 1);
+        const int64_t sr = g.nx + 1, sm = sr * (g.nr + 1), sl = sm * NMOD;
         const double2* base = Fld + pg.ix + pg.ir * sr;
         for (int l = 0; l < 6; l++) {
             double acc = 0.0;
-            for (int iO = 0; iO <= g.nkO; iO++) {
+#pragma unroll
+            for (int iO = 0; iO < NMOD; iO++) {
                 const double2* fl = base + iO * sm + l * sl;
                 for (int b = 0; b < 2; b++)
                     for (int a = 0; a < 2; a++) {
@@ -66,10 +69,18 @@ __global__ void k_gather_push(const double* __restrict__ coord, PStride sx, doub
 // Phase 1 optionally pushes positions (particle_tools.f90:58-82) and finds the
 // block's cell bounding box for J (at the half-step positions) and rho (at the
 // new positions).  Phase 2 accumulates into LDS tiles covering those boxes with
-// LDS FP64 atomics; phase 3 adds the tiles to the grid with one global atomic
-// per node and component.  A box too large for the LDS budget makes that block
-// deposit straight to the grid with global atomics.  Deposits are unfolded; the
-// forward FB transform applies the ghost fold.
+// LDS FP64 atomics.  Phase 3 hands the tiles on:
+//  * a block whose tile spans at most DEP_SLOT_R node rows and DEP_SLOT_W node
+//    columns (the common case for particles in the reference's generation
+//    order or sorted by cell: 256 particles cover about 16 cells of one or two
+//    cell rows) stores its tile with plain stores in a slot of its own and
+//    records (first node row, first node column, width, rows) in a
+//    descriptor; k_dep_combine then adds the slots into the grid, so no
+//    global atomics;
+//  * any other block adds its tile to the grid with one global atomic per
+//    node and component, and a box too large for the LDS budget makes the
+//    block deposit straight to the grid with global atomics.
+// Deposits are unfolded; the forward FB transform applies the ghost fold.
 
 
 struct DepArgs {
@@ -84,7 +95,19 @@ struct DepArgs {
     GridGeom gJ, gR;    // geometry for J (leftX at the current deposit) and rho
     double2* J;         // (nx+1, nr+1, M, 3) or null
     double2* R;         // (nx+1, nr+1, M) or null
+    double2* slotJ;     // [block][3*M][DEP_SLOT_R][DEP_SLOT_W] (with J)
+    double2* slotR;     // [block][M][DEP_SLOT_R][DEP_SLOT_W] (with R)
+    int4* desc;         // [2][blocks]: (first node row or -1, first node column, width, rows) of J, then of R
+    int nblocks;
+    int vec4;           // particle arrays allow 32-byte vector access (ld % 4 == 0, 32-byte aligned bases)
 };
+
+typedef double dv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ dv4 ld4(const double* p) { return *reinterpret_cast<const dv4*>(p); }
+__device__ __forceinline__ void st4(double* p, dv4 v) { *reinterpret_cast<dv4*>(p) = v; }
+
+constexpr int DEP_THREADS = 64, DEP_PPT = 4, DEP_LDS_BYTES = 12 * 1024;
+constexpr int DEP_SLOT_W = 32, DEP_SLOT_R = 4;   // node columns and rows of one slot
 
 __device__ __forceinline__ void lds_add(double* p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -109,50 +132,117 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-template <int DEP_THREADS, int DEP_PPT, int DEP_LDS_BYTES>
+template <int NMOD>
 __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
     constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
-    extern __shared__ double tile[];
+    __shared__ double tile[DEP_LDS_BYTES / sizeof(double)];
     __shared__ int red[8][DEP_THREADS / 64];
     __shared__ int box[8];
     const int tid = threadIdx.x;
-    const int64_t p0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * DEP_CHUNK;
-    const int nmod = a.gJ.nkO + 1;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t p0 = (int64_t)blk * DEP_CHUNK;
+    constexpr int nmod = NMOD;
     const int64_t ld = a.ld;
     const bool doJ = a.J != nullptr, doR = a.R != nullptr;
 
-    // ---- phase 1: push, bounding boxes
-    int bx[8] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
-    for (int j = 0; j < DEP_PPT; j++) {
-        const int64_t ip = p0 + (int64_t)tid * DEP_PPT + j;
-        if (ip >= a.np) break;
-        const double wp = a.w[ip];
-        double xo[3] = {a.x[ip], a.x[ld + ip], a.x[2 * ld + ip]};
-        double xn[3], xhh[3];
-        const double u0 = a.u[ip], u1 = a.u[ld + ip], u2 = a.u[2 * ld + ip];
-        if (a.push) {
-            const double dt_gp = a.dt / sqrt(1.0 + (u0 * u0 + u1 * u1 + u2 * u2));
-            const double uu[3] = {u0, u1, u2};
+    // ---- phase 1: load the thread's DEP_PPT consecutive particles (32-byte
+    // vector loads when the arrays allow: a wave then reads 2 KB contiguous per
+    // component), push, store, bounding boxes.  The particles stay in
+    // registers for phase 2.
+    const int64_t ib = p0 + (int64_t)tid * DEP_PPT;
+    const bool full = a.vec4 && ib + DEP_PPT <= a.np;
+    // the thread's particles: component c of array p at particle ib + j
+    auto load_parts = [&](const double* p, double (&v)[3][DEP_PPT], int ncomp) __attribute__((always_inline)) {
+        if (full) {
+#pragma unroll
             for (int c = 0; c < 3; c++) {
-                xn[c] = xo[c] + uu[c] * dt_gp;
-                xhh[c] = 0.5 * (xo[c] + xn[c]);
-                a.x[c * ld + ip] = xn[c];
-                a.xh[c * ld + ip] = xhh[c];
+                if (c >= ncomp) break;
+                const dv4 t = ld4(p + c * ld + ib);
+                v[c][0] = t.x, v[c][1] = t.y, v[c][2] = t.z, v[c][3] = t.w;
             }
         } else {
-            for (int c = 0; c < 3; c++) {
-                xn[c] = xo[c];
-                xhh[c] = a.xh[c * ld + ip];
+#pragma unroll
+            for (int j = 0; j < DEP_PPT; j++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    if (c >= ncomp) break;
+                    v[c][j] = ib + j < a.np ? p[c * ld + ib + j] : 0.0;
+                }
+        }
+    };
+    int bx[8] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    {
+    double X[3][DEP_PPT], XH[3][DEP_PPT], U[3][DEP_PPT], W[DEP_PPT];
+    if (full) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const dv4 x4 = ld4(a.x + c * ld + ib), u4 = ld4(a.u + c * ld + ib);
+            X[c][0] = x4.x, X[c][1] = x4.y, X[c][2] = x4.z, X[c][3] = x4.w;
+            U[c][0] = u4.x, U[c][1] = u4.y, U[c][2] = u4.z, U[c][3] = u4.w;
+            if (!a.push) {
+                const dv4 h4 = ld4(a.xh + c * ld + ib);
+                XH[c][0] = h4.x, XH[c][1] = h4.y, XH[c][2] = h4.z, XH[c][3] = h4.w;
             }
         }
-        if (wp == 0.0) continue;
+        const dv4 w4 = ld4(a.w + ib);
+        W[0] = w4.x, W[1] = w4.y, W[2] = w4.z, W[3] = w4.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < DEP_PPT; j++) {
+            const int64_t ip = ib + j;
+            const bool ok = ip < a.np;
+            W[j] = ok ? a.w[ip] : 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                X[c][j] = ok ? a.x[c * ld + ip] : 0.0;
+                U[c][j] = ok ? a.u[c * ld + ip] : 0.0;
+                XH[c][j] = (ok && !a.push) ? a.xh[c * ld + ip] : 0.0;
+            }
+        }
+    }
+    if (a.push) {
+        // particle_tools.f90:58-82: x += u*dt/gamma; x_half = (x_old + x_new)/2
+#pragma unroll
+        for (int j = 0; j < DEP_PPT; j++) {
+            const double dt_gp = a.dt / sqrt(1.0 + (U[0][j] * U[0][j] + U[1][j] * U[1][j] + U[2][j] * U[2][j]));
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double xn = X[c][j] + U[c][j] * dt_gp;
+                XH[c][j] = 0.5 * (X[c][j] + xn);
+                X[c][j] = xn;
+            }
+        }
+        if (full) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                st4(a.x + c * ld + ib, dv4{X[c][0], X[c][1], X[c][2], X[c][3]});
+                st4(a.xh + c * ld + ib, dv4{XH[c][0], XH[c][1], XH[c][2], XH[c][3]});
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < DEP_PPT; j++) {
+                const int64_t ip = ib + j;
+                if (ip >= a.np) break;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    a.x[c * ld + ip] = X[c][j];
+                    a.xh[c * ld + ip] = XH[c][j];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < DEP_PPT; j++) {
+        if (W[j] == 0.0) continue;
         int ix, ir;
-        if (doJ && (fabs(u0) + fabs(u1) + fabs(u2)) != 0.0 && cell_of(xhh[0], xhh[1], xhh[2], a.gJ, ix, ir)) {
+        if (doJ && (fabs(U[0][j]) + fabs(U[1][j]) + fabs(U[2][j])) != 0.0 &&
+            cell_of(XH[0][j], XH[1][j], XH[2][j], a.gJ, ix, ir)) {
             bx[0] = min(bx[0], ix); bx[1] = max(bx[1], ix); bx[2] = min(bx[2], ir); bx[3] = max(bx[3], ir);
         }
-        if (doR && cell_of(xn[0], xn[1], xn[2], a.gR, ix, ir)) {
+        if (doR && cell_of(X[0][j], X[1][j], X[2][j], a.gR, ix, ir)) {
             bx[4] = min(bx[4], ix); bx[5] = max(bx[5], ix); bx[6] = min(bx[6], ir); bx[7] = max(bx[7], ir);
         }
+    }
     }
     for (int q = 0; q < 8; q++) {
         const int v = (q & 1) ? wave_max(bx[q]) : wave_min(bx[q]);
@@ -177,87 +267,254 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
     for (int64_t i = tid; i < nz; i += DEP_THREADS) tile[i] = 0.0;
     __syncthreads();
 
-    // ---- phase 2: deposit
+    // ---- phase 2: deposit.  A thread's consecutive particles usually share a
+    // cell (the lattice of a cell is ordered with the azimuthal index
+    // innermost), so each thread sums its per-corner contributions in
+    // registers while the cell stays the same and adds them to the LDS tile
+    // once per cell.  Four passes over the thread's particles (J_x, J_y, J_z,
+    // rho) keep the register sums to one component.
     const int64_t sr = a.gJ.nx + 1, sm = sr * (a.gJ.nr + 1), sl = sm * nmod;
-    for (int j = 0; j < DEP_PPT; j++) {
-        const int64_t ip = p0 + (int64_t)tid * DEP_PPT + j;
-        if (ip >= a.np) break;
-        const double wp = a.w[ip];
-        if (wp == 0.0) continue;
-        PartGeom pg;
-        if (doJ) {
-            double v[3] = {a.u[ip], a.u[ld + ip], a.u[2 * ld + ip]};
-            if ((fabs(v[0]) + fabs(v[1]) + fabs(v[2])) != 0.0 &&
-                part_geom(pg, a.xh[ip], a.xh[ld + ip], a.xh[2 * ld + ip], a.gJ, -1.0)) {
-                const double gp = sqrt(1 + v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                for (int c = 0; c < 3; c++) v[c] = v[c] * wp / gp;
-                // in-tile test on the cell found here (phase 1 may round differently at a cell edge)
-                const int64_t tx = pg.ix - box[0], tr = pg.ir - box[2];
-                const bool inJ = tileJ && tx >= 0 && tx + 1 < TXJ && tr >= 0 && tr + 1 < TRJ;
-                for (int l = 0; l < 3; l++)
-                    for (int m = 0; m < nmod; m++) {
-                        const double2 pv = crs(v[l], pg.ph[m]);
-                        for (int b = 0; b < 2; b++)
-                            for (int aa = 0; aa < 2; aa++) {
-                                const double2 d = crs(pg.S[0][aa] * pg.S[1][b], pv);
-                                if (inJ) {
-                                    const int64_t t = (((int64_t)(l * nmod + m) * TRJ + (tr + b)) * TXJ + (tx + aa)) * 2;
-                                    lds_add(tJ + t, d.x);
-                                    lds_add(tJ + t + 1, d.y);
-                                } else {
-                                    double* dst = reinterpret_cast<double*>(a.J + (pg.ix + aa) + (pg.ir + b) * sr +
-                                                                            m * sm + l * sl);
-                                    unsafeAtomicAdd(dst, d.x);
-                                    unsafeAtomicAdd(dst + 1, d.y);
-                                }
-                            }
+    for (int l = 0; l < 4; l++) {
+        const bool isJ = l < 3;
+        if ((isJ && !doJ) || (!isJ && !doR)) continue;
+        const GridGeom& gg = isJ ? a.gJ : a.gR;
+        double* tile_l = isJ ? tJ + (int64_t)l * nmod * TRJ * TXJ * 2 : tR;
+        double2* grid_l = isJ ? a.J + l * sl : a.R;
+        const int bx0 = isJ ? box[0] : box[4], br0 = isJ ? box[2] : box[6];
+        const int TX = isJ ? TXJ : TXR, TR = isJ ? TRJ : TRR;
+        const bool tiled = isJ ? tileJ : tileR;
+        double2 acc[NMOD][4];
+        int cx = -1, cr = 0;
+        auto flush = [&]() __attribute__((always_inline)) {
+            if (cx < 0) return;
+            const int tx = cx - bx0, tr = cr - br0;
+#pragma unroll
+            for (int m = 0; m < NMOD; m++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int t = ((m * TR + (tr + (q >> 1))) * TX + (tx + (q & 1))) * 2;
+                    lds_add(tile_l + t, acc[m][q].x);
+                    lds_add(tile_l + t + 1, acc[m][q].y);
+                }
+        };
+        double P[3][DEP_PPT], V[3][DEP_PPT], Wt[3][DEP_PPT];   // Wt: row 0 only
+        load_parts(isJ ? a.xh : a.x, P, 3);
+        if (isJ) load_parts(a.u, V, 3);
+        load_parts(a.w, Wt, 1);
+#pragma unroll
+        for (int j = 0; j < DEP_PPT; j++) {
+            const double wp = Wt[0][j];
+            if (wp == 0.0) continue;
+            double q0;   // weight of this component: w*u_l/gamma for J, w for rho
+            PartGeom pg;
+            if (isJ) {
+                const double u0 = V[0][j], u1 = V[1][j], u2 = V[2][j];
+                if ((fabs(u0) + fabs(u1) + fabs(u2)) == 0.0) continue;
+                if (!part_geom(pg, P[0][j], P[1][j], P[2][j], gg, -1.0)) continue;
+                const double gp = sqrt(1 + u0 * u0 + u1 * u1 + u2 * u2);
+                q0 = (l == 0 ? u0 : l == 1 ? u1 : u2) * wp / gp;
+            } else {
+                if (!part_geom(pg, P[0][j], P[1][j], P[2][j], gg, -1.0)) continue;
+                q0 = wp;
+            }
+            // in-tile test on the cell found here (phase 1 may round differently at a cell edge)
+            const int64_t tx = pg.ix - bx0, tr = pg.ir - br0;
+            if (tiled && tx >= 0 && tx + 1 < TX && tr >= 0 && tr + 1 < TR) {
+                if (pg.ix != cx || pg.ir != cr) {
+                    flush();
+#pragma unroll
+                    for (int m = 0; m < NMOD; m++)
+#pragma unroll
+                        for (int q = 0; q < 4; q++) acc[m][q] = cmk(0.0, 0.0);
+                    cx = (int)pg.ix;
+                    cr = (int)pg.ir;
+                }
+#pragma unroll
+                for (int m = 0; m < NMOD; m++) {
+                    const double2 pv = crs(q0, pg.ph[m]);
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        acc[m][q] = cadd(acc[m][q], crs(pg.S[0][q & 1] * pg.S[1][q >> 1], pv));
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < NMOD; m++) {
+                    const double2 pv = crs(q0, pg.ph[m]);
+                    for (int q = 0; q < 4; q++) {
+                        const double2 d = crs(pg.S[0][q & 1] * pg.S[1][q >> 1], pv);
+                        double* dst = reinterpret_cast<double*>(grid_l + (pg.ix + (q & 1)) + (pg.ir + (q >> 1)) * sr +
+                                                                m * sm);
+                        unsafeAtomicAdd(dst, d.x);
+                        unsafeAtomicAdd(dst + 1, d.y);
                     }
+                }
             }
         }
-        if (doR && part_geom(pg, a.x[ip], a.x[ld + ip], a.x[2 * ld + ip], a.gR, -1.0)) {
-            const int64_t tx = pg.ix - box[4], tr = pg.ir - box[6];
-            const bool inR = tileR && tx >= 0 && tx + 1 < TXR && tr >= 0 && tr + 1 < TRR;
-            for (int m = 0; m < nmod; m++)
-                for (int b = 0; b < 2; b++)
-                    for (int aa = 0; aa < 2; aa++) {
-                        const double2 d = crs(pg.S[0][aa] * pg.S[1][b] * wp, pg.ph[m]);
-                        if (inR) {
-                            const int64_t t = (((int64_t)m * TRR + (tr + b)) * TXR + (tx + aa)) * 2;
-                            lds_add(tR + t, d.x);
-                            lds_add(tR + t + 1, d.y);
-                        } else {
-                            double* dst = reinterpret_cast<double*>(a.R + (pg.ix + aa) + (pg.ir + b) * sr + m * sm);
-                            unsafeAtomicAdd(dst, d.x);
-                            unsafeAtomicAdd(dst + 1, d.y);
-                        }
-                    }
-        }
+        flush();
     }
     __syncthreads();
 
-    // ---- phase 3: flush the tiles (x fastest: coalesced atomics)
-    if (tileJ) {
-        const int64_t n = nJ / 2;
-        for (int64_t i = tid; i < n; i += DEP_THREADS) {
-            const double re = tJ[2 * i], im = tJ[2 * i + 1];
-            if (re == 0.0 && im == 0.0) continue;
-            const int64_t xx = i % TXJ, rest = i / TXJ, rr = rest % TRJ, lm = rest / TRJ;
-            double* dst = reinterpret_cast<double*>(a.J + (box[0] + xx) + (box[2] + rr) * sr + lm * sm);
-            unsafeAtomicAdd(dst, re);
-            unsafeAtomicAdd(dst + 1, im);
+    // ---- phase 3: row-pure tiles to the block's slots, other tiles by global atomics
+    const int4 none = make_int4(-1, 0, 0, 0);
+    if (doJ) {
+        const bool slot = tileJ && TRJ <= DEP_SLOT_R && TXJ <= DEP_SLOT_W;
+        if (slot) {
+            // tile index (lm*TRJ + rr)*TXJ + xx -> slot index (lm*DEP_SLOT_R + rr)*DEP_SLOT_W + xx
+            double2* sl = a.slotJ + (int64_t)blk * (3 * nmod * DEP_SLOT_R * DEP_SLOT_W);
+            const int64_t n = nJ / 2;
+            for (int64_t i = tid; i < n; i += DEP_THREADS) {
+                const int64_t rest = i / TXJ;
+                sl[((rest / TRJ) * DEP_SLOT_R + rest % TRJ) * DEP_SLOT_W + i % TXJ] = cmk(tJ[2 * i], tJ[2 * i + 1]);
+            }
+        } else if (tileJ) {
+            const int64_t n = nJ / 2;
+            for (int64_t i = tid; i < n; i += DEP_THREADS) {
+                const double re = tJ[2 * i], im = tJ[2 * i + 1];
+                if (re == 0.0 && im == 0.0) continue;
+                const int64_t xx = i % TXJ, rest = i / TXJ, rr = rest % TRJ, lm = rest / TRJ;
+                double* dst = reinterpret_cast<double*>(a.J + (box[0] + xx) + (box[2] + rr) * sr + lm * sm);
+                unsafeAtomicAdd(dst, re);
+                unsafeAtomicAdd(dst + 1, im);
+            }
+        }
+        if (tid == 0) a.desc[blk] = slot ? make_int4(box[2], box[0], TXJ, TRJ) : none;
+    }
+    if (doR) {
+        const bool slot = tileR && TRR <= DEP_SLOT_R && TXR <= DEP_SLOT_W;
+        if (slot) {
+            double2* sl = a.slotR + (int64_t)blk * (nmod * DEP_SLOT_R * DEP_SLOT_W);
+            const int64_t n = nR / 2;
+            for (int64_t i = tid; i < n; i += DEP_THREADS) {
+                const int64_t rest = i / TXR;
+                sl[((rest / TRR) * DEP_SLOT_R + rest % TRR) * DEP_SLOT_W + i % TXR] = cmk(tR[2 * i], tR[2 * i + 1]);
+            }
+        } else if (tileR) {
+            const int64_t n = nR / 2;
+            for (int64_t i = tid; i < n; i += DEP_THREADS) {
+                const double re = tR[2 * i], im = tR[2 * i + 1];
+                if (re == 0.0 && im == 0.0) continue;
+                const int64_t xx = i % TXR, rest = i / TXR, rr = rest % TRR, m = rest / TRR;
+                double* dst = reinterpret_cast<double*>(a.R + (box[4] + xx) + (box[6] + rr) * sr + m * sm);
+                unsafeAtomicAdd(dst, re);
+                unsafeAtomicAdd(dst + 1, im);
+            }
+        }
+        if (tid == 0) a.desc[a.nblocks + blk] = slot ? make_int4(box[6], box[4], TXR, TRR) : none;
+    }
+}
+
+// ------------------------------------------------------------------ slot combine
+//
+// Descriptor d (0 <= d < 2*nblocks; J first, then rho) gets the bucket key
+// which*nrb + first node row (nrb = nr + 1 buckets per grid), or 2*nrb
+// without a slot.
+// A stable radix sort of the keys lists every bucket's slots in block order.
+__global__ void k_slot_keys(const int4* __restrict__ desc, int nb, int nrb, uint32_t* __restrict__ keys) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= 2 * nb) return;
+    const int row = desc[d].x;
+    keys[d] = row >= 0 ? (uint32_t)((d >= nb ? nrb : 0) + row) : (uint32_t)(2 * nrb);
+}
+
+// bounds[2*k], bounds[2*k+1]: first and one-past-last sorted position of bucket k (zeroed before)
+__global__ void k_slot_bounds(const uint32_t* __restrict__ keys, int n, uint32_t nkeys, int32_t* __restrict__ bounds) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = keys[p];
+    if (k >= nkeys) return;
+    if (p == 0 || keys[p - 1] != k) bounds[2 * k] = p;
+    if (p == n - 1 || keys[p + 1] != k) bounds[2 * k + 1] = p + 1;
+}
+
+constexpr int CMB_T = 256, CMB_LIST = 1024;
+
+// One workgroup per (256 node columns, node row r, grid): the slots whose
+// first node row is r - layer (layer < DEP_SLOT_R, their node row `layer` is
+// r) and whose columns overlap the workgroup's are listed in LDS in sorted
+// order (stable compaction with ballots), with their column range; each
+// thread then sums the listed slots that cover its node and adds the sum to
+// the grid once.  The order of the sum is fixed: layer 0 first, blocks in
+// index order within a layer.
+template <int NMOD>
+__global__ void __launch_bounds__(CMB_T) k_dep_combine(double2* __restrict__ J, double2* __restrict__ R,
+                                                        const double2* __restrict__ slotJ,
+                                                        const double2* __restrict__ slotR,
+                                                        const int4* __restrict__ desc, const int32_t* __restrict__ sidx,
+                                                        const int32_t* __restrict__ bounds, int nb, int nrb,
+                                                        int64_t sr, int64_t sm, int nx1) {
+    constexpr int NC = 3 * NMOD;
+    const int which = blockIdx.z;
+    double2* grid = which ? R : J;
+    if (grid == nullptr) return;
+    const double2* slot = which ? slotR : slotJ;
+    const int ncomp = which ? NMOD : NC;
+    const int64_t sstride = (int64_t)ncomp * DEP_SLOT_R * DEP_SLOT_W;
+    const int r = blockIdx.y, xs = blockIdx.x * CMB_T, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x = xs + tid;
+    __shared__ int4 list[CMB_LIST];   // (4*block + layer, first column, width, 0)
+    __shared__ int wcount[CMB_T / 64];
+    __shared__ int nlist;
+    double2 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = cmk(0.0, 0.0);
+    bool hit = false;
+    auto drain = [&](int n) __attribute__((always_inline)) {
+        for (int e = 0; e < n; e++) {
+            const int4 ent = list[e];
+            if (x >= ent.y && x < ent.y + ent.z) {
+                hit = true;
+                const int b = ent.x >> 2, layer = ent.x & 3;
+                const double2* sp = slot + (int64_t)b * sstride + layer * DEP_SLOT_W + (x - ent.y);
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+                    if (c < ncomp) acc[c] = cadd(acc[c], sp[c * DEP_SLOT_R * DEP_SLOT_W]);
+            }
+        }
+    };
+    if (tid == 0) nlist = 0;
+    __syncthreads();
+    for (int layer = 0; layer < DEP_SLOT_R; layer++) {
+        const int row = r - layer;
+        if (row < 0 || row >= nrb) continue;
+        const int k = which * nrb + row;
+        const int b0 = bounds[2 * k], b1 = bounds[2 * k + 1];
+        for (int q0 = b0; q0 < b1; q0 += CMB_T) {
+            const int q = q0 + tid;
+            int4 ent = make_int4(-1, 0, 0, 0);
+            if (q < b1) {
+                const int b = sidx[q] - which * nb;
+                const int4 d = desc[which * nb + b];
+                if (layer < d.w && d.y < xs + CMB_T && d.y + d.z > xs) ent = make_int4(4 * b + layer, d.y, d.z, 0);
+            }
+            // stable compaction: wave ballot + per-wave offsets
+            const unsigned long long m = __ballot(ent.x >= 0);
+            const int before = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wcount[wv] = __popcll(m);
+            __syncthreads();
+            int off = nlist;
+            for (int w = 0; w < wv; w++) off += wcount[w];
+            if (ent.x >= 0) list[off + before] = ent;
+            __syncthreads();
+            if (tid == 0) {
+                int tot = 0;
+                for (int w = 0; w < CMB_T / 64; w++) tot += wcount[w];
+                nlist += tot;
+            }
+            __syncthreads();
+            if (nlist > CMB_LIST - CMB_T) {
+                if (x < nx1) drain(nlist);
+                __syncthreads();
+                if (tid == 0) nlist = 0;
+                __syncthreads();
+            }
         }
     }
-    if (tileR) {
-        const int64_t n = nR / 2;
-        for (int64_t i = tid; i < n; i += DEP_THREADS) {
-            const double re = tR[2 * i], im = tR[2 * i + 1];
-            if (re == 0.0 && im == 0.0) continue;
-            const int64_t xx = i % TXR, rest = i / TXR, rr = rest % TRR, m = rest / TRR;
-            double* dst = reinterpret_cast<double*>(a.R + (box[4] + xx) + (box[6] + rr) * sr + m * sm);
-            unsafeAtomicAdd(dst, re);
-            unsafeAtomicAdd(dst + 1, im);
-        }
-    }
+    if (x < nx1) drain(nlist);
+    if (x >= nx1 || !hit) return;
+    double2* gp = grid + x + (int64_t)r * sr;
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+        if (c < ncomp) gp[c * sm] = cadd(gp[c * sm], acc[c]);
 }
 
 // Stable compaction: flags, exclusive scan, scatter of every particle array.
@@ -278,6 +535,19 @@ __global__ void k_compact_scatter(const int32_t* __restrict__ flag, const int64_
     const int64_t o = pos[ip];
     for (int c = 0; c < ncomp; c++) dst[c * ld_dst + o] = src[c * ld_src + ip];
 }
+
+// Launch kernel template K<NMOD> for NMOD = nkO + 1 (check_resident bounds nkO).
+#define CH_MODE_SWITCH(nkO, LAUNCH)                  \
+    switch ((nkO) + 1) {                             \
+        case 1: LAUNCH(1); break;                    \
+        case 2: LAUNCH(2); break;                    \
+        case 3: LAUNCH(3); break;                    \
+        case 4: LAUNCH(4); break;                    \
+        case 5: LAUNCH(5); break;                    \
+        case 6: LAUNCH(6); break;                    \
+        case 7: LAUNCH(7); break;                    \
+        default: LAUNCH(8); break;                   \
+    }
 
 static GridGeom resident_geom(double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
                               int nx, int nr, int nkO) {
@@ -402,8 +672,11 @@ int32_t chimera_soa_gather_push(const double* coord, double* momenta, const doub
         nu = 1;
     }
     KTimer kt(KF_GATHER_PUSH);
-    hipLaunchKernelGGL(k_gather_push, blocks_for(np, 256), 256, 0, stream(), coord, s, momenta, s, wghts,
-                       reinterpret_cast<const double2*>(Fld), g, push_dt, up, nu, np);
+#define GATHER_LAUNCH(N)                                                                                      \
+    hipLaunchKernelGGL((k_gather_push<N>), blocks_for(np, 256), 256, 0, stream(), coord, s, momenta, s, wghts, \
+                       reinterpret_cast<const double2*>(Fld), g, push_dt, up, nu, np)
+    CH_MODE_SWITCH(nkO, GATHER_LAUNCH)
+#undef GATHER_LAUNCH
     CH_LAUNCH_CHECK("k_gather_push");
     return ST_OK;
 }
@@ -463,33 +736,51 @@ int32_t chimera_soa_push_deposit(double* coord, double* coord_half, const double
     a.gR = resident_geom(leftX_rho, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
     a.J = reinterpret_cast<double2*>(curr);
     a.R = reinterpret_cast<double2*>(dens);
-    // block shape: threads x consecutive particles per thread, LDS tile budget;
-    // CHIMERA_DEP_SHAPE selects one for sweeps
-    static int shape = -1;
-    if (shape < 0) {
-        const char* e = getenv("CHIMERA_DEP_SHAPE");
-        shape = e ? atoi(e) : 4;
-        if (shape < 0 || shape > 4) shape = 4;
+    {
+        auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 31) == 0; };
+        a.vec4 = (ld % 4 == 0) && al(coord) && al(coord_half) && al(momenta) && al(wghts);
     }
+    const int nb = (int)blocks_for(np, DEP_THREADS * DEP_PPT);
+    const int nmod = nkO + 1, nrb = nr + 1;
+    a.nblocks = nb;
+    a.slotJ = a.J ? static_cast<double2*>(
+                        scratch(16, (size_t)nb * 3 * nmod * DEP_SLOT_R * DEP_SLOT_W * sizeof(double2)))
+                  : nullptr;
+    a.slotR = a.R ? static_cast<double2*>(scratch(17, (size_t)nb * nmod * DEP_SLOT_R * DEP_SLOT_W * sizeof(double2)))
+                  : nullptr;
+    a.desc = static_cast<int4*>(scratch(18, (size_t)2 * nb * sizeof(int4)));
+    uint32_t* keys = static_cast<uint32_t*>(scratch(19, (size_t)2 * nb * 4 * sizeof(uint32_t)));
+    int32_t* bounds = static_cast<int32_t*>(scratch(23, (size_t)2 * 2 * nrb * sizeof(int32_t)));
+    if ((a.J && !a.slotJ) || (a.R && !a.slotR) || !a.desc || !keys || !bounds) return ST_ENOMEM;
+    if (!a.J && !a.R) return ST_OK;
     KTimer kt(a.J ? KF_DEP_CURR : KF_DEP_DENS);
-    switch (shape) {
-        case 0:
-            hipLaunchKernelGGL((k_push_deposit<256, 4, 48 * 1024>), blocks_for(np, 1024), 256, 48 * 1024, stream(), a);
-            break;
-        case 1:
-            hipLaunchKernelGGL((k_push_deposit<64, 16, 48 * 1024>), blocks_for(np, 1024), 64, 48 * 1024, stream(), a);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_push_deposit<64, 8, 24 * 1024>), blocks_for(np, 512), 64, 24 * 1024, stream(), a);
-            break;
-        case 3:
-            hipLaunchKernelGGL((k_push_deposit<128, 4, 24 * 1024>), blocks_for(np, 512), 128, 24 * 1024, stream(), a);
-            break;
-        default:
-            hipLaunchKernelGGL((k_push_deposit<64, 4, 12 * 1024>), blocks_for(np, 256), 64, 12 * 1024, stream(), a);
-            break;
-    }
+    // descriptors of a grid not deposited here read "no slot" (row -1)
+    CH_TRY(fill_bytes(a.desc, 0xFF, (size_t)2 * nb * sizeof(int4)));
+#define DEP_LAUNCH(N) hipLaunchKernelGGL((k_push_deposit<N>), nb, DEP_THREADS, 0, stream(), a)
+    CH_MODE_SWITCH(nkO, DEP_LAUNCH)
+#undef DEP_LAUNCH
     CH_LAUNCH_CHECK("k_push_deposit");
+    // bucket the slots by (grid, cell row), in block order
+    const int nd = 2 * nb;
+    uint32_t* keys_tmp = keys + nd;
+    int32_t* idx = reinterpret_cast<int32_t*>(keys + 2 * (size_t)nd);
+    int32_t* idx_tmp = idx + nd;
+    hipLaunchKernelGGL(k_slot_keys, blocks_for(nd, 256), 256, 0, stream(), a.desc, nb, nrb, keys);
+    CH_LAUNCH_CHECK("k_slot_keys");
+    int bits = 1;
+    while ((1u << bits) <= (uint32_t)(2 * nrb)) bits++;
+    CH_TRY(radix_sort_keys(keys, idx, keys_tmp, idx_tmp, nd, bits, 20, 21));
+    CH_TRY(fill_bytes(bounds, 0, (size_t)2 * 2 * nrb * sizeof(int32_t)));
+    hipLaunchKernelGGL(k_slot_bounds, blocks_for(nd, 256), 256, 0, stream(), keys, nd, (uint32_t)(2 * nrb), bounds);
+    CH_LAUNCH_CHECK("k_slot_bounds");
+    const int64_t sr = nx + 1, sm = sr * (nr + 1);
+    dim3 grid((unsigned)((nx + 1 + CMB_T - 1) / CMB_T), (unsigned)(nr + 1), 2u);
+#define CMB_LAUNCH(N)                                                                                           \
+    hipLaunchKernelGGL((k_dep_combine<N>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR, a.desc, idx, bounds, \
+                       nb, nrb, sr, sm, nx + 1)
+    CH_MODE_SWITCH(nkO, CMB_LAUNCH)
+#undef CMB_LAUNCH
+    CH_LAUNCH_CHECK("k_dep_combine");
     return ST_OK;
 }
 

@@ -73,6 +73,7 @@ class Specie:
     # coords, coords_halfstep, momenta: component c of particle i at c*cap + i
     # (a Fortran (cap, 3) array); weights: (cap,).
     def _alloc(self, cap):
+        cap = -(-int(cap) // 64) * 64        # components start on 512-byte boundaries (vector loads)
         old, oldcap = (self.Data, self.cap) if self.cap else (None, 0)
         D = {k: DeviceArray((cap, 3), np.float64) for k in ("coords", "coords_halfstep", "momenta")}
         D["weights"] = DeviceArray((cap,), np.float64)
