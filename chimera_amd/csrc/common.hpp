@@ -157,3 +157,41 @@ __device__ __forceinline__ void gst2(double2* p, int64_t i, double2 v) {
     w.y = v.y;
     ((CH_GLOBAL ch_vd2*)p)[i] = w;
 }
+
+// ---------------------------------------------------------------- single-precision complex (mixed precision)
+__host__ __device__ inline float2 cmkf(float re, float im) { return make_float2(re, im); }
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 crs(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ inline float2 cneg(float2 a) { return make_float2(-a.x, -a.y); }
+
+// Complex type of a real type: cplx_t<double> = double2, cplx_t<float> = float2.
+template <typename R> struct CplxOf;
+template <> struct CplxOf<double> { typedef double2 type; };
+template <> struct CplxOf<float> { typedef float2 type; };
+template <typename R> using cplx_t = typename CplxOf<R>::type;
+template <typename R> __host__ __device__ inline cplx_t<R> cmake(R re, R im);
+template <> __host__ __device__ inline double2 cmake<double>(double re, double im) { return make_double2(re, im); }
+template <> __host__ __device__ inline float2 cmake<float>(float re, float im) { return make_float2(re, im); }
+// complex conversion between precisions
+__host__ __device__ inline float2 to_f(double2 a) { return make_float2((float)a.x, (float)a.y); }
+template <typename R> __host__ __device__ inline cplx_t<R> cconv(double2 a);
+template <> __host__ __device__ inline double2 cconv<double>(double2 a) { return a; }
+template <> __host__ __device__ inline float2 cconv<float>(double2 a) { return to_f(a); }
+
+typedef float ch_vf2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gld2(const float2* p, int64_t i) {
+    const ch_vf2 v = ((const CH_GLOBAL ch_vf2*)p)[i];
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float gld1(const float* p, int64_t i) { return ((const CH_GLOBAL float*)p)[i]; }
+__device__ __forceinline__ void gst2(float2* p, int64_t i, float2 v) {
+    ch_vf2 w;
+    w.x = v.x;
+    w.y = v.y;
+    ((CH_GLOBAL ch_vf2*)p)[i] = w;
+}

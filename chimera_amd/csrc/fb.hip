@@ -43,10 +43,28 @@ const double2* fft_twiddles(int N) {
     return d;
 }
 
-template <int EPT>
-__global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgs a) {
+const float2* fft_twiddles_f(int N) {
+    static std::mutex mu;
+    static std::map<int, const float2*> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(N);
+    if (it != cache.end()) return it->second;
+    std::vector<float2> h(N > 0 ? N : 1);
+    for (int k = 0; k < N; k++) {
+        const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)N;
+        h[k] = make_float2((float)cosl(a), (float)sinl(a));
+    }
+    const float2* d = static_cast<const float2*>(device_table(h.data(), h.size() * sizeof(float2)));
+    cache[N] = d;
+    return d;
+}
+
+template <int EPT, typename R>
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
+    typedef cplx_t<R> C;
     constexpr int TOT = EPT * FFT_THREADS;
-    extern __shared__ double2 sbuf[];
+    extern __shared__ __align__(16) unsigned char fft_smem[];
+    C* const sbuf = reinterpret_cast<C*>(fft_smem);
     const int N = a.N;
     const int lpb = TOT / N;
     const int64_t line0 = (int64_t)blockIdx.x * lpb;
@@ -54,7 +72,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgs a) {
     for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
         const int64_t line = line0 + i / N;
         const int x = i % N;
-        double2 v = cmk(0.0, 0.0);
+        C v = cmake<R>(0, 0);
         if (line < a.nlines) {
             const int64_t r = line % a.n_inner, q = line / a.n_inner;
             v = a.in[r * a.in_r + q * a.in_q + x];
@@ -63,7 +81,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgs a) {
         sbuf[i] = v;
     }
     __syncthreads();
-    fft_lds<EPT>(sbuf, N, a.log2N, a.sign, a.tw);
+    fft_lds<EPT, R>(sbuf, N, a.log2N, a.sign, a.tw);
     if (a.filtr) {
         // fb_io.f90:263: multiply the first nxfilt points, transform back
         for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
@@ -71,7 +89,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgs a) {
             if (x < a.nxfilt) sbuf[i] = crs(a.filtr[x], sbuf[i]);
         }
         __syncthreads();
-        fft_lds<EPT>(sbuf, N, a.log2N, -a.sign, a.tw);
+        fft_lds<EPT, R>(sbuf, N, a.log2N, -a.sign, a.tw);
     }
     // ---- store (with post-multiplier, real scale table, mode normalisation)
     for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
@@ -79,24 +97,26 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgs a) {
         if (line >= a.nlines) continue;
         const int x = i % N;
         const int64_t r = line % a.n_inner, q = line / a.n_inner;
-        double2 v = sbuf[i];
+        C v = sbuf[i];
         if (a.post) v = cmul(v, a.post[x]);
         if (a.scale) v = crs(a.scale[x + (int64_t)N * r], v);
         int m = 0;
         if (a.ghost) {
             m = (int)(q % a.nmodes);
-            if (a.ghost == 2) v = crs(m == 0 ? 0.5 / M_PI : 1.0 / M_PI, v);
+            if (a.ghost == 2) v = crs((R)(m == 0 ? 0.5 / M_PI : 1.0 / M_PI), v);
         }
-        double2* o = a.out + r * a.out_r + q * a.out_q + x;
+        C* o = a.out + r * a.out_r + q * a.out_q + x;
         *o = v;
         if (a.ghost && r == 0) {
             // ghost radial row below row 1: zero (fb_io.f90:199) or +-row 1 (grid_deps.f90:248-265)
-            o[-a.out_r] = a.ghost == 1 ? cmk(0.0, 0.0) : (m == 0 ? v : cneg(v));
+            o[-a.out_r] = a.ghost == 1 ? cmake<R>(0, 0) : (m == 0 ? v : cneg(v));
         }
     }
 }
 
-int32_t launch_fft(FftArgs a) {
+template <typename R>
+int32_t launch_fft(FftArgsT<R> a) {
+    typedef cplx_t<R> C;
     if (a.nlines <= 0) return ST_OK;
     const int N = a.N;
     if (N < 1 || (N & (N - 1)) != 0 || N > 8192)
@@ -104,31 +124,35 @@ int32_t launch_fft(FftArgs a) {
     int l2 = 0;
     while ((1 << l2) < N) l2++;
     a.log2N = l2;
-    a.tw = fft_twiddles(N);
+    if constexpr (sizeof(R) == 8) a.tw = fft_twiddles(N);
+    else a.tw = fft_twiddles_f(N);
     if (!a.tw) return ST_ENOMEM;
     int ept = N <= 2048 ? 8 : N / FFT_THREADS;
     const int tot = ept * FFT_THREADS;
     const int lpb = tot / N;
     const unsigned grid = (unsigned)((a.nlines + lpb - 1) / lpb);
-    const size_t shmem = (size_t)tot * sizeof(double2);
+    const size_t shmem = (size_t)tot * sizeof(C);
     static bool attr_done = false;
     if (!attr_done) {
-        // lines of 4096 and 8192 points use 64 and 128 KiB of LDS per workgroup
-        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 32768));
-        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<16>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
-        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+        // complex128 lines of 4096 and 8192 points use 64 and 128 KiB of LDS per workgroup
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<8, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(8 * FFT_THREADS * sizeof(C))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<16, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(16 * FFT_THREADS * sizeof(C))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<32, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(32 * FFT_THREADS * sizeof(C))));
         attr_done = true;
     }
     KTimer kt(KF_FFT);
     switch (ept) {
         case 8:
-            hipLaunchKernelGGL(k_fft_lines<8>, grid, FFT_THREADS, shmem, stream(), a);
+            hipLaunchKernelGGL((k_fft_lines<8, R>), grid, FFT_THREADS, shmem, stream(), a);
             break;
         case 16:
-            hipLaunchKernelGGL(k_fft_lines<16>, grid, FFT_THREADS, shmem, stream(), a);
+            hipLaunchKernelGGL((k_fft_lines<16, R>), grid, FFT_THREADS, shmem, stream(), a);
             break;
         case 32:
-            hipLaunchKernelGGL(k_fft_lines<32>, grid, FFT_THREADS, shmem, stream(), a);
+            hipLaunchKernelGGL((k_fft_lines<32, R>), grid, FFT_THREADS, shmem, stream(), a);
             break;
         default:
             return fail(ST_EUNSUP, "FFT length %d", N);
@@ -136,32 +160,39 @@ int32_t launch_fft(FftArgs a) {
     CH_LAUNCH_CHECK("k_fft_lines");
     return ST_OK;
 }
+template int32_t launch_fft<double>(FftArgsT<double>);
+template int32_t launch_fft<float>(FftArgsT<float>);
 
 // Per-x phase exp(s*i*kx*leftX) (fb_io.f90:40, :120, :202), on the device.
-__global__ void k_phase(double2* __restrict__ ph, const double* __restrict__ kx, double leftX, double s, int n) {
+template <typename R>
+__global__ void k_phase(cplx_t<R>* __restrict__ ph, const double* __restrict__ kx, double leftX, double s, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double sn, cs;
     sincos(leftX * kx[i], &sn, &cs);
-    ph[i] = cmk(cs, s * sn);
+    ph[i] = cmake<R>((R)cs, (R)(s * sn));
 }
 
 // 1/(N*exp(i*kx*leftX)) as the reference forms it (fb_io.f90:247).
-__global__ void k_phase_inv(double2* __restrict__ ph, const double* __restrict__ kx, double leftX, int n) {
+template <typename R>
+__global__ void k_phase_inv(cplx_t<R>* __restrict__ ph, const double* __restrict__ kx, double leftX, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double sn, cs;
     sincos(leftX * kx[i], &sn, &cs);
     const double a = (double)n * cs, b = (double)n * sn;
     const double d = a * a + b * b;
-    ph[i] = cmk(a / d, -b / d);
+    ph[i] = cmake<R>((R)(a / d), (R)(-b / d));
 }
 
-int32_t make_phase(double2* ph, const double* kx, double leftX, double s, int n) {
-    hipLaunchKernelGGL(k_phase, blocks_for(n, 256), 256, 0, stream(), ph, kx, leftX, s, n);
+template <typename R>
+int32_t make_phase(cplx_t<R>* ph, const double* kx, double leftX, double s, int n) {
+    hipLaunchKernelGGL((k_phase<R>), blocks_for(n, 256), 256, 0, stream(), ph, kx, leftX, s, n);
     CH_LAUNCH_CHECK("k_phase");
     return ST_OK;
 }
+template int32_t make_phase<double>(double2*, const double*, double, double, int);
+template int32_t make_phase<float>(float2*, const double*, double, double, int);
 
 // ================================================================== FB GEMM
 
@@ -183,7 +214,8 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // Element (x, k') of segment s reads a0/a1 at seg_off; seg_finish turns the raw
 // loads into c0*T(a0) + c1*T(a1).  Loads and arithmetic are split so that the
 // loads of the next K-stage stay in flight while the MFMAs run.
-__device__ __forceinline__ int64_t seg_off(const FbSeg& s, int64_t x, int64_t kk, int nx, int64_t lda) {
+template <typename R>
+__device__ __forceinline__ int64_t seg_off(const FbSegT<R>& s, int64_t x, int64_t kk, int nx, int64_t lda) {
     const int64_t xr = s.refl ? (x == 0 ? 0 : nx - x) : x;
     return xr + lda * kk;
 }
@@ -195,6 +227,16 @@ __device__ __forceinline__ double2 seg_finish(const FbSeg& s, double2 v0, double
     }
     double2 v = cmul(s.c0, v0);
     if (s.a1) v = cadd(v, cmul(s.c1, v1));
+    return v;
+}
+
+__device__ __forceinline__ float2 seg_finish(const FbSegT<float>& s, float2 v0, float2 v1) {
+    if (s.refl) {
+        v0 = cneg(cconj(v0));
+        v1 = cneg(cconj(v1));
+    }
+    float2 v = cmul(to_f(s.c0), v0);
+    if (s.a1) v = cadd(v, cmul(to_f(s.c1), v1));
     return v;
 }
 
@@ -379,14 +421,13 @@ static int32_t launch_gemm_variant(const FbJob* dj, size_t njobs, int maxN, cons
     return ST_OK;
 }
 
-int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family) {
-    if (jobs.empty()) return ST_OK;
-    KTimer kt(family);
-    // Jobs whose segments all feed one accumulator run in the one-accumulator
-    // kernel (with the output coefficients of accumulator 1 moved to 0 where
-    // needed); the others are ordered so that segment s feeds accumulator s.
-    std::vector<FbJob> one, two;
-    for (FbJob j : jobs) {
+// Jobs whose segments all feed one accumulator go to `one` (with the output
+// coefficients of accumulator 1 moved to 0 where needed); the others to `two`,
+// ordered so that segment s feeds accumulator s.
+template <typename R>
+static int32_t split_jobs(const std::vector<FbJobT<R>>& jobs, std::vector<FbJobT<R>>& one,
+                          std::vector<FbJobT<R>>& two) {
+    for (FbJobT<R> j : jobs) {
         bool any0 = false, any1 = false;
         for (int s = 0; s < j.nseg; s++) (j.seg[s].acc ? any1 : any0) = true;
         if (any1 && !any0) {
@@ -402,6 +443,15 @@ int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int 
         if (j.seg[0].acc == 1) std::swap(j.seg[0], j.seg[1]);
         two.push_back(j);
     }
+    return ST_OK;
+}
+
+template <>
+int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family) {
+    if (jobs.empty()) return ST_OK;
+    KTimer kt(family);
+    std::vector<FbJob> one, two;
+    CH_TRY(split_jobs<double>(jobs, one, two));
     for (int pass = 0; pass < 2; pass++) {
         const std::vector<FbJob>& js = pass == 0 ? one : two;
         if (js.empty()) continue;
@@ -421,11 +471,220 @@ int32_t launch_fb_gemm(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int 
     return ST_OK;
 }
 
+// ---------------------------------------------------------------- FP32 (mixed precision)
+//
+// Same job structure on complex64 data with v_mfma_f32_16x16x4f32.  Lane
+// layout on gfx950 (checked on the device): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and holds D[4*(l/16) + i][l%16], i = 0..3.  A rows stay in
+// memory order (re, im interleaved), so a lane's items (0,1) and (2,3) are the
+// (re, im) of x = 2*(l/16) and 2*(l/16) + 1 within the 8-complex MFMA tile.
+// Accumulators take half the registers of FP64, so both accumulator sets fit
+// with 64x64 wave tiles.
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <int WN>
+__device__ __forceinline__ void mfma_stage_f32(flt4 (&acc)[4][WN], const float* __restrict__ As,
+                                               const float* __restrict__ Ds, int lane, int wr, int wc) {
+    constexpr int LDA = GB_ROWS + GB_PAD, LDD = 32 * WN + GB_PAD;
+#pragma unroll
+    for (int kq = 0; kq < GB_K / 4; kq++) {
+        const int kk = kq * 4 + (lane >> 4);
+        float a[4], b[WN];
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = As[kk * LDA + wr * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < WN; j++) b[j] = Ds[kk * LDD + wc * 16 * WN + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < WN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+constexpr size_t gb_lds_bytes_f32(int WN) {
+    return 2 * (size_t)GB_K * ((GB_ROWS + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(float);
+}
+
+template <int NACC, int WN>
+__global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<float>* __restrict__ jobs, FbGemmArgs g) {
+    typedef FbSegT<float> Seg;
+    constexpr int BN = 32 * WN, LDA = GB_ROWS + GB_PAD, LDD = BN + GB_PAD;
+    constexpr int A_STAGE = GB_K * LDA, D_STAGE = GB_K * LDD;
+    const FbJobT<float>& J = jobs[blockIdx.z];
+    const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
+    const int n0 = blockIdx.y * BN;
+    if (n0 >= Ncol) return;
+    const int x0 = blockIdx.x * GB_X;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
+
+    extern __shared__ __align__(16) unsigned char gbf_smem[];
+    float* const As = reinterpret_cast<float*>(gbf_smem);  // [2][GB_K][LDA]
+    float* const Ds = As + 2 * A_STAGE;                       // [2][GB_K][LDD]
+
+    flt4 acc0[4][WN], acc1[4][NACC == 2 ? WN : 1];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < WN; j++) acc0[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+
+    constexpr int NA = (GB_K * GB_X) / GB_THREADS, ND = (GB_K * BN) / GB_THREADS;
+    const int stages_per_seg = (K + GB_K - 1) / GB_K;
+    const int nstages = stages_per_seg * nseg;
+    const Seg S0 = J.seg[0];
+    const Seg S1 = J.seg[nseg > 1 ? 1 : 0];
+    float2 ra[NA], rb[NA];
+    float rd[ND];
+    auto fetch_seg = [&](const Seg& S, int k0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + GB_THREADS * i;
+            const int xl = e % GB_X, kk = e / GB_X;
+            const int64_t xg = x0 + xl;
+            const bool ok = xg < g.nx && k0 + kk < K;
+            ra[i] = ok ? gld2(S.a0, seg_off(S, xg, k0 + kk, g.nx, g.lda)) : cmkf(0.f, 0.f);
+        }
+        if (S.a1) {
+#pragma unroll
+            for (int i = 0; i < NA; i++) {
+                const int e = tid + GB_THREADS * i;
+                const int xl = e % GB_X, kk = e / GB_X;
+                const int64_t xg = x0 + xl;
+                const bool ok = xg < g.nx && k0 + kk < K;
+                rb[i] = ok ? gld2(S.a1, seg_off(S, xg, k0 + kk, g.nx, g.lda)) : cmkf(0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            const int e = tid + GB_THREADS * i;
+            const int kk = e % GB_K, nl = e / GB_K;
+            rd[i] = (k0 + kk < K && n0 + nl < Ncol) ? gld1(S.D, (int64_t)(k0 + kk) + S.ldd * (int64_t)(n0 + nl)) : 0.f;
+        }
+    };
+    auto fetch = [&](int t) __attribute__((always_inline)) {
+        const int k0 = (t % stages_per_seg) * GB_K;
+        if (t < stages_per_seg) fetch_seg(S0, k0);
+        else fetch_seg(S1, k0);
+    };
+    auto stash_seg = [&](const Seg& S, int buf) __attribute__((always_inline)) {
+        float* A = As + buf * A_STAGE;
+        float* D = Ds + buf * D_STAGE;
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + GB_THREADS * i;
+            const int xl = e % GB_X, kk = e / GB_X;
+            const float2 v = seg_finish(S, ra[i], rb[i]);
+            *reinterpret_cast<float2*>(A + kk * LDA + 2 * xl) = v;
+        }
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            const int e = tid + GB_THREADS * i;
+            D[(e % GB_K) * LDD + e / GB_K] = rd[i];
+        }
+    };
+    auto stash = [&](int buf, int t) __attribute__((always_inline)) {
+        if (t < stages_per_seg) stash_seg(S0, buf);
+        else stash_seg(S1, buf);
+    };
+    auto pipe = [&](flt4 (&acc)[4][WN], int tb, int te) __attribute__((always_inline)) {
+        for (int t = tb; t < te; t++) {
+            if (t + 1 < nstages) fetch(t + 1);
+            mfma_stage_f32<WN>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
+            if (t + 1 < nstages) stash((t + 1) & 1, t + 1);
+            __syncthreads();
+        }
+    };
+    if (nstages > 0) {
+        fetch(0);
+        stash(0, 0);
+    }
+    __syncthreads();
+    if constexpr (NACC == 2) {
+        pipe(acc0, 0, stages_per_seg);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < WN; j++) acc1[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+        pipe(acc1, stages_per_seg, nstages);
+    } else {
+        pipe(acc0, 0, nstages);
+    }
+    // ---- epilogue: lane holds complex outputs at x = x0 + wr*32 + i*8 + 2*(lane/16) + h, h = 0,1
+    for (int o = 0; o < J.nout; o++) {
+        const FbOutT<float>& O = J.out[o];
+        const float2 al0 = to_f(O.alpha0), al1 = to_f(O.alpha1), gam = to_f(O.gamma);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < WN; j++) {
+                const int64_t ng = n0 + wc * 16 * WN + j * 16 + (lane & 15);
+                if (ng >= Ncol) continue;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int64_t xg = x0 + wr * 32 + i * 8 + 2 * (lane >> 4) + h;
+                    if (xg >= g.nx) continue;
+                    float2 v = cmul(al0, cmkf(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]));
+                    if constexpr (NACC == 2) v = cadd(v, cmul(al1, cmkf(acc1[i][j][2 * h], acc1[i][j][2 * h + 1])));
+                    const int64_t off = xg + g.ldo * ng;
+                    if (O.diag) v = cadd(v, crs((float)g.kx[xg], cmul(gam, gld2(O.diag, off))));
+                    if (O.scale) v = crs(gld1(O.scale, off), v);
+                    if (O.poiss) {
+                        const float2 d = crs((float)g.dt_inv, csub(gld2(O.poiss_np1, off), gld2(O.poiss_n, off)));
+                        gst2(O.p, off, cadd(gld2(O.p, off), crs(gld1(O.poiss_w2, off), cadd(v, d))));
+                    } else {
+                        gst2(O.p, off, J.beta ? cadd(gld2(O.p, off), v) : v);
+                    }
+                }
+            }
+    }
+}
+
+template <int NACC, int WN>
+static int32_t launch_gemm_variant_f32(const FbJobT<float>* dj, size_t njobs, int maxN, const FbGemmArgs& g) {
+    constexpr int BN = 32 * WN;
+    static bool attr = false;
+    if (!attr) {
+        CH_HIP(hipFuncSetAttribute((const void*)k_fb_gemm_f32<NACC, WN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)gb_lds_bytes_f32(WN)));
+        attr = true;
+    }
+    dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
+    hipLaunchKernelGGL((k_fb_gemm_f32<NACC, WN>), grid, GB_THREADS, gb_lds_bytes_f32(WN), stream(), dj, g);
+    CH_LAUNCH_CHECK("k_fb_gemm_f32");
+    return ST_OK;
+}
+
+template <>
+int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGemmArgs& g, int family) {
+    if (jobs.empty()) return ST_OK;
+    KTimer kt(family);
+    std::vector<FbJobT<float>> one, two;
+    CH_TRY(split_jobs<float>(jobs, one, two));
+    for (int pass = 0; pass < 2; pass++) {
+        const std::vector<FbJobT<float>>& js = pass == 0 ? one : two;
+        if (js.empty()) continue;
+        int maxN = 0;
+        for (const auto& j : js) maxN = j.Ncol > maxN ? j.Ncol : maxN;
+        const FbJobT<float>* dj =
+            static_cast<const FbJobT<float>*>(device_table(js.data(), js.size() * sizeof(FbJobT<float>)));
+        if (!dj) return ST_ENOMEM;
+        const int64_t xb = (g.nx + GB_X - 1) / GB_X;
+        const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
+        int32_t st;
+        if (pass == 0) st = wide ? launch_gemm_variant_f32<1, 4>(dj, js.size(), maxN, g)
+                                 : launch_gemm_variant_f32<1, 2>(dj, js.size(), maxN, g);
+        else st = wide ? launch_gemm_variant_f32<2, 4>(dj, js.size(), maxN, g)
+                       : launch_gemm_variant_f32<2, 2>(dj, js.size(), maxN, g);
+        CH_TRY(st);
+    }
+    return ST_OK;
+}
+
 // ================================================================== job builders
 
-static FbSeg seg(const double2* a0, double2 c0, const double2* a1, double2 c1, const double* D, int64_t ldd, int refl,
-                 int acc) {
-    FbSeg s;
+template <typename R>
+static FbSegT<R> seg(const cplx_t<R>* a0, double2 c0, const cplx_t<R>* a1, double2 c1, const R* D, int64_t ldd,
+                     int refl, int acc) {
+    FbSegT<R> s;
     s.a0 = a0;
     s.a1 = a1;
     s.c0 = c0;
@@ -437,8 +696,10 @@ static FbSeg seg(const double2* a0, double2 c0, const double2* a1, double2 c1, c
     return s;
 }
 
-static FbOut outp(double2* p, double2 al0, double2 al1, const double2* diag = nullptr, double2 gamma = {0, 0}) {
-    FbOut o;
+template <typename R>
+static FbOutT<R> outp(cplx_t<R>* p, double2 al0, double2 al1, const cplx_t<R>* diag = nullptr,
+                      double2 gamma = {0, 0}) {
+    FbOutT<R> o;
     memset(&o, 0, sizeof o);
     o.p = p;
     o.alpha0 = al0;
@@ -448,23 +709,25 @@ static FbOut outp(double2* p, double2 al0, double2 al1, const double2* diag = nu
     return o;
 }
 
-static FbJob job0() {
-    FbJob j;
+template <typename R>
+static FbJobT<R> job0() {
+    FbJobT<R> j;
     memset(&j, 0, sizeof j);
     return j;
 }
 
 static const double2 ONE = {1, 0}, MONE = {-1, 0}, I1 = {0, 1}, MI = {0, -1}, ZERO = {0, 0};
 
-void build_dht_jobs(std::vector<FbJob>& jobs, const double2* in, int64_t in_m, int64_t in_l, const double* D,
-                    int64_t ldd, int64_t d_m, int K, double2* out, int64_t out_m, int64_t out_l, int Ncol,
+template <typename R>
+void build_dht_jobs(std::vector<FbJobT<R>>& jobs, const cplx_t<R>* in, int64_t in_m, int64_t in_l, const R* D,
+                    int64_t ldd, int64_t d_m, int K, cplx_t<R>* out, int64_t out_m, int64_t out_l, int Ncol,
                     int nmodes, int ncomp) {
     for (int l = 0; l < ncomp; l++)
         for (int m = 0; m < nmodes; m++) {
-            FbJob j = job0();
-            j.seg[0] = seg(in + m * in_m + l * in_l, ONE, nullptr, ZERO, D + m * d_m, ldd, 0, 0);
+            FbJobT<R> j = job0<R>();
+            j.seg[0] = seg<R>(in + m * in_m + l * in_l, ONE, nullptr, ZERO, D + m * d_m, ldd, 0, 0);
             j.nseg = 1;
-            j.out[0] = outp(out + m * out_m + l * out_l, ONE, ZERO);
+            j.out[0] = outp<R>(out + m * out_m + l * out_l, ONE, ZERO);
             j.nout = 1;
             j.K = K;
             j.Ncol = Ncol;
@@ -473,42 +736,45 @@ void build_dht_jobs(std::vector<FbJob>& jobs, const double2* in, int64_t in_m, i
 }
 
 // Views of an FB-space array a(nkx, ncol, 0:nmod-1, ncomp): column base of (mode, comp).
+template <typename R>
 struct FbView {
-    const double2* p;
+    const cplx_t<R>* p;
     int64_t nkx, ncol, nmod;
-    const double2* at(int m, int c) const { return p + nkx * ncol * (m + nmod * c); }
+    const cplx_t<R>* at(int m, int c) const { return p + nkx * ncol * (m + nmod * c); }
 };
+template <typename R>
 struct FbViewW {
-    double2* p;
+    cplx_t<R>* p;
     int64_t nkx, ncol, nmod;
-    double2* at(int m, int c) const { return p + nkx * ncol * (m + nmod * c); }
+    cplx_t<R>* at(int m, int c) const { return p + nkx * ncol * (m + nmod * c); }
 };
 
-void build_rot_jobs(std::vector<FbJob>& jobs, FbViewW loc, FbView v, const double* Dp, const double* Dm, int nkr,
-                    int nkr_loc, int nkO, const double* scale_tab) {
-    auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
+template <typename R>
+void build_rot_jobs(std::vector<FbJobT<R>>& jobs, FbViewW<R> loc, FbView<R> v, const R* Dp, const R* Dm, int nkr,
+                    int nkr_loc, int nkO, const R* scale_tab) {
+    auto D = [&](const R* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
     for (int m = 0; m <= nkO; m++) {
         // loc_x = -D+ (i v_y - v_z)[m+1] - D- (i v_y + v_z)[m-1]      fb_math.f90:47-53, :63-69, :76-82
-        FbJob a = job0();
-        if (m < nkO) a.seg[a.nseg++] = seg(v.at(m + 1, 1), I1, v.at(m + 1, 2), MONE, D(Dp, m), nkr, 0, 0);
-        if (m > 0) a.seg[a.nseg++] = seg(v.at(m - 1, 1), I1, v.at(m - 1, 2), ONE, D(Dm, m), nkr, 0, 0);
-        else if (nkO >= 1) a.seg[a.nseg++] = seg(v.at(1, 1), I1, v.at(1, 2), ONE, D(Dm, 0), nkr, 1, 0);
-        a.out[0] = outp(loc.at(m, 0), MONE, ZERO);
+        FbJobT<R> a = job0<R>();
+        if (m < nkO) a.seg[a.nseg++] = seg<R>(v.at(m + 1, 1), I1, v.at(m + 1, 2), MONE, D(Dp, m), nkr, 0, 0);
+        if (m > 0) a.seg[a.nseg++] = seg<R>(v.at(m - 1, 1), I1, v.at(m - 1, 2), ONE, D(Dm, m), nkr, 0, 0);
+        else if (nkO >= 1) a.seg[a.nseg++] = seg<R>(v.at(1, 1), I1, v.at(1, 2), ONE, D(Dm, 0), nkr, 1, 0);
+        a.out[0] = outp<R>(loc.at(m, 0), MONE, ZERO);
         a.nout = 1;
         a.K = nkr;
         a.Ncol = nkr_loc;
         // loc_y = -i kx v_z + i W + i V ; loc_z = i kx v_y - W + V     fb_math.f90:42-45, :55-60, :70-75
-        FbJob b = job0();
-        if (m < nkO) b.seg[b.nseg++] = seg(v.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 0);
-        if (m > 0) b.seg[b.nseg++] = seg(v.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 1);
-        else if (nkO >= 1) b.seg[b.nseg++] = seg(v.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 1);
-        b.out[0] = outp(loc.at(m, 1), I1, I1, v.at(m, 2), MI);
-        b.out[1] = outp(loc.at(m, 2), MONE, ONE, v.at(m, 1), I1);
+        FbJobT<R> b = job0<R>();
+        if (m < nkO) b.seg[b.nseg++] = seg<R>(v.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 0);
+        if (m > 0) b.seg[b.nseg++] = seg<R>(v.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 1);
+        else if (nkO >= 1) b.seg[b.nseg++] = seg<R>(v.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 1);
+        b.out[0] = outp<R>(loc.at(m, 1), I1, I1, v.at(m, 2), MI);
+        b.out[1] = outp<R>(loc.at(m, 2), MONE, ONE, v.at(m, 1), I1);
         b.nout = 2;
         b.K = nkr;
         b.Ncol = nkr_loc;
         if (scale_tab) {
-            const double* sc = scale_tab + (int64_t)v.nkx * nkr_loc * m;
+            const R* sc = scale_tab + (int64_t)v.nkx * nkr_loc * m;
             a.out[0].scale = sc;
             b.out[0].scale = sc;
             b.out[1].scale = sc;
@@ -518,19 +784,20 @@ void build_rot_jobs(std::vector<FbJob>& jobs, FbViewW loc, FbView v, const doubl
     }
 }
 
-void build_grad_jobs(std::vector<FbJob>& jobs, FbViewW loc, FbView s, const double* Dp, const double* Dm, int nkr,
+template <typename R>
+void build_grad_jobs(std::vector<FbJobT<R>>& jobs, FbViewW<R> loc, FbView<R> s, const R* Dp, const R* Dm, int nkr,
                      int nkr_loc, int nkO, int nmod_in) {
     (void)nmod_in;
-    auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
+    auto D = [&](const R* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
     for (int m = 0; m <= nkO; m++) {
         // fb_math.f90:119-144: loc_x = i kx s[m]; loc_y = -V + W; loc_z = i V + i W
-        FbJob j = job0();
-        if (m > 0) j.seg[j.nseg++] = seg(s.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 0);
-        else if (nkO >= 1) j.seg[j.nseg++] = seg(s.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 0);
-        if (m < nkO) j.seg[j.nseg++] = seg(s.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 1);
-        j.out[0] = outp(loc.at(m, 0), ZERO, ZERO, s.at(m, 0), I1);
-        j.out[1] = outp(loc.at(m, 1), MONE, ONE);
-        j.out[2] = outp(loc.at(m, 2), I1, I1);
+        FbJobT<R> j = job0<R>();
+        if (m > 0) j.seg[j.nseg++] = seg<R>(s.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 0);
+        else if (nkO >= 1) j.seg[j.nseg++] = seg<R>(s.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 0);
+        if (m < nkO) j.seg[j.nseg++] = seg<R>(s.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 1);
+        j.out[0] = outp<R>(loc.at(m, 0), ZERO, ZERO, s.at(m, 0), I1);
+        j.out[1] = outp<R>(loc.at(m, 1), MONE, ONE);
+        j.out[2] = outp<R>(loc.at(m, 2), I1, I1);
         j.nout = 3;
         j.K = nkr;
         j.Ncol = nkr_loc;
@@ -539,17 +806,18 @@ void build_grad_jobs(std::vector<FbJob>& jobs, FbViewW loc, FbView s, const doub
 }
 
 // div of v (modes 0..nkO) into out modes 0..nmod_out-1; D+ exists for m < nkO.
-void build_div_jobs(std::vector<FbJob>& jobs, FbViewW out, FbView v, const double* Dp, const double* Dm, int nkr,
+template <typename R>
+void build_div_jobs(std::vector<FbJobT<R>>& jobs, FbViewW<R> out, FbView<R> v, const R* Dp, const R* Dm, int nkr,
                     int nkr_loc, int nkO, int nmod_out) {
-    auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
+    auto D = [&](const R* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
     for (int m = 0; m < nmod_out; m++) {
         // fb_math.f90:173-194
-        FbJob j = job0();
-        if (m > 0) j.seg[j.nseg++] = seg(v.at(m - 1, 2), I1, v.at(m - 1, 1), MONE, D(Dm, m), nkr, 0, 0);
-        else if (nkO >= 1) j.seg[j.nseg++] = seg(v.at(1, 2), I1, v.at(1, 1), MONE, D(Dm, 0), nkr, 1, 0);
-        if (m < nkO) j.seg[j.nseg++] = seg(v.at(m + 1, 2), I1, v.at(m + 1, 1), ONE, D(Dp, m), nkr, 0, 0);
-        if (m <= nkO) j.out[0] = outp(out.at(m, 0), ONE, ZERO, v.at(m, 0), I1);
-        else j.out[0] = outp(out.at(m, 0), ONE, ZERO);
+        FbJobT<R> j = job0<R>();
+        if (m > 0) j.seg[j.nseg++] = seg<R>(v.at(m - 1, 2), I1, v.at(m - 1, 1), MONE, D(Dm, m), nkr, 0, 0);
+        else if (nkO >= 1) j.seg[j.nseg++] = seg<R>(v.at(1, 2), I1, v.at(1, 1), MONE, D(Dm, 0), nkr, 1, 0);
+        if (m < nkO) j.seg[j.nseg++] = seg<R>(v.at(m + 1, 2), I1, v.at(m + 1, 1), ONE, D(Dp, m), nkr, 0, 0);
+        if (m <= nkO) j.out[0] = outp<R>(out.at(m, 0), ONE, ZERO, v.at(m, 0), I1);
+        else j.out[0] = outp<R>(out.at(m, 0), ONE, ZERO);
         j.nout = 1;
         j.K = nkr;
         j.Ncol = nkr_loc;
@@ -560,20 +828,22 @@ void build_div_jobs(std::vector<FbJob>& jobs, FbViewW out, FbView v, const doubl
 // Second half of grad-div: gradient of s (modes 0..nkO+1) into v (modes 0..nkO).
 // With poiss != null the outputs take the Poisson-correction epilogue
 // v += (graddiv + (gp - gn)*dt_inv)*w2_inv (maxwell_solvers.f90:153-158).
-struct PoissArgs { const double2* gn; const double2* gp; const double* w2; };
+template <typename R>
+struct PoissArgs { const cplx_t<R>* gn; const cplx_t<R>* gp; const R* w2; };
 
-void build_graddiv2_jobs(std::vector<FbJob>& jobs, FbViewW v, FbView s, const double* Dp, const double* Dm, int nkr,
-                         int nkr_loc, int nkO, const PoissArgs* poiss = nullptr) {
-    auto D = [&](const double* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
+template <typename R>
+void build_graddiv2_jobs(std::vector<FbJobT<R>>& jobs, FbViewW<R> v, FbView<R> s, const R* Dp, const R* Dm, int nkr,
+                         int nkr_loc, int nkO, const PoissArgs<R>* poiss = nullptr) {
+    auto D = [&](const R* B, int m) { return B + (int64_t)nkr * nkr_loc * m; };
     for (int m = 0; m <= nkO; m++) {
         // fb_math.f90:266-288
-        FbJob j = job0();
-        if (m > 0) j.seg[j.nseg++] = seg(s.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 0);
-        else if (nkO >= 1) j.seg[j.nseg++] = seg(s.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 0);
-        j.seg[j.nseg++] = seg(s.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 1);
-        j.out[0] = outp(v.at(m, 0), ZERO, ZERO, s.at(m, 0), I1);
-        j.out[1] = outp(v.at(m, 1), MONE, ONE);
-        j.out[2] = outp(v.at(m, 2), I1, I1);
+        FbJobT<R> j = job0<R>();
+        if (m > 0) j.seg[j.nseg++] = seg<R>(s.at(m - 1, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 0);
+        else if (nkO >= 1) j.seg[j.nseg++] = seg<R>(s.at(1, 0), ONE, nullptr, ZERO, D(Dm, 0), nkr, 1, 0);
+        j.seg[j.nseg++] = seg<R>(s.at(m + 1, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 1);
+        j.out[0] = outp<R>(v.at(m, 0), ZERO, ZERO, s.at(m, 0), I1);
+        j.out[1] = outp<R>(v.at(m, 1), MONE, ONE);
+        j.out[2] = outp<R>(v.at(m, 2), I1, I1);
         j.nout = 3;
         j.K = nkr_loc;
         j.Ncol = nkr_loc;
@@ -593,14 +863,16 @@ void build_graddiv2_jobs(std::vector<FbJob>& jobs, FbViewW v, FbView s, const do
 
 // ================================================================== composite device-side operations
 
-int32_t fb_forward(double2* out, double2* in, double leftX, const double* kx_dev, const double* In_dev, int nkx,
-                   int nr, int nmodes, int nkr, int ncomp, const double* scale_tab, int fold) {
+template <typename R>
+int32_t fb_forward_t(cplx_t<R>* out, cplx_t<R>* in, double leftX, const double* kx_dev, const R* In_dev, int nkx,
+                     int nr, int nmodes, int nkr, int ncomp, const R* scale_tab, int fold) {
+    typedef cplx_t<R> C;
     const size_t tmp_elems = (size_t)nkx * nkr * nmodes * ncomp;
-    double2* tmp = static_cast<double2*>(scratch(0, tmp_elems * sizeof(double2)));
-    double2* ph = static_cast<double2*>(scratch(1, (size_t)nkx * sizeof(double2)));
+    C* tmp = static_cast<C*>(scratch(0, tmp_elems * sizeof(C)));
+    C* ph = static_cast<C*>(scratch(1, (size_t)nkx * sizeof(C)));
     if (!tmp || !ph) return ST_ENOMEM;
-    if (fold) CH_TRY(ghost_fold(in, nkx, nr + 1, (int64_t)nmodes * ncomp));
-    std::vector<FbJob> jobs;
+    if (fold) CH_TRY(ghost_fold<R>(in, nkx, nr + 1, (int64_t)nmodes * ncomp));
+    std::vector<FbJobT<R>> jobs;
     // vec(nkx, 0:nr, nmodes, ncomp): row 1 (first real node) is the DHT's r = 1
     build_dht_jobs(jobs, in + nkx, (int64_t)nkx * (nr + 1), (int64_t)nkx * (nr + 1) * nmodes, In_dev, nr,
                    (int64_t)nr * nkr, nr, tmp, (int64_t)nkx * nkr, (int64_t)nkx * nkr * nmodes, nkr, nmodes, ncomp);
@@ -610,9 +882,9 @@ int32_t fb_forward(double2* out, double2* in, double leftX, const double* kx_dev
     g.lda = nkx;
     g.ldo = nkx;
     g.kx = kx_dev;
-    CH_TRY(launch_fb_gemm(jobs, g, KF_DHT));
-    CH_TRY(make_phase(ph, kx_dev, leftX, -1.0, nkx));
-    FftArgs a;
+    CH_TRY(launch_fb_gemm<R>(jobs, g, KF_DHT));
+    CH_TRY(make_phase<R>(ph, kx_dev, leftX, -1.0, nkx));
+    FftArgsT<R> a;
     memset(&a, 0, sizeof a);
     a.in = tmp;
     a.out = out;
@@ -626,16 +898,18 @@ int32_t fb_forward(double2* out, double2* in, double leftX, const double* kx_dev
     a.scale = scale_tab;
     a.sign = -1;
     a.N = nkx;
-    return launch_fft(a);
+    return launch_fft<R>(a);
 }
 
-int32_t fb_inverse(double2* eb, const double2* e_fb, const double2* b_fb, int64_t b_comp_stride, double leftX,
-                   const double* kx_dev, const double* Out_dev, int nkx, int nr, int nmodes, int nkr, int ghost) {
+template <typename R>
+int32_t fb_inverse_t(cplx_t<R>* eb, const cplx_t<R>* e_fb, const cplx_t<R>* b_fb, int64_t b_comp_stride, double leftX,
+                     const double* kx_dev, const R* Out_dev, int nkx, int nr, int nmodes, int nkr, int ghost) {
+    typedef cplx_t<R> C;
     const size_t tmp_elems = (size_t)nkx * nr * nmodes * 6;
-    double2* tmp = static_cast<double2*>(scratch(0, tmp_elems * sizeof(double2)));
-    double2* ph = static_cast<double2*>(scratch(1, (size_t)nkx * sizeof(double2)));
+    C* tmp = static_cast<C*>(scratch(0, tmp_elems * sizeof(C)));
+    C* ph = static_cast<C*>(scratch(1, (size_t)nkx * sizeof(C)));
     if (!tmp || !ph) return ST_ENOMEM;
-    std::vector<FbJob> jobs;
+    std::vector<FbJobT<R>> jobs;
     const int64_t fbc = (int64_t)nkx * nkr * nmodes;  // e_fb component stride
     build_dht_jobs(jobs, e_fb, (int64_t)nkx * nkr, fbc, Out_dev, nkr, (int64_t)nkr * nr, nkr, tmp, (int64_t)nkx * nr,
                    (int64_t)nkx * nr * nmodes, nr, nmodes, 3);
@@ -647,9 +921,9 @@ int32_t fb_inverse(double2* eb, const double2* e_fb, const double2* b_fb, int64_
     g.lda = nkx;
     g.ldo = nkx;
     g.kx = kx_dev;
-    CH_TRY(launch_fb_gemm(jobs, g, KF_DHT));
-    CH_TRY(make_phase(ph, kx_dev, leftX, +1.0, nkx));
-    FftArgs a;
+    CH_TRY(launch_fb_gemm<R>(jobs, g, KF_DHT));
+    CH_TRY(make_phase<R>(ph, kx_dev, leftX, +1.0, nkx));
+    FftArgsT<R> a;
     memset(&a, 0, sizeof a);
     a.in = tmp;
     a.out = eb + nkx;  // row 1
@@ -664,18 +938,20 @@ int32_t fb_inverse(double2* eb, const double2* e_fb, const double2* b_fb, int64_
     a.ghost = ghost;
     a.nmodes = nmodes;
     a.N = nkx;
-    return launch_fft(a);
+    return launch_fft<R>(a);
 }
 
-int32_t fb_filter(double2* vec, double leftX, const double* kx_dev, const double* filtr_dev, int nxfilt, int nkx,
-                  int nkr, int nmodes, int ncomp) {
-    double2* ph = static_cast<double2*>(scratch(1, (size_t)nkx * sizeof(double2)));
-    double2* phi = static_cast<double2*>(scratch(2, (size_t)nkx * sizeof(double2)));
+template <typename R>
+int32_t fb_filter_t(cplx_t<R>* vec, double leftX, const double* kx_dev, const R* filtr_dev, int nxfilt, int nkx,
+                    int nkr, int nmodes, int ncomp) {
+    typedef cplx_t<R> C;
+    C* ph = static_cast<C*>(scratch(1, (size_t)nkx * sizeof(C)));
+    C* phi = static_cast<C*>(scratch(2, (size_t)nkx * sizeof(C)));
     if (!ph || !phi) return ST_ENOMEM;
-    CH_TRY(make_phase(ph, kx_dev, leftX, +1.0, nkx));
-    hipLaunchKernelGGL(k_phase_inv, blocks_for(nkx, 256), 256, 0, stream(), phi, kx_dev, leftX, nkx);
+    CH_TRY(make_phase<R>(ph, kx_dev, leftX, +1.0, nkx));
+    hipLaunchKernelGGL((k_phase_inv<R>), blocks_for(nkx, 256), 256, 0, stream(), phi, kx_dev, leftX, nkx);
     CH_LAUNCH_CHECK("k_phase_inv");
-    FftArgs a;
+    FftArgsT<R> a;
     memset(&a, 0, sizeof a);
     a.in = vec;
     a.out = vec;
@@ -691,12 +967,14 @@ int32_t fb_filter(double2* vec, double leftX, const double* kx_dev, const double
     a.filtr = filtr_dev;
     a.nxfilt = nxfilt < nkx ? nxfilt : nkx;
     a.N = nkx;
-    return launch_fft(a);
+    return launch_fft<R>(a);
 }
 
-int32_t fb_op(int op, double2* out, const double2* in, const double* Dp, const double* Dm, const double* kx_dev,
-              int nkx, int nkr, int nkO, int nkr_loc, const double* scale_tab) {
-    std::vector<FbJob> jobs;
+template <typename R>
+int32_t fb_op_t(int op, cplx_t<R>* out, const cplx_t<R>* in, const R* Dp, const R* Dm, const double* kx_dev, int nkx,
+                int nkr, int nkO, int nkr_loc, const R* scale_tab) {
+    typedef cplx_t<R> C;
+    std::vector<FbJobT<R>> jobs;
     FbGemmArgs g;
     memset(&g, 0, sizeof g);
     g.nx = nkx;
@@ -706,28 +984,28 @@ int32_t fb_op(int op, double2* out, const double2* in, const double* Dp, const d
     const int nmod = nkO + 1;
     switch (op) {
         case FB_ROT:
-            build_rot_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
+            build_rot_jobs(jobs, FbViewW<R>{out, nkx, nkr_loc, nmod}, FbView<R>{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
                            nkO, scale_tab);
-            return launch_fb_gemm(jobs, g, KF_FBOPS);
+            return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
         case FB_GRAD:
-            build_grad_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
+            build_grad_jobs(jobs, FbViewW<R>{out, nkx, nkr_loc, nmod}, FbView<R>{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
                             nkO, nmod);
-            return launch_fb_gemm(jobs, g, KF_FBOPS);
+            return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
         case FB_DIV:
-            build_div_jobs(jobs, FbViewW{out, nkx, nkr_loc, nmod}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
+            build_div_jobs(jobs, FbViewW<R>{out, nkx, nkr_loc, nmod}, FbView<R>{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
                            nkO, nmod);
-            return launch_fb_gemm(jobs, g, KF_FBOPS);
+            return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
         case FB_GRADDIV: {
             if (nkr_loc != nkr) return fail(ST_EUNSUP, "fb_graddiv needs nkr_loc == nkr (got %d, %d)", nkr_loc, nkr);
-            double2* s = static_cast<double2*>(scratch(3, (size_t)nkx * nkr_loc * (nkO + 2) * sizeof(double2)));
+            C* s = static_cast<C*>(scratch(3, (size_t)nkx * nkr_loc * (nkO + 2) * sizeof(C)));
             if (!s) return ST_ENOMEM;
-            build_div_jobs(jobs, FbViewW{s, nkx, nkr_loc, nkO + 2}, FbView{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
+            build_div_jobs(jobs, FbViewW<R>{s, nkx, nkr_loc, nkO + 2}, FbView<R>{in, nkx, nkr, nmod}, Dp, Dm, nkr, nkr_loc,
                            nkO, nkO + 2);
-            CH_TRY(launch_fb_gemm(jobs, g, KF_FBOPS));
+            CH_TRY(launch_fb_gemm<R>(jobs, g, KF_FBOPS));
             jobs.clear();
-            build_graddiv2_jobs(jobs, FbViewW{out, nkx, nkr, nmod}, FbView{s, nkx, nkr_loc, nkO + 2}, Dp, Dm, nkr,
+            build_graddiv2_jobs(jobs, FbViewW<R>{out, nkx, nkr, nmod}, FbView<R>{s, nkx, nkr_loc, nkO + 2}, Dp, Dm, nkr,
                                 nkr_loc, nkO);
-            return launch_fb_gemm(jobs, g, KF_FBOPS);
+            return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
         }
         default:
             return fail(ST_EINVAL, "unknown FB operator %d", op);
@@ -736,9 +1014,11 @@ int32_t fb_op(int op, double2* out, const double2* in, const double* Dp, const d
 
 // j += (graddiv j + (gp - gn)*dt_inv)*w2_inv, one Poisson-correction iteration
 // (solvers.py:317-326 with maxwell_solvers.f90:131-164), graddiv fused.
-int32_t fb_graddiv_poisson(double2* j, const double2* gn, const double2* gp, double dt_inv, const double* w2,
-                           const double* Dp, const double* Dm, const double* kx_dev, int nkx, int nkr, int nkO) {
-    std::vector<FbJob> jobs;
+template <typename R>
+int32_t fb_graddiv_poisson_t(cplx_t<R>* j, const cplx_t<R>* gn, const cplx_t<R>* gp, double dt_inv, const R* w2,
+                             const R* Dp, const R* Dm, const double* kx_dev, int nkx, int nkr, int nkO) {
+    typedef cplx_t<R> C;
+    std::vector<FbJobT<R>> jobs;
     FbGemmArgs g;
     memset(&g, 0, sizeof g);
     g.nx = nkx;
@@ -747,15 +1027,30 @@ int32_t fb_graddiv_poisson(double2* j, const double2* gn, const double2* gp, dou
     g.kx = kx_dev;
     g.dt_inv = dt_inv;
     const int nmod = nkO + 1;
-    double2* s = static_cast<double2*>(scratch(3, (size_t)nkx * nkr * (nkO + 2) * sizeof(double2)));
+    C* s = static_cast<C*>(scratch(3, (size_t)nkx * nkr * (nkO + 2) * sizeof(C)));
     if (!s) return ST_ENOMEM;
-    build_div_jobs(jobs, FbViewW{s, nkx, nkr, nkO + 2}, FbView{j, nkx, nkr, nmod}, Dp, Dm, nkr, nkr, nkO, nkO + 2);
-    CH_TRY(launch_fb_gemm(jobs, g, KF_FBOPS));
+    build_div_jobs(jobs, FbViewW<R>{s, nkx, nkr, nkO + 2}, FbView<R>{j, nkx, nkr, nmod}, Dp, Dm, nkr, nkr, nkO, nkO + 2);
+    CH_TRY(launch_fb_gemm<R>(jobs, g, KF_FBOPS));
     jobs.clear();
-    PoissArgs pa{gn, gp, w2};
-    build_graddiv2_jobs(jobs, FbViewW{j, nkx, nkr, nmod}, FbView{s, nkx, nkr, nkO + 2}, Dp, Dm, nkr, nkr, nkO, &pa);
-    return launch_fb_gemm(jobs, g, KF_FBOPS);
+    PoissArgs<R> pa{gn, gp, w2};
+    build_graddiv2_jobs(jobs, FbViewW<R>{j, nkx, nkr, nmod}, FbView<R>{s, nkx, nkr, nkO + 2}, Dp, Dm, nkr, nkr, nkO, &pa);
+    return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
 }
+
+
+#define CH_FB_INSTANTIATE(R)                                                                                          \
+    template int32_t fb_forward_t<R>(cplx_t<R>*, cplx_t<R>*, double, const double*, const R*, int, int, int, int, int, \
+                                     const R*, int);                                                                  \
+    template int32_t fb_inverse_t<R>(cplx_t<R>*, const cplx_t<R>*, const cplx_t<R>*, int64_t, double, const double*,  \
+                                     const R*, int, int, int, int, int);                                              \
+    template int32_t fb_filter_t<R>(cplx_t<R>*, double, const double*, const R*, int, int, int, int, int);            \
+    template int32_t fb_op_t<R>(int, cplx_t<R>*, const cplx_t<R>*, const R*, const R*, const double*, int, int, int,  \
+                                int, const R*);                                                                       \
+    template int32_t fb_graddiv_poisson_t<R>(cplx_t<R>*, const cplx_t<R>*, const cplx_t<R>*, double, const R*,         \
+                                             const R*, const R*, const double*, int, int, int);
+CH_FB_INSTANTIATE(double)
+CH_FB_INSTANTIATE(float)
+#undef CH_FB_INSTANTIATE
 
 }  // namespace chimera
 
@@ -918,7 +1213,7 @@ int32_t chimera_fb_graddiv_poisson(double* j_fb, const double* grad_rho_n_fb, co
     if (!is_device_ptr(j_fb) || !is_device_ptr(grad_rho_n_fb) || !is_device_ptr(grad_rho_np1_fb) ||
         !is_device_ptr(w2_inv))
         return fail(ST_EINVAL, "fb_graddiv_poisson: resident entry points take device pointers");
-    return fb_graddiv_poisson(reinterpret_cast<double2*>(j_fb), reinterpret_cast<const double2*>(grad_rho_n_fb),
+    return fb_graddiv_poisson_t<double>(reinterpret_cast<double2*>(j_fb), reinterpret_cast<const double2*>(grad_rho_n_fb),
                               reinterpret_cast<const double2*>(grad_rho_np1_fb), dt_inv, w2_inv, DpS2S, DmS2S, kx,
                               nkx, nkr, nkO);
 }
@@ -939,6 +1234,73 @@ int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const dou
         return fail(ST_EINVAL, "fb_filter: resident entry points take device pointers");
     return fb_filter(reinterpret_cast<double2*>(vec), leftX, kx, nxfilt > 0 ? filtr : nullptr, nxfilt, nkx, nkr,
                      nmodes, ncomp);
+}
+
+/* ---------------------------------------------------------------- resident tier, complex64 (mixed precision) */
+
+int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double* kx, const float* In,
+                              const float* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
+                              int32_t fold) {
+    if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ncomp < 1) return fail(ST_EINVAL, "fb_forward_c8: bad sizes");
+    CH_TRY(pow2_check(nkx));
+    if (!is_device_ptr(out) || !is_device_ptr(in) || !is_device_ptr(kx) || !is_device_ptr(In) ||
+        (scale && !is_device_ptr(scale)))
+        return fail(ST_EINVAL, "fb_forward_c8: resident entry points take device pointers");
+    return fb_forward_t<float>(reinterpret_cast<float2*>(out), reinterpret_cast<float2*>(in), leftX, kx, In, nkx, nr,
+                               nmodes, nkr, ncomp, scale, fold);
+}
+
+int32_t chimera_fb_inverse_c8(float* eb_spc, const float* e_fb, const float* b_fb, double leftX, const double* kx,
+                              const float* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost) {
+    if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 2)
+        return fail(ST_EINVAL, "fb_inverse_c8: bad sizes or ghost mode");
+    CH_TRY(pow2_check(nkx));
+    if (!is_device_ptr(eb_spc) || !is_device_ptr(e_fb) || !is_device_ptr(b_fb) || !is_device_ptr(kx) ||
+        !is_device_ptr(Out))
+        return fail(ST_EINVAL, "fb_inverse_c8: resident entry points take device pointers");
+    return fb_inverse_t<float>(reinterpret_cast<float2*>(eb_spc), reinterpret_cast<const float2*>(e_fb),
+                               reinterpret_cast<const float2*>(b_fb), (int64_t)nkx * nkr * nmodes, leftX, kx, Out,
+                               nkx, nr, nmodes, nkr, ghost);
+}
+
+int32_t chimera_fb_rot_scaled_c8(float* vec_fb_loc, const float* vec_fb, const float* DpS2S, const float* DmS2S,
+                                 const double* kx, const float* scale, int32_t nkx, int32_t nkr, int32_t nkO) {
+    if (nkx < 1 || nkr < 1 || nkO < 0) return fail(ST_EINVAL, "fb_rot_scaled_c8: bad sizes");
+    if (!is_device_ptr(vec_fb_loc) || !is_device_ptr(vec_fb) || !is_device_ptr(scale))
+        return fail(ST_EINVAL, "fb_rot_scaled_c8: resident entry points take device pointers");
+    return fb_op_t<float>(FB_ROT, reinterpret_cast<float2*>(vec_fb_loc), reinterpret_cast<const float2*>(vec_fb),
+                          DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr, scale);
+}
+
+int32_t chimera_fb_graddiv_poisson_c8(float* j_fb, const float* grad_rho_n_fb, const float* grad_rho_np1_fb,
+                                      double dt_inv, const float* w2_inv, const float* DpS2S, const float* DmS2S,
+                                      const double* kx, int32_t nkx, int32_t nkr, int32_t nkO) {
+    if (nkx < 1 || nkr < 1 || nkO < 0) return fail(ST_EINVAL, "fb_graddiv_poisson_c8: bad sizes");
+    if (!is_device_ptr(j_fb) || !is_device_ptr(grad_rho_n_fb) || !is_device_ptr(grad_rho_np1_fb) ||
+        !is_device_ptr(w2_inv))
+        return fail(ST_EINVAL, "fb_graddiv_poisson_c8: resident entry points take device pointers");
+    return fb_graddiv_poisson_t<float>(reinterpret_cast<float2*>(j_fb), reinterpret_cast<const float2*>(grad_rho_n_fb),
+                                       reinterpret_cast<const float2*>(grad_rho_np1_fb), dt_inv, w2_inv, DpS2S, DmS2S,
+                                       kx, nkx, nkr, nkO);
+}
+
+int32_t chimera_fb_op_c8(int32_t op, float* out, const float* in, const float* DpS2S, const float* DmS2S,
+                         const double* kx, int32_t nkx, int32_t nkr, int32_t nkO) {
+    if (nkx < 1 || nkr < 1 || nkO < 0 || op < 0 || op > 3) return fail(ST_EINVAL, "fb_op_c8: bad arguments");
+    if (!is_device_ptr(out) || !is_device_ptr(in))
+        return fail(ST_EINVAL, "fb_op_c8: resident entry points take device pointers");
+    return fb_op_t<float>(op, reinterpret_cast<float2*>(out), reinterpret_cast<const float2*>(in), DpS2S, DmS2S, kx,
+                          nkx, nkr, nkO, nkr, nullptr);
+}
+
+int32_t chimera_fb_filter_c8(float* vec, double leftX, const double* kx, const float* filtr, int32_t nxfilt,
+                             int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp) {
+    if (nkx < 1 || nkr < 1 || nmodes < 1 || ncomp < 1 || nxfilt < 0) return fail(ST_EINVAL, "fb_filter_c8: bad sizes");
+    CH_TRY(pow2_check(nkx));
+    if (!is_device_ptr(vec) || !is_device_ptr(kx) || (nxfilt > 0 && !is_device_ptr(filtr)))
+        return fail(ST_EINVAL, "fb_filter_c8: resident entry points take device pointers");
+    return fb_filter_t<float>(reinterpret_cast<float2*>(vec), leftX, kx, nxfilt > 0 ? filtr : nullptr, nxfilt, nkx,
+                              nkr, nmodes, ncomp);
 }
 
 }  // extern "C"

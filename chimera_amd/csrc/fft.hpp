@@ -1,5 +1,6 @@
-// fft.hpp -- batched complex FP64 FFT along x, one block of 256 threads per
-// TOT = 256*EPT points of LDS (one line of length N = TOT, or TOT/N lines).
+// fft.hpp -- batched complex FFT along x (R = double: complex128, R = float:
+// complex64), one block of 256 threads per TOT = 256*EPT points of LDS (one
+// line of length N = TOT, or TOT/N lines).
 //
 // Stockham autosort formulation: every pass reads R points at stride N/R,
 // twiddles them, does a radix-R DFT and writes them back in natural order, so
@@ -7,7 +8,7 @@
 // between one barrier and the next, so a single LDS buffer suffices and a line
 // of 8192 points (128 KiB) fits in one workgroup.  Radix 4, with one radix-2
 // pass first when log2 N is odd.  Twiddles come from a table tw[k] =
-// exp(-2 pi i k/N) built on the host in double precision.  Sign conventions are
+// exp(-2 pi i k/N) built on the host in long double and rounded to R.  Sign conventions are
 // FFTW's: FORWARD = exp(-2 pi i jk/N), BACKWARD = exp(+...), both unnormalised.
 #pragma once
 
@@ -17,22 +18,24 @@ namespace chimera {
 
 constexpr int FFT_THREADS = 256;
 
-__device__ __forceinline__ double2 tw_get(const double2* __restrict__ tw, int idx, int sign) {
-    const double2 w = tw[idx];
+template <typename C>
+__device__ __forceinline__ C tw_get(const C* __restrict__ tw, int idx, int sign) {
+    const C w = tw[idx];
     return sign < 0 ? w : cconj(w);
 }
 
-template <int EPT>
-__device__ __forceinline__ void fft_pass_r4(double2* buf, int N, int Ns, int sign, const double2* __restrict__ tw) {
+template <int EPT, typename R>
+__device__ __forceinline__ void fft_pass_r4(cplx_t<R>* buf, int N, int Ns, int sign, const cplx_t<R>* __restrict__ tw) {
+    typedef cplx_t<R> C;
     constexpr int NB = EPT / 4;
     const int tid = threadIdx.x;
     const int q4 = N >> 2;
-    double2 v[NB][4];
+    C v[NB][4];
 #pragma unroll
     for (int q = 0; q < NB; q++) {
         const int b = tid + FFT_THREADS * q;
         const int line = b / q4, j = b - line * q4;
-        const double2* src = buf + line * N + j;
+        const C* src = buf + line * N + j;
 #pragma unroll
         for (int r = 0; r < 4; r++) v[q][r] = src[r * q4];
     }
@@ -47,11 +50,11 @@ __device__ __forceinline__ void fft_pass_r4(double2* buf, int N, int Ns, int sig
 #pragma unroll
             for (int r = 1; r < 4; r++) v[q][r] = cmul(v[q][r], tw_get(tw, k * r * step, sign));
         }
-        const double2 a0 = cadd(v[q][0], v[q][2]), a1 = csub(v[q][0], v[q][2]);
-        const double2 a2 = cadd(v[q][1], v[q][3]);
-        double2 a3 = csub(v[q][1], v[q][3]);
-        a3 = sign < 0 ? cmk(a3.y, -a3.x) : cmk(-a3.y, a3.x);  // times -i (forward) or +i
-        double2* dst = buf + line * N + (j / Ns) * Ns * 4 + k;
+        const C a0 = cadd(v[q][0], v[q][2]), a1 = csub(v[q][0], v[q][2]);
+        const C a2 = cadd(v[q][1], v[q][3]);
+        C a3 = csub(v[q][1], v[q][3]);
+        a3 = sign < 0 ? cmake<R>(a3.y, -a3.x) : cmake<R>(-a3.y, a3.x);  // times -i (forward) or +i
+        C* dst = buf + line * N + (j / Ns) * Ns * 4 + k;
         dst[0] = cadd(a0, a2);
         dst[Ns] = cadd(a1, a3);
         dst[2 * Ns] = csub(a0, a2);
@@ -60,12 +63,13 @@ __device__ __forceinline__ void fft_pass_r4(double2* buf, int N, int Ns, int sig
     __syncthreads();
 }
 
-template <int EPT>
-__device__ __forceinline__ void fft_pass_r2(double2* buf, int N, int Ns, int sign, const double2* __restrict__ tw) {
+template <int EPT, typename R>
+__device__ __forceinline__ void fft_pass_r2(cplx_t<R>* buf, int N, int Ns, int sign, const cplx_t<R>* __restrict__ tw) {
+    typedef cplx_t<R> C;
     constexpr int NB = EPT / 2;
     const int tid = threadIdx.x;
     const int q2 = N >> 1;
-    double2 v[NB][2];
+    C v[NB][2];
 #pragma unroll
     for (int q = 0; q < NB; q++) {
         const int b = tid + FFT_THREADS * q;
@@ -80,7 +84,7 @@ __device__ __forceinline__ void fft_pass_r2(double2* buf, int N, int Ns, int sig
         const int line = b / q2, j = b - line * q2;
         const int k = j % Ns;
         if (Ns > 1) v[q][1] = cmul(v[q][1], tw_get(tw, k * (N / (Ns * 2)), sign));
-        double2* dst = buf + line * N + (j / Ns) * Ns * 2 + k;
+        C* dst = buf + line * N + (j / Ns) * Ns * 2 + k;
         dst[0] = cadd(v[q][0], v[q][1]);
         dst[Ns] = csub(v[q][0], v[q][1]);
     }
@@ -88,20 +92,21 @@ __device__ __forceinline__ void fft_pass_r2(double2* buf, int N, int Ns, int sig
 }
 
 // In-place transform of the TOT/N lines held in buf (N a power of two).
-template <int EPT>
-__device__ __forceinline__ void fft_lds(double2* buf, int N, int log2N, int sign, const double2* __restrict__ tw) {
+template <int EPT, typename R>
+__device__ __forceinline__ void fft_lds(cplx_t<R>* buf, int N, int log2N, int sign, const cplx_t<R>* __restrict__ tw) {
     int Ns = 1;
     if (log2N & 1) {
-        fft_pass_r2<EPT>(buf, N, Ns, sign, tw);
+        fft_pass_r2<EPT, R>(buf, N, Ns, sign, tw);
         Ns = 2;
     }
     while (Ns < N) {
-        fft_pass_r4<EPT>(buf, N, Ns, sign, tw);
+        fft_pass_r4<EPT, R>(buf, N, Ns, sign, tw);
         Ns *= 4;
     }
 }
 
 // Twiddle table exp(-2 pi i k / N), k < N, cached per N on the device.
 const double2* fft_twiddles(int N);
+const float2* fft_twiddles_f(int N);
 
 }  // namespace chimera

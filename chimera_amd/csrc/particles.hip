@@ -126,22 +126,27 @@ __global__ void k_dep_dens(const double* __restrict__ coord, PStride sx, const d
 }
 
 // Ghost fold after deposition: row 1 -= row 0, row 0 = 0.  grid_deps.f90:80-85, :144-145.
-__global__ void k_ghost_fold(double2* __restrict__ a, int nx1, int nr1, int nlines) {
+template <typename C>
+__global__ void k_ghost_fold(C* __restrict__ a, int nx1, int nr1, int nlines) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nx1 * nlines) return;
     const int64_t ix = t % nx1, q = t / nx1;
-    double2* r0 = a + ix + q * (int64_t)nx1 * nr1;
+    C* r0 = a + ix + q * (int64_t)nx1 * nr1;
     r0[nx1] = csub(r0[nx1], r0[0]);
-    r0[0] = cmk(0.0, 0.0);
+    r0[0].x = 0;
+    r0[0].y = 0;
 }
 
-int32_t ghost_fold(double2* a, int nx1, int nr1, int64_t nlines) {
+template <typename R>
+int32_t ghost_fold(cplx_t<R>* a, int nx1, int nr1, int64_t nlines) {
     if (nlines <= 0) return ST_OK;
-    hipLaunchKernelGGL(k_ghost_fold, blocks_for((int64_t)nx1 * nlines, 256), 256, 0, stream(), a, nx1, nr1,
+    hipLaunchKernelGGL((k_ghost_fold<cplx_t<R>>), blocks_for((int64_t)nx1 * nlines, 256), 256, 0, stream(), a, nx1, nr1,
                        (int)nlines);
     CH_LAUNCH_CHECK("k_ghost_fold");
     return ST_OK;
 }
+template int32_t ghost_fold<double>(double2*, int, int, int64_t);
+template int32_t ghost_fold<float>(float2*, int, int, int64_t);
 
 // Mode normalisation and ghost row of the gathered field.  grid_deps.f90:219-266.
 // One thread per (ix, mode, component) column, so the ghost row is written
@@ -317,7 +322,7 @@ static int32_t dep_curr_impl(const double* coord, const double* momenta, const d
         CH_LAUNCH_CHECK("k_dep_curr");
     }
     const int nlines = (nkO + 1) * 3;
-    hipLaunchKernelGGL(k_ghost_fold, blocks_for((int64_t)(nx + 1) * nlines, 256), 256, 0, stream(), J, nx + 1, nr + 1,
+    hipLaunchKernelGGL(k_ghost_fold<double2>, blocks_for((int64_t)(nx + 1) * nlines, 256), 256, 0, stream(), J, nx + 1, nr + 1,
                        nlines);
     CH_LAUNCH_CHECK("k_ghost_fold");
     return st.finish();
@@ -341,7 +346,7 @@ static int32_t dep_dens_impl(const double* coord, const double* wghts, double* d
         hipLaunchKernelGGL(k_dep_dens, blocks_for(ip1 - ip0, 256), 256, 0, stream(), x, AOS3, w, D, g, ip0, ip1);
         CH_LAUNCH_CHECK("k_dep_dens");
     }
-    hipLaunchKernelGGL(k_ghost_fold, blocks_for((int64_t)(nx + 1) * (nkO + 1), 256), 256, 0, stream(), D, nx + 1,
+    hipLaunchKernelGGL(k_ghost_fold<double2>, blocks_for((int64_t)(nx + 1) * (nkO + 1), 256), 256, 0, stream(), D, nx + 1,
                        nr + 1, nkO + 1);
     CH_LAUNCH_CHECK("k_ghost_fold");
     return st.finish();

@@ -83,26 +83,27 @@ __device__ __forceinline__ bool part_geom(PartGeom& pg, double xp, double yp, do
     return true;
 }
 
-// Relativistic Boris rotation with half-step dt_2 = dt/2 (particle_tools.f90:37-51).
-__device__ __forceinline__ void boris(double u[3], const double f[6], double dt_2) {
-    double Um[3], t[3], s[3], U0[3];
+// Relativistic Boris rotation with half-step dt_2 = dt/2 (particle_tools.f90:37-51),
+// in the precision R of the momenta.
+template <typename R>
+__device__ __forceinline__ void boris(R u[3], const R f[6], R dt_2) {
+    R Um[3], t[3], s[3], U0[3];
     for (int c = 0; c < 3; c++) Um[c] = u[c] + dt_2 * f[c];
-    const double gamma = sqrt(1.0 + (Um[0] * Um[0] + Um[1] * Um[1] + Um[2] * Um[2]));
+    const R gamma = sqrt((R)1 + (Um[0] * Um[0] + Um[1] * Um[1] + Um[2] * Um[2]));
     for (int c = 0; c < 3; c++) t[c] = dt_2 * f[3 + c] / gamma;
-    const double t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    const R t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
     for (int c = 0; c < 3; c++) s[c] = 2 * t[c] / (1 + t2);
     U0[0] = Um[0] + Um[1] * t[2] - Um[2] * t[1];
     U0[1] = Um[1] - Um[0] * t[2] + Um[2] * t[0];
     U0[2] = Um[2] + Um[0] * t[1] - Um[1] * t[0];
-    const double Up0 = Um[0] + U0[1] * s[2] - U0[2] * s[1];
-    const double Up1 = Um[1] - U0[0] * s[2] + U0[2] * s[0];
-    const double Up2 = Um[2] + U0[0] * s[1] - U0[1] * s[0];
+    const R Up0 = Um[0] + U0[1] * s[2] - U0[2] * s[1];
+    const R Up1 = Um[1] - U0[0] * s[2] + U0[2] * s[0];
+    const R Up2 = Um[2] + U0[0] * s[1] - U0[1] * s[0];
     u[0] = Up0 + dt_2 * f[0];
     u[1] = Up1 + dt_2 * f[1];
     u[2] = Up2 + dt_2 * f[2];
 }
 
-// Analytic planar undulator (devices.f90:162-203).
 // Workgroups of a 1-D launch go to the 8 XCDs round-robin (workgroup b to XCD
 // b % 8), and each XCD has its own L2.  xcd_block renumbers them so that XCD x
 // takes one contiguous range of the logical block indices: with particles in
@@ -124,15 +125,18 @@ inline UndulParams make_undul(const double p[4]) {
     return u;
 }
 
-__device__ __forceinline__ void undul_field(const UndulParams& up, double x, double y, double f[6]) {
+// Analytic planar undulator (devices.f90:162-203), evaluated in FP64 and added
+// to the gathered fields.
+template <typename R>
+__device__ __forceinline__ void undul_field(const UndulParams& up, double x, double y, R f[6]) {
     double ampl;
     if (x <= up.X0 || x >= up.X0 + up.Lx) ampl = 0.0;
     else if (x > up.X0 && x < up.X0 + up.lambda) ampl = (x - up.X0) / up.lambda;
     else if (x > up.X0 + up.Lx - up.lambda && x < up.X0 + up.Lx) ampl = (up.X0 + up.Lx - x) / up.lambda;
     else ampl = 1.0;
     ampl = ampl * up.a0;
-    f[4] += ampl * sin(up.ku * (x - up.X0)) * cosh(up.ku * y);
-    f[3] += ampl * cos(up.ku * (x - up.X0)) * sinh(up.ku * y);
+    f[4] += (R)(ampl * sin(up.ku * (x - up.X0)) * cosh(up.ku * y));
+    f[3] += (R)(ampl * cos(up.ku * (x - up.X0)) * sinh(up.ku * y));
 }
 
 // Launchers of the particle kernels (particles.hip).

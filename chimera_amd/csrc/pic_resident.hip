@@ -18,6 +18,7 @@
 //                              (particle_tools.f90:130-153 + the permutation
 //                              gathers of species.py:385-398)
 #include <climits>
+#include <type_traits>
 
 #include "common.hpp"
 #include "pic_device.hpp"
@@ -28,36 +29,40 @@ namespace chimera {
 
 // NMOD = number of azimuthal modes, a template parameter so that every mode
 // index is a constant and the per-particle phases stay in registers.
-template <int NMOD>
-__global__ void k_gather_push(const double* __restrict__ coord, PStride sx, double* __restrict__ u, PStride su,
-                              const double* __restrict__ w, const double2* __restrict__ Fld, GridGeom g,
+// R: precision of momenta, weights and fields (double, or float in mixed
+// precision, where the Boris push runs in FP32); positions and the cell
+// geometry are FP64.
+template <int NMOD, typename R>
+__global__ void k_gather_push(const double* __restrict__ coord, PStride sx, R* __restrict__ u, PStride su,
+                              const R* __restrict__ w, const cplx_t<R>* __restrict__ Fld, GridGeom g,
                               double push_dt, UndulParams up, int n_undul, int np) {
+    typedef cplx_t<R> C;
     const int ip = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (ip >= np) return;
     const double xp = coord[sx.at(ip, 0)], yp = coord[sx.at(ip, 1)], zp = coord[sx.at(ip, 2)];
-    double f[6] = {0, 0, 0, 0, 0, 0};
+    R f[6] = {0, 0, 0, 0, 0, 0};
     PartGeom pg;
-    if (w[ip] != 0.0 && part_geom(pg, xp, yp, zp, g, +1.0)) {
+    if (w[ip] != 0 && part_geom(pg, xp, yp, zp, g, +1.0)) {
         const int64_t sr = g.nx + 1, sm = sr * (g.nr + 1), sl = sm * NMOD;
-        const double2* base = Fld + pg.ix + pg.ir * sr;
+        const C* base = Fld + pg.ix + pg.ir * sr;
         for (int l = 0; l < 6; l++) {
-            double acc = 0.0;
+            R acc = 0;
 #pragma unroll
             for (int iO = 0; iO < NMOD; iO++) {
-                const double2* fl = base + iO * sm + l * sl;
+                const C* fl = base + iO * sm + l * sl;
                 for (int b = 0; b < 2; b++)
                     for (int a = 0; a < 2; a++) {
-                        const double2 pc = crs(pg.S[1][b] * pg.S[0][a], pg.ph[iO]);
-                        const double2 v = fl[a + b * sr];
+                        const C pc = cconv<R>(crs(pg.S[1][b] * pg.S[0][a], pg.ph[iO]));
+                        const C v = fl[a + b * sr];
                         acc += pc.x * v.x - pc.y * v.y;
                     }
             }
             f[l] = acc;
         }
     }
-    if (n_undul) undul_field(up, xp, yp, f);
-    double m[3] = {u[su.at(ip, 0)], u[su.at(ip, 1)], u[su.at(ip, 2)]};
-    boris(m, f, 0.5 * push_dt);
+    if (n_undul) undul_field<R>(up, xp, yp, f);
+    R m[3] = {u[su.at(ip, 0)], u[su.at(ip, 1)], u[su.at(ip, 2)]};
+    boris<R>(m, f, (R)(0.5 * push_dt));
     for (int c = 0; c < 3; c++) u[su.at(ip, c)] = m[c];
 }
 
@@ -83,33 +88,40 @@ __global__ void k_gather_push(const double* __restrict__ coord, PStride sx, doub
 // Deposits are unfolded; the forward FB transform applies the ghost fold.
 
 
-struct DepArgs {
+template <typename RT>
+struct DepArgsT {
+    typedef cplx_t<RT> C;
     double* x;          // positions (3, ld), updated when push
     double* xh;         // half-step positions (3, ld): written when push, read otherwise
-    const double* u;
-    const double* w;
+    const RT* u;        // momenta (3, ld)
+    const RT* w;
     int64_t ld;
     int np;
     int push;
     double dt;
     GridGeom gJ, gR;    // geometry for J (leftX at the current deposit) and rho
-    double2* J;         // (nx+1, nr+1, M, 3) or null
-    double2* R;         // (nx+1, nr+1, M) or null
-    double2* slotJ;     // [block][3*M][DEP_SLOT_R][DEP_SLOT_W] (with J)
-    double2* slotR;     // [block][M][DEP_SLOT_R][DEP_SLOT_W] (with R)
+    C* J;               // (nx+1, nr+1, M, 3) or null
+    C* R;               // (nx+1, nr+1, M) or null
+    C* slotJ;           // [block][3*M][DEP_SLOT_R][DEP_SLOT_W] (with J)
+    C* slotR;           // [block][M][DEP_SLOT_R][DEP_SLOT_W] (with R)
     int4* desc;         // [2][blocks]: (first node row or -1, first node column, width, rows) of J, then of R
     int nblocks;
     int vec4;           // particle arrays allow 32-byte vector access (ld % 4 == 0, 32-byte aligned bases)
 };
 
 typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ dv4 ld4(const double* p) { return *reinterpret_cast<const dv4*>(p); }
+__device__ __forceinline__ fv4 ld4(const float* p) { return *reinterpret_cast<const fv4*>(p); }
 __device__ __forceinline__ void st4(double* p, dv4 v) { *reinterpret_cast<dv4*>(p) = v; }
 
 constexpr int DEP_THREADS = 64, DEP_PPT = 4, DEP_LDS_BYTES = 12 * 1024;
 constexpr int DEP_SLOT_W = 32, DEP_SLOT_R = 4;   // node columns and rows of one slot
 
 __device__ __forceinline__ void lds_add(double* p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(float* p, float v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
@@ -132,10 +144,14 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-template <int NMOD>
-__global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
+// R: precision of momenta, weights, tiles and grids (double, or float in mixed
+// precision).  Positions, the push and the cell geometry stay FP64; each
+// corner contribution is formed in FP64 and accumulated in R.
+template <int NMOD, typename R>
+__global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
+    typedef cplx_t<R> C;
     constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
-    __shared__ double tile[DEP_LDS_BYTES / sizeof(double)];
+    __shared__ R tile[DEP_LDS_BYTES / sizeof(R)];
     __shared__ int red[8][DEP_THREADS / 64];
     __shared__ int box[8];
     const int tid = threadIdx.x;
@@ -152,12 +168,12 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
     const int64_t ib = p0 + (int64_t)tid * DEP_PPT;
     const bool full = a.vec4 && ib + DEP_PPT <= a.np;
     // the thread's particles: component c of array p at particle ib + j
-    auto load_parts = [&](const double* p, double (&v)[3][DEP_PPT], int ncomp) __attribute__((always_inline)) {
+    auto load_parts = [&](const auto* p, double (&v)[3][DEP_PPT], int ncomp) __attribute__((always_inline)) {
         if (full) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 if (c >= ncomp) break;
-                const dv4 t = ld4(p + c * ld + ib);
+                const auto t = ld4(p + c * ld + ib);
                 v[c][0] = t.x, v[c][1] = t.y, v[c][2] = t.z, v[c][3] = t.w;
             }
         } else {
@@ -172,34 +188,11 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
     };
     int bx[8] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
     {
-    double X[3][DEP_PPT], XH[3][DEP_PPT], U[3][DEP_PPT], W[DEP_PPT];
-    if (full) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const dv4 x4 = ld4(a.x + c * ld + ib), u4 = ld4(a.u + c * ld + ib);
-            X[c][0] = x4.x, X[c][1] = x4.y, X[c][2] = x4.z, X[c][3] = x4.w;
-            U[c][0] = u4.x, U[c][1] = u4.y, U[c][2] = u4.z, U[c][3] = u4.w;
-            if (!a.push) {
-                const dv4 h4 = ld4(a.xh + c * ld + ib);
-                XH[c][0] = h4.x, XH[c][1] = h4.y, XH[c][2] = h4.z, XH[c][3] = h4.w;
-            }
-        }
-        const dv4 w4 = ld4(a.w + ib);
-        W[0] = w4.x, W[1] = w4.y, W[2] = w4.z, W[3] = w4.w;
-    } else {
-#pragma unroll
-        for (int j = 0; j < DEP_PPT; j++) {
-            const int64_t ip = ib + j;
-            const bool ok = ip < a.np;
-            W[j] = ok ? a.w[ip] : 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                X[c][j] = ok ? a.x[c * ld + ip] : 0.0;
-                U[c][j] = ok ? a.u[c * ld + ip] : 0.0;
-                XH[c][j] = (ok && !a.push) ? a.xh[c * ld + ip] : 0.0;
-            }
-        }
-    }
+    double X[3][DEP_PPT], XH[3][DEP_PPT], U[3][DEP_PPT], W[3][DEP_PPT];   // W: row 0 only
+    load_parts(a.x, X, 3);
+    load_parts(a.u, U, 3);
+    load_parts(a.w, W, 1);
+    if (!a.push) load_parts(a.xh, XH, 3);
     if (a.push) {
         // particle_tools.f90:58-82: x += u*dt/gamma; x_half = (x_old + x_new)/2
 #pragma unroll
@@ -233,7 +226,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
     }
 #pragma unroll
     for (int j = 0; j < DEP_PPT; j++) {
-        if (W[j] == 0.0) continue;
+        if (W[0][j] == 0.0) continue;
         int ix, ir;
         if (doJ && (fabs(U[0][j]) + fabs(U[1][j]) + fabs(U[2][j])) != 0.0 &&
             cell_of(XH[0][j], XH[1][j], XH[2][j], a.gJ, ix, ir)) {
@@ -259,10 +252,10 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
     const int TXJ = box[1] >= box[0] ? box[1] - box[0] + 2 : 0, TRJ = box[3] >= box[2] ? box[3] - box[2] + 2 : 0;
     const int TXR = box[5] >= box[4] ? box[5] - box[4] + 2 : 0, TRR = box[7] >= box[6] ? box[7] - box[6] + 2 : 0;
     const int64_t nJ = (int64_t)TXJ * TRJ * 3 * nmod * 2, nR = (int64_t)TXR * TRR * nmod * 2;
-    const bool tileJ = doJ && nJ > 0 && nJ * 8 <= DEP_LDS_BYTES;
-    const bool tileR = doR && nR > 0 && (nR + (tileJ ? nJ : 0)) * 8 <= DEP_LDS_BYTES;
-    double* tJ = tile;
-    double* tR = tile + (tileJ ? nJ : 0);
+    const bool tileJ = doJ && nJ > 0 && nJ * (int64_t)sizeof(R) <= DEP_LDS_BYTES;
+    const bool tileR = doR && nR > 0 && (nR + (tileJ ? nJ : 0)) * (int64_t)sizeof(R) <= DEP_LDS_BYTES;
+    R* tJ = tile;
+    R* tR = tile + (tileJ ? nJ : 0);
     const int64_t nz = (tileJ ? nJ : 0) + (tileR ? nR : 0);
     for (int64_t i = tid; i < nz; i += DEP_THREADS) tile[i] = 0.0;
     __syncthreads();
@@ -278,12 +271,12 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
         const bool isJ = l < 3;
         if ((isJ && !doJ) || (!isJ && !doR)) continue;
         const GridGeom& gg = isJ ? a.gJ : a.gR;
-        double* tile_l = isJ ? tJ + (int64_t)l * nmod * TRJ * TXJ * 2 : tR;
-        double2* grid_l = isJ ? a.J + l * sl : a.R;
+        R* tile_l = isJ ? tJ + (int64_t)l * nmod * TRJ * TXJ * 2 : tR;
+        C* grid_l = isJ ? a.J + l * sl : a.R;
         const int bx0 = isJ ? box[0] : box[4], br0 = isJ ? box[2] : box[6];
         const int TX = isJ ? TXJ : TXR, TR = isJ ? TRJ : TRR;
         const bool tiled = isJ ? tileJ : tileR;
-        double2 acc[NMOD][4];
+        C acc[NMOD][4];
         int cx = -1, cr = 0;
         auto flush = [&]() __attribute__((always_inline)) {
             if (cx < 0) return;
@@ -325,7 +318,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
 #pragma unroll
                     for (int m = 0; m < NMOD; m++)
 #pragma unroll
-                        for (int q = 0; q < 4; q++) acc[m][q] = cmk(0.0, 0.0);
+                        for (int q = 0; q < 4; q++) acc[m][q] = cmake<R>(0, 0);
                     cx = (int)pg.ix;
                     cr = (int)pg.ir;
                 }
@@ -334,7 +327,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
                     const double2 pv = crs(q0, pg.ph[m]);
 #pragma unroll
                     for (int q = 0; q < 4; q++)
-                        acc[m][q] = cadd(acc[m][q], crs(pg.S[0][q & 1] * pg.S[1][q >> 1], pv));
+                        acc[m][q] = cadd(acc[m][q], cconv<R>(crs(pg.S[0][q & 1] * pg.S[1][q >> 1], pv)));
                 }
             } else {
 #pragma unroll
@@ -342,10 +335,9 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
                     const double2 pv = crs(q0, pg.ph[m]);
                     for (int q = 0; q < 4; q++) {
                         const double2 d = crs(pg.S[0][q & 1] * pg.S[1][q >> 1], pv);
-                        double* dst = reinterpret_cast<double*>(grid_l + (pg.ix + (q & 1)) + (pg.ir + (q >> 1)) * sr +
-                                                                m * sm);
-                        unsafeAtomicAdd(dst, d.x);
-                        unsafeAtomicAdd(dst + 1, d.y);
+                        R* dst = reinterpret_cast<R*>(grid_l + (pg.ix + (q & 1)) + (pg.ir + (q >> 1)) * sr + m * sm);
+                        unsafeAtomicAdd(dst, (R)d.x);
+                        unsafeAtomicAdd(dst + 1, (R)d.y);
                     }
                 }
             }
@@ -360,19 +352,19 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
         const bool slot = tileJ && TRJ <= DEP_SLOT_R && TXJ <= DEP_SLOT_W;
         if (slot) {
             // tile index (lm*TRJ + rr)*TXJ + xx -> slot index (lm*DEP_SLOT_R + rr)*DEP_SLOT_W + xx
-            double2* sl = a.slotJ + (int64_t)blk * (3 * nmod * DEP_SLOT_R * DEP_SLOT_W);
+            C* sl = a.slotJ + (int64_t)blk * (3 * nmod * DEP_SLOT_R * DEP_SLOT_W);
             const int64_t n = nJ / 2;
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
                 const int64_t rest = i / TXJ;
-                sl[((rest / TRJ) * DEP_SLOT_R + rest % TRJ) * DEP_SLOT_W + i % TXJ] = cmk(tJ[2 * i], tJ[2 * i + 1]);
+                sl[((rest / TRJ) * DEP_SLOT_R + rest % TRJ) * DEP_SLOT_W + i % TXJ] = cmake<R>(tJ[2 * i], tJ[2 * i + 1]);
             }
         } else if (tileJ) {
             const int64_t n = nJ / 2;
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
-                const double re = tJ[2 * i], im = tJ[2 * i + 1];
-                if (re == 0.0 && im == 0.0) continue;
+                const R re = tJ[2 * i], im = tJ[2 * i + 1];
+                if (re == 0 && im == 0) continue;
                 const int64_t xx = i % TXJ, rest = i / TXJ, rr = rest % TRJ, lm = rest / TRJ;
-                double* dst = reinterpret_cast<double*>(a.J + (box[0] + xx) + (box[2] + rr) * sr + lm * sm);
+                R* dst = reinterpret_cast<R*>(a.J + (box[0] + xx) + (box[2] + rr) * sr + lm * sm);
                 unsafeAtomicAdd(dst, re);
                 unsafeAtomicAdd(dst + 1, im);
             }
@@ -382,19 +374,19 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgs a) {
     if (doR) {
         const bool slot = tileR && TRR <= DEP_SLOT_R && TXR <= DEP_SLOT_W;
         if (slot) {
-            double2* sl = a.slotR + (int64_t)blk * (nmod * DEP_SLOT_R * DEP_SLOT_W);
+            C* sl = a.slotR + (int64_t)blk * (nmod * DEP_SLOT_R * DEP_SLOT_W);
             const int64_t n = nR / 2;
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
                 const int64_t rest = i / TXR;
-                sl[((rest / TRR) * DEP_SLOT_R + rest % TRR) * DEP_SLOT_W + i % TXR] = cmk(tR[2 * i], tR[2 * i + 1]);
+                sl[((rest / TRR) * DEP_SLOT_R + rest % TRR) * DEP_SLOT_W + i % TXR] = cmake<R>(tR[2 * i], tR[2 * i + 1]);
             }
         } else if (tileR) {
             const int64_t n = nR / 2;
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
-                const double re = tR[2 * i], im = tR[2 * i + 1];
-                if (re == 0.0 && im == 0.0) continue;
+                const R re = tR[2 * i], im = tR[2 * i + 1];
+                if (re == 0 && im == 0) continue;
                 const int64_t xx = i % TXR, rest = i / TXR, rr = rest % TRR, m = rest / TRR;
-                double* dst = reinterpret_cast<double*>(a.R + (box[4] + xx) + (box[6] + rr) * sr + m * sm);
+                R* dst = reinterpret_cast<R*>(a.R + (box[4] + xx) + (box[6] + rr) * sr + m * sm);
                 unsafeAtomicAdd(dst, re);
                 unsafeAtomicAdd(dst + 1, im);
             }
@@ -435,18 +427,19 @@ constexpr int CMB_T = 256, CMB_LIST = 1024;
 // thread then sums the listed slots that cover its node and adds the sum to
 // the grid once.  The order of the sum is fixed: layer 0 first, blocks in
 // index order within a layer.
-template <int NMOD>
-__global__ void __launch_bounds__(CMB_T) k_dep_combine(double2* __restrict__ J, double2* __restrict__ R,
-                                                        const double2* __restrict__ slotJ,
-                                                        const double2* __restrict__ slotR,
+template <int NMOD, typename RT>
+__global__ void __launch_bounds__(CMB_T) k_dep_combine(cplx_t<RT>* __restrict__ J, cplx_t<RT>* __restrict__ R,
+                                                        const cplx_t<RT>* __restrict__ slotJ,
+                                                        const cplx_t<RT>* __restrict__ slotR,
                                                         const int4* __restrict__ desc, const int32_t* __restrict__ sidx,
                                                         const int32_t* __restrict__ bounds, int nb, int nrb,
                                                         int64_t sr, int64_t sm, int nx1) {
     constexpr int NC = 3 * NMOD;
     const int which = blockIdx.z;
-    double2* grid = which ? R : J;
+    typedef cplx_t<RT> C;
+    C* grid = which ? R : J;
     if (grid == nullptr) return;
-    const double2* slot = which ? slotR : slotJ;
+    const C* slot = which ? slotR : slotJ;
     const int ncomp = which ? NMOD : NC;
     const int64_t sstride = (int64_t)ncomp * DEP_SLOT_R * DEP_SLOT_W;
     const int r = blockIdx.y, xs = blockIdx.x * CMB_T, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -454,9 +447,9 @@ __global__ void __launch_bounds__(CMB_T) k_dep_combine(double2* __restrict__ J, 
     __shared__ int4 list[CMB_LIST];   // (4*block + layer, first column, width, 0)
     __shared__ int wcount[CMB_T / 64];
     __shared__ int nlist;
-    double2 acc[NC];
+    C acc[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) acc[c] = cmk(0.0, 0.0);
+    for (int c = 0; c < NC; c++) acc[c] = cmake<RT>(0, 0);
     bool hit = false;
     auto drain = [&](int n) __attribute__((always_inline)) {
         for (int e = 0; e < n; e++) {
@@ -464,7 +457,7 @@ __global__ void __launch_bounds__(CMB_T) k_dep_combine(double2* __restrict__ J, 
             if (x >= ent.y && x < ent.y + ent.z) {
                 hit = true;
                 const int b = ent.x >> 2, layer = ent.x & 3;
-                const double2* sp = slot + (int64_t)b * sstride + layer * DEP_SLOT_W + (x - ent.y);
+                const C* sp = slot + (int64_t)b * sstride + layer * DEP_SLOT_W + (x - ent.y);
 #pragma unroll
                 for (int c = 0; c < NC; c++)
                     if (c < ncomp) acc[c] = cadd(acc[c], sp[c * DEP_SLOT_R * DEP_SLOT_W]);
@@ -511,7 +504,7 @@ __global__ void __launch_bounds__(CMB_T) k_dep_combine(double2* __restrict__ J, 
     }
     if (x < nx1) drain(nlist);
     if (x >= nx1 || !hit) return;
-    double2* gp = grid + x + (int64_t)r * sr;
+    C* gp = grid + x + (int64_t)r * sr;
 #pragma unroll
     for (int c = 0; c < NC; c++)
         if (c < ncomp) gp[c * sm] = cadd(gp[c * sm], acc[c]);
@@ -527,8 +520,9 @@ __global__ void k_keep_flags(const double* __restrict__ coord, int64_t ld, doubl
     flag[ip] = (x >= l0 && x <= l1 && r2 >= l2 && r2 <= l3) ? 1 : 0;
 }
 
+template <typename T>
 __global__ void k_compact_scatter(const int32_t* __restrict__ flag, const int64_t* __restrict__ pos,
-                                  const double* __restrict__ src, double* __restrict__ dst, int64_t ld_src,
+                                  const T* __restrict__ src, T* __restrict__ dst, int64_t ld_src,
                                   int64_t ld_dst, int ncomp, int np) {
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= np || !flag[ip]) return;
@@ -627,6 +621,139 @@ __global__ void k_gather_particles(const int32_t* __restrict__ idx, int n, const
 
 using namespace chimera;
 
+template <typename R>
+static int32_t gather_push_impl(const double* coord, R* momenta, const R* wghts, int64_t ld, int32_t np,
+                                const cplx_t<R>* Fld, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
+                                double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
+                                const double* undul_params) {
+    CH_TRY(check_resident(np, ld, nkO));
+    if (np == 0) return ST_OK;
+    const PStride s = {1, ld};
+    const GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    UndulParams up = {0, 1, 0, 0, 0};
+    int nu = 0;
+    if (undul_params) {
+        up = make_undul(undul_params);
+        nu = 1;
+    }
+    KTimer kt(KF_GATHER_PUSH);
+#define GATHER_LAUNCH(N)                                                                                      \
+    hipLaunchKernelGGL((k_gather_push<N, R>), blocks_for(np, 256), 256, 0, stream(), coord, s, momenta, s, wghts, \
+                       Fld, g, push_dt, up, nu, np)
+    CH_MODE_SWITCH(nkO, GATHER_LAUNCH)
+#undef GATHER_LAUNCH
+    CH_LAUNCH_CHECK("k_gather_push");
+    return ST_OK;
+}
+
+template <typename R>
+static int32_t compact_impl(double* coord, double* coord_half, R* momenta, R* wghts, int64_t ld, int32_t np,
+                            const double* lims, int32_t test_half, int32_t* np_out) {
+    CH_TRY(check_resident(np, ld, 0));
+    if (lims == nullptr || np_out == nullptr) return fail(ST_EINVAL, "soa_compact: null lims/np_out");
+    if (np == 0) {
+        *np_out = 0;
+        return ST_OK;
+    }
+    int32_t* flag = static_cast<int32_t*>(scratch(4, (size_t)np * sizeof(int32_t)));
+    int64_t* pos = static_cast<int64_t*>(scratch(5, ((size_t)np + 8) * sizeof(int64_t)));
+    double* tmp = static_cast<double*>(scratch(7, (size_t)np * 3 * sizeof(double)));
+    if (!flag || !pos || !tmp) return ST_ENOMEM;
+    int64_t* total = pos + np;
+    const double* test = test_half ? coord_half : coord;
+    KTimer kt(KF_SORT);
+    hipLaunchKernelGGL(k_keep_flags, blocks_for(np, 256), 256, 0, stream(), test, ld, lims[0], lims[1], lims[2],
+                       lims[3], flag, np);
+    CH_LAUNCH_CHECK("k_keep_flags");
+    CH_TRY(exclusive_scan(flag, pos, np, total, 6));
+    // each array: scatter the kept particles to the work buffer, copy back
+    auto one = [&](auto* arr, int nc) -> int32_t {
+        typedef typename std::remove_pointer<decltype(arr)>::type T;
+        T* t = reinterpret_cast<T*>(tmp);
+        hipLaunchKernelGGL((k_compact_scatter<T>), blocks_for(np, 256), 256, 0, stream(), flag, pos, arr, t, ld,
+                           (int64_t)np, nc, np);
+        CH_LAUNCH_CHECK("k_compact_scatter");
+        CH_HIP(hipMemcpy2DAsync(arr, ld * sizeof(T), t, (size_t)np * sizeof(T), (size_t)np * sizeof(T), nc,
+                                hipMemcpyDeviceToDevice, stream()));
+        return ST_OK;
+    };
+    CH_TRY(one(coord, 3));
+    CH_TRY(one(coord_half, 3));
+    CH_TRY(one(momenta, 3));
+    CH_TRY(one(wghts, 1));
+    int64_t h = 0;
+    CH_HIP(hipMemcpyAsync(&h, total, sizeof h, hipMemcpyDeviceToHost, stream()));
+    CH_HIP(hipStreamSynchronize(stream()));
+    *np_out = (int32_t)h;
+    return ST_OK;
+}
+
+template <typename R>
+static int32_t push_deposit_impl(double* coord, double* coord_half, const R* momenta, const R* wghts, int64_t ld,
+                                 int32_t np, double dt, double leftX_J, double leftX_rho, const double* Rgrid,
+                                 double dx_inv, double dr_inv, double r0, double rmax, int32_t nx, int32_t nr,
+                                 int32_t nkO, cplx_t<R>* curr, cplx_t<R>* dens) {
+    typedef cplx_t<R> C;
+    CH_TRY(check_resident(np, ld, nkO));
+    if (np == 0) return ST_OK;
+    DepArgsT<R> a;
+    a.x = coord;
+    a.xh = coord_half;
+    a.u = momenta;
+    a.w = wghts;
+    a.ld = ld;
+    a.np = np;
+    a.push = dt != 0.0;
+    a.dt = dt;
+    a.gJ = resident_geom(leftX_J, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    a.gR = resident_geom(leftX_rho, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    a.J = curr;
+    a.R = dens;
+    {
+        auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 31) == 0; };
+        a.vec4 = (ld % 4 == 0) && al(coord) && al(coord_half) && al(momenta) && al(wghts);
+    }
+    const int nb = (int)blocks_for(np, DEP_THREADS * DEP_PPT);
+    const int nmod = nkO + 1, nrb = nr + 1;
+    a.nblocks = nb;
+    a.slotJ = a.J ? static_cast<C*>(scratch(16, (size_t)nb * 3 * nmod * DEP_SLOT_R * DEP_SLOT_W * sizeof(C))) : nullptr;
+    a.slotR = a.R ? static_cast<C*>(scratch(17, (size_t)nb * nmod * DEP_SLOT_R * DEP_SLOT_W * sizeof(C))) : nullptr;
+    a.desc = static_cast<int4*>(scratch(18, (size_t)2 * nb * sizeof(int4)));
+    uint32_t* keys = static_cast<uint32_t*>(scratch(19, (size_t)2 * nb * 4 * sizeof(uint32_t)));
+    int32_t* bounds = static_cast<int32_t*>(scratch(23, (size_t)2 * 2 * nrb * sizeof(int32_t)));
+    if ((a.J && !a.slotJ) || (a.R && !a.slotR) || !a.desc || !keys || !bounds) return ST_ENOMEM;
+    if (!a.J && !a.R) return ST_OK;
+    KTimer kt(a.J ? KF_DEP_CURR : KF_DEP_DENS);
+    // descriptors of a grid not deposited here read "no slot" (row -1)
+    CH_TRY(fill_bytes(a.desc, 0xFF, (size_t)2 * nb * sizeof(int4)));
+#define DEP_LAUNCH(N) hipLaunchKernelGGL((k_push_deposit<N, R>), nb, DEP_THREADS, 0, stream(), a)
+    CH_MODE_SWITCH(nkO, DEP_LAUNCH)
+#undef DEP_LAUNCH
+    CH_LAUNCH_CHECK("k_push_deposit");
+    // bucket the slots by (grid, cell row), in block order
+    const int nd = 2 * nb;
+    uint32_t* keys_tmp = keys + nd;
+    int32_t* idx = reinterpret_cast<int32_t*>(keys + 2 * (size_t)nd);
+    int32_t* idx_tmp = idx + nd;
+    hipLaunchKernelGGL(k_slot_keys, blocks_for(nd, 256), 256, 0, stream(), a.desc, nb, nrb, keys);
+    CH_LAUNCH_CHECK("k_slot_keys");
+    int bits = 1;
+    while ((1u << bits) <= (uint32_t)(2 * nrb)) bits++;
+    CH_TRY(radix_sort_keys(keys, idx, keys_tmp, idx_tmp, nd, bits, 20, 21));
+    CH_TRY(fill_bytes(bounds, 0, (size_t)2 * 2 * nrb * sizeof(int32_t)));
+    hipLaunchKernelGGL(k_slot_bounds, blocks_for(nd, 256), 256, 0, stream(), keys, nd, (uint32_t)(2 * nrb), bounds);
+    CH_LAUNCH_CHECK("k_slot_bounds");
+    const int64_t sr = nx + 1, sm = sr * (nr + 1);
+    dim3 grid((unsigned)((nx + 1 + CMB_T - 1) / CMB_T), (unsigned)(nr + 1), 2u);
+#define CMB_LAUNCH(N)                                                                                           \
+    hipLaunchKernelGGL((k_dep_combine<N, R>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR, a.desc, idx, bounds, \
+                       nb, nrb, sr, sm, nx + 1)
+    CH_MODE_SWITCH(nkO, CMB_LAUNCH)
+#undef CMB_LAUNCH
+    CH_LAUNCH_CHECK("k_dep_combine");
+    return ST_OK;
+}
+
 extern "C" {
 
 int32_t chimera_soa_push_coords(double* coord, double* coord_half, const double* momenta, int64_t ld, int32_t np,
@@ -661,59 +788,26 @@ int32_t chimera_soa_gather_push(const double* coord, double* momenta, const doub
                                 const double* Fld, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
                                 double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
                                 const double* undul_params) {
-    CH_TRY(check_resident(np, ld, nkO));
-    if (np == 0) return ST_OK;
-    const PStride s = {1, ld};
-    const GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
-    UndulParams up = {0, 1, 0, 0, 0};
-    int nu = 0;
-    if (undul_params) {
-        up = make_undul(undul_params);
-        nu = 1;
-    }
-    KTimer kt(KF_GATHER_PUSH);
-#define GATHER_LAUNCH(N)                                                                                      \
-    hipLaunchKernelGGL((k_gather_push<N>), blocks_for(np, 256), 256, 0, stream(), coord, s, momenta, s, wghts, \
-                       reinterpret_cast<const double2*>(Fld), g, push_dt, up, nu, np)
-    CH_MODE_SWITCH(nkO, GATHER_LAUNCH)
-#undef GATHER_LAUNCH
-    CH_LAUNCH_CHECK("k_gather_push");
-    return ST_OK;
+    return gather_push_impl<double>(coord, momenta, wghts, ld, np, reinterpret_cast<const double2*>(Fld), leftX, Rgrid,
+                                    dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, undul_params);
+}
+
+int32_t chimera_soa_gather_push_mixed(const double* coord, float* momenta, const float* wghts, int64_t ld, int32_t np,
+                                      const float* Fld, double leftX, const double* Rgrid, double dx_inv,
+                                      double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
+                                      double push_dt, const double* undul_params) {
+    return gather_push_impl<float>(coord, momenta, wghts, ld, np, reinterpret_cast<const float2*>(Fld), leftX, Rgrid,
+                                   dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, undul_params);
 }
 
 int32_t chimera_soa_compact(double* coord, double* coord_half, double* momenta, double* wghts, int64_t ld, int32_t np,
                             const double* lims, int32_t test_half, int32_t* np_out) {
-    CH_TRY(check_resident(np, ld, 0));
-    if (lims == nullptr || np_out == nullptr) return fail(ST_EINVAL, "soa_compact: null lims/np_out");
-    if (np == 0) {
-        *np_out = 0;
-        return ST_OK;
-    }
-    int32_t* flag = static_cast<int32_t*>(scratch(4, (size_t)np * sizeof(int32_t)));
-    int64_t* pos = static_cast<int64_t*>(scratch(5, ((size_t)np + 8) * sizeof(int64_t)));
-    double* tmp = static_cast<double*>(scratch(7, (size_t)np * 3 * sizeof(double)));
-    if (!flag || !pos || !tmp) return ST_ENOMEM;
-    int64_t* total = pos + np;
-    const double* test = test_half ? coord_half : coord;
-    KTimer kt(KF_SORT);
-    hipLaunchKernelGGL(k_keep_flags, blocks_for(np, 256), 256, 0, stream(), test, ld, lims[0], lims[1], lims[2],
-                       lims[3], flag, np);
-    CH_LAUNCH_CHECK("k_keep_flags");
-    CH_TRY(exclusive_scan(flag, pos, np, total, 6));
-    double* arrays[4] = {coord, coord_half, momenta, wghts};
-    const int ncomp[4] = {3, 3, 3, 1};
-    for (int a = 0; a < 4; a++) {
-        hipLaunchKernelGGL(k_compact_scatter, blocks_for(np, 256), 256, 0, stream(), flag, pos, arrays[a], tmp, ld,
-                           (int64_t)np, ncomp[a], np);
-        CH_LAUNCH_CHECK("k_compact_scatter");
-        CH_HIP(hipMemcpy2DAsync(arrays[a], ld * sizeof(double), tmp, (size_t)np * sizeof(double),
-                                (size_t)np * sizeof(double), ncomp[a], hipMemcpyDeviceToDevice, stream()));
-    }
-    int64_t h = 0;
-    CH_HIP(hipMemcpyAsync(&h, total, sizeof h, hipMemcpyDeviceToHost, stream()));
-    CH_HIP(hipStreamSynchronize(stream()));
-    *np_out = (int32_t)h;
-    return ST_OK;
+    return compact_impl<double>(coord, coord_half, momenta, wghts, ld, np, lims, test_half, np_out);
+}
+
+int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* momenta, float* wghts, int64_t ld,
+                                  int32_t np, const double* lims, int32_t test_half, int32_t* np_out) {
+    return compact_impl<float>(coord, coord_half, momenta, wghts, ld, np, lims, test_half, np_out);
 }
 
 
@@ -721,67 +815,18 @@ int32_t chimera_soa_push_deposit(double* coord, double* coord_half, const double
                                  int64_t ld, int32_t np, double dt, double leftX_J, double leftX_rho,
                                  const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax, int32_t nx,
                                  int32_t nr, int32_t nkO, double* curr, double* dens) {
-    CH_TRY(check_resident(np, ld, nkO));
-    if (np == 0) return ST_OK;
-    DepArgs a;
-    a.x = coord;
-    a.xh = coord_half;
-    a.u = momenta;
-    a.w = wghts;
-    a.ld = ld;
-    a.np = np;
-    a.push = dt != 0.0;
-    a.dt = dt;
-    a.gJ = resident_geom(leftX_J, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
-    a.gR = resident_geom(leftX_rho, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
-    a.J = reinterpret_cast<double2*>(curr);
-    a.R = reinterpret_cast<double2*>(dens);
-    {
-        auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 31) == 0; };
-        a.vec4 = (ld % 4 == 0) && al(coord) && al(coord_half) && al(momenta) && al(wghts);
-    }
-    const int nb = (int)blocks_for(np, DEP_THREADS * DEP_PPT);
-    const int nmod = nkO + 1, nrb = nr + 1;
-    a.nblocks = nb;
-    a.slotJ = a.J ? static_cast<double2*>(
-                        scratch(16, (size_t)nb * 3 * nmod * DEP_SLOT_R * DEP_SLOT_W * sizeof(double2)))
-                  : nullptr;
-    a.slotR = a.R ? static_cast<double2*>(scratch(17, (size_t)nb * nmod * DEP_SLOT_R * DEP_SLOT_W * sizeof(double2)))
-                  : nullptr;
-    a.desc = static_cast<int4*>(scratch(18, (size_t)2 * nb * sizeof(int4)));
-    uint32_t* keys = static_cast<uint32_t*>(scratch(19, (size_t)2 * nb * 4 * sizeof(uint32_t)));
-    int32_t* bounds = static_cast<int32_t*>(scratch(23, (size_t)2 * 2 * nrb * sizeof(int32_t)));
-    if ((a.J && !a.slotJ) || (a.R && !a.slotR) || !a.desc || !keys || !bounds) return ST_ENOMEM;
-    if (!a.J && !a.R) return ST_OK;
-    KTimer kt(a.J ? KF_DEP_CURR : KF_DEP_DENS);
-    // descriptors of a grid not deposited here read "no slot" (row -1)
-    CH_TRY(fill_bytes(a.desc, 0xFF, (size_t)2 * nb * sizeof(int4)));
-#define DEP_LAUNCH(N) hipLaunchKernelGGL((k_push_deposit<N>), nb, DEP_THREADS, 0, stream(), a)
-    CH_MODE_SWITCH(nkO, DEP_LAUNCH)
-#undef DEP_LAUNCH
-    CH_LAUNCH_CHECK("k_push_deposit");
-    // bucket the slots by (grid, cell row), in block order
-    const int nd = 2 * nb;
-    uint32_t* keys_tmp = keys + nd;
-    int32_t* idx = reinterpret_cast<int32_t*>(keys + 2 * (size_t)nd);
-    int32_t* idx_tmp = idx + nd;
-    hipLaunchKernelGGL(k_slot_keys, blocks_for(nd, 256), 256, 0, stream(), a.desc, nb, nrb, keys);
-    CH_LAUNCH_CHECK("k_slot_keys");
-    int bits = 1;
-    while ((1u << bits) <= (uint32_t)(2 * nrb)) bits++;
-    CH_TRY(radix_sort_keys(keys, idx, keys_tmp, idx_tmp, nd, bits, 20, 21));
-    CH_TRY(fill_bytes(bounds, 0, (size_t)2 * 2 * nrb * sizeof(int32_t)));
-    hipLaunchKernelGGL(k_slot_bounds, blocks_for(nd, 256), 256, 0, stream(), keys, nd, (uint32_t)(2 * nrb), bounds);
-    CH_LAUNCH_CHECK("k_slot_bounds");
-    const int64_t sr = nx + 1, sm = sr * (nr + 1);
-    dim3 grid((unsigned)((nx + 1 + CMB_T - 1) / CMB_T), (unsigned)(nr + 1), 2u);
-#define CMB_LAUNCH(N)                                                                                           \
-    hipLaunchKernelGGL((k_dep_combine<N>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR, a.desc, idx, bounds, \
-                       nb, nrb, sr, sm, nx + 1)
-    CH_MODE_SWITCH(nkO, CMB_LAUNCH)
-#undef CMB_LAUNCH
-    CH_LAUNCH_CHECK("k_dep_combine");
-    return ST_OK;
+    return push_deposit_impl<double>(coord, coord_half, momenta, wghts, ld, np, dt, leftX_J, leftX_rho, Rgrid, dx_inv,
+                                     dr_inv, r0, rmax, nx, nr, nkO, reinterpret_cast<double2*>(curr),
+                                     reinterpret_cast<double2*>(dens));
+}
+
+int32_t chimera_soa_push_deposit_mixed(double* coord, double* coord_half, const float* momenta, const float* wghts,
+                                       int64_t ld, int32_t np, double dt, double leftX_J, double leftX_rho,
+                                       const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
+                                       int32_t nx, int32_t nr, int32_t nkO, float* curr, float* dens) {
+    return push_deposit_impl<float>(coord, coord_half, momenta, wghts, ld, np, dt, leftX_J, leftX_rho, Rgrid, dx_inv,
+                                    dr_inv, r0, rmax, nx, nr, nkO, reinterpret_cast<float2*>(curr),
+                                    reinterpret_cast<float2*>(dens));
 }
 
 

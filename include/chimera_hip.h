@@ -311,6 +311,50 @@ int32_t chimera_fb_op(int32_t op, double* out, const double* in, const double* D
 int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const double* filtr, int32_t nxfilt,
                           int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp);
 
+/* ======================================================================
+ * Mixed-precision resident tier (no counterpart in the reference, which is
+ * FP64 throughout).  Particle positions stay FP64; momenta and weights are
+ * float32; every grid and FB-space array is complex64 (interleaved float
+ * pairs) with the same Fortran layout as the FP64 entry points above; the
+ * Hankel, D+- and PSATD tables are the FP64 tables rounded to float32; kx
+ * stays FP64.  The GEMMs run on FP32 MFMA with FP32 accumulation.  Each
+ * function has the argument list and meaning of the FP64 function of the same
+ * name without the suffix.
+ * ====================================================================== */
+int32_t chimera_soa_push_deposit_mixed(double* coord, double* coord_half, const float* momenta, const float* wghts,
+                                       int64_t ld, int32_t np, double dt, double leftX_J, double leftX_rho,
+                                       const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
+                                       int32_t nx, int32_t nr, int32_t nkO, float* curr, float* dens);
+int32_t chimera_soa_gather_push_mixed(const double* coord, float* momenta, const float* wghts, int64_t ld, int32_t np,
+                                      const float* Fld, double leftX, const double* Rgrid, double dx_inv,
+                                      double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
+                                      double push_dt, const double* undul_params);
+int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* momenta, float* wghts, int64_t ld,
+                                  int32_t np, const double* lims, int32_t test_half, int32_t* np_out);
+int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double* kx, const float* In,
+                              const float* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
+                              int32_t fold);
+int32_t chimera_fb_inverse_c8(float* eb_spc, const float* e_fb, const float* b_fb, double leftX, const double* kx,
+                              const float* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost);
+int32_t chimera_fb_rot_scaled_c8(float* vec_fb_loc, const float* vec_fb, const float* DpS2S, const float* DmS2S,
+                                 const double* kx, const float* scale, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_fb_graddiv_poisson_c8(float* j_fb, const float* grad_rho_n_fb, const float* grad_rho_np1_fb,
+                                      double dt_inv, const float* w2_inv, const float* DpS2S, const float* DmS2S,
+                                      const double* kx, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_fb_op_c8(int32_t op, float* out, const float* in, const float* DpS2S, const float* DmS2S,
+                         const double* kx, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_fb_filter_c8(float* vec, double leftX, const double* kx, const float* filtr, int32_t nxfilt,
+                             int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp);
+int32_t chimera_maxwell_push_with_spchrg_c8(float* EG_fb, const float* j_fb, const float* grad_rho_n_fb,
+                                            const float* grad_rho_np1_fb, const float* C1, const float* C2,
+                                            int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_maxwell_push_wo_spchrg_c8(float* EG_fb, const float* j_fb, const float* C1, const float* C2,
+                                          int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_maxwell_init_push_c8(float* EG_fb, const float* j_fb, const float* grad_rho_n_fb, const float* C1,
+                                     const float* C2, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_omp_mult_vec_c8(float* vec_fb, const float* A, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_omp_add_vec_c8(float* vec_fb, const float* A, int32_t nkx, int32_t nkr, int32_t nkO);
+
 #ifdef __cplusplus
 }
 #endif
