@@ -65,8 +65,8 @@ def run_gpu(args):
     from chimera_amd import workloads as W
     from chimera_amd._lib import lib
 
-    spec = W.WORKLOADS[args.workload]()
-    log(f"building {args.workload}")
+    spec = W.with_precision(W.WORKLOADS[args.workload](), args.precision)
+    log(f"building {args.workload} ({args.precision})")
     t0 = time.time()
     run = W.build(G, spec, seed=args.seed)
     L = lib()
@@ -222,6 +222,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="lpa_cfg1")
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",

@@ -30,6 +30,13 @@ class ChimeraRun:
         self.MovingFrames = SimComps["MovingFrames"] if "MovingFrames" in SimComps else ({},)
         self.Features = SimComps.get("Features", ())
         self.SortEvery = int(SimComps.get("SortEvery", 0))
+        # one precision for the whole run ('fp64' or 'mixed', the Precision key of the components)
+        precs = {c.Args.get("Precision", "fp64") for c in tuple(self.Solvers) + tuple(self.Particles)}
+        if len(precs) > 1:
+            raise ValueError(f"solvers and species must share one Precision, got {sorted(precs)}")
+        self.mixed = precs == {"mixed"}
+        if self.mixed and self.SortEvery:
+            raise NotImplementedError("SortEvery is implemented for FP64 runs")
         self.init_Moving_Frames()
         self.init_damp_profile()
         if "PreDepositBackground" in self.Features:
@@ -37,6 +44,10 @@ class ChimeraRun:
                 if "StillAsBackground" in solver.Args["Features"]:
                     self.dep_bg(solver)
         self.make_halfstep()
+
+    def _pfn(self, name):
+        """Particle entry point of the run's precision."""
+        return name + "_mixed" if self.mixed else name
 
     def init_Moving_Frames(self):
         for wind in self.MovingFrames:                                     # chimera_main.py:41-52
@@ -140,7 +151,7 @@ class ChimeraRun:
         for sp in self.Particles:
             if sp.np == 0 or "Still" in sp.Args["Features"]:
                 continue
-            call("chimera_soa_push_deposit", sp.ptr("coords"), sp.ptr("coords_halfstep"), sp.ptr("momenta"),
+            call(self._pfn("chimera_soa_push_deposit"), sp.ptr("coords"), sp.ptr("coords_halfstep"), sp.ptr("momenta"),
                  sp.ptr("weights"), sp.ld, sp.np, float(sp.Args["TimeStep"]), g[0], float(leftX_rho), *g[1:],
                  curr.ptr if curr is not None else None, dens.ptr if dens is not None else None)
 
@@ -172,7 +183,7 @@ class ChimeraRun:
         for species in self.Particles:
             if species.np == 0 or "Still" in species.Args["Features"]:
                 continue
-            call("chimera_soa_push_deposit", species.ptr("coords"), species.ptr("coords_halfstep"),
+            call(self._pfn("chimera_soa_push_deposit"), species.ptr("coords"), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
                  J.ptr, None)
 
@@ -186,7 +197,7 @@ class ChimeraRun:
         for species in self.Particles:
             if species.np == 0 or "Still" in species.Args["Features"]:
                 continue
-            call("chimera_soa_push_deposit", species.ptr(component), species.ptr("coords_halfstep"),
+            call(self._pfn("chimera_soa_push_deposit"), species.ptr(component), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
                  None, R.ptr)
 
@@ -197,7 +208,7 @@ class ChimeraRun:
         for species in self.Particles:
             if species.np == 0 or "Still" not in species.Args["Features"]:
                 continue
-            call("chimera_soa_push_deposit", species.ptr("coords"), species.ptr("coords_halfstep"),
+            call(self._pfn("chimera_soa_push_deposit"), species.ptr("coords"), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
                  None, B.ptr)
 
@@ -231,7 +242,7 @@ class ChimeraRun:
                 EB, g = solver.Data["EB"].ptr, solver.geom_args()
             else:
                 EB, g = 0, (0.0, 0, 1.0, 1.0, 0.0, -1.0, 1, 1, 0)   # rmax < 0: nothing gathered
-            call("chimera_soa_gather_push", species.ptr("coords"), species.ptr("momenta"), species.ptr("weights"),
+            call(self._pfn("chimera_soa_gather_push"), species.ptr("coords"), species.ptr("momenta"), species.ptr("weights"),
                  species.ld, species.np, EB, *g, push_dt, und.ctypes.data if und is not None else None)
 
     # ------------------------------------------------------------------ moving frames (chimera_main.py:250-318)

@@ -7,6 +7,11 @@ Supported: the plain (non-KxShift) solver with or without SpaceCharge and
 StillAsBackground, Rcut, NoPoissonCorrection, Xchunked grids, CoPropagative.
 The envelope (KxShift) and StaticKick paths are not part of the PIC step and
 raise NotImplementedError.
+
+Extra key 'Precision': 'fp64' (default, the reference's arithmetic) or
+'mixed': every field and FB-space array complex64, the Hankel, D+- and PSATD
+tables built in FP64 and rounded to float32 (kx stays FP64), transforms and
+operators on FP32 MFMA with FP32 accumulation.
 """
 from __future__ import annotations
 
@@ -29,6 +34,12 @@ class Solver:
             raise NotImplementedError("the envelope (KxShift) solver is outside the PIC-step path")
         if "StaticKick" in A["Features"]:
             raise NotImplementedError("StaticKick is outside the PIC-step path")
+        A.setdefault("Precision", "fp64")
+        if A["Precision"] not in ("fp64", "mixed"):
+            raise ValueError(f"Precision {A['Precision']!r}: 'fp64' or 'mixed'")
+        self.mixed = A["Precision"] == "mixed"
+        self.rt = np.float32 if self.mixed else np.float64          # real tables
+        self.ct = np.complex64 if self.mixed else np.complex128     # fields
 
         A["Nko"] = A["MaxAzimuthMode"]
         A["leftX"], A["rightX"], A["lengthR"], A["dx"], A["dr"] = A["Grid"]
@@ -89,33 +100,40 @@ class Solver:
         A["FBout"] = (A["kx"], A["Out"])
 
         # ---- device copies of the operators
-        self.dev = {kw: DeviceArray.from_numpy(A[kw], np.float64)
-                    for kw in ("DpS2S", "DmS2S", "kx", "In", "InCurr", "Out", "DepFact", "PoissFact", "Rgrid")}
+        self.dev = {kw: DeviceArray.from_numpy(A[kw], self.rt)
+                    for kw in ("DpS2S", "DmS2S", "In", "InCurr", "Out", "DepFact", "PoissFact")}
+        for kw in ("kx", "Rgrid"):
+            self.dev[kw] = DeviceArray.from_numpy(A[kw], np.float64)
 
         # ---- device state (solvers.py:183-212)
         Nx, Nr, Nkr, M = A["Nx"], A["Nr"], A["Nkr"], A["Mtot"]
         sp, fb = (Nx, Nr, M), (Nx, Nkr, M)
         D = self.Data = {}
-        D["EB"] = DeviceArray(sp + (6,))
-        D["J"] = DeviceArray(sp + (3,))
-        D["scl_spc"] = DeviceArray(sp)
-        D["EG_fb"] = DeviceArray(fb + (6,))
-        D["vec_fb"] = DeviceArray(fb + (3,))
-        D["scl_fb"] = DeviceArray(fb)
-        D["J_fb"] = DeviceArray(fb + (3,))
-        D["B_fb"] = DeviceArray(fb + (3,))
+        ct = self.ct
+        D["EB"] = DeviceArray(sp + (6,), ct)
+        D["J"] = DeviceArray(sp + (3,), ct)
+        D["scl_spc"] = DeviceArray(sp, ct)
+        D["EG_fb"] = DeviceArray(fb + (6,), ct)
+        D["vec_fb"] = DeviceArray(fb + (3,), ct)
+        D["scl_fb"] = DeviceArray(fb, ct)
+        D["J_fb"] = DeviceArray(fb + (3,), ct)
+        D["B_fb"] = DeviceArray(fb + (3,), ct)
         if "SpaceCharge" in A["Features"]:
-            D["Rho"] = DeviceArray(sp)
-            D["Rho_fb"] = DeviceArray(fb)
-            D["gradRho_fb_nxt"] = DeviceArray(fb + (3,))
-            D["gradRho_fb_prv"] = DeviceArray(fb + (3,))
+            D["Rho"] = DeviceArray(sp, ct)
+            D["Rho_fb"] = DeviceArray(fb, ct)
+            D["gradRho_fb_nxt"] = DeviceArray(fb + (3,), ct)
+            D["gradRho_fb_prv"] = DeviceArray(fb + (3,), ct)
         if "StillAsBackground" in A["Features"]:
-            D["BckGrndRho"] = DeviceArray(sp)
+            D["BckGrndRho"] = DeviceArray(sp, ct)
 
         if "CoPropagative" in A:
             self.PSATD_coeffs(A["CoPropagative"])
         else:
             self.PSATD_coeffs()
+
+    def fn(self, name):
+        """Library entry point of this solver's precision."""
+        return name + "_c8" if self.mixed else name
 
     # ------------------------------------------------------------------ geometry helpers
     @property
@@ -134,17 +152,17 @@ class Solver:
     def PSATD_coeffs(self, beta=1.0):
         A = self.Args
         E, G = ops.psatd_tables(A["w"], A["TimeStep"], "SpaceCharge" in A["Features"])
-        self.Data["PSATD_E"] = DeviceArray.from_numpy(E, np.float64)
-        self.Data["PSATD_G"] = DeviceArray.from_numpy(G, np.float64)
+        self.Data["PSATD_E"] = DeviceArray.from_numpy(E, self.rt)
+        self.Data["PSATD_G"] = DeviceArray.from_numpy(G, self.rt)
 
     def maxwell_solver(self):
         D = self.Data
         Nx, Nkr, M = self.n_fb
         if "SpaceCharge" in self.Args["Features"]:
-            call("chimera_maxwell_push_with_spchrg", D["EG_fb"].ptr, D["J_fb"].ptr, D["gradRho_fb_prv"].ptr,
+            call(self.fn("chimera_maxwell_push_with_spchrg"), D["EG_fb"].ptr, D["J_fb"].ptr, D["gradRho_fb_prv"].ptr,
                  D["gradRho_fb_nxt"].ptr, D["PSATD_E"].ptr, D["PSATD_G"].ptr, Nx, Nkr, M)
         else:
-            call("chimera_maxwell_push_wo_spchrg", D["EG_fb"].ptr, D["J_fb"].ptr, D["PSATD_E"].ptr,
+            call(self.fn("chimera_maxwell_push_wo_spchrg"), D["EG_fb"].ptr, D["J_fb"].ptr, D["PSATD_E"].ptr,
                  D["PSATD_G"].ptr, Nx, Nkr, M)
 
     def poiss_corr(self, poiss_corr_num=3):
@@ -155,14 +173,14 @@ class Solver:
         Nx, Nkr, M = self.n_fb
         for _ in range(poiss_corr_num):
             if "SpaceCharge" in A["Features"]:
-                call("chimera_fb_graddiv_poisson", D["J_fb"].ptr, D["gradRho_fb_prv"].ptr, D["gradRho_fb_nxt"].ptr,
+                call(self.fn("chimera_fb_graddiv_poisson"), D["J_fb"].ptr, D["gradRho_fb_prv"].ptr, D["gradRho_fb_nxt"].ptr,
                      float(A["dt_inv"]), self.dev["PoissFact"].ptr, self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr,
                      self.dev["kx"].ptr, Nx, Nkr, M - 1)
             else:
                 D["vec_fb"].copy_(D["J_fb"])
                 self.FBGradDiv()
-                call("chimera_omp_mult_vec", D["vec_fb"].ptr, self.dev["PoissFact"].ptr, Nx, Nkr, M)
-                call("chimera_omp_add_vec", D["J_fb"].ptr, D["vec_fb"].ptr, Nx, Nkr, M)
+                call(self.fn("chimera_omp_mult_vec"), D["vec_fb"].ptr, self.dev["PoissFact"].ptr, Nx, Nkr, M)
+                call(self.fn("chimera_omp_add_vec"), D["J_fb"].ptr, D["vec_fb"].ptr, Nx, Nkr, M)
 
     def maxwell_solver_stat(self, px0):
         """solvers.py:333-362 (complex tables built on the host, once)."""
@@ -170,9 +188,9 @@ class Solver:
         if "SpaceCharge" not in A["Features"]:
             return
         C1, C2 = ops.static_tables(A["kx_g"], A["w"], px0)
-        c1, c2 = DeviceArray.from_numpy(C1), DeviceArray.from_numpy(C2)
+        c1, c2 = DeviceArray.from_numpy(C1, self.ct), DeviceArray.from_numpy(C2, self.ct)
         Nx, Nkr, M = self.n_fb
-        call("chimera_maxwell_init_push", D["EG_fb"].ptr, D["J_fb"].ptr, D["gradRho_fb_nxt"].ptr, c1.ptr, c2.ptr, Nx,
+        call(self.fn("chimera_maxwell_init_push"), D["EG_fb"].ptr, D["J_fb"].ptr, D["gradRho_fb_nxt"].ptr, c1.ptr, c2.ptr, Nx,
              Nkr, M)
 
     # ------------------------------------------------------------------ FB transforms (solvers.py:407-465)
@@ -180,26 +198,26 @@ class Solver:
         """J (unfolded deposit) -> J_fb, with the ghost fold and DepFact fused."""
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
-        call("chimera_fb_forward", D["J_fb"].ptr, D["J"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
+        call(self.fn("chimera_fb_forward"), D["J_fb"].ptr, D["J"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
              self.dev["InCurr"].ptr, self.dev["DepFact"].ptr, Nx, A["Nr"] - 1, M, Nkr, 3, 1)
 
     def fb_dens_in(self):
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
-        call("chimera_fb_forward", D["Rho_fb"].ptr, D["Rho"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
+        call(self.fn("chimera_fb_forward"), D["Rho_fb"].ptr, D["Rho"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
              self.dev["InCurr"].ptr, self.dev["DepFact"].ptr, Nx, A["Nr"] - 1, M, Nkr, 1, 1)
 
     def fb_fld_out(self):
         """EG_fb, B_fb -> EB with eb_correction fused."""
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
-        call("chimera_fb_inverse", D["EB"].ptr, D["EG_fb"].ptr, D["B_fb"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
+        call(self.fn("chimera_fb_inverse"), D["EB"].ptr, D["EG_fb"].ptr, D["B_fb"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
              self.dev["Out"].ptr, Nx, A["Nr"] - 1, M, Nkr, 2)
 
     # ------------------------------------------------------------------ FB operators (solvers.py:467-553)
     def _op(self, op, out, inp):
         Nx, Nkr, M = self.n_fb
-        call("chimera_fb_op", op, out.ptr, inp.ptr, self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr, self.dev["kx"].ptr,
+        call(self.fn("chimera_fb_op"), op, out.ptr, inp.ptr, self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr, self.dev["kx"].ptr,
              Nx, Nkr, M - 1)
 
     def FBGrad(self):
@@ -219,7 +237,7 @@ class Solver:
         D = self.Data
         Nx, Nkr, M = self.n_fb
         G = D["EG_fb"].offset_ptr(3 * Nx * Nkr * M)
-        call("chimera_fb_rot_scaled", D["B_fb"].ptr, G, self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr,
+        call(self.fn("chimera_fb_rot_scaled"), D["B_fb"].ptr, G, self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr,
              self.dev["kx"].ptr, self.dev["PoissFact"].ptr, Nx, Nkr, M - 1)
 
     # ------------------------------------------------------------------ laser and absorbing layer
@@ -248,7 +266,7 @@ class Solver:
         GG = -w * np.sin(w * X_focus) * EE + np.cos(w * X_focus) * GG
         EE = vf * np.exp(1.0j * kx_g[:, :, None, None] * X_focus)
         GG = GG * np.exp(1.0j * kx_g[:, :, None, None] * X_focus)
-        EG = D["EG_fb"].numpy()
+        EG = D["EG_fb"].numpy().astype(np.complex128)
         EG[..., :3] += EE
         EG[..., 3:] += GG
         D["EG_fb"].upload(EG)
@@ -274,7 +292,7 @@ class Solver:
         A, D = self.Args, self.Data
         prof = A["damp_profile"]
         if "damp_profile_dev" not in self.dev or self.dev["damp_profile_dev"].shape != prof.shape:
-            self.dev["damp_profile_dev"] = DeviceArray.from_numpy(np.asarray(prof, np.float64))
+            self.dev["damp_profile_dev"] = DeviceArray.from_numpy(np.asarray(prof, np.float64), self.rt)
         Nx, Nkr, M = self.n_fb
-        call("chimera_fb_filter", D["EG_fb"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
+        call(self.fn("chimera_fb_filter"), D["EG_fb"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
              self.dev["damp_profile_dev"].ptr, len(prof), Nx, Nkr, M, 6)

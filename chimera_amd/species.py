@@ -7,6 +7,9 @@ Same constructor dictionary, attributes and methods as the reference's Specie
 arrays exactly as the reference does (the random draws are NumPy's legacy
 generator in the reference's order, the FixedCell lattice is built by
 libchimera_hip); add_particles uploads them.
+
+Extra key 'Precision': 'fp64' (default) or 'mixed' (momenta and weights
+float32 on the device; positions stay FP64).
 """
 from __future__ import annotations
 
@@ -31,6 +34,11 @@ class Specie:
         A.setdefault("MomentaSpreads", (0.0, 0.0, 0.0))
         A.setdefault("Grid", (0.0, 0.0, 0.0, 1.0, 1.0))
         self.Devices = A["Devices"] if "Devices" in A else ()
+        A.setdefault("Precision", "fp64")
+        if A["Precision"] not in ("fp64", "mixed"):
+            raise ValueError(f"Precision {A['Precision']!r}: 'fp64' or 'mixed'")
+        self.mixed = A["Precision"] == "mixed"
+        self.rt = np.float32 if self.mixed else np.float64
 
         A["leftX"], A["rightX"], A["lengthR"], A["dx"], A["dr"] = A["Grid"]
         A["push_fact"] = 2 * np.pi * A["Charge"] / A["Mass"]             # species.py:64
@@ -75,8 +83,9 @@ class Specie:
     def _alloc(self, cap):
         cap = -(-int(cap) // 64) * 64        # components start on 512-byte boundaries (vector loads)
         old, oldcap = (self.Data, self.cap) if self.cap else (None, 0)
-        D = {k: DeviceArray((cap, 3), np.float64) for k in ("coords", "coords_halfstep", "momenta")}
-        D["weights"] = DeviceArray((cap,), np.float64)
+        D = {k: DeviceArray((cap, 3), np.float64) for k in ("coords", "coords_halfstep")}
+        D["momenta"] = DeviceArray((cap, 3), self.rt)
+        D["weights"] = DeviceArray((cap,), self.rt)
         if old is not None and self.np:
             for k in ("coords", "coords_halfstep", "momenta"):
                 for c in range(3):
@@ -158,6 +167,8 @@ class Specie:
 
     def denoise(self, WaveNums2Kill):
         """Mirror particles in anti-phase (species.py:322-349)."""
+        if self.mixed:
+            raise NotImplementedError("denoise is implemented for FP64 species")
         for k_supp in WaveNums2Kill:
             n = self.np
             if 2 * n > self.cap:
@@ -187,7 +198,7 @@ class Specie:
         half = ("Xchunked" in A and "NoSorting" not in wind["Features"] and position == "cntr")
         lims = np.ascontiguousarray(SimDom, dtype=np.float64)
         nout = ctypes.c_int32(0)
-        call("chimera_soa_compact", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"),
+        call("chimera_soa_compact_mixed" if self.mixed else "chimera_soa_compact", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"),
              self.ptr("weights"), self.ld, self.np, lims.ctypes.data, 1 if half else 0, ctypes.addressof(nout))
         self.np = int(nout.value)
 
@@ -198,6 +209,8 @@ class Specie:
         that deposition and gather touch neighbouring nodes."""
         if self.np == 0:
             return
+        if self.mixed:
+            raise NotImplementedError("sorting by cell is implemented for FP64 species")
         spare = getattr(self, "_spare", None)
         if spare is None or spare["weights"].shape[0] != self.cap:
             spare = {k: DeviceArray((self.cap, 3), np.float64) for k in ("coords", "coords_halfstep", "momenta")}
@@ -217,6 +230,8 @@ class Specie:
     def push_coords(self, dt_frac=1.0):
         if self.np == 0 or "Still" in self.Args["Features"]:
             return
+        if self.mixed:
+            raise NotImplementedError("mixed-precision species push positions inside the fused deposition")
         call("chimera_soa_push_coords", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"), self.ld,
              self.np, float(dt_frac * self.Args["TimeStep"]))
 

@@ -65,6 +65,15 @@ def lpa_small(Nx=128, Nkr=32, nko=1, moving=False, steps=10):
 WORKLOADS = {"lpa_cfg0": lpa_cfg0, "lpa_cfg1": lpa_cfg1, "lpa_small": lpa_small}
 
 
+def with_precision(spec, precision):
+    """A copy of spec whose solver and species run in `precision` ('fp64' or
+    'mixed'; the oracle ignores the key and always runs FP64)."""
+    spec = copy.deepcopy(spec)
+    for key in ("solver", "electrons", "ions"):
+        spec[key]["Precision"] = precision
+    return spec
+
+
 def build(mod, spec, seed=12345, **run_options):
     """Instantiate a workload with mod.Solver / mod.Specie / mod.ChimeraRun;
     run_options are extra ChimeraRun keys (e.g. SortEvery)."""

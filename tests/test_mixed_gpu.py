@@ -1,0 +1,95 @@
+"""Mixed precision (float32 momenta and weights, complex64 fields, FP32 MFMA
+transforms, FP64 positions and tables) against the FP64 GPU path on the same
+seeded inputs.
+
+The errors are reported per field component and per particle moment.  Bar:
+relative L2 <= 1e-5 for E, B (each component of the gathered field EB), the
+FB-space E/G and J after 100 steps of lpa_cfg0, and <= 1e-5 for the particle
+moments (total charge, mean momentum relative to the RMS spread, RMS momentum).
+After the first step a component whose field is still ~1e-3 of the others
+(E_y of the linearly polarised laser) is compared against the norm of all six
+components instead of its own.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tests._cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _moments(u, w):
+    W = w.sum()
+    mean = (u * w).sum(1) / W
+    rms = np.sqrt(((u - mean[:, None]) ** 2 * w).sum(1) / W)
+    return W, mean, rms
+
+
+def _errors(mx, fp):
+    out = {}
+    EBm = mx.Solvers[0].Data["EB"].numpy().astype(np.complex128)
+    EBd = fp.Solvers[0].Data["EB"].numpy()
+    allnorm = np.linalg.norm(EBd.ravel())
+    for l, name in enumerate(("Ex", "Ey", "Ez", "Bx", "By", "Bz")):
+        d = np.linalg.norm((EBm[..., l] - EBd[..., l]).ravel())
+        out[name] = d / max(np.linalg.norm(EBd[..., l].ravel()), 1e-300)
+        out[name + "/all"] = d / allnorm
+    for key in ("EG_fb", "J_fb"):
+        out[key] = rel_l2(mx.Solvers[0].Data[key].numpy().astype(np.complex128), fp.Solvers[0].Data[key].numpy())
+    um = mx.Particles[0].host("momenta").astype(np.float64)
+    wm = mx.Particles[0].host("weights").astype(np.float64)
+    Wm, mm, rm = _moments(um, wm)
+    Wd, md, rd = _moments(fp.Particles[0].host("momenta"), fp.Particles[0].host("weights"))
+    out["charge"] = abs(Wm - Wd) / abs(Wd)
+    out["mean/rms"] = float(np.max(np.abs(mm - md) / rd))
+    out["rms"] = float(np.max(np.abs(rm - rd) / rd))
+    return out
+
+
+@pytest.mark.timeout(600)
+def test_mixed_against_fp64_cfg0_100_steps():
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    say("case: lpa_cfg0 mixed vs fp64, 100 steps")
+    spec = W.lpa_cfg0()
+    fp = W.build(G, W.with_precision(spec, "fp64"), 31)
+    mx = W.build(G, W.with_precision(spec, "mixed"), 31)
+    assert mx.Solvers[0].Data["EB"].dtype == np.complex64 and mx.Particles[0].Data["momenta"].dtype == np.float32
+    for i in range(1, 101):
+        fp.make_step(i)
+        mx.make_step(i)
+        if i in (1, 100):
+            e = _errors(mx, fp)
+            say(f"  step {i}: " + " ".join(f"{k}={v:.2e}" for k, v in e.items()))
+            assert mx.Particles[0].np == fp.Particles[0].np
+            for name in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
+                err = e[name] if i == 100 else e[name + "/all"]
+                assert err <= 1e-5, (i, name, err)
+            for key in ("EG_fb", "J_fb", "charge", "mean/rms", "rms"):
+                assert e[key] <= (1e-5 if i == 100 else 1e-4), (i, key, e[key])
+
+
+def test_mixed_moving_window():
+    """Window moves, plasma injection and absorption in mixed precision keep
+    the particle count of the FP64 run and stay within 1e-4 of its fields."""
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    say("case: lpa_small moving window mixed vs fp64, 21 steps")
+    spec = W.lpa_small(Nx=128, Nkr=32, nko=2, moving=True)
+    fp = W.build(G, W.with_precision(spec, "fp64"), 5)
+    mx = W.build(G, W.with_precision(spec, "mixed"), 5)
+    for i in range(1, 22):
+        fp.make_step(i)
+        mx.make_step(i)
+    e = _errors(mx, fp)
+    say("  step 21: " + " ".join(f"{k}={v:.2e}" for k, v in e.items()))
+    assert mx.Particles[0].np == fp.Particles[0].np and mx.Particles[1].np == fp.Particles[1].np
+    for key in ("EG_fb", "charge", "rms"):
+        assert e[key] <= 1e-4, (key, e[key])
