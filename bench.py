@@ -1,7 +1,7 @@
 """Benchmark of the PIC step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lpa_cfg1]
-                    [--no-cpu-baseline] [--cpu-steps 1]
+                    [--precision both|fp64|mixed] [--no-cpu-baseline] [--cpu-steps 1]
 
 A step is one ChimeraRun.make_step of the workload (sort/absorb, deposition,
 FB transforms, Poisson correction, PSATD, inverse transforms, gather + push,
@@ -10,8 +10,11 @@ steps, then K timed steps between two device synchronisations.  Prints ONE
 JSON line on stdout; progress goes to stderr.
 
 value = pushed-particle steps per second over the timed steps (electrons; the
-still ions are not pushed).  Per-kernel-family times come from HIP events that
-libchimera_hip records on its own stream around every launch.
+still ions are not pushed) of the FP64 run.  With --precision both (default)
+the same workload then runs in mixed precision (float32 momenta and weights,
+complex64 fields, FP32 MFMA transforms, FP64 positions) and its figures go in
+the "mixed" object of the same line.  Per-kernel-family times come from HIP
+events that libchimera_hip records on its own stream around every launch.
 """
 from __future__ import annotations
 
@@ -28,7 +31,8 @@ sys.path.insert(0, HERE)
 
 METRIC = "particle-steps/s + FB-transform ms/step, FP64 vs mixed; p50/p99 scan latency"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFS = 78.6
+FP64_PEAK_TFS = 78.6      # dense MFMA
+FP32_PEAK_TFS = 157.3
 FAMILIES = ["push_coords", "dep_curr", "dep_dens", "gather_push", "dht", "fft", "fb_ops", "psatd", "sort", "other"]
 
 
@@ -43,30 +47,47 @@ def prof_read(lib, fam):
     return t.value, n.value
 
 
-def algorithmic(fam, P, Gr, Gk, Nx, Nkr, M, K):
+def algorithmic(fam, P, Gr, Gk, Nx, Nkr, M, K, prec="fp64"):
     """(amount, unit, bound) per timed region of a family, from the per-unit
     figures: particles P (per launch), real-space nodes Gr = Nx*(Nkr+1)*M,
-    spectral elements Gk = Nx*Nkr*M (complex double = 16 B)."""
-    if fam == "gather_push":      # x,y,z + u read, u written, w read; the 6-component field read once
-        return P * 80 + 6 * Gr * 16, "B", "hbm"
+    spectral elements Gk = Nx*Nkr*M.  s_u = bytes per momentum component or
+    weight (8 FP64, 4 mixed), s_f = bytes per complex field value (16, 8);
+    positions are FP64 in both."""
+    s_u, s_f = (8, 16) if prec == "fp64" else (4, 8)
+    if fam == "gather_push":      # x read, u read and written, w read; the 6-component field read once
+        return P * (24 + 7 * s_u) + 6 * Gr * s_f, "B", "hbm"
     if fam == "dep_curr":         # fused push + J + rho: x, u, w read, x and x_half written;
-        return P * 104 + 4 * Gr * 32, "B", "hbm"   # J (3 components) and rho read+written once
-    if fam == "dep_dens":         # x,y,z + w read; rho read+written once
-        return P * 32 + Gr * 32, "B", "hbm"
-    if fam == "push_coords":      # x,u read; x, x_half written
-        return P * 96, "B", "hbm"
-    if fam == "psatd":            # E,G,J,grad rho_n, grad rho_n+1 read, E,G written, 10 table doubles
-        return Gk * (21 * 16 + 80), "B", "hbm"
+        return P * (72 + 4 * s_u) + 4 * Gr * 2 * s_f, "B", "hbm"   # J (3 components) and rho read+written once
+    if fam == "dep_dens":         # x + w read; rho read+written once
+        return P * (24 + s_u) + Gr * 2 * s_f, "B", "hbm"
+    if fam == "psatd":            # E,G,J,grad rho_n, grad rho_n+1 read, E,G written, 10 table reals
+        return Gk * (21 * s_f + 10 * s_u), "B", "hbm"
     return None
 
 
-def run_gpu(args):
+def mfma_rates(res, K):
+    """Achieved FLOP rate of the Hankel transforms (10 M transforms of 4 Nx Nkr^2
+    flops per step) and of the FB-space operators (4 (18 M - 3) Nx Nkr^2) against
+    the dense MFMA peak of the precision they run in."""
+    Nx, Nkr, M = res["Nx"], res["Nkr"], res["M"]
+    peak = FP64_PEAK_TFS if res["precision"] == "fp64" else FP32_PEAK_TFS
+    out = {}
+    for fam, flops in (("dht", 40.0 * M * Nx * Nkr ** 2), ("fb_ops", 4.0 * (18 * M - 3) * Nx * Nkr ** 2)):
+        ms = res["fams"][fam][0] / K
+        if ms > 0:
+            tf = flops / (ms * 1e-3) / 1e12
+            out[fam] = {"tflops": round(tf, 2), "peak": peak, "frac": round(tf / peak, 4),
+                        "flop_per_step": flops}
+    return out
+
+
+def run_gpu(args, precision):
     import chimera_amd as G
     from chimera_amd import workloads as W
     from chimera_amd._lib import lib
 
-    spec = W.with_precision(W.WORKLOADS[args.workload](), args.precision)
-    log(f"building {args.workload} ({args.precision})")
+    spec = W.with_precision(W.WORKLOADS[args.workload](), precision)
+    log(f"building {args.workload} ({precision})")
     t0 = time.time()
     run = W.build(G, spec, seed=args.seed)
     L = lib()
@@ -99,13 +120,13 @@ def run_gpu(args):
         if n:
             log(f"  {name:12s} {ms / args.steps:9.3f} ms/step  ({n} launches)")
     return dict(run=run, wall=wall, psteps=psteps, fams=fams, Nx=Nx, Nkr=Nkr, M=M, P=e.np, workload=args.workload,
-                Gr=Nx * A["Nr"] * M, Gk=Nx * Nkr * M)
+                Gr=Nx * A["Nr"] * M, Gk=Nx * Nkr * M, precision=precision)
 
 
 def roofline(res, K):
     ranked = sorted(((ms, name) for name, (ms, n) in res["fams"].items() if n), reverse=True)
     for ms, name in ranked:
-        alg = algorithmic(name, res["P"], res["Gr"], res["Gk"], res["Nx"], res["Nkr"], res["M"], K)
+        alg = algorithmic(name, res["P"], res["Gr"], res["Gk"], res["Nx"], res["Nkr"], res["M"], K, res["precision"])
         if alg is None:
             continue
         amount, unit, bound = alg
@@ -115,7 +136,8 @@ def roofline(res, K):
         per_launch = amount / launches_per_step if bound == "hbm" and name == "psatd" else amount
         achieved = per_launch / avg_s / 1e9
         return {"kernel": name, "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(name, res["workload"]),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_for(name, traffic_key(res["workload"], res["precision"])),
                 "avg_launch_ms": round(avg_s * 1e3, 4), "algorithmic_bytes_per_launch": int(per_launch),
                 "share_of_step": round(ms / sum(v[0] for v in res["fams"].values()), 3)}
     return None
@@ -157,7 +179,8 @@ def copy_calibration(gib):
 
 def traffic_for(kernel, workload):
     """HBM bytes per launch of a kernel family, measured with rocprofv3 --pmc and
-    calibrated against a plain copy (profiles/traffic.json), if recorded."""
+    calibrated against a plain copy (profiles/traffic.json), if recorded.
+    workload is the traffic_key (workload, or workload:mixed)."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     try:
         rec = json.load(open(path))
@@ -167,18 +190,24 @@ def traffic_for(kernel, workload):
     return int(e["bytes_per_launch"]) if e else None
 
 
-TRAFFIC_KERNELS = {"dep_curr": "k_push_deposit", "gather_push": "k_gather_push"}
+# kernels whose HBM traffic makes up a family's figure (the deposit family also
+# runs a small bucket sort of the slot descriptors, left out)
+TRAFFIC_KERNELS = {"dep_curr": ("k_push_deposit", "k_dep_combine"), "gather_push": ("k_gather_push",)}
 
 
-def summarize_traffic(root, workload, steps, out_dir):
+def traffic_key(workload, precision):
+    return workload if precision == "fp64" else f"{workload}:{precision}"
+
+
+def summarize_traffic(root, workload, steps, out_dir, precision="fp64"):
     """Turn four rocprofv3 --pmc runs into HBM bytes per launch.
 
     root holds pmc_copy_fetch/, pmc_copy_write/ (python bench.py --copy-calibration 1
-    under --pmc FETCH_SIZE, then WRITE_SIZE) and pmc_step_fetch/, pmc_step_write/
-    (python bench.py --steps S --warmup 0 --no-cpu-baseline, same counters).  Each
-    counter is scaled by (known copy bytes) / (what it read for the copy); a kernel's
-    figure is the mean over its last S launches (the launches of the S steps after
-    the initial half step)."""
+    under --pmc FETCH_SIZE, then WRITE_SIZE) and pmc_<precision>_fetch/,
+    pmc_<precision>_write/ (python bench.py --steps S --warmup 0 --no-cpu-baseline
+    --precision P, same counters).  Each counter is scaled by (known copy bytes) /
+    (what it read for the copy); a kernel's figure is the mean over its last S
+    launches (the launches of the S steps after the initial half step)."""
     import csv
 
     def rows(tag):
@@ -190,27 +219,32 @@ def summarize_traffic(root, workload, steps, out_dir):
     cf = [float(r["Counter_Value"]) for r in rows("copy_fetch") if "copyBuffer" in r["Kernel_Name"]][0]
     cw = [float(r["Counter_Value"]) for r in rows("copy_write") if "copyBuffer" in r["Kernel_Name"]][0]
     f_fetch, f_write = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
-    fetch, write = rows("step_fetch"), rows("step_write")
+    fetch, write = rows(f"{precision}_fetch"), rows(f"{precision}_write")
     rec = {"_calibration": {"copy_bytes": copy_bytes, "fetch_KiB": cf, "write_KiB": cw,
                             "fetch_factor": round(f_fetch, 4), "write_factor": round(f_write, 4)}}
     lines = []
-    for fam, kname in TRAFFIC_KERNELS.items():
-        fv = [float(r["Counter_Value"]) for r in fetch if kname in r["Kernel_Name"]]
-        wv = [float(r["Counter_Value"]) for r in write if kname in r["Kernel_Name"]]
-        if not fv or not wv:
-            continue
-        n = min(len(fv), steps)            # one launch per step for these kernels
-        fb = np.mean(fv[-n:]) * 1024 * f_fetch
-        wb = np.mean(wv[-n:]) * 1024 * f_write
-        rec[fam] = {"kernel": kname, "launches": n, "fetch_bytes": int(fb), "write_bytes": int(wb),
-                    "bytes_per_launch": int(fb + wb)}
-        lines.append(f"| {fam} | `{kname}` | {n} | {fb / 1e9:.3f} | {wb / 1e9:.3f} | {(fb + wb) / 1e9:.3f} |")
+    for fam, knames in TRAFFIC_KERNELS.items():
+        fb = wb = 0.0
+        parts = []
+        for kname in knames:
+            fv = [float(r["Counter_Value"]) for r in fetch if kname in r["Kernel_Name"]]
+            wv = [float(r["Counter_Value"]) for r in write if kname in r["Kernel_Name"]]
+            if not fv or not wv:
+                continue
+            n = min(len(fv), steps)            # one launch per step for these kernels
+            kf, kw = np.mean(fv[-n:]) * 1024 * f_fetch, np.mean(wv[-n:]) * 1024 * f_write
+            fb, wb = fb + kf, wb + kw
+            parts.append({"kernel": kname, "launches": n, "fetch_bytes": int(kf), "write_bytes": int(kw)})
+            lines.append(f"| {fam} | `{kname}` | {n} | {kf / 1e9:.3f} | {kw / 1e9:.3f} | {(kf + kw) / 1e9:.3f} |")
+        if parts:
+            rec[fam] = {"kernels": parts, "fetch_bytes": int(fb), "write_bytes": int(wb),
+                        "bytes_per_launch": int(fb + wb)}
     os.makedirs(out_dir, exist_ok=True)
     path = os.path.join(out_dir, "traffic.json")
     allrec = {}
     if os.path.exists(path):
         allrec = json.load(open(path))
-    allrec[workload] = rec
+    allrec[traffic_key(workload, precision)] = rec
     json.dump(allrec, open(path, "w"), indent=1)
     return rec, lines
 
@@ -222,7 +256,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="lpa_cfg1")
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -231,8 +265,9 @@ def main():
                     help="write profiles/traffic.json from the four rocprofv3 --pmc runs under DIR")
     args = ap.parse_args()
     if args.summarize_traffic:
+        prec = "fp64" if args.precision == "both" else args.precision
         rec, lines = summarize_traffic(args.summarize_traffic, args.workload, args.steps,
-                                       os.path.join(HERE, "profiles"))
+                                       os.path.join(HERE, "profiles"), prec)
         print(json.dumps(rec, indent=1))
         print("\n".join(lines))
         return
@@ -243,26 +278,44 @@ def main():
         print(f"bench.py measures one GPU: the PIC step of one simulation runs on one MI355X and this build "
               f"has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
-    res = run_gpu(args)
     K = args.steps
-    value = res["psteps"] / res["wall"]
-    fb_ms = (res["fams"]["dht"][0] + res["fams"]["fft"][0]) / K
-    ops_ms = res["fams"]["fb_ops"][0] / K
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    summaries = {}
+    for prec in precs:
+        res = run_gpu(args, prec)
+        value = res["psteps"] / res["wall"]
+        summaries[prec] = {
+            "value": round(value, 1), "ms_per_step": round(res["wall"] / K * 1e3, 3),
+            "fb_transform_ms_per_step": round((res["fams"]["dht"][0] + res["fams"]["fft"][0]) / K, 3),
+            "spectral_operator_ms_per_step": round(res["fams"]["fb_ops"][0] / K, 3),
+            "kernel_ms_per_step": {k: round(v[0] / K, 3) for k, v in res["fams"].items() if v[1]},
+            "roofline": roofline(res, K), "mfma": mfma_rates(res, K), "P": res["P"],
+            "Nx": res["Nx"], "Nkr": res["Nkr"], "M": res["M"]}
+        del res["run"]
+    main_prec = precs[0]
+    m = summaries[main_prec]
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
-        "warmup": args.warmup, "ms_per_step": round(res["wall"] / K * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": METRIC, "value": m["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if main_prec == "fp64" else "f32+c64 (f64 positions)",
         "data": "synthetic: seeded FixedCell 2x2x4 plasma (n=0.005 n_c) + Gaussian laser, random draws from NumPy",
-        "config": {"workload": args.workload, "Nx": res["Nx"], "Nkr": res["Nkr"], "modes": res["M"],
-                   "electrons": res["P"], "precision": "fp64", "moving_window": True},
-        "fb_transform_ms_per_step": round(fb_ms, 3),
-        "spectral_operator_ms_per_step": round(ops_ms, 3),
-        "kernel_ms_per_step": {k: round(v[0] / K, 3) for k, v in res["fams"].items() if v[1]},
-        "roofline": roofline(res, K),
+        "config": {"workload": args.workload, "Nx": m["Nx"], "Nkr": m["Nkr"], "modes": m["M"],
+                   "electrons": m["P"], "precision": main_prec, "moving_window": True},
+        "fb_transform_ms_per_step": m["fb_transform_ms_per_step"],
+        "spectral_operator_ms_per_step": m["spectral_operator_ms_per_step"],
+        "kernel_ms_per_step": m["kernel_ms_per_step"],
+        "roofline": m["roofline"],
+        "mfma": m["mfma"],
     }
-    del res["run"]
+    if len(precs) == 2:
+        mx = summaries["mixed"]
+        out["mixed"] = {k: mx[k] for k in ("value", "ms_per_step", "fb_transform_ms_per_step",
+                                           "spectral_operator_ms_per_step", "kernel_ms_per_step", "roofline",
+                                           "mfma")}
+        out["mixed"]["dtype"] = "f32 momenta/weights, c64 fields, FP32 MFMA; f64 positions and tables"
+        out["mixed"]["speedup_vs_fp64"] = round(mx["value"] / m["value"], 3)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args, res["P"])
+        out["cpu_baseline"] = cpu_baseline(args, m["P"])
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -144,14 +144,16 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// R: precision of momenta, weights, tiles and grids (double, or float in mixed
+// R: precision of momenta, weights and grids (double, or float in mixed
 // precision).  Positions, the push and the cell geometry stay FP64; each
-// corner contribution is formed in FP64 and accumulated in R.
+// corner contribution is formed in FP64, summed per cell in R, and the LDS tile
+// is FP64 in both precisions (FP64 LDS atomics measured far cheaper here than
+// FP32 ones under this access pattern), rounded to R when it leaves the block.
 template <int NMOD, typename R>
 __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     typedef cplx_t<R> C;
     constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
-    __shared__ R tile[DEP_LDS_BYTES / sizeof(R)];
+    __shared__ double tile[DEP_LDS_BYTES / sizeof(double)];
     __shared__ int red[8][DEP_THREADS / 64];
     __shared__ int box[8];
     const int tid = threadIdx.x;
@@ -252,10 +254,10 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     const int TXJ = box[1] >= box[0] ? box[1] - box[0] + 2 : 0, TRJ = box[3] >= box[2] ? box[3] - box[2] + 2 : 0;
     const int TXR = box[5] >= box[4] ? box[5] - box[4] + 2 : 0, TRR = box[7] >= box[6] ? box[7] - box[6] + 2 : 0;
     const int64_t nJ = (int64_t)TXJ * TRJ * 3 * nmod * 2, nR = (int64_t)TXR * TRR * nmod * 2;
-    const bool tileJ = doJ && nJ > 0 && nJ * (int64_t)sizeof(R) <= DEP_LDS_BYTES;
-    const bool tileR = doR && nR > 0 && (nR + (tileJ ? nJ : 0)) * (int64_t)sizeof(R) <= DEP_LDS_BYTES;
-    R* tJ = tile;
-    R* tR = tile + (tileJ ? nJ : 0);
+    const bool tileJ = doJ && nJ > 0 && nJ * (int64_t)sizeof(double) <= DEP_LDS_BYTES;
+    const bool tileR = doR && nR > 0 && (nR + (tileJ ? nJ : 0)) * (int64_t)sizeof(double) <= DEP_LDS_BYTES;
+    double* tJ = tile;
+    double* tR = tile + (tileJ ? nJ : 0);
     const int64_t nz = (tileJ ? nJ : 0) + (tileR ? nR : 0);
     for (int64_t i = tid; i < nz; i += DEP_THREADS) tile[i] = 0.0;
     __syncthreads();
@@ -271,7 +273,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
         const bool isJ = l < 3;
         if ((isJ && !doJ) || (!isJ && !doR)) continue;
         const GridGeom& gg = isJ ? a.gJ : a.gR;
-        R* tile_l = isJ ? tJ + (int64_t)l * nmod * TRJ * TXJ * 2 : tR;
+        double* tile_l = isJ ? tJ + (int64_t)l * nmod * TRJ * TXJ * 2 : tR;
         C* grid_l = isJ ? a.J + l * sl : a.R;
         const int bx0 = isJ ? box[0] : box[4], br0 = isJ ? box[2] : box[6];
         const int TX = isJ ? TXJ : TXR, TR = isJ ? TRJ : TRR;
@@ -286,8 +288,8 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int t = ((m * TR + (tr + (q >> 1))) * TX + (tx + (q & 1))) * 2;
-                    lds_add(tile_l + t, acc[m][q].x);
-                    lds_add(tile_l + t + 1, acc[m][q].y);
+                    lds_add(tile_l + t, (double)acc[m][q].x);
+                    lds_add(tile_l + t + 1, (double)acc[m][q].y);
                 }
         };
         double P[3][DEP_PPT], V[3][DEP_PPT], Wt[3][DEP_PPT];   // Wt: row 0 only
@@ -356,12 +358,13 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
             const int64_t n = nJ / 2;
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
                 const int64_t rest = i / TXJ;
-                sl[((rest / TRJ) * DEP_SLOT_R + rest % TRJ) * DEP_SLOT_W + i % TXJ] = cmake<R>(tJ[2 * i], tJ[2 * i + 1]);
+                sl[((rest / TRJ) * DEP_SLOT_R + rest % TRJ) * DEP_SLOT_W + i % TXJ] =
+                    cmake<R>((R)tJ[2 * i], (R)tJ[2 * i + 1]);
             }
         } else if (tileJ) {
             const int64_t n = nJ / 2;
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
-                const R re = tJ[2 * i], im = tJ[2 * i + 1];
+                const R re = (R)tJ[2 * i], im = (R)tJ[2 * i + 1];
                 if (re == 0 && im == 0) continue;
                 const int64_t xx = i % TXJ, rest = i / TXJ, rr = rest % TRJ, lm = rest / TRJ;
                 R* dst = reinterpret_cast<R*>(a.J + (box[0] + xx) + (box[2] + rr) * sr + lm * sm);
@@ -378,12 +381,13 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
             const int64_t n = nR / 2;
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
                 const int64_t rest = i / TXR;
-                sl[((rest / TRR) * DEP_SLOT_R + rest % TRR) * DEP_SLOT_W + i % TXR] = cmake<R>(tR[2 * i], tR[2 * i + 1]);
+                sl[((rest / TRR) * DEP_SLOT_R + rest % TRR) * DEP_SLOT_W + i % TXR] =
+                    cmake<R>((R)tR[2 * i], (R)tR[2 * i + 1]);
             }
         } else if (tileR) {
             const int64_t n = nR / 2;
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
-                const R re = tR[2 * i], im = tR[2 * i + 1];
+                const R re = (R)tR[2 * i], im = (R)tR[2 * i + 1];
                 if (re == 0 && im == 0) continue;
                 const int64_t xx = i % TXR, rest = i / TXR, rr = rest % TRR, m = rest / TRR;
                 R* dst = reinterpret_cast<R*>(a.R + (box[4] + xx) + (box[6] + rr) * sr + m * sm);
