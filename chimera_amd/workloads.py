@@ -8,7 +8,12 @@ instantiates them with any module exposing Solver / Specie / ChimeraRun.
   lpa_cfg1   production drop-in: Nx=4096 x Nkr=256, modes m=0..2, 16.7M
              electrons, a0=2, waist 8, duration 6, moving window (plasma
              injected on the right, absorbing layer on the left), 200 steps
+  lpa_cfg2   the cfg1 physics at the precision-form size: Nx=8192 x Nkr=512,
+             modes m=0..3, 67.0M electrons, moving window, 500 steps
   lpa_small  the cfg0 physics on a reduced grid, for quick parity runs
+  beam       one relativistic bunch with space charge in an undulator, in a
+             window moving with it; beam_scan draws the 64 bunches of a scan
+             (Nx 256-2048 x Nkr 32-256, modes m=0,1, 1e4-2e6 particles, 300 steps)
 
 Plasma density n0 = 0.005 n_c (as doc/tests/lpa-testrun.py:27) and thermal
 spread 1e-3 m_e c.  Random draws come from NumPy's legacy generator, seeded
@@ -57,20 +62,92 @@ def lpa_cfg1():
                absorb=256)
 
 
+def lpa_cfg2():
+    return lpa(8192, 512, 3, a0=2.0, waist=8.0, duration=6.0, x0_from_right=30.0, steps=500, moving=True,
+               absorb=512)
+
+
 def lpa_small(Nx=128, Nkr=32, nko=1, moving=False, steps=10):
     return lpa(Nx, Nkr, nko, a0=1.0, waist=2.0, duration=1.5, x0_from_right=2.5, steps=steps, moving=moving,
                absorb=16 if moving else 0)
 
 
-WORKLOADS = {"lpa_cfg0": lpa_cfg0, "lpa_cfg1": lpa_cfg1, "lpa_small": lpa_small}
+WORKLOADS = {"lpa_cfg0": lpa_cfg0, "lpa_cfg1": lpa_cfg1, "lpa_cfg2": lpa_cfg2, "lpa_small": lpa_small}
+
+
+def beam(Nx, Nkr, gamma, sig_x, sig_r, emittance, n_target, K, lambda_u, dens=1e-3, steps=300):
+    """A relativistic electron bunch with space charge in an analytic
+    undulator, in a window that follows it (the layout of the reference's
+    space-charge demo, doc/space-charge-demo(vs_ocelot).ipynb, with the
+    undulator device of doc/tests/fel-testrun.py).  The Gaussian bunch (rms
+    length sig_x, rms radius sig_r) fills the box +-3.5 sig_x, 0..3.5 sig_r on
+    a FixedCell lattice sized for about n_target particles; the transverse
+    momentum spread is emittance / sig_r, the longitudinal one 1e-3 of the mean.
+    Modes m = 0, 1; dt = dx; the undulator amplitude is K k_u (k_u = 2 pi / lambda_u)."""
+    ncx = max(8, min(int(0.7 * Nx), int(round(np.sqrt(n_target / 4.0 * Nx / Nkr)))))
+    ncr = max(4, min(int(0.7 * Nkr), int(round(n_target / 4.0 / ncx))))
+    per = max(1.0, n_target / (4.0 * ncx * ncr))
+    fx = max(1, int(round(np.sqrt(per))))
+    fr = max(1, int(round(per / fx)))
+    dx = 7.0 * sig_x / ncx
+    dr = 3.5 * sig_r / ncr
+    xmin, xmax, rmax = -0.5 * Nx * dx, 0.5 * Nx * dx, Nkr * dr
+    solver_in = {"Grid": (xmin, xmax, rmax, dx, dr), "TimeStep": dx, "MaxAzimuthMode": 1,
+                 "Features": ("SpaceCharge",)}
+    pz0 = float(np.sqrt(gamma ** 2 - 1.0))
+    pt = emittance / sig_r
+    ku = 2.0 * np.pi / lambda_u
+    undulator = np.array([K * ku, lambda_u, xmin - 2.0 * lambda_u, 1.0e12])
+    beam_in = {"Grid": (xmin, xmax, rmax, dx, dr), "TimeStep": dx, "Charge": -1.0, "Mass": 1.0, "Density": dens,
+               "FixedCell": (fx, fr, 4), "MomentaMeans": (pz0, 0.0, 0.0),
+               "MomentaSpreads": (1e-3 * pz0, pt, pt), "Features": ("NoSorting",),
+               "Devices": (("undul_analytic", undulator),)}
+    frame = {"TimeStep": dx, "Steps": 1, "Velocity": 1.0, "Features": ("NoSorting", "Staged")}
+    return {"kind": "beam", "solver": solver_in, "beam": beam_in, "frames": (frame,), "steps": steps,
+            "domain": (-3.5 * sig_x, 3.5 * sig_x, 0.0, 3.5 * sig_r), "sigmas": (sig_x, sig_r),
+            "params": {"Nx": Nx, "Nkr": Nkr, "gamma": gamma, "sig_x": sig_x, "sig_r": sig_r,
+                       "emittance": emittance, "n_target": n_target, "K": K, "lambda_u": lambda_u}}
+
+
+def beam_scan(n=64, seed=2026, steps=300):
+    """n bunches with the parameters of a scan drawn from one seed: gamma
+    uniform 50-500, sig_x uniform 1-10, sig_r uniform 0.5-5, normalised
+    emittance log-uniform 0.1-2, particle count log-uniform 1e4-2e6, Nx a power
+    of two in 256-2048, Nkr uniform 32-256, K uniform 1-3, undulator period
+    uniform 100-1000."""
+    rng = np.random.RandomState(seed)
+    out = []
+    for _ in range(n):
+        out.append(beam(Nx=int(2 ** rng.randint(8, 12)), Nkr=int(rng.randint(32, 257)),
+                        gamma=float(rng.uniform(50, 500)), sig_x=float(rng.uniform(1, 10)),
+                        sig_r=float(rng.uniform(0.5, 5)), emittance=float(np.exp(rng.uniform(np.log(0.1), np.log(2)))),
+                        n_target=float(np.exp(rng.uniform(np.log(1e4), np.log(2e6)))), K=float(rng.uniform(1, 3)),
+                        lambda_u=float(rng.uniform(100, 1000)), steps=steps))
+    return out
+
+
+def build_beam(mod, spec, seed=12345):
+    """Instantiate a beam workload with mod.Solver / mod.Specie / mod.ChimeraRun."""
+    spec = copy.deepcopy(spec)
+    solver = mod.Solver(spec["solver"])
+    bunch = mod.Specie(spec["beam"])
+    sx, sr = spec["sigmas"]
+
+    def profile(x, y, z):
+        return np.exp(-0.5 * x ** 2 / sx ** 2 - 0.5 * (y ** 2 + z ** 2) / sr ** 2)
+
+    np.random.seed(seed)
+    bunch.add_particles(*bunch.gen_parts(Domain=spec["domain"], ProfileFunc=profile))
+    return mod.ChimeraRun({"Solvers": (solver,), "Particles": (bunch,), "MovingFrames": spec["frames"]})
 
 
 def with_precision(spec, precision):
     """A copy of spec whose solver and species run in `precision` ('fp64' or
     'mixed'; the oracle ignores the key and always runs FP64)."""
     spec = copy.deepcopy(spec)
-    for key in ("solver", "electrons", "ions"):
-        spec[key]["Precision"] = precision
+    for key in ("solver", "electrons", "ions", "beam"):
+        if key in spec:
+            spec[key]["Precision"] = precision
     return spec
 
 
