@@ -165,6 +165,73 @@ def cpu_baseline(args, P_gpu):
                       f"driven by oracle/driver.py, OMP_NUM_THREADS={threads}"}
 
 
+def scan_bench(args):
+    """The batched scan: args.scan_n bunches of workloads.beam_scan (uneven
+    grids and particle counts, modes m=0,1, space charge, undulator, moving
+    window), args.scan_steps steps each.  Four runs: one simulation after
+    another with eager steps (the plain path) and all at once with recorded,
+    interleaved steps (the batched path), each in FP64 and in mixed precision.
+    value = particle-steps/s over the stepping (building not timed);
+    time-to-result of each simulation from the start of the stepping."""
+    from chimera_amd import serve as S
+    from chimera_amd import workloads as W
+    specs = W.beam_scan(args.scan_n, seed=args.seed % 100000, steps=args.scan_steps)
+    out = {"workload": f"beam_scan: {args.scan_n} bunches x {args.scan_steps} steps",
+           "grids": f"Nx {min(sp['params']['Nx'] for sp in specs)}-{max(sp['params']['Nx'] for sp in specs)} x "
+                    f"Nkr {min(sp['params']['Nkr'] for sp in specs)}-{max(sp['params']['Nkr'] for sp in specs)}, "
+                    f"modes m=0,1"}
+    runs = [("plain_fp64", "fp64", False), ("plain_mixed", "mixed", False), ("batched_fp64", "fp64", True),
+            ("batched_mixed", "mixed", True)]
+    for label, prec, batched in runs:
+        if args.scan_only and label not in args.scan_only.split(","):
+            continue
+        reqs = [S.Request(f"beam{k}", sp, 1000 + k, sp["steps"], prec) for k, sp in enumerate(specs)]
+        log(f"scan {label}: {len(reqs)} simulations")
+        rate, ttr, wall = S.run_scan(reqs, batched=batched, graphs=batched)
+        parts = sum(r.particle_steps for r in reqs) // args.scan_steps
+        out[label] = {"value": round(rate, 1), "unit": "particle-steps/s", "stepping_s": round(wall, 3),
+                      "p50_s": round(S.percentile(ttr, 50), 4), "p99_s": round(S.percentile(ttr, 99), 4),
+                      "particles": int(parts)}
+        log(f"scan {label}: {rate:.3g} particle-steps/s, {wall:.2f} s, p50 {S.percentile(ttr, 50):.3f} s "
+            f"p99 {S.percentile(ttr, 99):.3f} s ({parts} particles)")
+    if "plain_fp64" in out and "batched_mixed" in out:
+        out["batched_mixed_vs_plain_fp64"] = round(out["batched_mixed"]["value"] / out["plain_fp64"]["value"], 3)
+    return out
+
+
+def serve_bench(args):
+    """The request loop: a Poisson stream of args.serve_rate requests per second
+    for args.serve seconds (mix and scan parameters of serve.poisson_requests),
+    served with continuous batching until every request has finished; once in
+    FP64 and once in mixed precision.  Throughput = particle-steps of the
+    finished requests over the wall time from the first arrival to the last
+    completion; time-to-result = completion - arrival (queueing, building and
+    stepping)."""
+    from chimera_amd import serve as S
+    out = {"rate_per_s": args.serve_rate, "arrival_window_s": args.serve, "mix": S.MIX}
+    for prec in ("fp64", "mixed"):
+        reqs = S.poisson_requests(args.serve_rate, args.serve, args.seed % 100000, prec)
+        log(f"serve {prec}: {len(reqs)} requests over {args.serve} s")
+        server = S.Server(log=log)
+        try:
+            wall = S.serve(reqs, server, log=log)
+        finally:
+            server.close()
+        fin = server.finished
+        ttr = [r.t_done - r.t_arrival_host for r in fin]
+        kinds = {}
+        for r in fin:
+            kinds.setdefault(r.name.split("#")[0], []).append(r.t_done - r.t_arrival_host)
+        rec = {"requests": len(reqs), "finished": len(fin), "wall_s": round(wall, 2),
+               "value": round(sum(r.particle_steps for r in fin) / wall, 1), "unit": "particle-steps/s",
+               "p50_s": round(S.percentile(ttr, 50), 3), "p99_s": round(S.percentile(ttr, 99), 3),
+               "by_kind": {k: {"n": len(v), "p50_s": round(S.percentile(v, 50), 3),
+                               "p99_s": round(S.percentile(v, 99), 3)} for k, v in sorted(kinds.items())}}
+        log(f"serve {prec}: {rec}")
+        out[prec] = rec
+    return out
+
+
 def copy_calibration(gib):
     """One device-to-device copy of a known size: the FETCH_SIZE / WRITE_SIZE it
     reads under rocprofv3 --pmc calibrate the counters of the PIC kernels."""
@@ -261,6 +328,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB gibibytes device to device once (reference kernel for the HBM counters)")
+    ap.add_argument("--scan", action="store_true", help="also run the batched bunch scan (scan object)")
+    ap.add_argument("--scan-n", type=int, default=64)
+    ap.add_argument("--scan-steps", type=int, default=300)
+    ap.add_argument("--scan-only", default="", help="comma list of scan runs (plain_fp64, plain_mixed, "
+                                                    "batched_fp64, batched_mixed) and no PIC-step bench")
+    ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
+                    help="only run the request loop for SECONDS of Poisson arrivals (serve object)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--summarize-traffic", metavar="DIR",
                     help="write profiles/traffic.json from the four rocprofv3 --pmc runs under DIR")
     args = ap.parse_args()
@@ -278,6 +353,12 @@ def main():
         print(f"bench.py measures one GPU: the PIC step of one simulation runs on one MI355X and this build "
               f"has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.scan_only:
+        print(json.dumps({"scan": scan_bench(args)}), flush=True)
+        return
+    if args.serve > 0:
+        print(json.dumps({"serve": serve_bench(args)}), flush=True)
+        return
     K = args.steps
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     summaries = {}
@@ -314,6 +395,8 @@ def main():
                                            "mfma")}
         out["mixed"]["dtype"] = "f32 momenta/weights, c64 fields, FP32 MFMA; f64 positions and tables"
         out["mixed"]["speedup_vs_fp64"] = round(mx["value"] / m["value"], 3)
+    if args.scan:
+        out["scan"] = scan_bench(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, m["P"])
     else:

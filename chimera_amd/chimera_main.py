@@ -81,6 +81,8 @@ class ChimeraRun:
         self.gather_push(0, dt_frac=0.5)
 
     def make_step(self, istep):
+        if self.Solvers and self.Solvers[0].frame is not None:
+            call("chimera_frame_step", self.Solvers[0].frame.ptr, *self._frame_shifts(istep))
         self.frame_act(istep)                                              # chimera_main.py:82-92
         if self.SortEvery and np.mod(istep, self.SortEvery) == 0 and self.Solvers:
             for species in self.Particles:
@@ -153,7 +155,7 @@ class ChimeraRun:
                 continue
             call(self._pfn("chimera_soa_push_deposit"), sp.ptr("coords"), sp.ptr("coords_halfstep"), sp.ptr("momenta"),
                  sp.ptr("weights"), sp.ld, sp.np, float(sp.Args["TimeStep"]), g[0], float(leftX_rho), *g[1:],
-                 curr.ptr if curr is not None else None, dens.ptr if dens is not None else None)
+                 curr.ptr if curr is not None else None, dens.ptr if dens is not None else None, solver.frame_ptr(0))
 
     # ------------------------------------------------------------------ sources
     def project_current(self):
@@ -185,7 +187,7 @@ class ChimeraRun:
                 continue
             call(self._pfn("chimera_soa_push_deposit"), species.ptr("coords"), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
-                 J.ptr, None)
+                 J.ptr, None, None)
 
     def dep_dens(self, solver, component="coords"):
         R = solver.Data["Rho"]
@@ -199,7 +201,7 @@ class ChimeraRun:
                 continue
             call(self._pfn("chimera_soa_push_deposit"), species.ptr(component), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
-                 None, R.ptr)
+                 None, R.ptr, None)
 
     def dep_bg(self, solver):
         B = solver.Data["BckGrndRho"]
@@ -210,7 +212,7 @@ class ChimeraRun:
                 continue
             call(self._pfn("chimera_soa_push_deposit"), species.ptr("coords"), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
-                 None, B.ptr)
+                 None, B.ptr, None)
 
     # ------------------------------------------------------------------ fields
     def update_fields(self):
@@ -237,13 +239,14 @@ class ChimeraRun:
             A = species.Args
             und = species.undulator_params()
             push_dt = float(A["push_fact"] * dt_frac * A["TimeStep"])
+            frame = None
             if self.Solvers and "NoPush" not in self.Solvers[0].Args["Features"]:
                 solver = self.Solvers[0]
-                EB, g = solver.Data["EB"].ptr, solver.geom_args()
+                EB, g, frame = solver.Data["EB"].ptr, solver.geom_args(), solver.frame_ptr(1)
             else:
                 EB, g = 0, (0.0, 0, 1.0, 1.0, 0.0, -1.0, 1, 1, 0)   # rmax < 0: nothing gathered
             call(self._pfn("chimera_soa_gather_push"), species.ptr("coords"), species.ptr("momenta"), species.ptr("weights"),
-                 species.ld, species.np, EB, *g, push_dt, und.ctypes.data if und is not None else None)
+                 species.ld, species.np, EB, *g, push_dt, und.ctypes.data if und is not None else None, frame)
 
     # ------------------------------------------------------------------ moving frames (chimera_main.py:250-318)
     def damp_fields(self, wind):
@@ -299,6 +302,80 @@ class ChimeraRun:
                 self.damp_plasma(wind)
                 self.postframe_corr(wind)
             elif act == "stage2" and "Staged" in wind["Features"]:
+                self.move_frame(wind)
+
+    # ------------------------------------------------------------------ device frame and captured steps
+    def graph_blockers(self):
+        """Why a step of this run cannot be recorded once and replayed (an
+        empty list when it can): every step must launch the same work on the
+        same arrays, so nothing may add, absorb or re-sort particles, and the
+        window may move only by the same amount every step."""
+        out = []
+        if len(self.Solvers) != 1:
+            out.append("needs exactly one field solver")
+        if self.SortEvery:
+            out.append("SortEvery re-sorts on some steps")
+        for sp in self.Particles:
+            if "Xchunked" in sp.Args and "NoSorting" not in sp.Args["Features"]:
+                out.append("Xchunked sorting runs on some steps")
+        if len(self.MovingFrames) > 1:
+            out.append("more than one moving frame")
+        for wind in self.MovingFrames:
+            if wind["Steps"] != 1 or wind["TimeActive"][0] > 1 or wind["TimeActive"][1] != np.inf:
+                out.append("the window does not move on every step")
+            if "AddPlasma" in wind or "AbsorbLayer" in wind:
+                out.append("plasma injection or absorption changes the particle count")
+            if self.Solvers and "StillAsBackground" in self.Solvers[0].Args["Features"]:
+                out.append("the still background is re-deposited after each move")
+        return out
+
+    def enable_device_frame(self):
+        """Keep the window edge in a device frame that a kernel advances at the
+        start of every step (frame[0]: after the stage-1 move, frame[1]: after
+        the stage-2 move), so that a recorded step replays with the moving
+        window.  The host Args keep moving exactly as before."""
+        from .device import DeviceArray
+        blockers = self.graph_blockers()
+        if blockers:
+            raise ValueError("device frame not possible: " + "; ".join(blockers))
+        s = self.Solvers[0]
+        if s.frame is None:
+            x = float(s.Args["leftX"])
+            s.frame = DeviceArray.from_numpy(np.array([x, x]), np.float64)
+
+    def _frame_shifts(self, istep):
+        """(shift at stage 1, shift at stage 2) of step istep, as frame_act moves the window."""
+        s1 = s2 = 0.0
+        for wind in self.MovingFrames:
+            if istep < wind["TimeActive"][0] or istep > wind["TimeActive"][1]:
+                continue
+            if np.mod(istep - wind["TimeActive"][0], wind["Steps"]) != 0:
+                continue
+            s1 = float(wind["shiftX"])
+            if "Staged" in wind["Features"]:
+                s2 = float(wind["shiftX"])
+        return s1, s2
+
+    def host_step(self, istep):
+        """The host-side effects of make_step(istep) -- window moves and the
+        swap of the two density-gradient buffers -- without launching anything:
+        the device work of the step is replayed from a recorded graph."""
+        for wind in self.MovingFrames:
+            if istep < wind["TimeActive"][0] or istep > wind["TimeActive"][1]:
+                continue
+            if np.mod(istep - wind["TimeActive"][0], wind["Steps"]) != 0:
+                continue
+            self.move_frame(wind)
+        for solver in self.Solvers:
+            if "SpaceCharge" in solver.Args["Features"]:
+                D = solver.Data
+                D["gradRho_fb_prv"], D["gradRho_fb_nxt"] = D["gradRho_fb_nxt"], D["gradRho_fb_prv"]
+        for wind in self.MovingFrames:
+            if istep < wind["TimeActive"][0] or istep > wind["TimeActive"][1]:
+                continue
+            if np.mod(istep - wind["TimeActive"][0], wind["Steps"]) != 0:
+                continue
+            if "Staged" in wind["Features"]:
                 self.move_frame(wind)
 
     def chunk_particles(self, istep=0):

@@ -29,8 +29,11 @@ void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 int32_t fail(int32_t code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int32_t hip_fail(hipError_t e, const char* what, const char* file, int line);
 
+// Stream of the calling thread's current context (chimera_context_bind).
 hipStream_t stream();
 void set_stream(hipStream_t s);
+// True while the current context records a graph (chimera_capture_begin).
+bool capturing();
 
 #define CH_HIP(expr)                                                                  \
     do {                                                                              \
@@ -84,16 +87,21 @@ class Stage {
     bool any_host_ = false;
 };
 
-// Scratch device memory owned by the library, grown on demand and reused
-// across calls on the same stream (the stream orders every use).
+// Scratch device memory of the current context, grown on demand and reused
+// across calls on its stream (the stream orders every use).  It cannot grow
+// while the context captures a step or while graphs captured on it are alive.
 void* scratch(int slot, size_t bytes);
 // Byte fill of device memory by a kernel on the library stream (no hipMemsetAsync,
 // so that a captured graph replays it correctly).
 int32_t fill_bytes(void* p, int value, size_t bytes);
 constexpr int SCRATCH_SLOTS = 24;
 
-// Device copy of a small host table (job lists, twiddles), cached by content.
+// Device copy of a small host table (job lists), cached by content in the
+// current context and freed with it.
 const void* device_table(const void* host, size_t bytes);
+// Device copy of a constant table that any context may read at any time (FFT
+// twiddles), cached by content for the life of the process.
+const void* shared_table(const void* host, size_t bytes);
 
 inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 

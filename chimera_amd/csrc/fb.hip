@@ -38,8 +38,8 @@ const double2* fft_twiddles(int N) {
         const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)N;
         h[k] = make_double2((double)cosl(a), (double)sinl(a));
     }
-    const double2* d = static_cast<const double2*>(device_table(h.data(), h.size() * sizeof(double2)));
-    cache[N] = d;
+    const double2* d = static_cast<const double2*>(shared_table(h.data(), h.size() * sizeof(double2)));
+    if (d) cache[N] = d;
     return d;
 }
 
@@ -54,8 +54,8 @@ const float2* fft_twiddles_f(int N) {
         const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)N;
         h[k] = make_float2((float)cosl(a), (float)sinl(a));
     }
-    const float2* d = static_cast<const float2*>(device_table(h.data(), h.size() * sizeof(float2)));
-    cache[N] = d;
+    const float2* d = static_cast<const float2*>(shared_table(h.data(), h.size() * sizeof(float2)));
+    if (d) cache[N] = d;
     return d;
 }
 
@@ -165,11 +165,13 @@ template int32_t launch_fft<float>(FftArgsT<float>);
 
 // Per-x phase exp(s*i*kx*leftX) (fb_io.f90:40, :120, :202), on the device.
 template <typename R>
-__global__ void k_phase(cplx_t<R>* __restrict__ ph, const double* __restrict__ kx, double leftX, double s, int n) {
+__global__ void k_phase(cplx_t<R>* __restrict__ ph, const double* __restrict__ kx, double leftX,
+                        const double* __restrict__ leftX_dev, double s, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double lx = leftX_dev ? *leftX_dev : leftX;
     double sn, cs;
-    sincos(leftX * kx[i], &sn, &cs);
+    sincos(lx * kx[i], &sn, &cs);
     ph[i] = cmake<R>((R)cs, (R)(s * sn));
 }
 
@@ -186,13 +188,13 @@ __global__ void k_phase_inv(cplx_t<R>* __restrict__ ph, const double* __restrict
 }
 
 template <typename R>
-int32_t make_phase(cplx_t<R>* ph, const double* kx, double leftX, double s, int n) {
-    hipLaunchKernelGGL((k_phase<R>), blocks_for(n, 256), 256, 0, stream(), ph, kx, leftX, s, n);
+int32_t make_phase(cplx_t<R>* ph, const double* kx, double leftX, double s, int n, const double* leftX_dev) {
+    hipLaunchKernelGGL((k_phase<R>), blocks_for(n, 256), 256, 0, stream(), ph, kx, leftX, leftX_dev, s, n);
     CH_LAUNCH_CHECK("k_phase");
     return ST_OK;
 }
-template int32_t make_phase<double>(double2*, const double*, double, double, int);
-template int32_t make_phase<float>(float2*, const double*, double, double, int);
+template int32_t make_phase<double>(double2*, const double*, double, double, int, const double*);
+template int32_t make_phase<float>(float2*, const double*, double, double, int, const double*);
 
 // ================================================================== FB GEMM
 
@@ -865,7 +867,7 @@ void build_graddiv2_jobs(std::vector<FbJobT<R>>& jobs, FbViewW<R> v, FbView<R> s
 
 template <typename R>
 int32_t fb_forward_t(cplx_t<R>* out, cplx_t<R>* in, double leftX, const double* kx_dev, const R* In_dev, int nkx,
-                     int nr, int nmodes, int nkr, int ncomp, const R* scale_tab, int fold) {
+                     int nr, int nmodes, int nkr, int ncomp, const R* scale_tab, int fold, const double* leftX_dev) {
     typedef cplx_t<R> C;
     const size_t tmp_elems = (size_t)nkx * nkr * nmodes * ncomp;
     C* tmp = static_cast<C*>(scratch(0, tmp_elems * sizeof(C)));
@@ -883,7 +885,7 @@ int32_t fb_forward_t(cplx_t<R>* out, cplx_t<R>* in, double leftX, const double* 
     g.ldo = nkx;
     g.kx = kx_dev;
     CH_TRY(launch_fb_gemm<R>(jobs, g, KF_DHT));
-    CH_TRY(make_phase<R>(ph, kx_dev, leftX, -1.0, nkx));
+    CH_TRY(make_phase<R>(ph, kx_dev, leftX, -1.0, nkx, leftX_dev));
     FftArgsT<R> a;
     memset(&a, 0, sizeof a);
     a.in = tmp;
@@ -903,7 +905,8 @@ int32_t fb_forward_t(cplx_t<R>* out, cplx_t<R>* in, double leftX, const double* 
 
 template <typename R>
 int32_t fb_inverse_t(cplx_t<R>* eb, const cplx_t<R>* e_fb, const cplx_t<R>* b_fb, int64_t b_comp_stride, double leftX,
-                     const double* kx_dev, const R* Out_dev, int nkx, int nr, int nmodes, int nkr, int ghost) {
+                     const double* kx_dev, const R* Out_dev, int nkx, int nr, int nmodes, int nkr, int ghost,
+                     const double* leftX_dev) {
     typedef cplx_t<R> C;
     const size_t tmp_elems = (size_t)nkx * nr * nmodes * 6;
     C* tmp = static_cast<C*>(scratch(0, tmp_elems * sizeof(C)));
@@ -922,7 +925,7 @@ int32_t fb_inverse_t(cplx_t<R>* eb, const cplx_t<R>* e_fb, const cplx_t<R>* b_fb
     g.ldo = nkx;
     g.kx = kx_dev;
     CH_TRY(launch_fb_gemm<R>(jobs, g, KF_DHT));
-    CH_TRY(make_phase<R>(ph, kx_dev, leftX, +1.0, nkx));
+    CH_TRY(make_phase<R>(ph, kx_dev, leftX, +1.0, nkx, leftX_dev));
     FftArgsT<R> a;
     memset(&a, 0, sizeof a);
     a.in = tmp;
@@ -1040,9 +1043,9 @@ int32_t fb_graddiv_poisson_t(cplx_t<R>* j, const cplx_t<R>* gn, const cplx_t<R>*
 
 #define CH_FB_INSTANTIATE(R)                                                                                          \
     template int32_t fb_forward_t<R>(cplx_t<R>*, cplx_t<R>*, double, const double*, const R*, int, int, int, int, int, \
-                                     const R*, int);                                                                  \
+                                     const R*, int, const double*);                                                   \
     template int32_t fb_inverse_t<R>(cplx_t<R>*, const cplx_t<R>*, const cplx_t<R>*, int64_t, double, const double*,  \
-                                     const R*, int, int, int, int, int);                                              \
+                                     const R*, int, int, int, int, int, const double*);                               \
     template int32_t fb_filter_t<R>(cplx_t<R>*, double, const double*, const R*, int, int, int, int, int);            \
     template int32_t fb_op_t<R>(int, cplx_t<R>*, const cplx_t<R>*, const R*, const R*, const double*, int, int, int,  \
                                 int, const R*);                                                                       \
@@ -1174,27 +1177,28 @@ int32_t chimera_fb_graddiv(double* vec_fb, const double* DpS2S, const double* Dm
 
 int32_t chimera_fb_forward(double* out, double* in, double leftX, const double* kx, const double* In,
                            const double* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
-                           int32_t fold) {
+                           int32_t fold, const double* frame) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ncomp < 1) return fail(ST_EINVAL, "fb_forward: bad sizes");
     CH_TRY(pow2_check(nkx));
     if (!is_device_ptr(out) || !is_device_ptr(in) || !is_device_ptr(kx) || !is_device_ptr(In) ||
         (scale && !is_device_ptr(scale)))
         return fail(ST_EINVAL, "fb_forward: resident entry points take device pointers");
-    return fb_forward(reinterpret_cast<double2*>(out), reinterpret_cast<double2*>(in), leftX, kx, In, nkx, nr,
-                      nmodes, nkr, ncomp, scale, fold);
+    return fb_forward_t<double>(reinterpret_cast<double2*>(out), reinterpret_cast<double2*>(in), leftX, kx, In, nkx,
+                                nr, nmodes, nkr, ncomp, scale, fold, frame);
 }
 
 int32_t chimera_fb_inverse(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
-                           const double* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost) {
+                           const double* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost,
+                           const double* frame) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 2)
         return fail(ST_EINVAL, "fb_inverse: bad sizes or ghost mode");
     CH_TRY(pow2_check(nkx));
     if (!is_device_ptr(eb_spc) || !is_device_ptr(e_fb) || !is_device_ptr(b_fb) || !is_device_ptr(kx) ||
         !is_device_ptr(Out))
         return fail(ST_EINVAL, "fb_inverse: resident entry points take device pointers");
-    return fb_inverse(reinterpret_cast<double2*>(eb_spc), reinterpret_cast<const double2*>(e_fb),
-                      reinterpret_cast<const double2*>(b_fb), (int64_t)nkx * nkr * nmodes, leftX, kx, Out, nkx, nr,
-                      nmodes, nkr, ghost);
+    return fb_inverse_t<double>(reinterpret_cast<double2*>(eb_spc), reinterpret_cast<const double2*>(e_fb),
+                                reinterpret_cast<const double2*>(b_fb), (int64_t)nkx * nkr * nmodes, leftX, kx, Out,
+                                nkx, nr, nmodes, nkr, ghost, frame);
 }
 
 int32_t chimera_fb_rot_scaled(double* vec_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
@@ -1240,18 +1244,19 @@ int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const dou
 
 int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double* kx, const float* In,
                               const float* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
-                              int32_t fold) {
+                              int32_t fold, const double* frame) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ncomp < 1) return fail(ST_EINVAL, "fb_forward_c8: bad sizes");
     CH_TRY(pow2_check(nkx));
     if (!is_device_ptr(out) || !is_device_ptr(in) || !is_device_ptr(kx) || !is_device_ptr(In) ||
         (scale && !is_device_ptr(scale)))
         return fail(ST_EINVAL, "fb_forward_c8: resident entry points take device pointers");
     return fb_forward_t<float>(reinterpret_cast<float2*>(out), reinterpret_cast<float2*>(in), leftX, kx, In, nkx, nr,
-                               nmodes, nkr, ncomp, scale, fold);
+                               nmodes, nkr, ncomp, scale, fold, frame);
 }
 
 int32_t chimera_fb_inverse_c8(float* eb_spc, const float* e_fb, const float* b_fb, double leftX, const double* kx,
-                              const float* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost) {
+                              const float* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost,
+                              const double* frame) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 2)
         return fail(ST_EINVAL, "fb_inverse_c8: bad sizes or ghost mode");
     CH_TRY(pow2_check(nkx));
@@ -1260,7 +1265,7 @@ int32_t chimera_fb_inverse_c8(float* eb_spc, const float* e_fb, const float* b_f
         return fail(ST_EINVAL, "fb_inverse_c8: resident entry points take device pointers");
     return fb_inverse_t<float>(reinterpret_cast<float2*>(eb_spc), reinterpret_cast<const float2*>(e_fb),
                                reinterpret_cast<const float2*>(b_fb), (int64_t)nkx * nkr * nmodes, leftX, kx, Out,
-                               nkx, nr, nmodes, nkr, ghost);
+                               nkx, nr, nmodes, nkr, ghost, frame);
 }
 
 int32_t chimera_fb_rot_scaled_c8(float* vec_fb_loc, const float* vec_fb, const float* DpS2S, const float* DmS2S,

@@ -79,7 +79,9 @@ struct FbGemmArgs {
 enum { FB_ROT = 0, FB_GRAD = 1, FB_DIV = 2, FB_GRADDIV = 3 };
 
 template <typename R> int32_t launch_fft(FftArgsT<R> a);
-template <typename R> int32_t make_phase(cplx_t<R>* ph, const double* kx, double leftX, double s, int n);
+// ph[i] = exp(s*i*kx[i]*leftX), leftX read from leftX_dev on the device when that is set.
+template <typename R>
+int32_t make_phase(cplx_t<R>* ph, const double* kx, double leftX, double s, int n, const double* leftX_dev = nullptr);
 template <typename R> int32_t launch_fb_gemm(const std::vector<FbJobT<R>>& jobs, const FbGemmArgs& g, int family);
 
 // Deposition ghost fold in place (grid_deps.f90:80-85): row 1 -= row 0, row 0
@@ -88,15 +90,18 @@ template <typename R> int32_t ghost_fold(cplx_t<R>* a, int nx1, int nr1, int64_t
 
 // Forward FB transform (fb_io.f90:18-98): in(nkx,0:nr,nmodes,ncomp) ->
 // out(nkx,nkr,nmodes,ncomp), optionally times scale_tab(nkx,nkr,nmodes); with
-// fold, the deposition ghost fold is first applied to in, in place.
+// fold, the deposition ghost fold is first applied to in, in place.  A
+// non-null leftX_dev replaces leftX by the device value at run time.
 template <typename R>
 int32_t fb_forward_t(cplx_t<R>* out, cplx_t<R>* in, double leftX, const double* kx_dev, const R* In_dev, int nkx,
-                     int nr, int nmodes, int nkr, int ncomp, const R* scale_tab, int fold);
+                     int nr, int nmodes, int nkr, int ncomp, const R* scale_tab, int fold,
+                     const double* leftX_dev = nullptr);
 // Inverse FB transform of E (first 3 components of e_fb) and B into
 // eb(nkx,0:nr,nmodes,6) (fb_io.f90:182-228); ghost as in FftArgs.
 template <typename R>
 int32_t fb_inverse_t(cplx_t<R>* eb, const cplx_t<R>* e_fb, const cplx_t<R>* b_fb, int64_t b_comp_stride, double leftX,
-                     const double* kx_dev, const R* Out_dev, int nkx, int nr, int nmodes, int nkr, int ghost);
+                     const double* kx_dev, const R* Out_dev, int nkx, int nr, int nmodes, int nkr, int ghost,
+                     const double* leftX_dev = nullptr);
 // Absorbing layer (fb_io.f90:230-270, modefilt 0) on ncomp components in place.
 template <typename R>
 int32_t fb_filter_t(cplx_t<R>* vec, double leftX, const double* kx_dev, const R* filtr_dev, int nxfilt, int nkx,

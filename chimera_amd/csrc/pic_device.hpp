@@ -20,6 +20,10 @@ struct GridGeom {
     double leftX, dx_inv, dr_inv;
     double r0, rmax;  // Rgrid(0), Rgrid(nr)
     const double* Rgrid;
+    // When set, the resident kernels take leftX from this device address at
+    // run time instead (a moving window advanced on the device, so that a
+    // captured step replays correctly); the drop-in kernels ignore it.
+    const double* leftX_dev;
     int nx, nr, nkO;
 };
 
@@ -29,6 +33,7 @@ inline GridGeom make_geom(double leftX, const double* Rgrid_dev, double dx_inv, 
     g.dx_inv = dx_inv;
     g.dr_inv = dr_inv;
     g.Rgrid = Rgrid_dev;
+    g.leftX_dev = nullptr;
     g.r0 = 0.0;
     g.rmax = 0.0;
     g.nx = nx;
@@ -54,6 +59,12 @@ struct PartGeom {
     double S[2][2];             // S[0][a]: x weight of node ix+a; S[1][b]: r weight of node ir+b
     double2 ph[MAX_MODES];      // ((y + sgn*i*z)/r)^m, phase 0 at r = 0
 };
+
+// The window's left edge a resident kernel uses: the device value when there is one.
+__device__ __forceinline__ GridGeom frame_geom(GridGeom g) {
+    if (g.leftX_dev) g.leftX = *g.leftX_dev;
+    return g;
+}
 
 // Cell indices, linear weights and azimuthal phases of one particle
 // (grid_deps.f90:41-66 and :175-199).  Returns false for particles the

@@ -34,11 +34,12 @@ namespace chimera {
 // geometry are FP64.
 template <int NMOD, typename R>
 __global__ void k_gather_push(const double* __restrict__ coord, PStride sx, R* __restrict__ u, PStride su,
-                              const R* __restrict__ w, const cplx_t<R>* __restrict__ Fld, GridGeom g,
+                              const R* __restrict__ w, const cplx_t<R>* __restrict__ Fld, GridGeom g_in,
                               double push_dt, UndulParams up, int n_undul, int np) {
     typedef cplx_t<R> C;
     const int ip = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (ip >= np) return;
+    const GridGeom g = frame_geom(g_in);
     const double xp = coord[sx.at(ip, 0)], yp = coord[sx.at(ip, 1)], zp = coord[sx.at(ip, 2)];
     R f[6] = {0, 0, 0, 0, 0, 0};
     PartGeom pg;
@@ -158,6 +159,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     __shared__ int box[8];
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const GridGeom gJ = frame_geom(a.gJ), gR = frame_geom(a.gR);
     const int64_t p0 = (int64_t)blk * DEP_CHUNK;
     constexpr int nmod = NMOD;
     const int64_t ld = a.ld;
@@ -231,10 +233,10 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
         if (W[0][j] == 0.0) continue;
         int ix, ir;
         if (doJ && (fabs(U[0][j]) + fabs(U[1][j]) + fabs(U[2][j])) != 0.0 &&
-            cell_of(XH[0][j], XH[1][j], XH[2][j], a.gJ, ix, ir)) {
+            cell_of(XH[0][j], XH[1][j], XH[2][j], gJ, ix, ir)) {
             bx[0] = min(bx[0], ix); bx[1] = max(bx[1], ix); bx[2] = min(bx[2], ir); bx[3] = max(bx[3], ir);
         }
-        if (doR && cell_of(X[0][j], X[1][j], X[2][j], a.gR, ix, ir)) {
+        if (doR && cell_of(X[0][j], X[1][j], X[2][j], gR, ix, ir)) {
             bx[4] = min(bx[4], ix); bx[5] = max(bx[5], ix); bx[6] = min(bx[6], ir); bx[7] = max(bx[7], ir);
         }
     }
@@ -268,11 +270,11 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     // registers while the cell stays the same and adds them to the LDS tile
     // once per cell.  Four passes over the thread's particles (J_x, J_y, J_z,
     // rho) keep the register sums to one component.
-    const int64_t sr = a.gJ.nx + 1, sm = sr * (a.gJ.nr + 1), sl = sm * nmod;
+    const int64_t sr = gJ.nx + 1, sm = sr * (gJ.nr + 1), sl = sm * nmod;
     for (int l = 0; l < 4; l++) {
         const bool isJ = l < 3;
         if ((isJ && !doJ) || (!isJ && !doR)) continue;
-        const GridGeom& gg = isJ ? a.gJ : a.gR;
+        const GridGeom& gg = isJ ? gJ : gR;
         double* tile_l = isJ ? tJ + (int64_t)l * nmod * TRJ * TXJ * 2 : tR;
         C* grid_l = isJ ? a.J + l * sl : a.R;
         const int bx0 = isJ ? box[0] : box[4], br0 = isJ ? box[2] : box[6];
@@ -629,11 +631,12 @@ template <typename R>
 static int32_t gather_push_impl(const double* coord, R* momenta, const R* wghts, int64_t ld, int32_t np,
                                 const cplx_t<R>* Fld, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
                                 double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
-                                const double* undul_params) {
+                                const double* undul_params, const double* frame) {
     CH_TRY(check_resident(np, ld, nkO));
     if (np == 0) return ST_OK;
     const PStride s = {1, ld};
-    const GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    g.leftX_dev = frame;
     UndulParams up = {0, 1, 0, 0, 0};
     int nu = 0;
     if (undul_params) {
@@ -696,7 +699,7 @@ template <typename R>
 static int32_t push_deposit_impl(double* coord, double* coord_half, const R* momenta, const R* wghts, int64_t ld,
                                  int32_t np, double dt, double leftX_J, double leftX_rho, const double* Rgrid,
                                  double dx_inv, double dr_inv, double r0, double rmax, int32_t nx, int32_t nr,
-                                 int32_t nkO, cplx_t<R>* curr, cplx_t<R>* dens) {
+                                 int32_t nkO, cplx_t<R>* curr, cplx_t<R>* dens, const double* frame) {
     typedef cplx_t<R> C;
     CH_TRY(check_resident(np, ld, nkO));
     if (np == 0) return ST_OK;
@@ -711,6 +714,8 @@ static int32_t push_deposit_impl(double* coord, double* coord_half, const R* mom
     a.dt = dt;
     a.gJ = resident_geom(leftX_J, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
     a.gR = resident_geom(leftX_rho, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    a.gJ.leftX_dev = frame;
+    a.gR.leftX_dev = frame ? frame + 1 : nullptr;
     a.J = curr;
     a.R = dens;
     {
@@ -791,17 +796,33 @@ int32_t chimera_soa_dep_dens(const double* coord, const double* wghts, int64_t l
 int32_t chimera_soa_gather_push(const double* coord, double* momenta, const double* wghts, int64_t ld, int32_t np,
                                 const double* Fld, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
                                 double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
-                                const double* undul_params) {
+                                const double* undul_params, const double* frame) {
     return gather_push_impl<double>(coord, momenta, wghts, ld, np, reinterpret_cast<const double2*>(Fld), leftX, Rgrid,
-                                    dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, undul_params);
+                                    dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, undul_params, frame);
 }
 
 int32_t chimera_soa_gather_push_mixed(const double* coord, float* momenta, const float* wghts, int64_t ld, int32_t np,
                                       const float* Fld, double leftX, const double* Rgrid, double dx_inv,
                                       double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
-                                      double push_dt, const double* undul_params) {
+                                      double push_dt, const double* undul_params, const double* frame) {
     return gather_push_impl<float>(coord, momenta, wghts, ld, np, reinterpret_cast<const float2*>(Fld), leftX, Rgrid,
-                                   dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, undul_params);
+                                   dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, undul_params, frame);
+}
+
+// Advance of a device-side window frame, as move_frame forms the new edge
+// (chimera_main.py:286-291: Xgrid + shiftX, leftX = Xgrid[0]).
+__global__ void k_frame_step(double* __restrict__ f, double shift1, double shift2) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double a = f[1] + shift1;
+    f[0] = a;
+    f[1] = a + shift2;
+}
+
+int32_t chimera_frame_step(double* frame, double shift1, double shift2) {
+    if (!is_device_ptr(frame)) return fail(ST_EINVAL, "frame_step: the frame is a device pointer");
+    hipLaunchKernelGGL(k_frame_step, 1, 64, 0, stream(), frame, shift1, shift2);
+    CH_LAUNCH_CHECK("k_frame_step");
+    return ST_OK;
 }
 
 int32_t chimera_soa_compact(double* coord, double* coord_half, double* momenta, double* wghts, int64_t ld, int32_t np,
@@ -818,19 +839,20 @@ int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* mome
 int32_t chimera_soa_push_deposit(double* coord, double* coord_half, const double* momenta, const double* wghts,
                                  int64_t ld, int32_t np, double dt, double leftX_J, double leftX_rho,
                                  const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax, int32_t nx,
-                                 int32_t nr, int32_t nkO, double* curr, double* dens) {
+                                 int32_t nr, int32_t nkO, double* curr, double* dens, const double* frame) {
     return push_deposit_impl<double>(coord, coord_half, momenta, wghts, ld, np, dt, leftX_J, leftX_rho, Rgrid, dx_inv,
                                      dr_inv, r0, rmax, nx, nr, nkO, reinterpret_cast<double2*>(curr),
-                                     reinterpret_cast<double2*>(dens));
+                                     reinterpret_cast<double2*>(dens), frame);
 }
 
 int32_t chimera_soa_push_deposit_mixed(double* coord, double* coord_half, const float* momenta, const float* wghts,
                                        int64_t ld, int32_t np, double dt, double leftX_J, double leftX_rho,
                                        const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
-                                       int32_t nx, int32_t nr, int32_t nkO, float* curr, float* dens) {
+                                       int32_t nx, int32_t nr, int32_t nkO, float* curr, float* dens,
+                                       const double* frame) {
     return push_deposit_impl<float>(coord, coord_half, momenta, wghts, ld, np, dt, leftX_J, leftX_rho, Rgrid, dx_inv,
                                     dr_inv, r0, rmax, nx, nr, nkO, reinterpret_cast<float2*>(curr),
-                                    reinterpret_cast<float2*>(dens));
+                                    reinterpret_cast<float2*>(dens), frame);
 }
 
 

@@ -1,0 +1,273 @@
+"""Scans and a request loop of independent PIC simulations on one GPU.
+
+The reference runs one simulation per process; this module keeps many
+resident at once (chimera_amd.batch: one library context and stream each,
+steps recorded into graphs and replayed) and schedules them:
+
+* run_scan: a fixed set of simulations (a parameter scan) to completion,
+  either one after another with eager steps (the plain path) or all at once
+  with their steps interleaved (the batched path); reports particle-steps per
+  second and the time-to-result of every simulation.
+* Server: continuous batching.  Requests arrive over time; a request is
+  admitted when a slot is free and its estimated device memory fits in the
+  free HBM (less a reserve); it is built (plasma generation, tables, initial
+  half step) on a builder thread, then advanced a few steps at a time in turn
+  with every other active simulation, so that the work of requests that
+  arrived at different times overlaps on the device.  Each simulation's
+  arrays and context are freed as soon as it completes.
+"""
+from __future__ import annotations
+
+import queue
+import threading
+import time
+
+import numpy as np
+
+from .batch import Simulation, free_device_bytes
+
+GIB = 1 << 30
+
+
+def estimate_bytes(spec, precision="fp64"):
+    """Device memory a workload needs, from its grid and particle counts, with a
+    1.5x margin for capacity growth and scratch memory."""
+    s = spec["solver"]
+    xmin, xmax, rmax, dx, dr = s["Grid"]
+    Nx = int(round((xmax - xmin) / dx))
+    Nkr = int(round(rmax / dr))
+    M = int(s["MaxAzimuthMode"]) + 1
+    cf = 16 if precision == "fp64" else 8
+    su = 8 if precision == "fp64" else 4
+    grid = Nx * (Nkr + 1) * M * cf * (6 + 3 + 2)              # EB, J, rho (+ background)
+    fb = Nx * Nkr * M * cf * (6 + 3 + 3 + 1 + 6 + 3 + 6)       # FB arrays + transform scratch
+    tables = Nx * Nkr * M * 10 * su + 4 * Nkr * Nkr * (M + 1) * su
+    if spec.get("kind") == "beam":
+        n = spec["params"]["n_target"] * 1.3
+        parts = n * (48 + 4 * su)
+    else:
+        fc = spec["electrons"]["FixedCell"]
+        n = Nx * Nkr * int(np.prod(fc))
+        parts = 2 * n * (48 + 4 * su)
+    return int(1.5 * (grid + fb + tables + parts))
+
+
+class Request:
+    """One simulation to run: a workload spec, its seed, precision and step count."""
+
+    def __init__(self, name, spec, seed, steps, precision="mixed", t_arrival=0.0):
+        from . import workloads as W
+        self.name, self.seed, self.steps, self.precision = name, int(seed), int(steps), precision
+        self.spec = W.with_precision(spec, precision)
+        self.t_arrival = float(t_arrival)
+        self.bytes = estimate_bytes(self.spec, precision)
+        self.sim = None
+        self.t_admit = self.t_built = self.t_done = None
+        self.particle_steps = 0
+
+    def build(self, graphs=True):
+        import chimera_amd as G
+        from . import workloads as W
+        if self.spec.get("kind") == "beam":
+            fn = lambda: W.build_beam(G, self.spec, self.seed, own_rng=True)
+        else:
+            fn = lambda: W.build(G, self.spec, self.seed, own_rng=True)
+        return Simulation(fn, self.steps, graphs=graphs, name=self.name)
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=float), q)) if len(xs) else None
+
+
+def run_scan(requests, batched=True, graphs=True, chunk=4, log=None):
+    """Run every request to completion and return (particle-steps/s over the
+    stepping, per-request time-to-result in s measured from the start of the
+    stepping, stepping seconds).  Building is not timed.  batched=False runs
+    the simulations one after another (each built, stepped and freed before
+    the next); batched=True builds them all, then interleaves their steps."""
+    if not batched:
+        psteps, wall, ttr = 0, 0.0, []
+        for r in requests:
+            sim = r.build(graphs=graphs)
+            t0 = time.perf_counter()
+            sim.advance(r.steps)
+            sim.result()
+            dt = time.perf_counter() - t0
+            wall += dt
+            ttr.append(wall)
+            psteps += sim.particles * r.steps
+            r.particle_steps = sim.particles * r.steps
+            sim.close()
+            if log:
+                log(f"  {r.name}: {sim.particles} particles, {r.steps} steps in {dt * 1e3:.1f} ms")
+        return psteps / wall, ttr, wall
+    sims = [r.build(graphs=graphs) for r in requests]
+    for r, s in zip(requests, sims):
+        r.particle_steps = s.particles * r.steps
+    # shortest first: small simulations reach the device first and finish early
+    order = sorted(range(len(sims)), key=lambda k: requests[k].particle_steps)
+    t0 = time.perf_counter()
+    done = {}
+    while len(done) < len(sims):
+        for k in order:
+            if k in done:
+                continue
+            s = sims[k]
+            if s.finished_issuing:
+                if s.done():
+                    done[k] = s.t_done - t0
+            elif s.ctx.done() or s.istep == 0:
+                s.advance(chunk)
+    wall = time.perf_counter() - t0
+    psteps = sum(r.particle_steps for r in requests)
+    for s in sims:
+        s.close()
+    return psteps / wall, [done[k] for k in range(len(sims))], wall
+
+
+class Server:
+    """Continuous batching of requests on one GPU (see the module docstring).
+
+    slots: most simulations resident at once; reserve: device bytes kept free;
+    chunk: steps issued per simulation per turn (a simulation gets its next
+    chunk once the previous one has finished)."""
+
+    def __init__(self, slots=64, reserve=16 * GIB, chunk=4, graphs=True, builder_thread=True, log=None):
+        self.slots, self.reserve, self.chunk, self.graphs = slots, reserve, chunk, graphs
+        self.log = log
+        self.pending = []          # arrived, not admitted (FIFO)
+        self.building = 0
+        self.reserved = 0          # estimated bytes of admitted requests not yet allocated
+        self.active = []
+        self.finished = []
+        self.built = queue.Queue()
+        self.errors = []
+        self.builder_thread = builder_thread
+        if builder_thread:
+            self.jobs = queue.Queue()
+            self.worker = threading.Thread(target=self._builder, daemon=True)
+            self.worker.start()
+
+    def _builder(self):
+        while True:
+            r = self.jobs.get()
+            if r is None:
+                return
+            try:
+                r.sim = r.build(graphs=self.graphs)
+                r.t_built = time.perf_counter()
+            except Exception as e:          # reported by the main loop
+                self.errors.append((r.name, repr(e)))
+                r.sim = None
+            self.built.put(r)
+
+    def submit(self, r, now):
+        r.t_arrival_host = now
+        self.pending.append(r)
+
+    def _admit(self, now):
+        free, _ = free_device_bytes()
+        while self.pending and len(self.active) + self.building < self.slots:
+            r = self.pending[0]
+            if r.bytes + self.reserve + self.reserved > free:
+                break
+            self.pending.pop(0)
+            r.t_admit = now
+            self.reserved += r.bytes
+            self.building += 1
+            if self.builder_thread:
+                self.jobs.put(r)
+            else:
+                r.sim = r.build(graphs=self.graphs)
+                r.t_built = time.perf_counter()
+                self.built.put(r)
+
+    def poll(self, now):
+        """One turn: take in built simulations, issue work, retire finished ones, admit."""
+        while not self.built.empty():
+            r = self.built.get()
+            self.building -= 1
+            self.reserved -= r.bytes        # now allocated: free_device_bytes() accounts for it
+            if r.sim is None:
+                continue
+            r.particle_steps = r.sim.particles * r.steps
+            self.active.append(r)
+        still = []
+        for r in self.active:
+            s = r.sim
+            if s.finished_issuing:
+                if s.done():
+                    r.t_done = s.t_done
+                    s.close()
+                    r.sim = None
+                    self.finished.append(r)
+                    continue
+            elif s.istep == 0 or s.ctx.done():
+                s.advance(self.chunk)
+            still.append(r)
+        self.active = still
+        self._admit(now)
+
+    @property
+    def idle(self):
+        return not (self.pending or self.active or self.building)
+
+    def close(self):
+        if self.builder_thread:
+            self.jobs.put(None)
+            self.worker.join()
+
+
+MIX = {"lpa_cfg0": 0.45, "beam": 0.5, "lpa_cfg1": 0.05}
+
+
+def poisson_requests(rate, seconds, seed, precision="mixed", mix=None):
+    """Requests arriving as a Poisson stream of `rate` per second for `seconds`,
+    each drawn from the mix of workloads (weights per kind) with scan
+    parameters from one seed: plasma density 1e-4..1e-2 n_c (log-uniform) and
+    a0 0.5..3 for the laser-plasma kinds; beam charge (density 1e-4..1e-2,
+    log-uniform), emittance and the other bunch parameters of
+    workloads.beam_scan for the bunches."""
+    from . import workloads as W
+    mix = MIX if mix is None else mix
+    kinds = sorted(mix)
+    p = np.array([mix[k] for k in kinds], dtype=float)
+    p /= p.sum()
+    rng = np.random.RandomState(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            return out
+        kind = kinds[rng.choice(len(kinds), p=p)]
+        if kind == "beam":
+            spec = W.beam_scan(1, seed=int(rng.randint(1 << 30)), steps=300)[0]
+            spec["beam"]["Density"] = float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-2))))
+        else:
+            dens = float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-2))))
+            a0 = float(rng.uniform(0.5, 3.0))
+            spec = W.WORKLOADS[kind](dens=dens, a0=a0)
+        out.append(Request(f"{kind}#{len(out)}", spec, int(rng.randint(1 << 30)), spec["steps"], precision, t))
+
+
+def serve(requests, server, log=None, t0=None):
+    """Feed requests (sorted by t_arrival, seconds after start) to the server in
+    real time and run until all have finished; returns the wall time."""
+    t0 = time.perf_counter() if t0 is None else t0
+    i = 0
+    last = t0
+    while i < len(requests) or not server.idle:
+        now = time.perf_counter()
+        while i < len(requests) and requests[i].t_arrival <= now - t0:
+            server.submit(requests[i], now)
+            i += 1
+        server.poll(now)
+        if log and now - last > 5.0:
+            last = now
+            log(f"  t={now - t0:6.1f}s arrived {i} pending {len(server.pending)} building {server.building} "
+                f"active {len(server.active)} finished {len(server.finished)}")
+        if server.errors:
+            raise RuntimeError(f"building {server.errors[0][0]} failed: {server.errors[0][1]}")
+        if not server.active and not server.built.qsize() and i < len(requests) and not server.pending:
+            time.sleep(min(0.002, max(0.0, requests[i].t_arrival - (time.perf_counter() - t0))))
+    return time.perf_counter() - t0

@@ -25,6 +25,7 @@ from .device import DeviceArray, call
 class Solver:
     def __init__(self, solver_in):
         self.Args = solver_in
+        self.frame = None     # device window frame (ChimeraRun.enable_device_frame), else None
         A = self.Args
         if "Features" not in A:
             A["Features"] = ()
@@ -148,6 +149,12 @@ class Solver:
         return (float(A["leftX"]), self.dev["Rgrid"].ptr, float(A["dx_inv"]), float(A["dr_inv"]), float(R[0]),
                 float(R[-1]), A["Nx"] - 1, A["Nr"] - 1, A["Mtot"] - 1)
 
+    def frame_ptr(self, stage):
+        """Device address of the window edge after the stage-1 (0) or the
+        stage-2 (1) move of the current step when the run keeps its frame on
+        the device, else None (the kernels then use Args['leftX'])."""
+        return None if self.frame is None else self.frame.offset_ptr(stage)
+
     # ------------------------------------------------------------------ PSATD (solvers.py:227-299)
     def PSATD_coeffs(self, beta=1.0):
         A = self.Args
@@ -199,20 +206,20 @@ class Solver:
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
         call(self.fn("chimera_fb_forward"), D["J_fb"].ptr, D["J"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
-             self.dev["InCurr"].ptr, self.dev["DepFact"].ptr, Nx, A["Nr"] - 1, M, Nkr, 3, 1)
+             self.dev["InCurr"].ptr, self.dev["DepFact"].ptr, Nx, A["Nr"] - 1, M, Nkr, 3, 1, self.frame_ptr(0))
 
     def fb_dens_in(self):
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
         call(self.fn("chimera_fb_forward"), D["Rho_fb"].ptr, D["Rho"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
-             self.dev["InCurr"].ptr, self.dev["DepFact"].ptr, Nx, A["Nr"] - 1, M, Nkr, 1, 1)
+             self.dev["InCurr"].ptr, self.dev["DepFact"].ptr, Nx, A["Nr"] - 1, M, Nkr, 1, 1, self.frame_ptr(1))
 
     def fb_fld_out(self):
         """EG_fb, B_fb -> EB with eb_correction fused."""
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
         call(self.fn("chimera_fb_inverse"), D["EB"].ptr, D["EG_fb"].ptr, D["B_fb"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
-             self.dev["Out"].ptr, Nx, A["Nr"] - 1, M, Nkr, 2)
+             self.dev["Out"].ptr, Nx, A["Nr"] - 1, M, Nkr, 2, self.frame_ptr(1))
 
     # ------------------------------------------------------------------ FB operators (solvers.py:467-553)
     def _op(self, op, out, inp):

@@ -72,6 +72,10 @@ class Specie:
         A["wght0"] = (A["Charge"] * A["Density"] * A["dr"] * A["dx"] * 2 * np.pi / A["Num_p"]) if A["Num_p"] else 0.0
         A["NpSlice"] = A["Nx"] * A["Num_p"]
 
+        # random draws of gen_parts: NumPy's global legacy stream, as in the
+        # reference; a simulation that must not share it (several runs in one
+        # process) sets its own numpy.random.RandomState here
+        self.rng = np.random
         self.np = 0
         self.cap = 0
         self.Data = {}
@@ -126,7 +130,7 @@ class Specie:
         if "FixedCell" not in A:
             return [np.zeros((3, 0), order="F"), np.zeros((3, 0), order="F"), np.zeros(0)]
         coords = np.zeros((4, Xgrid.shape[0] * Rgrid.shape[0] * A["Num_p"]), order="F")
-        RandPackO = np.random.rand(Xgrid.shape[0], Rgrid.shape[0]).astype("d", order="F")
+        RandPackO = self.rng.rand(Xgrid.shape[0], Rgrid.shape[0]).astype("d", order="F")
         coords, n = F.genparts(coords, Xgrid, Rgrid, RandPackO, *A["Pax"])
         coords = coords[:, :n]
         if ProfileFunc is None:
@@ -142,9 +146,9 @@ class Specie:
             else:
                 raise ValueError("can't understand the ProfileFunc")
         if "FlatSpectrum" in A["Features"]:
-            rand_mom = 2 * np.random.rand(3, n) - 1.0
+            rand_mom = 2 * self.rng.rand(3, n) - 1.0
         else:
-            rand_mom = np.random.randn(3, n)
+            rand_mom = self.rng.randn(3, n)
         mm, ms = A["MomentaMeans"], A["MomentaSpreads"]
         momenta = np.vstack([mm[c] + ms[c] * rand_mom[c] for c in range(3)]).astype("d", order="F")
         weights = coords[-1].astype("d", order="F")

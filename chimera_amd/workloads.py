@@ -53,13 +53,16 @@ def _uniform(x):
     return 1.0 + 0 * x
 
 
-def lpa_cfg0():
-    return lpa(512, 64, 1, a0=1.0, waist=5.0, duration=4.0, x0_from_right=12.8, steps=100)
+def lpa_cfg0(**kw):
+    args = dict(a0=1.0, waist=5.0, duration=4.0, x0_from_right=12.8, steps=100)
+    args.update(kw)
+    return lpa(512, 64, 1, **args)
 
 
-def lpa_cfg1():
-    return lpa(4096, 256, 2, a0=2.0, waist=8.0, duration=6.0, x0_from_right=30.0, steps=200, moving=True,
-               absorb=256)
+def lpa_cfg1(**kw):
+    args = dict(a0=2.0, waist=8.0, duration=6.0, x0_from_right=30.0, steps=200, moving=True, absorb=256)
+    args.update(kw)
+    return lpa(4096, 256, 2, **args)
 
 
 def lpa_cfg2():
@@ -126,7 +129,19 @@ def beam_scan(n=64, seed=2026, steps=300):
     return out
 
 
-def build_beam(mod, spec, seed=12345):
+def _seed(species, seed, own_rng):
+    """Seed the draws of gen_parts: NumPy's global stream (the reference's), or
+    with own_rng one RandomState shared by the run's species in the same order,
+    which gives the same draws without touching the global stream."""
+    if own_rng:
+        rng = np.random.RandomState(seed)
+        for sp in species:
+            sp.rng = rng
+    else:
+        np.random.seed(seed)
+
+
+def build_beam(mod, spec, seed=12345, own_rng=False):
     """Instantiate a beam workload with mod.Solver / mod.Specie / mod.ChimeraRun."""
     spec = copy.deepcopy(spec)
     solver = mod.Solver(spec["solver"])
@@ -136,7 +151,7 @@ def build_beam(mod, spec, seed=12345):
     def profile(x, y, z):
         return np.exp(-0.5 * x ** 2 / sx ** 2 - 0.5 * (y ** 2 + z ** 2) / sr ** 2)
 
-    np.random.seed(seed)
+    _seed((bunch,), seed, own_rng)
     bunch.add_particles(*bunch.gen_parts(Domain=spec["domain"], ProfileFunc=profile))
     return mod.ChimeraRun({"Solvers": (solver,), "Particles": (bunch,), "MovingFrames": spec["frames"]})
 
@@ -151,15 +166,17 @@ def with_precision(spec, precision):
     return spec
 
 
-def build(mod, spec, seed=12345, **run_options):
+def build(mod, spec, seed=12345, own_rng=False, **run_options):
     """Instantiate a workload with mod.Solver / mod.Specie / mod.ChimeraRun;
-    run_options are extra ChimeraRun keys (e.g. SortEvery)."""
+    run_options are extra ChimeraRun keys (e.g. SortEvery).  With own_rng the
+    species draw from their own seeded stream (also for the plasma injected by
+    the moving window), not NumPy's global one."""
     spec = copy.deepcopy(spec)
     solver = mod.Solver(spec["solver"])
     solver.add_gauss_beam(spec["laser"])
     electrons = mod.Specie(spec["electrons"])
     ions = mod.Specie(spec["ions"])
-    np.random.seed(seed)
+    _seed((electrons, ions), seed, own_rng)
     electrons.add_particles(*electrons.gen_parts())
     ions.add_particles(*ions.gen_parts())
     comps = {"Solvers": (solver,), "Particles": (electrons, ions), "MovingFrames": spec["frames"],

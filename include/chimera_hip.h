@@ -48,6 +48,35 @@ int32_t chimera_set_stream(void* hip_stream);
 void* chimera_get_stream(void);
 int32_t chimera_synchronize(void);
 
+/* Contexts and captured steps (no counterpart in the reference, which runs one
+ * simulation per process).  A context owns a stream (default priority, not
+ * blocking on the null stream), the scratch memory its launches share and the
+ * device copies of the small tables they read.  chimera_context_bind makes a
+ * context current for the calling thread (NULL = the default context, whose
+ * stream chimera_set_stream sets); every later call of the thread launches on
+ * it.  Independent simulations on different contexts run concurrently.
+ * chimera_context_query sets *done (HOST) to 1 when all work queued on the
+ * context has finished, else 0, without waiting.
+ * chimera_capture_begin / _end record the launches of the current context into
+ * a graph instead of running them (nothing may allocate, upload a new table or
+ * synchronise meanwhile: run the same step once uncaptured first);
+ * chimera_graph_launch replays it on its context's stream.  A graph reads the
+ * device memory it was recorded with: its arrays must stay allocated and its
+ * context alive until chimera_graph_destroy.  chimera_mem_info reports free
+ * and total device memory in bytes (HOST outputs). */
+int32_t chimera_context_create(void** ctx);
+int32_t chimera_context_destroy(void* ctx);
+int32_t chimera_context_bind(void* ctx);
+void* chimera_context_current(void);
+int32_t chimera_context_query(void* ctx, int32_t* done);
+int32_t chimera_capture_begin(void);
+int32_t chimera_capture_end(void** graph);
+int32_t chimera_capture_abort(void);
+int32_t chimera_graph_launch(void* graph);
+int32_t chimera_graph_destroy(void* graph);
+int32_t chimera_device_synchronize(void);
+int32_t chimera_mem_info(int64_t* free_bytes, int64_t* total_bytes);
+
 /* Device memory for resident simulations (no counterpart in the reference,
  * whose arrays are NumPy-owned).  chimera_malloc returns zero-filled device
  * memory; chimera_memcpy copies between any two of host/device memory (device
@@ -232,17 +261,31 @@ int32_t chimera_soa_dep_dens(const double* coord, const double* wghts, int64_t l
                              double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
                              int32_t nx, int32_t nr, int32_t nkO);
 
+/* Window frames on the device.  The resident entry points below that take a
+ * window edge (leftX) also take `frame`, a DEVICE pointer or NULL: when it is
+ * not NULL the kernels read the edge from it at run time and ignore the
+ * by-value argument, so that a step captured into a graph
+ * (chimera_capture_begin) replays correctly while the window moves.  A frame
+ * is two doubles: frame[0] = the edge for the stage-1 consumers of a step
+ * (J deposit and its forward transform), frame[1] = the edge after the staged
+ * stage-2 move (rho deposit, its transform, the inverse transform, gather).
+ * chimera_frame_step advances it as move_frame does (chimera_main.py:286-291,
+ * Xgrid + shiftX): frame[0] = frame[1] + shift1, then frame[1] = frame[0] +
+ * shift2 (a shift of 0 is no move), in FP64. */
+int32_t chimera_frame_step(double* frame, double shift1, double shift2);
+
 /* Fused position push and deposition: with dt != 0 the positions are pushed
  * first (coord -> coord at n+1, coord_half = midpoint), otherwise coord_half is
  * read as given; then J (curr, if not NULL) is deposited from coord_half and
- * momenta with leftX_J, and rho (dens, if not NULL) from coord with leftX_rho.
+ * momenta with leftX_J, and rho (dens, if not NULL) from coord with leftX_rho
+ * (frame[0] and frame[1] instead when frame is not NULL).
  * Deposits are accumulated per block in LDS tiles and added to the grid
  * unfolded.  = push_coords + dep_curr + dep_dens (particle_tools.f90:58-82,
  * grid_deps.f90:18-147), as make_step runs them (chimera_main.py:84-88). */
 int32_t chimera_soa_push_deposit(double* coord, double* coord_half, const double* momenta, const double* wghts,
                                  int64_t ld, int32_t np, double dt, double leftX_J, double leftX_rho,
                                  const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax, int32_t nx,
-                                 int32_t nr, int32_t nkO, double* curr, double* dens);
+                                 int32_t nr, int32_t nkO, double* curr, double* dens, const double* frame);
 
 /* Gather of Fld(0:nx,0:nr,0:nkO,6) at the particles, plus the analytic
  * undulator when undul_params (HOST double[4]) is not NULL, then the Boris push
@@ -252,7 +295,7 @@ int32_t chimera_soa_push_deposit(double* coord, double* coord_half, const double
 int32_t chimera_soa_gather_push(const double* coord, double* momenta, const double* wghts, int64_t ld, int32_t np,
                                 const double* Fld, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
                                 double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
-                                const double* undul_params);
+                                const double* undul_params, const double* frame);
 
 /* Stable removal of the particles outside lims (HOST double[4]: x range,
  * r^2 range), tested on coord (test_half=0) or coord_half (1); *np_out (HOST)
@@ -283,13 +326,14 @@ int32_t chimera_soa_sort_cells(const double* coord, const double* coord_half, co
  * solvers.py:417-435). */
 int32_t chimera_fb_forward(double* out, double* in, double leftX, const double* kx, const double* In,
                            const double* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
-                           int32_t fold);
+                           int32_t fold, const double* frame);
 
 /* Inverse FB transform into eb_spc(nkx,0:nr,nmodes,6); ghost=1: ghost row zero
  * (fb_eb_out alone), ghost=2: eb_correction applied as well.
  * = fb_eb_out + eb_correction (fb_io.f90:182-228, grid_deps.f90:219-266). */
 int32_t chimera_fb_inverse(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
-                           const double* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost);
+                           const double* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost,
+                           const double* frame);
 
 /* B = rot(G) * scale (PoissFact).  = fb_rot + omp_mult_vec (solvers.py:536-553). */
 int32_t chimera_fb_rot_scaled(double* vec_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
@@ -324,18 +368,20 @@ int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const dou
 int32_t chimera_soa_push_deposit_mixed(double* coord, double* coord_half, const float* momenta, const float* wghts,
                                        int64_t ld, int32_t np, double dt, double leftX_J, double leftX_rho,
                                        const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
-                                       int32_t nx, int32_t nr, int32_t nkO, float* curr, float* dens);
+                                       int32_t nx, int32_t nr, int32_t nkO, float* curr, float* dens,
+                                       const double* frame);
 int32_t chimera_soa_gather_push_mixed(const double* coord, float* momenta, const float* wghts, int64_t ld, int32_t np,
                                       const float* Fld, double leftX, const double* Rgrid, double dx_inv,
                                       double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
-                                      double push_dt, const double* undul_params);
+                                      double push_dt, const double* undul_params, const double* frame);
 int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* momenta, float* wghts, int64_t ld,
                                   int32_t np, const double* lims, int32_t test_half, int32_t* np_out);
 int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double* kx, const float* In,
                               const float* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
-                              int32_t fold);
+                              int32_t fold, const double* frame);
 int32_t chimera_fb_inverse_c8(float* eb_spc, const float* e_fb, const float* b_fb, double leftX, const double* kx,
-                              const float* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost);
+                              const float* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost,
+                              const double* frame);
 int32_t chimera_fb_rot_scaled_c8(float* vec_fb_loc, const float* vec_fb, const float* DpS2S, const float* DmS2S,
                                  const double* kx, const float* scale, int32_t nkx, int32_t nkr, int32_t nkO);
 int32_t chimera_fb_graddiv_poisson_c8(float* j_fb, const float* grad_rho_n_fb, const float* grad_rho_np1_fb,
