@@ -116,7 +116,12 @@ __device__ __forceinline__ dv4 ld4(const double* p) { return *reinterpret_cast<c
 __device__ __forceinline__ fv4 ld4(const float* p) { return *reinterpret_cast<const fv4*>(p); }
 __device__ __forceinline__ void st4(double* p, dv4 v) { *reinterpret_cast<dv4*>(p) = v; }
 
-constexpr int DEP_THREADS = 64, DEP_PPT = 4, DEP_LDS_BYTES = 12 * 1024;
+constexpr int DEP_THREADS = 64, DEP_PPT = 4;
+// LDS tile budget of a block: 64 nodes for each of J_x, J_y, J_z and rho in
+// each mode (complex FP64), i.e. nmod * 4 KiB.  A block's 256 cell-ordered
+// particles span about 17 x 3 nodes, so the tiles fit for any mode count, and
+// eight blocks per CU stay within the 160 KiB of LDS up to nmod = 5.
+constexpr int dep_lds_bytes(int nmod) { return 4 * nmod * 64 * 16; }
 constexpr int DEP_SLOT_W = 32, DEP_SLOT_R = 4;   // node columns and rows of one slot
 
 __device__ __forceinline__ void lds_add(double* p, double v) {
@@ -154,6 +159,7 @@ template <int NMOD, typename R>
 __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     typedef cplx_t<R> C;
     constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
+    constexpr int DEP_LDS_BYTES = dep_lds_bytes(NMOD);
     __shared__ double tile[DEP_LDS_BYTES / sizeof(double)];
     __shared__ int red[8][DEP_THREADS / 64];
     __shared__ int box[8];
