@@ -15,6 +15,17 @@ the same workload then runs in mixed precision (float32 momenta and weights,
 complex64 fields, FP32 MFMA transforms, FP64 positions) and its figures go in
 the "mixed" object of the same line.  Per-kernel-family times come from HIP
 events that libchimera_hip records on its own stream around every launch.
+
+The "scan" object (skipped with --no-scan) is the batched scan of 64 uneven
+bunches x 300 steps (workloads.beam_scan): particle-steps/s and p50/p99
+time-to-result of the plain path (one simulation after another, eager steps)
+and of the batched path (all resident, recorded steps replayed on one stream
+each, interleaved), in FP64 and mixed precision.  --serve SECONDS runs the
+request loop instead (Poisson arrivals, continuous batching).
+
+HIP's GPU_MAX_HW_QUEUES (hardware queues per process) is set to 16 (or to
+CHIMERA_HW_QUEUES): the simulations of a batch run on separate streams, and
+with HIP's default of 4 queues at most four of them overlap.
 """
 from __future__ import annotations
 
@@ -23,6 +34,8 @@ import json
 import os
 import sys
 import time
+
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("CHIMERA_HW_QUEUES", "16")
 
 import numpy as np
 
@@ -187,7 +200,8 @@ def scan_bench(args):
             continue
         reqs = [S.Request(f"beam{k}", sp, 1000 + k, sp["steps"], prec) for k, sp in enumerate(specs)]
         log(f"scan {label}: {len(reqs)} simulations")
-        rate, ttr, wall = S.run_scan(reqs, batched=batched, graphs=batched)
+        rate, ttr, wall = S.run_scan(reqs, batched=batched, graphs=batched, chunk=args.scan_chunk,
+                                     gate=bool(args.scan_gate))
         parts = sum(r.particle_steps for r in reqs) // args.scan_steps
         out[label] = {"value": round(rate, 1), "unit": "particle-steps/s", "stepping_s": round(wall, 3),
                       "p50_s": round(S.percentile(ttr, 50), 4), "p99_s": round(S.percentile(ttr, 99), 4),
@@ -328,9 +342,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB gibibytes device to device once (reference kernel for the HBM counters)")
-    ap.add_argument("--scan", action="store_true", help="also run the batched bunch scan (scan object)")
+    ap.add_argument("--no-scan", action="store_true", help="skip the batched bunch scan (scan object)")
     ap.add_argument("--scan-n", type=int, default=64)
     ap.add_argument("--scan-steps", type=int, default=300)
+    ap.add_argument("--scan-chunk", type=int, default=4, help="steps issued per simulation per turn")
+    ap.add_argument("--scan-gate", type=int, default=1, help="1: a simulation's next chunk waits for the last")
     ap.add_argument("--scan-only", default="", help="comma list of scan runs (plain_fp64, plain_mixed, "
                                                     "batched_fp64, batched_mixed) and no PIC-step bench")
     ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
@@ -395,7 +411,7 @@ def main():
                                            "mfma")}
         out["mixed"]["dtype"] = "f32 momenta/weights, c64 fields, FP32 MFMA; f64 positions and tables"
         out["mixed"]["speedup_vs_fp64"] = round(mx["value"] / m["value"], 3)
-    if args.scan:
+    if not args.no_scan:
         out["scan"] = scan_bench(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, m["P"])

@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes
 import os
 import re
+import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -42,6 +43,13 @@ def parse_header(path: str = HEADER):
 
 
 _lib = None
+
+# Held around every library call made from Python, and by a step capture for
+# its whole length: HIP invalidates a stream capture when another thread
+# allocates, frees or copies synchronously meanwhile, and any library call may
+# grow scratch memory or upload a table.  Reentrant, so the calls inside a
+# capture take it again.
+DEVICE_LOCK = threading.RLock()
 
 
 def lib():

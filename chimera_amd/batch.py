@@ -23,7 +23,7 @@ from __future__ import annotations
 import ctypes
 import time
 
-from ._lib import check, lib
+from ._lib import DEVICE_LOCK, check, lib
 
 
 class Context:
@@ -31,7 +31,9 @@ class Context:
 
     def __init__(self):
         h = ctypes.c_void_p(0)
-        check("chimera_context_create", lib().chimera_context_create(ctypes.byref(h)))
+        with DEVICE_LOCK:
+            st = lib().chimera_context_create(ctypes.byref(h))
+        check("chimera_context_create", st)
         self.handle = h.value
         self._prev = []
 
@@ -55,7 +57,9 @@ class Context:
 
     def close(self):
         if self.handle:
-            check("chimera_context_destroy", lib().chimera_context_destroy(self.handle))
+            with DEVICE_LOCK:
+                st = lib().chimera_context_destroy(self.handle)
+            check("chimera_context_destroy", st)
             self.handle = None
 
 
@@ -93,14 +97,16 @@ class Simulation:
         parity = i % 2
         g = self.graphs.get(parity)
         if g is None:
-            check("chimera_capture_begin", L.chimera_capture_begin())
-            try:
-                self.run.make_step(i)
-            except Exception:
-                L.chimera_capture_abort()
-                raise
-            h = ctypes.c_void_p(0)
-            check("chimera_capture_end", L.chimera_capture_end(ctypes.byref(h)))
+            # no other thread may touch the device while the step is recorded
+            with DEVICE_LOCK:
+                check("chimera_capture_begin", L.chimera_capture_begin())
+                try:
+                    self.run.make_step(i)
+                except Exception:
+                    L.chimera_capture_abort()
+                    raise
+                h = ctypes.c_void_p(0)
+                check("chimera_capture_end", L.chimera_capture_end(ctypes.byref(h)))
             g = self.graphs[parity] = h.value
         else:
             self.run.host_step(i)
@@ -140,7 +146,9 @@ class Simulation:
         L = lib()
         self.ctx.synchronize()
         for g in self.graphs.values():
-            check("chimera_graph_destroy", L.chimera_graph_destroy(g))
+            with DEVICE_LOCK:
+                st = L.chimera_graph_destroy(g)
+            check("chimera_graph_destroy", st)
         self.graphs = {}
         with self.ctx:
             self.run = None      # arrays are freed while the context is current

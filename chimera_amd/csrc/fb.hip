@@ -461,13 +461,17 @@ int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs&
         for (const auto& j : js) maxN = j.Ncol > maxN ? j.Ncol : maxN;
         const FbJob* dj = static_cast<const FbJob*>(device_table(js.data(), js.size() * sizeof(FbJob)));
         if (!dj) return ST_ENOMEM;
-        // 128-column tiles when they still give every CU two blocks, else 64-column tiles
+        // 128-column tiles when they still give every CU two blocks, 32-column
+        // tiles when 64-column ones would leave CUs without a block (small grids)
         const int64_t xb = (g.nx + GB_X - 1) / GB_X;
         const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
+        const bool narrow = xb * ((maxN + 63) / 64) * (int64_t)js.size() < 256;
         int32_t st;
         if (pass == 0) st = wide ? launch_gemm_variant<1, 4>(dj, js.size(), maxN, g)
-                                 : launch_gemm_variant<1, 2>(dj, js.size(), maxN, g);
-        else st = launch_gemm_variant<2, 2>(dj, js.size(), maxN, g);
+                            : narrow ? launch_gemm_variant<1, 1>(dj, js.size(), maxN, g)
+                                     : launch_gemm_variant<1, 2>(dj, js.size(), maxN, g);
+        else st = narrow ? launch_gemm_variant<2, 1>(dj, js.size(), maxN, g)
+                         : launch_gemm_variant<2, 2>(dj, js.size(), maxN, g);
         CH_TRY(st);
     }
     return ST_OK;
@@ -483,13 +487,17 @@ int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs&
 // Accumulators take half the registers of FP64, so both accumulator sets fit
 // with 64x64 wave tiles.
 typedef float flt4 __attribute__((ext_vector_type(4)));
+// K-stage depth of the FP32 kernel: 16 like the FP64 kernel, and 32 for the
+// 32-column tiles of small grids, whose launches are too few blocks to hide
+// the global-load latency of a stage and gain from half as many stages.
+constexpr int gbf_k(int nacc, int wn) { return wn == 1 ? 32 : 16; }
 
-template <int WN>
+template <int WN, int GBF_K>
 __device__ __forceinline__ void mfma_stage_f32(flt4 (&acc)[4][WN], const float* __restrict__ As,
                                                const float* __restrict__ Ds, int lane, int wr, int wc) {
     constexpr int LDA = GB_ROWS + GB_PAD, LDD = 32 * WN + GB_PAD;
 #pragma unroll
-    for (int kq = 0; kq < GB_K / 4; kq++) {
+    for (int kq = 0; kq < GBF_K / 4; kq++) {
         const int kk = kq * 4 + (lane >> 4);
         float a[4], b[WN];
 #pragma unroll
@@ -503,15 +511,16 @@ __device__ __forceinline__ void mfma_stage_f32(flt4 (&acc)[4][WN], const float* 
     }
 }
 
-constexpr size_t gb_lds_bytes_f32(int WN) {
-    return 2 * (size_t)GB_K * ((GB_ROWS + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(float);
+constexpr size_t gb_lds_bytes_f32(int WN, int GBF_K) {
+    return 2 * (size_t)GBF_K * ((GB_ROWS + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(float);
 }
 
 template <int NACC, int WN>
 __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<float>* __restrict__ jobs, FbGemmArgs g) {
     typedef FbSegT<float> Seg;
+    constexpr int GBF_K = gbf_k(NACC, WN);
     constexpr int BN = 32 * WN, LDA = GB_ROWS + GB_PAD, LDD = BN + GB_PAD;
-    constexpr int A_STAGE = GB_K * LDA, D_STAGE = GB_K * LDD;
+    constexpr int A_STAGE = GBF_K * LDA, D_STAGE = GBF_K * LDD;
     const FbJobT<float>& J = jobs[blockIdx.z];
     const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
     const int n0 = blockIdx.y * BN;
@@ -520,8 +529,8 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
 
     extern __shared__ __align__(16) unsigned char gbf_smem[];
-    float* const As = reinterpret_cast<float*>(gbf_smem);  // [2][GB_K][LDA]
-    float* const Ds = As + 2 * A_STAGE;                       // [2][GB_K][LDD]
+    float* const As = reinterpret_cast<float*>(gbf_smem);  // [2][GBF_K][LDA]
+    float* const Ds = As + 2 * A_STAGE;                       // [2][GBF_K][LDD]
 
     flt4 acc0[4][WN], acc1[4][NACC == 2 ? WN : 1];
 #pragma unroll
@@ -529,8 +538,8 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
 #pragma unroll
         for (int j = 0; j < WN; j++) acc0[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
 
-    constexpr int NA = (GB_K * GB_X) / GB_THREADS, ND = (GB_K * BN) / GB_THREADS;
-    const int stages_per_seg = (K + GB_K - 1) / GB_K;
+    constexpr int NA = (GBF_K * GB_X) / GB_THREADS, ND = (GBF_K * BN) / GB_THREADS;
+    const int stages_per_seg = (K + GBF_K - 1) / GBF_K;
     const int nstages = stages_per_seg * nseg;
     const Seg S0 = J.seg[0];
     const Seg S1 = J.seg[nseg > 1 ? 1 : 0];
@@ -558,12 +567,12 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
 #pragma unroll
         for (int i = 0; i < ND; i++) {
             const int e = tid + GB_THREADS * i;
-            const int kk = e % GB_K, nl = e / GB_K;
+            const int kk = e % GBF_K, nl = e / GBF_K;
             rd[i] = (k0 + kk < K && n0 + nl < Ncol) ? gld1(S.D, (int64_t)(k0 + kk) + S.ldd * (int64_t)(n0 + nl)) : 0.f;
         }
     };
     auto fetch = [&](int t) __attribute__((always_inline)) {
-        const int k0 = (t % stages_per_seg) * GB_K;
+        const int k0 = (t % stages_per_seg) * GBF_K;
         if (t < stages_per_seg) fetch_seg(S0, k0);
         else fetch_seg(S1, k0);
     };
@@ -580,7 +589,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
 #pragma unroll
         for (int i = 0; i < ND; i++) {
             const int e = tid + GB_THREADS * i;
-            D[(e % GB_K) * LDD + e / GB_K] = rd[i];
+            D[(e % GBF_K) * LDD + e / GBF_K] = rd[i];
         }
     };
     auto stash = [&](int buf, int t) __attribute__((always_inline)) {
@@ -590,7 +599,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     auto pipe = [&](flt4 (&acc)[4][WN], int tb, int te) __attribute__((always_inline)) {
         for (int t = tb; t < te; t++) {
             if (t + 1 < nstages) fetch(t + 1);
-            mfma_stage_f32<WN>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
+            mfma_stage_f32<WN, GBF_K>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
             if (t + 1 < nstages) stash((t + 1) & 1, t + 1);
             __syncthreads();
         }
@@ -646,11 +655,11 @@ static int32_t launch_gemm_variant_f32(const FbJobT<float>* dj, size_t njobs, in
     static bool attr = false;
     if (!attr) {
         CH_HIP(hipFuncSetAttribute((const void*)k_fb_gemm_f32<NACC, WN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)gb_lds_bytes_f32(WN)));
+                                   (int)gb_lds_bytes_f32(WN, gbf_k(NACC, WN))));
         attr = true;
     }
     dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
-    hipLaunchKernelGGL((k_fb_gemm_f32<NACC, WN>), grid, GB_THREADS, gb_lds_bytes_f32(WN), stream(), dj, g);
+    hipLaunchKernelGGL((k_fb_gemm_f32<NACC, WN>), grid, GB_THREADS, gb_lds_bytes_f32(WN, gbf_k(NACC, WN)), stream(), dj, g);
     CH_LAUNCH_CHECK("k_fb_gemm_f32");
     return ST_OK;
 }
@@ -671,11 +680,14 @@ int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGe
         if (!dj) return ST_ENOMEM;
         const int64_t xb = (g.nx + GB_X - 1) / GB_X;
         const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
+        const bool narrow = xb * ((maxN + 63) / 64) * (int64_t)js.size() < 256;
         int32_t st;
         if (pass == 0) st = wide ? launch_gemm_variant_f32<1, 4>(dj, js.size(), maxN, g)
-                                 : launch_gemm_variant_f32<1, 2>(dj, js.size(), maxN, g);
+                            : narrow ? launch_gemm_variant_f32<1, 1>(dj, js.size(), maxN, g)
+                                     : launch_gemm_variant_f32<1, 2>(dj, js.size(), maxN, g);
         else st = wide ? launch_gemm_variant_f32<2, 4>(dj, js.size(), maxN, g)
-                       : launch_gemm_variant_f32<2, 2>(dj, js.size(), maxN, g);
+                  : narrow ? launch_gemm_variant_f32<2, 1>(dj, js.size(), maxN, g)
+                           : launch_gemm_variant_f32<2, 2>(dj, js.size(), maxN, g);
         CH_TRY(st);
     }
     return ST_OK;

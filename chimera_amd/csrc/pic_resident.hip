@@ -74,8 +74,9 @@ __global__ void k_gather_push(const double* __restrict__ coord, PStride sx, R* _
 // apart and, with particles in cell order, mostly in different cells).
 // Phase 1 optionally pushes positions (particle_tools.f90:58-82) and finds the
 // block's cell bounding box for J (at the half-step positions) and rho (at the
-// new positions).  Phase 2 accumulates into LDS tiles covering those boxes with
-// LDS FP64 atomics.  Phase 3 hands the tiles on:
+// new positions).  Then, one component (J_x, J_y, J_z, rho) at a time, the
+// block accumulates into an LDS tile covering the box with LDS FP64 atomics
+// and hands the tile on:
 //  * a block whose tile spans at most DEP_SLOT_R node rows and DEP_SLOT_W node
 //    columns (the common case for particles in the reference's generation
 //    order or sorted by cell: 256 particles cover about 16 cells of one or two
@@ -116,12 +117,14 @@ __device__ __forceinline__ dv4 ld4(const double* p) { return *reinterpret_cast<c
 __device__ __forceinline__ fv4 ld4(const float* p) { return *reinterpret_cast<const fv4*>(p); }
 __device__ __forceinline__ void st4(double* p, dv4 v) { *reinterpret_cast<dv4*>(p) = v; }
 
-constexpr int DEP_THREADS = 64, DEP_PPT = 4;
-// LDS tile budget of a block: 64 nodes for each of J_x, J_y, J_z and rho in
-// each mode (complex FP64), i.e. nmod * 4 KiB.  A block's 256 cell-ordered
-// particles span about 17 x 3 nodes, so the tiles fit for any mode count, and
-// eight blocks per CU stay within the 160 KiB of LDS up to nmod = 5.
-constexpr int dep_lds_bytes(int nmod) { return 4 * nmod * 64 * 16; }
+// A thread owns DEP_PPT consecutive particles, handled in groups of DEP_G (one
+// 32-byte vector load per component and group).
+constexpr int DEP_THREADS = 64, DEP_PPT = 4, DEP_G = 4;
+// LDS tile of a block: one component (J_x, J_y, J_z or rho) at a time, all
+// modes, complex FP64: 16 KiB hold 1024 / nmod nodes -- the 17 x 3 nodes of 256
+// cell-ordered particles at 16 per cell, or the 65 x 3 at 4 per cell up to
+// nmod = 4 -- and eight blocks per CU use 128 of the 160 KiB of LDS.
+constexpr int DEP_LDS_BYTES = 16 * 1024;
 constexpr int DEP_SLOT_W = 32, DEP_SLOT_R = 4;   // node columns and rows of one slot
 
 __device__ __forceinline__ void lds_add(double* p, double v) {
@@ -152,14 +155,13 @@ __device__ __forceinline__ int wave_max(int v) {
 
 // R: precision of momenta, weights and grids (double, or float in mixed
 // precision).  Positions, the push and the cell geometry stay FP64; each
-// corner contribution is formed in FP64, summed per cell in R, and the LDS tile
+// corner contribution is formed and summed per cell in FP64, and the LDS tile
 // is FP64 in both precisions (FP64 LDS atomics measured far cheaper here than
 // FP32 ones under this access pattern), rounded to R when it leaves the block.
 template <int NMOD, typename R>
 __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     typedef cplx_t<R> C;
     constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
-    constexpr int DEP_LDS_BYTES = dep_lds_bytes(NMOD);
     __shared__ double tile[DEP_LDS_BYTES / sizeof(double)];
     __shared__ int red[8][DEP_THREADS / 64];
     __shared__ int box[8];
@@ -171,15 +173,15 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     const int64_t ld = a.ld;
     const bool doJ = a.J != nullptr, doR = a.R != nullptr;
 
-    // ---- phase 1: load the thread's DEP_PPT consecutive particles (32-byte
-    // vector loads when the arrays allow: a wave then reads 2 KB contiguous per
-    // component), push, store, bounding boxes.  The particles stay in
-    // registers for phase 2.
-    const int64_t ib = p0 + (int64_t)tid * DEP_PPT;
-    const bool full = a.vec4 && ib + DEP_PPT <= a.np;
-    // the thread's particles: component c of array p at particle ib + j
-    auto load_parts = [&](const auto* p, double (&v)[3][DEP_PPT], int ncomp) __attribute__((always_inline)) {
-        if (full) {
+    // ---- phase 1: load the thread's DEP_PPT consecutive particles, DEP_G at a
+    // time (32-byte vector loads when the arrays allow: with DEP_PPT = DEP_G a
+    // wave then reads 2 KB contiguous per component), push, store, bounding
+    // boxes.  (Eight particles per thread measured slower in FP64.)
+    const int64_t ib0 = p0 + (int64_t)tid * DEP_PPT;
+    // group h of the thread's particles: component c of array p at particle ib0 + h*DEP_G + j
+    auto load_parts = [&](const auto* p, double (&v)[3][DEP_G], int ncomp, int h) __attribute__((always_inline)) {
+        const int64_t ib = ib0 + h * DEP_G;
+        if (a.vec4 && ib + DEP_G <= a.np) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 if (c >= ncomp) break;
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < DEP_PPT; j++)
+            for (int j = 0; j < DEP_G; j++)
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     if (c >= ncomp) break;
@@ -197,16 +199,18 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
         }
     };
     int bx[8] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
-    {
-    double X[3][DEP_PPT], XH[3][DEP_PPT], U[3][DEP_PPT], W[3][DEP_PPT];   // W: row 0 only
-    load_parts(a.x, X, 3);
-    load_parts(a.u, U, 3);
-    load_parts(a.w, W, 1);
-    if (!a.push) load_parts(a.xh, XH, 3);
+    for (int h = 0; h < DEP_PPT / DEP_G; h++) {
+    const int64_t ib = ib0 + h * DEP_G;
+    const bool full = a.vec4 && ib + DEP_G <= a.np;
+    double X[3][DEP_G], XH[3][DEP_G], U[3][DEP_G], W[3][DEP_G];   // W: row 0 only
+    load_parts(a.x, X, 3, h);
+    load_parts(a.u, U, 3, h);
+    load_parts(a.w, W, 1, h);
+    if (!a.push) load_parts(a.xh, XH, 3, h);
     if (a.push) {
         // particle_tools.f90:58-82: x += u*dt/gamma; x_half = (x_old + x_new)/2
 #pragma unroll
-        for (int j = 0; j < DEP_PPT; j++) {
+        for (int j = 0; j < DEP_G; j++) {
             const double dt_gp = a.dt / sqrt(1.0 + (U[0][j] * U[0][j] + U[1][j] * U[1][j] + U[2][j] * U[2][j]));
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -223,7 +227,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < DEP_PPT; j++) {
+            for (int j = 0; j < DEP_G; j++) {
                 const int64_t ip = ib + j;
                 if (ip >= a.np) break;
 #pragma unroll
@@ -235,7 +239,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
         }
     }
 #pragma unroll
-    for (int j = 0; j < DEP_PPT; j++) {
+    for (int j = 0; j < DEP_G; j++) {
         if (W[0][j] == 0.0) continue;
         int ix, ir;
         if (doJ && (fabs(U[0][j]) + fabs(U[1][j]) + fabs(U[2][j])) != 0.0 &&
@@ -258,35 +262,39 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
         box[tid] = v;
     }
     __syncthreads();
-    // tiles: J nodes (TXJ x TRJ) x 3 comps x nmod modes, then rho nodes x nmod; complex = 2 doubles
+    // tiles: one component at a time, (TX x TR nodes) x nmod modes, complex = 2 doubles
     const int TXJ = box[1] >= box[0] ? box[1] - box[0] + 2 : 0, TRJ = box[3] >= box[2] ? box[3] - box[2] + 2 : 0;
     const int TXR = box[5] >= box[4] ? box[5] - box[4] + 2 : 0, TRR = box[7] >= box[6] ? box[7] - box[6] + 2 : 0;
-    const int64_t nJ = (int64_t)TXJ * TRJ * 3 * nmod * 2, nR = (int64_t)TXR * TRR * nmod * 2;
+    const int64_t nJ = (int64_t)TXJ * TRJ * nmod * 2, nR = (int64_t)TXR * TRR * nmod * 2;
     const bool tileJ = doJ && nJ > 0 && nJ * (int64_t)sizeof(double) <= DEP_LDS_BYTES;
-    const bool tileR = doR && nR > 0 && (nR + (tileJ ? nJ : 0)) * (int64_t)sizeof(double) <= DEP_LDS_BYTES;
-    double* tJ = tile;
-    double* tR = tile + (tileJ ? nJ : 0);
-    const int64_t nz = (tileJ ? nJ : 0) + (tileR ? nR : 0);
-    for (int64_t i = tid; i < nz; i += DEP_THREADS) tile[i] = 0.0;
-    __syncthreads();
+    const bool tileR = doR && nR > 0 && nR * (int64_t)sizeof(double) <= DEP_LDS_BYTES;
+    const bool slotJ = tileJ && TRJ <= DEP_SLOT_R && TXJ <= DEP_SLOT_W;
+    const bool slotR = tileR && TRR <= DEP_SLOT_R && TXR <= DEP_SLOT_W;
 
-    // ---- phase 2: deposit.  A thread's consecutive particles usually share a
-    // cell (the lattice of a cell is ordered with the azimuthal index
-    // innermost), so each thread sums its per-corner contributions in
-    // registers while the cell stays the same and adds them to the LDS tile
-    // once per cell.  Four passes over the thread's particles (J_x, J_y, J_z,
-    // rho) keep the register sums to one component.
+    // ---- phases 2 and 3, one pass per component (J_x, J_y, J_z, rho): zero
+    // the tile, deposit into it, then write it to the block's slot (a tile no
+    // taller than DEP_SLOT_R rows and no wider than DEP_SLOT_W columns) or add
+    // it to the grid with global atomics.  A thread's consecutive particles
+    // usually share a cell (the lattice of a cell is ordered with the
+    // azimuthal index innermost), so each thread sums its per-corner
+    // contributions in registers while the cell stays the same and adds them
+    // to the LDS tile once per cell.
     const int64_t sr = gJ.nx + 1, sm = sr * (gJ.nr + 1), sl = sm * nmod;
     for (int l = 0; l < 4; l++) {
         const bool isJ = l < 3;
         if ((isJ && !doJ) || (!isJ && !doR)) continue;
         const GridGeom& gg = isJ ? gJ : gR;
-        double* tile_l = isJ ? tJ + (int64_t)l * nmod * TRJ * TXJ * 2 : tR;
+        double* tile_l = tile;
         C* grid_l = isJ ? a.J + l * sl : a.R;
         const int bx0 = isJ ? box[0] : box[4], br0 = isJ ? box[2] : box[6];
         const int TX = isJ ? TXJ : TXR, TR = isJ ? TRJ : TRR;
         const bool tiled = isJ ? tileJ : tileR;
-        C acc[NMOD][4];
+        const int64_t nt = isJ ? nJ : nR;
+        if (tiled) {
+            for (int64_t i = tid; i < nt; i += DEP_THREADS) tile[i] = 0.0;
+            __syncthreads();
+        }
+        double2 acc[NMOD][4];   // per-cell sums in FP64 in both precisions
         int cx = -1, cr = 0;
         auto flush = [&]() __attribute__((always_inline)) {
             if (cx < 0) return;
@@ -300,12 +308,13 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
                     lds_add(tile_l + t + 1, (double)acc[m][q].y);
                 }
         };
-        double P[3][DEP_PPT], V[3][DEP_PPT], Wt[3][DEP_PPT];   // Wt: row 0 only
-        load_parts(isJ ? a.xh : a.x, P, 3);
-        if (isJ) load_parts(a.u, V, 3);
-        load_parts(a.w, Wt, 1);
+        for (int h = 0; h < DEP_PPT / DEP_G; h++) {
+        double P[3][DEP_G], V[3][DEP_G], Wt[3][DEP_G];   // Wt: row 0 only
+        load_parts(isJ ? a.xh : a.x, P, 3, h);
+        if (isJ) load_parts(a.u, V, 3, h);
+        load_parts(a.w, Wt, 1, h);
 #pragma unroll
-        for (int j = 0; j < DEP_PPT; j++) {
+        for (int j = 0; j < DEP_G; j++) {
             const double wp = Wt[0][j];
             if (wp == 0.0) continue;
             double q0;   // weight of this component: w*u_l/gamma for J, w for rho
@@ -328,7 +337,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
 #pragma unroll
                     for (int m = 0; m < NMOD; m++)
 #pragma unroll
-                        for (int q = 0; q < 4; q++) acc[m][q] = cmake<R>(0, 0);
+                        for (int q = 0; q < 4; q++) acc[m][q] = cmk(0.0, 0.0);
                     cx = (int)pg.ix;
                     cr = (int)pg.ir;
                 }
@@ -337,7 +346,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
                     const double2 pv = crs(q0, pg.ph[m]);
 #pragma unroll
                     for (int q = 0; q < 4; q++)
-                        acc[m][q] = cadd(acc[m][q], cconv<R>(crs(pg.S[0][q & 1] * pg.S[1][q >> 1], pv)));
+                        acc[m][q] = cadd(acc[m][q], crs(pg.S[0][q & 1] * pg.S[1][q >> 1], pv));
                 }
             } else {
 #pragma unroll
@@ -352,58 +361,40 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
                 }
             }
         }
+        }
         flush();
+        if (!tiled) continue;
+        __syncthreads();
+        const bool slot = isJ ? slotJ : slotR;
+        const int64_t n = nt / 2;
+        if (slot) {
+            // tile index (m*TR + rr)*TX + xx -> slot index ((l*nmod + m)*DEP_SLOT_R + rr)*DEP_SLOT_W + xx
+            // (rho: l = 0 of its own slot array)
+            C* sl_base = isJ ? a.slotJ + (int64_t)blk * (3 * nmod * DEP_SLOT_R * DEP_SLOT_W) +
+                                   (int64_t)l * nmod * DEP_SLOT_R * DEP_SLOT_W
+                             : a.slotR + (int64_t)blk * (nmod * DEP_SLOT_R * DEP_SLOT_W);
+            for (int64_t i = tid; i < n; i += DEP_THREADS) {
+                const int64_t rest = i / TX;
+                sl_base[((rest / TR) * DEP_SLOT_R + rest % TR) * DEP_SLOT_W + i % TX] =
+                    cmake<R>((R)tile[2 * i], (R)tile[2 * i + 1]);
+            }
+        } else {
+            for (int64_t i = tid; i < n; i += DEP_THREADS) {
+                const R re = (R)tile[2 * i], im = (R)tile[2 * i + 1];
+                if (re == 0 && im == 0) continue;
+                const int64_t xx = i % TX, rest = i / TX, rr = rest % TR, m = rest / TR;
+                R* dst = reinterpret_cast<R*>(grid_l + (bx0 + xx) + (br0 + rr) * sr + m * sm);
+                unsafeAtomicAdd(dst, re);
+                unsafeAtomicAdd(dst + 1, im);
+            }
+        }
+        __syncthreads();   // the tile is zeroed again by the next pass
     }
-    __syncthreads();
 
-    // ---- phase 3: row-pure tiles to the block's slots, other tiles by global atomics
     const int4 none = make_int4(-1, 0, 0, 0);
-    if (doJ) {
-        const bool slot = tileJ && TRJ <= DEP_SLOT_R && TXJ <= DEP_SLOT_W;
-        if (slot) {
-            // tile index (lm*TRJ + rr)*TXJ + xx -> slot index (lm*DEP_SLOT_R + rr)*DEP_SLOT_W + xx
-            C* sl = a.slotJ + (int64_t)blk * (3 * nmod * DEP_SLOT_R * DEP_SLOT_W);
-            const int64_t n = nJ / 2;
-            for (int64_t i = tid; i < n; i += DEP_THREADS) {
-                const int64_t rest = i / TXJ;
-                sl[((rest / TRJ) * DEP_SLOT_R + rest % TRJ) * DEP_SLOT_W + i % TXJ] =
-                    cmake<R>((R)tJ[2 * i], (R)tJ[2 * i + 1]);
-            }
-        } else if (tileJ) {
-            const int64_t n = nJ / 2;
-            for (int64_t i = tid; i < n; i += DEP_THREADS) {
-                const R re = (R)tJ[2 * i], im = (R)tJ[2 * i + 1];
-                if (re == 0 && im == 0) continue;
-                const int64_t xx = i % TXJ, rest = i / TXJ, rr = rest % TRJ, lm = rest / TRJ;
-                R* dst = reinterpret_cast<R*>(a.J + (box[0] + xx) + (box[2] + rr) * sr + lm * sm);
-                unsafeAtomicAdd(dst, re);
-                unsafeAtomicAdd(dst + 1, im);
-            }
-        }
-        if (tid == 0) a.desc[blk] = slot ? make_int4(box[2], box[0], TXJ, TRJ) : none;
-    }
-    if (doR) {
-        const bool slot = tileR && TRR <= DEP_SLOT_R && TXR <= DEP_SLOT_W;
-        if (slot) {
-            C* sl = a.slotR + (int64_t)blk * (nmod * DEP_SLOT_R * DEP_SLOT_W);
-            const int64_t n = nR / 2;
-            for (int64_t i = tid; i < n; i += DEP_THREADS) {
-                const int64_t rest = i / TXR;
-                sl[((rest / TRR) * DEP_SLOT_R + rest % TRR) * DEP_SLOT_W + i % TXR] =
-                    cmake<R>((R)tR[2 * i], (R)tR[2 * i + 1]);
-            }
-        } else if (tileR) {
-            const int64_t n = nR / 2;
-            for (int64_t i = tid; i < n; i += DEP_THREADS) {
-                const R re = (R)tR[2 * i], im = (R)tR[2 * i + 1];
-                if (re == 0 && im == 0) continue;
-                const int64_t xx = i % TXR, rest = i / TXR, rr = rest % TRR, m = rest / TRR;
-                R* dst = reinterpret_cast<R*>(a.R + (box[4] + xx) + (box[6] + rr) * sr + m * sm);
-                unsafeAtomicAdd(dst, re);
-                unsafeAtomicAdd(dst + 1, im);
-            }
-        }
-        if (tid == 0) a.desc[a.nblocks + blk] = slot ? make_int4(box[6], box[4], TXR, TRR) : none;
+    if (tid == 0) {
+        if (doJ) a.desc[blk] = slotJ ? make_int4(box[2], box[0], TXJ, TRJ) : none;
+        if (doR) a.desc[a.nblocks + blk] = slotR ? make_int4(box[6], box[4], TXR, TRR) : none;
     }
 }
 

@@ -12,7 +12,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import DEVICE_LOCK, check, lib
 
 
 _shutting_down = [False]
@@ -21,7 +21,9 @@ atexit.register(lambda: _shutting_down.__setitem__(0, True))
 
 def call(name: str, *args):
     """Call an entry point of libchimera_hip; a non-zero status raises."""
-    check(name, getattr(lib(), name)(*args))
+    with DEVICE_LOCK:
+        st = getattr(lib(), name)(*args)
+    check(name, st)
 
 
 class DeviceArray:
@@ -41,7 +43,8 @@ class DeviceArray:
         p, self._ptr = getattr(self, "_ptr", 0), 0
         if p and not _shutting_down[0]:
             try:
-                self._lib.chimera_free(p)
+                with DEVICE_LOCK:
+                    self._lib.chimera_free(p)
             except Exception:
                 pass
 

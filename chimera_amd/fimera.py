@@ -18,7 +18,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import DEVICE_LOCK, check, lib
 
 __all__ = [
     "push_velocs", "push_coords", "genparts", "sortpartsout", "chunk_coords_boundaries", "align_data_vec",
@@ -41,7 +41,9 @@ def _p(a):
 
 
 def _call(name, *args):
-    check(name, getattr(lib(), name)(*args))
+    with DEVICE_LOCK:
+        st = getattr(lib(), name)(*args)
+    check(name, st)
 
 
 # ------------------------------------------------------------------ particle_tools.f90

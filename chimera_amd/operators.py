@@ -12,8 +12,44 @@ Restates (non-KxShift branch):
 """
 from __future__ import annotations
 
+import functools
+import hashlib
+import threading
+from collections import OrderedDict
+
 import numpy as np
 from scipy.special import jn, jn_zeros
+
+
+def _key(v):
+    if isinstance(v, np.ndarray):
+        return ("a", v.shape, v.dtype.str, hashlib.sha1(np.ascontiguousarray(v).view(np.uint8)).hexdigest())
+    return v
+
+
+def _memo(fn):
+    """Keep the last few results of a table builder, keyed by its arguments'
+    values: simulations of one grid (the requests of a scan) share the tables
+    instead of rebuilding them.  The cached arrays are read-only."""
+    cache, lock = OrderedDict(), threading.Lock()
+
+    @functools.wraps(fn)
+    def wrapper(*args, **kw):
+        key = tuple(_key(a) for a in args) + tuple((k, _key(v)) for k, v in sorted(kw.items()))
+        with lock:
+            if key in cache:
+                cache.move_to_end(key)
+                return cache[key]
+        out = fn(*args, **kw)
+        for a in (out.values() if isinstance(out, dict) else out):
+            if isinstance(a, np.ndarray):
+                a.setflags(write=False)
+        with lock:
+            cache[key] = out
+            while len(cache) > 8:
+                cache.popitem(last=False)
+        return out
+    return wrapper
 
 
 def x_grid(leftX, rightX, dx, nthrds=None):
@@ -36,6 +72,7 @@ def r_grid(lengthR, dr):
     return Nkr, RgridFull, lengthR
 
 
+@_memo
 def spectral_operators(Nx, Nkr, Nko, lengthR, kx, RgridFull, ext=0):
     """Bessel roots, DHT matrices and the m+-1 coupling matrices
     (solvers.py:717-759), in the reference's axis order before the swap of
@@ -66,6 +103,7 @@ def spectral_operators(Nx, Nkr, Nko, lengthR, kx, RgridFull, ext=0):
     return dict(In=In, Out=Out, DpS2S=DpS2S, DmS2S=DmS2S, w=w, kr=kr, kr_g=kr_g, kx_g=kx_g)
 
 
+@_memo
 def psatd_tables(w, dt, space_charge=True):
     """PSATD_E / PSATD_G (solvers.py:244-279), (Nx,Nkr,M,5) with space
     charge, else (...,3)."""
@@ -87,6 +125,7 @@ def psatd_tables(w, dt, space_charge=True):
     return E, G
 
 
+@_memo
 def static_tables(kx_g, w, px0):
     """Complex tables of maxwell_solver_stat (solvers.py:350-359)."""
     beta0 = px0 / np.sqrt(1 + px0 ** 2)

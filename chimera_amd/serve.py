@@ -11,7 +11,7 @@ steps recorded into graphs and replayed) and schedules them:
 * Server: continuous batching.  Requests arrive over time; a request is
   admitted when a slot is free and its estimated device memory fits in the
   free HBM (less a reserve); it is built (plasma generation, tables, initial
-  half step) on a builder thread, then advanced a few steps at a time in turn
+  half step) on one of a few builder threads, then advanced a few steps at a time in turn
   with every other active simulation, so that the work of requests that
   arrived at different times overlaps on the device.  Each simulation's
   arrays and context are freed as soon as it completes.
@@ -79,12 +79,14 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=float), q)) if len(xs) else None
 
 
-def run_scan(requests, batched=True, graphs=True, chunk=4, log=None):
+def run_scan(requests, batched=True, graphs=True, chunk=4, gate=True, log=None):
     """Run every request to completion and return (particle-steps/s over the
     stepping, per-request time-to-result in s measured from the start of the
     stepping, stepping seconds).  Building is not timed.  batched=False runs
     the simulations one after another (each built, stepped and freed before
-    the next); batched=True builds them all, then interleaves their steps."""
+    the next); batched=True builds them all, then interleaves their steps,
+    `chunk` steps of each in turn (with gate, a simulation's next chunk waits
+    until its previous one has finished on the device)."""
     if not batched:
         psteps, wall, ttr = 0, 0.0, []
         for r in requests:
@@ -116,7 +118,7 @@ def run_scan(requests, batched=True, graphs=True, chunk=4, log=None):
             if s.finished_issuing:
                 if s.done():
                     done[k] = s.t_done - t0
-            elif s.ctx.done() or s.istep == 0:
+            elif not gate or s.istep == 0 or s.ctx.done():
                 s.advance(chunk)
     wall = time.perf_counter() - t0
     psteps = sum(r.particle_steps for r in requests)
@@ -132,8 +134,10 @@ class Server:
     chunk: steps issued per simulation per turn (a simulation gets its next
     chunk once the previous one has finished)."""
 
-    def __init__(self, slots=64, reserve=16 * GIB, chunk=4, graphs=True, builder_thread=True, log=None):
+    def __init__(self, slots=64, reserve=16 * GIB, chunk=4, graphs=True, builder_thread=True, log=None,
+                 on_finish=None, builders=4):
         self.slots, self.reserve, self.chunk, self.graphs = slots, reserve, chunk, graphs
+        self.on_finish = on_finish     # called with each finished request before its simulation is freed
         self.log = log
         self.pending = []          # arrived, not admitted (FIFO)
         self.building = 0
@@ -144,9 +148,12 @@ class Server:
         self.errors = []
         self.builder_thread = builder_thread
         if builder_thread:
+            # several builders: plasma generation and table construction are
+            # NumPy/SciPy work that runs outside the interpreter lock
             self.jobs = queue.Queue()
-            self.worker = threading.Thread(target=self._builder, daemon=True)
-            self.worker.start()
+            self.workers = [threading.Thread(target=self._builder, daemon=True) for _ in range(max(1, builders))]
+            for w in self.workers:
+                w.start()
 
     def _builder(self):
         while True:
@@ -198,6 +205,8 @@ class Server:
             if s.finished_issuing:
                 if s.done():
                     r.t_done = s.t_done
+                    if self.on_finish is not None:
+                        self.on_finish(r)
                     s.close()
                     r.sim = None
                     self.finished.append(r)
@@ -214,8 +223,10 @@ class Server:
 
     def close(self):
         if self.builder_thread:
-            self.jobs.put(None)
-            self.worker.join()
+            for _ in self.workers:
+                self.jobs.put(None)
+            for w in self.workers:
+                w.join()
 
 
 MIX = {"lpa_cfg0": 0.45, "beam": 0.5, "lpa_cfg1": 0.05}

@@ -168,3 +168,35 @@ def test_context_and_capture_errors():
     f, t = ctypes.c_int64(0), ctypes.c_int64(0)
     assert L.chimera_mem_info(ctypes.byref(f), ctypes.byref(t)) == 0
     assert 0 < f.value <= t.value
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_results_match_alone(precision):
+    """Requests arriving while others run (continuous batching, builder thread,
+    admission by free memory) give the results each gives alone."""
+    import time
+    from chimera_amd import serve as S
+    specs = [small_beam(Nx=nx, Nkr=nkr, n_target=n, gamma=gm, steps=st)
+             for nx, nkr, n, gm, st in ((256, 40, 3e4, 80.0, 10), (128, 64, 8e4, 300.0, 14), (512, 32, 2e4, 60.0, 8),
+                                        (256, 96, 1e5, 450.0, 12), (128, 32, 1e4, 120.0, 16))]
+    say(f"case: 5 beams {precision} served with staggered arrivals vs alone")
+    reqs = [S.Request(f"r{k}", sp, 50 + k, sp["steps"], precision, t_arrival=0.05 * k) for k, sp in enumerate(specs)]
+    alone = {}
+    for r in reqs:
+        sim = r.build()
+        sim.advance(r.steps)
+        alone[r.name] = snap(sim.result())
+        sim.close()
+    served = {}
+    server = S.Server(slots=3, reserve=0, chunk=2, on_finish=lambda r: served.__setitem__(r.name, snap(r.sim.run)))
+    try:
+        S.serve(reqs, server)
+    finally:
+        server.close()
+    assert sorted(served) == sorted(alone)
+    assert all(r.t_done >= r.t_arrival_host for r in server.finished)
+    for name in alone:
+        for k in ("EB", "EG_fb", "momenta", "coords"):
+            e = rel_l2(served[name][k], alone[name][k])
+            assert e <= TOL[precision], (name, k, e)
+    say("  all requests match their runs alone")
