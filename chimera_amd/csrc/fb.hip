@@ -448,12 +448,36 @@ static int32_t split_jobs(const std::vector<FbJobT<R>>& jobs, std::vector<FbJobT
     return ST_OK;
 }
 
+// Small grids: 64-column tiles of all the jobs together would leave CUs without a block.
+template <typename J>
+static bool gemm_narrow(int nx, const std::vector<J>& jobs) {
+    int maxN = 0;
+    for (const auto& j : jobs) maxN = j.Ncol > maxN ? j.Ncol : maxN;
+    return (int64_t)((nx + GB_X - 1) / GB_X) * ((maxN + 63) / 64) * (int64_t)jobs.size() < 256;
+}
+
+// For one launch of everything: a one-accumulator job becomes a two-accumulator
+// job whose second segment (if any) feeds accumulator 1 with the same output
+// coefficients, so that sum_o alpha0*acc0 + alpha1*acc1 is unchanged; with one
+// segment accumulator 1 stays zero.
+template <typename R>
+static void merge_jobs(std::vector<FbJobT<R>>& two, const std::vector<FbJobT<R>>& one) {
+    for (FbJobT<R> j : one) {
+        for (int o = 0; o < j.nout; o++) j.out[o].alpha1 = j.nseg == 2 ? j.out[o].alpha0 : make_double2(0.0, 0.0);
+        two.push_back(j);
+    }
+}
+
 template <>
 int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family) {
     if (jobs.empty()) return ST_OK;
     KTimer kt(family);
     std::vector<FbJob> one, two;
     CH_TRY(split_jobs<double>(jobs, one, two));
+    if (!two.empty() && !one.empty() && gemm_narrow(g.nx, jobs)) {
+        merge_jobs<double>(two, one);   // small grids: one launch instead of two
+        one.clear();
+    }
     for (int pass = 0; pass < 2; pass++) {
         const std::vector<FbJob>& js = pass == 0 ? one : two;
         if (js.empty()) continue;
@@ -670,6 +694,10 @@ int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGe
     KTimer kt(family);
     std::vector<FbJobT<float>> one, two;
     CH_TRY(split_jobs<float>(jobs, one, two));
+    if (!two.empty() && !one.empty() && gemm_narrow(g.nx, jobs)) {
+        merge_jobs<float>(two, one);    // small grids: one launch instead of two
+        one.clear();
+    }
     for (int pass = 0; pass < 2; pass++) {
         const std::vector<FbJobT<float>>& js = pass == 0 ? one : two;
         if (js.empty()) continue;

@@ -32,33 +32,114 @@ namespace chimera {
 // R: precision of momenta, weights and fields (double, or float in mixed
 // precision, where the Boris push runs in FP32); positions and the cell
 // geometry are FP64.
+//
+// A block takes GP_T consecutive (cell-ordered) particles, one per thread,
+// finds the bounding box of their cells, and copies the field nodes of that
+// box (all six components and modes) into LDS with coalesced loads; each
+// particle then reads its 4 x 6 x NMOD values from LDS instead of from L2 (at
+// 16 particles per cell the block's 256 particles share 17 x 3 nodes).  A box
+// too large for GP_LDS_BYTES reads the field from global memory.  The values
+// and the order of the sums are the same either way.
+constexpr int GP_T = 256, GP_LDS_BYTES = 24 * 1024;
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int block_min_max(int v, bool is_max, int (*red)[GP_T / 64], int q) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = is_max ? wave_max(v) : wave_min(v);
+    if (lane == 0) red[q][wv] = v;
+    return v;
+}
+
 template <int NMOD, typename R>
-__global__ void k_gather_push(const double* __restrict__ coord, PStride sx, R* __restrict__ u, PStride su,
-                              const R* __restrict__ w, const cplx_t<R>* __restrict__ Fld, GridGeom g_in,
-                              double push_dt, UndulParams up, int n_undul, int np) {
+__global__ void __launch_bounds__(GP_T) k_gather_push(const double* __restrict__ coord, PStride sx, R* __restrict__ u,
+                                                      PStride su, const R* __restrict__ w,
+                                                      const cplx_t<R>* __restrict__ Fld, GridGeom g_in, double push_dt,
+                                                      UndulParams up, int n_undul, int np) {
     typedef cplx_t<R> C;
-    const int ip = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (ip >= np) return;
+    constexpr int NC = 6 * NMOD;   // (component, mode) planes
+    __shared__ C tile[GP_LDS_BYTES / sizeof(C)];
+    __shared__ int red[4][GP_T / 64];
+    __shared__ int box[4];
+    const int tid = threadIdx.x;
+    const int ip = xcd_block(blockIdx.x, gridDim.x) * GP_T + tid;
+    const bool live = ip < np;
     const GridGeom g = frame_geom(g_in);
-    const double xp = coord[sx.at(ip, 0)], yp = coord[sx.at(ip, 1)], zp = coord[sx.at(ip, 2)];
-    R f[6] = {0, 0, 0, 0, 0, 0};
+    double xp = 0, yp = 0, zp = 0;
+    if (live) {
+        xp = coord[sx.at(ip, 0)];
+        yp = coord[sx.at(ip, 1)];
+        zp = coord[sx.at(ip, 2)];
+    }
     PartGeom pg;
-    if (w[ip] != 0 && part_geom(pg, xp, yp, zp, g, +1.0)) {
-        const int64_t sr = g.nx + 1, sm = sr * (g.nr + 1), sl = sm * NMOD;
-        const C* base = Fld + pg.ix + pg.ir * sr;
-        for (int l = 0; l < 6; l++) {
-            R acc = 0;
+    const bool has = live && w[ip] != 0 && part_geom(pg, xp, yp, zp, g, +1.0);
+    block_min_max(has ? (int)pg.ix : INT_MAX, false, red, 0);
+    block_min_max(has ? (int)pg.ix : INT_MIN, true, red, 1);
+    block_min_max(has ? (int)pg.ir : INT_MAX, false, red, 2);
+    block_min_max(has ? (int)pg.ir : INT_MIN, true, red, 3);
+    __syncthreads();
+    if (tid < 4) {
+        int v = red[tid][0];
+        for (int k = 1; k < GP_T / 64; k++) v = (tid & 1) ? max(v, red[tid][k]) : min(v, red[tid][k]);
+        box[tid] = v;
+    }
+    __syncthreads();
+    const int64_t sr = g.nx + 1, sm = sr * (g.nr + 1), sl = sm * NMOD;
+    const int bx0 = box[0], br0 = box[2];
+    const int TX = box[1] >= box[0] ? box[1] - box[0] + 2 : 0, TR = box[3] >= box[2] ? box[3] - box[2] + 2 : 0;
+    const int nt = TX * TR * NC;
+    const bool tiled = nt > 0 && (int64_t)nt * (int64_t)sizeof(C) <= GP_LDS_BYTES;
+    if (tiled) {
+        // plane lm = l*NMOD + m of the field starts at lm*sm (sl = NMOD*sm)
+        for (int e = tid; e < nt; e += GP_T) {
+            const int xx = e % TX, rest = e / TX, rr = rest % TR, lm = rest / TR;
+            tile[e] = Fld[(bx0 + xx) + (int64_t)(br0 + rr) * sr + (int64_t)lm * sm];
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    R f[6] = {0, 0, 0, 0, 0, 0};
+    if (has) {
+        if (tiled) {
+            const C* base = tile + (pg.ix - bx0) + (pg.ir - br0) * TX;
+            const int tsl = TX * TR;
+            for (int l = 0; l < 6; l++) {
+                R acc = 0;
 #pragma unroll
-            for (int iO = 0; iO < NMOD; iO++) {
-                const C* fl = base + iO * sm + l * sl;
-                for (int b = 0; b < 2; b++)
-                    for (int a = 0; a < 2; a++) {
-                        const C pc = cconv<R>(crs(pg.S[1][b] * pg.S[0][a], pg.ph[iO]));
-                        const C v = fl[a + b * sr];
-                        acc += pc.x * v.x - pc.y * v.y;
-                    }
+                for (int iO = 0; iO < NMOD; iO++) {
+                    const C* fl = base + (l * NMOD + iO) * tsl;
+                    for (int b = 0; b < 2; b++)
+                        for (int a = 0; a < 2; a++) {
+                            const C pc = cconv<R>(crs(pg.S[1][b] * pg.S[0][a], pg.ph[iO]));
+                            const C v = fl[a + b * TX];
+                            acc += pc.x * v.x - pc.y * v.y;
+                        }
+                }
+                f[l] = acc;
             }
-            f[l] = acc;
+        } else {
+            const C* base = Fld + pg.ix + pg.ir * sr;
+            for (int l = 0; l < 6; l++) {
+                R acc = 0;
+#pragma unroll
+                for (int iO = 0; iO < NMOD; iO++) {
+                    const C* fl = base + iO * sm + l * sl;
+                    for (int b = 0; b < 2; b++)
+                        for (int a = 0; a < 2; a++) {
+                            const C pc = cconv<R>(crs(pg.S[1][b] * pg.S[0][a], pg.ph[iO]));
+                            const C v = fl[a + b * sr];
+                            acc += pc.x * v.x - pc.y * v.y;
+                        }
+                }
+                f[l] = acc;
+            }
         }
     }
     if (n_undul) undul_field<R>(up, xp, yp, f);
@@ -144,14 +225,6 @@ __device__ __forceinline__ bool cell_of(double xp, double yp, double zp, const G
     return true;
 }
 
-__device__ __forceinline__ int wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
 
 // R: precision of momenta, weights and grids (double, or float in mixed
 // precision).  Positions, the push and the cell geometry stay FP64; each
@@ -429,7 +502,11 @@ constexpr int CMB_T = 256, CMB_LIST = 1024;
 // order (stable compaction with ballots), with their column range; each
 // thread then sums the listed slots that cover its node and adds the sum to
 // the grid once.  The order of the sum is fixed: layer 0 first, blocks in
-// index order within a layer.
+// index order within a layer.  The candidates of a layer come from the bucket
+// of row r - layer (sidx and bounds from the sort of the slot keys) or, when
+// bounds is null (small launches, where the sort's dozen launches cost more
+// than the scan), from all nb descriptors in block order: the same slots in
+// the same order, so the two forms give the same sums.
 template <int NMOD, typename RT>
 __global__ void __launch_bounds__(CMB_T) k_dep_combine(cplx_t<RT>* __restrict__ J, cplx_t<RT>* __restrict__ R,
                                                         const cplx_t<RT>* __restrict__ slotJ,
@@ -473,14 +550,15 @@ __global__ void __launch_bounds__(CMB_T) k_dep_combine(cplx_t<RT>* __restrict__ 
         const int row = r - layer;
         if (row < 0 || row >= nrb) continue;
         const int k = which * nrb + row;
-        const int b0 = bounds[2 * k], b1 = bounds[2 * k + 1];
+        const int b0 = bounds ? bounds[2 * k] : 0, b1 = bounds ? bounds[2 * k + 1] : nb;
         for (int q0 = b0; q0 < b1; q0 += CMB_T) {
             const int q = q0 + tid;
             int4 ent = make_int4(-1, 0, 0, 0);
             if (q < b1) {
-                const int b = sidx[q] - which * nb;
+                const int b = bounds ? sidx[q] - which * nb : q;
                 const int4 d = desc[which * nb + b];
-                if (layer < d.w && d.y < xs + CMB_T && d.y + d.z > xs) ent = make_int4(4 * b + layer, d.y, d.z, 0);
+                if (d.x == row && layer < d.w && d.y < xs + CMB_T && d.y + d.z > xs)
+                    ent = make_int4(4 * b + layer, d.y, d.z, 0);
             }
             // stable compaction: wave ballot + per-wave offsets
             const unsigned long long m = __ballot(ent.x >= 0);
@@ -642,7 +720,7 @@ static int32_t gather_push_impl(const double* coord, R* momenta, const R* wghts,
     }
     KTimer kt(KF_GATHER_PUSH);
 #define GATHER_LAUNCH(N)                                                                                      \
-    hipLaunchKernelGGL((k_gather_push<N, R>), blocks_for(np, 256), 256, 0, stream(), coord, s, momenta, s, wghts, \
+    hipLaunchKernelGGL((k_gather_push<N, R>), blocks_for(np, GP_T), GP_T, 0, stream(), coord, s, momenta, s, wghts, \
                        Fld, g, push_dt, up, nu, np)
     CH_MODE_SWITCH(nkO, GATHER_LAUNCH)
 #undef GATHER_LAUNCH
@@ -730,9 +808,29 @@ static int32_t push_deposit_impl(double* coord, double* coord_half, const R* mom
     if ((a.J && !a.slotJ) || (a.R && !a.slotR) || !a.desc || !keys || !bounds) return ST_ENOMEM;
     if (!a.J && !a.R) return ST_OK;
     KTimer kt(a.J ? KF_DEP_CURR : KF_DEP_DENS);
+    const int64_t sr = nx + 1, sm = sr * (nr + 1);
+    dim3 grid((unsigned)((nx + 1 + CMB_T - 1) / CMB_T), (unsigned)(nr + 1), 2u);
+    // small launches (up to 512 blocks, 131072 particles): every combine
+    // workgroup scans all descriptors (DEP_SLOT_R x nb / CMB_T rounds of
+    // coalesced L2 reads) instead of sorting them into buckets, which takes a
+    // dozen launches
+    const bool scan_all = nb <= 512;
+#define CMB_LAUNCH(N, IDX, BOUNDS)                                                                                \
+    hipLaunchKernelGGL((k_dep_combine<N, R>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR, a.desc, IDX, \
+                       BOUNDS, nb, nrb, sr, sm, nx + 1)
+    if (scan_all) {
+        // every descriptor of a deposited grid is written by its block
+#define DEP_LAUNCH(N) hipLaunchKernelGGL((k_push_deposit<N, R>), nb, DEP_THREADS, 0, stream(), a)
+        CH_MODE_SWITCH(nkO, DEP_LAUNCH)
+        CH_LAUNCH_CHECK("k_push_deposit");
+#define CMB_SCAN(N) CMB_LAUNCH(N, nullptr, nullptr)
+        CH_MODE_SWITCH(nkO, CMB_SCAN)
+#undef CMB_SCAN
+        CH_LAUNCH_CHECK("k_dep_combine");
+        return ST_OK;
+    }
     // descriptors of a grid not deposited here read "no slot" (row -1)
     CH_TRY(fill_bytes(a.desc, 0xFF, (size_t)2 * nb * sizeof(int4)));
-#define DEP_LAUNCH(N) hipLaunchKernelGGL((k_push_deposit<N, R>), nb, DEP_THREADS, 0, stream(), a)
     CH_MODE_SWITCH(nkO, DEP_LAUNCH)
 #undef DEP_LAUNCH
     CH_LAUNCH_CHECK("k_push_deposit");
@@ -749,12 +847,9 @@ static int32_t push_deposit_impl(double* coord, double* coord_half, const R* mom
     CH_TRY(fill_bytes(bounds, 0, (size_t)2 * 2 * nrb * sizeof(int32_t)));
     hipLaunchKernelGGL(k_slot_bounds, blocks_for(nd, 256), 256, 0, stream(), keys, nd, (uint32_t)(2 * nrb), bounds);
     CH_LAUNCH_CHECK("k_slot_bounds");
-    const int64_t sr = nx + 1, sm = sr * (nr + 1);
-    dim3 grid((unsigned)((nx + 1 + CMB_T - 1) / CMB_T), (unsigned)(nr + 1), 2u);
-#define CMB_LAUNCH(N)                                                                                           \
-    hipLaunchKernelGGL((k_dep_combine<N, R>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR, a.desc, idx, bounds, \
-                       nb, nrb, sr, sm, nx + 1)
-    CH_MODE_SWITCH(nkO, CMB_LAUNCH)
+#define CMB_SORTED(N) CMB_LAUNCH(N, idx, bounds)
+    CH_MODE_SWITCH(nkO, CMB_SORTED)
+#undef CMB_SORTED
 #undef CMB_LAUNCH
     CH_LAUNCH_CHECK("k_dep_combine");
     return ST_OK;
