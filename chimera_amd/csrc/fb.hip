@@ -265,6 +265,185 @@ __device__ __forceinline__ void mfma_stage(dbl4 (&acc)[4][WN], const double* __r
 // LDS bytes of one k_fb_gemm<NACC, WN> block: A and D stages, double-buffered.
 constexpr size_t gb_lds_bytes(int WN) { return 2 * (size_t)GB_K * ((GB_ROWS + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(double); }
 
+// Fast-path loads of the K-stages.  With the thread layout of the fetch
+// (element e = tid + GB_THREADS*i), a thread's A elements of a stage are
+// (x0 + tid%GB_X, k0 + tid/GB_X + (GB_THREADS/GB_X)*i) and its D elements
+// (k0 + tid%KD, n0 + tid/KD + (GB_THREADS/KD)*i): a per-lane byte offset that
+// is the same for every stage plus a uniform one.  They are issued as buffer
+// loads (segment base in a buffer resource, lane offset in a VGPR, uniform
+// offset in an SGPR) instead of a 64-bit address computation per element.
+// Used when the whole stage is in range (full x tile, full column tile, no K
+// remainder); the other stages take the guarded form.
+typedef float ch_vf2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ch_rsrc(const void* p) {
+    // raw buffer (stride 0), the whole 32-bit offset range, gfx9 word 3
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xFFFFFFFFu, 0x00020000);
+}
+// (the loaded vector is reinterpreted whole: taking its dwords one by one into
+// floats let the compiler narrow the 64-bit load to its first dword)
+__device__ __forceinline__ double2 ldb_c(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double2) {
+    const ch_vd2 v = __builtin_bit_cast(ch_vd2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ldb_c(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float2) {
+    const ch_vf2 v = __builtin_bit_cast(ch_vf2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ double ldb_r(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ float ldb_r(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+// Lane byte offset of a thread's A elements in segment S (x reflected for refl).
+template <typename R>
+__device__ __forceinline__ uint32_t seg_lane_a(const FbSegT<R>& S, int xg, int nx, int64_t lda, int kq0) {
+    const int xc = xg < nx ? xg : 0;
+    const int xr = S.refl ? (xc == 0 ? 0 : nx - xc) : xc;
+    return (uint32_t)((xr + lda * kq0) * (int64_t)sizeof(cplx_t<R>));
+}
+// c0*T(a0) + c1*T(a1) is a0 itself (bit for bit, for finite values)
+template <typename R>
+__device__ __forceinline__ bool seg_plain(const FbSegT<R>& S) {
+    return !S.refl && !S.a1 && S.c0.x == 1.0 && S.c0.y == 0.0;
+}
+
+// Output elements per batch of the epilogue, by element type and accumulator
+// registers (a batch holds up to seven loaded values and an offset per element).
+constexpr int gemm_eb(int rbytes, int nacc, int wn) {
+    return rbytes == 4 ? (nacc * wn <= 2 ? 8 : nacc * wn == 4 ? 4 : 2) : (nacc * wn <= 2 ? 4 : wn == 4 ? 1 : 2);
+}
+
+// Epilogue of both GEMM kernels.  For row tile i, column tile j and h = 0, 1 a
+// lane holds the complex output at column n0 + wc*16*WN + j*16 + lane%16 and
+// x = x0 + wr*32 + i*8 + (FP64: lane/16 + 4h; FP32: 2*(lane/16) + h).  The
+// output descriptor is copied to registers (read through the job pointer after
+// a store, it would be read again, since the outputs could alias the job
+// table), and the elements go in batches of EB whose loads are all issued
+// before the batch's stores: one memory round trip per batch instead of
+// several per element.  Every element is read and written by its own lane
+// only, so the results are those of one element at a time.
+template <typename R, int NACC, int WN, typename ACC>
+__device__ __forceinline__ void gemm_epilogue_serial(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[4][WN],
+                                                     const ACC (&acc1)[4][NACC == 2 ? WN : 1], int Ncol, int n0,
+                                                     int x0, int lane, int wr, int wc);
+
+template <typename R, int NACC, int WN, typename ACC>
+__device__ __forceinline__ void gemm_epilogue(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[4][WN],
+                                              const ACC (&acc1)[4][NACC == 2 ? WN : 1], int Ncol, int n0, int x0,
+                                              int lane, int wr, int wc) {
+    if constexpr (WN == 4) {
+        // the 128-column tiles of large grids have no registers to spare for a
+        // batch; their launches have enough blocks to hide the latency
+        gemm_epilogue_serial<R, NACC, WN>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
+        return;
+    }
+    typedef cplx_t<R> C;
+    constexpr int NE = 4 * WN * 2, EB = gemm_eb((int)sizeof(R), NACC, WN);
+    static_assert(NE % EB == 0, "epilogue batch");
+    const int nout = J.nout;
+    const bool beta = J.beta != 0;
+    for (int o = 0; o < nout; o++) {
+        const FbOutT<R> O = J.out[o];
+        const C al0 = cconv<R>(O.alpha0), al1 = cconv<R>(O.alpha1), gam = cconv<R>(O.gamma);
+        const R dti = (R)g.dt_inv;
+        const bool rd_p = O.poiss || beta;
+#pragma unroll
+        for (int b = 0; b < NE; b += EB) {
+            // out-of-range elements load element 0 (a valid address) and store nothing,
+            // so that the loads need no divergent branches
+            int64_t off[EB];
+            int xs[EB];
+            bool ok[EB];
+            C v[EB], dg[EB], pv[EB], n1[EB], n0v[EB];
+            R sc[EB], w2[EB];
+            double kxv[EB];
+#pragma unroll
+            for (int e = 0; e < EB; e++) {
+                const int q = b + e, i = q / (2 * WN), j = (q / 2) % WN, h = q % 2;
+                const int64_t ng = n0 + wc * 16 * WN + j * 16 + (lane & 15);
+                const int xg = x0 + wr * 32 + i * 8 + (sizeof(R) == 8 ? (lane >> 4) + 4 * h : 2 * (lane >> 4) + h);
+                ok[e] = ng < Ncol && xg < g.nx;
+                off[e] = ok[e] ? xg + g.ldo * ng : 0;
+                xs[e] = ok[e] ? xg : 0;
+                v[e] = cmul(al0, cmake<R>(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]));
+                if constexpr (NACC == 2) v[e] = cadd(v[e], cmul(al1, cmake<R>(acc1[i][j][2 * h], acc1[i][j][2 * h + 1])));
+            }
+            if (O.diag) {
+#pragma unroll
+                for (int e = 0; e < EB; e++) {
+                    dg[e] = gld2(O.diag, off[e]);
+                    kxv[e] = g.kx[xs[e]];
+                }
+            }
+            if (O.scale) {
+#pragma unroll
+                for (int e = 0; e < EB; e++) sc[e] = gld1(O.scale, off[e]);
+            }
+            if (O.poiss) {
+#pragma unroll
+                for (int e = 0; e < EB; e++) {
+                    n1[e] = gld2(O.poiss_np1, off[e]);
+                    n0v[e] = gld2(O.poiss_n, off[e]);
+                    w2[e] = gld1(O.poiss_w2, off[e]);
+                }
+            }
+            if (rd_p) {
+#pragma unroll
+                for (int e = 0; e < EB; e++) pv[e] = gld2(O.p, off[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < EB; e++) {
+                C w = v[e];
+                if (O.diag) w = cadd(w, crs((R)kxv[e], cmul(gam, dg[e])));
+                if (O.scale) w = crs(sc[e], w);
+                if (O.poiss) {
+                    // maxwell_solvers.f90:153-158: J += (v + (g_np1 - g_n)*dt_inv)*w2_inv
+                    w = cadd(pv[e], crs(w2[e], cadd(w, crs(dti, csub(n1[e], n0v[e])))));
+                } else if (beta) {
+                    w = cadd(pv[e], w);
+                }
+                if (ok[e]) gst2(O.p, off[e], w);
+            }
+        }
+    }
+}
+
+// One element at a time, in the order of the batched form.
+template <typename R, int NACC, int WN, typename ACC>
+__device__ __forceinline__ void gemm_epilogue_serial(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[4][WN],
+                                                     const ACC (&acc1)[4][NACC == 2 ? WN : 1], int Ncol, int n0,
+                                                     int x0, int lane, int wr, int wc) {
+    typedef cplx_t<R> C;
+    for (int o = 0; o < J.nout; o++) {
+        const FbOutT<R>& O = J.out[o];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < WN; j++) {
+                const int64_t ng = n0 + wc * 16 * WN + j * 16 + (lane & 15);
+                if (ng >= Ncol) continue;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int64_t xg = x0 + wr * 32 + i * 8 + (sizeof(R) == 8 ? (lane >> 4) + 4 * h : 2 * (lane >> 4) + h);
+                    if (xg >= g.nx) continue;
+                    C v = cmul(cconv<R>(O.alpha0), cmake<R>(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]));
+                    if constexpr (NACC == 2)
+                        v = cadd(v, cmul(cconv<R>(O.alpha1), cmake<R>(acc1[i][j][2 * h], acc1[i][j][2 * h + 1])));
+                    const int64_t off = xg + g.ldo * ng;
+                    if (O.diag) v = cadd(v, crs((R)g.kx[xg], cmul(cconv<R>(O.gamma), gld2(O.diag, off))));
+                    if (O.scale) v = crs(gld1(O.scale, off), v);
+                    if (O.poiss) {
+                        const C d = crs((R)g.dt_inv, csub(gld2(O.poiss_np1, off), gld2(O.poiss_n, off)));
+                        gst2(O.p, off, cadd(gld2(O.p, off), crs(gld1(O.poiss_w2, off), cadd(v, d))));
+                    } else {
+                        gst2(O.p, off, J.beta ? cadd(gld2(O.p, off), v) : v);
+                    }
+                }
+            }
+    }
+}
+
 // Block tile: 64 complex x (128 real rows) x 32*WN columns; 4 waves as 2 x 2,
 // each 64 rows x 16*WN columns.  NACC = 2 keeps a second accumulator set,
 // fed by segment 1 (launch_fb_gemm orders the segments so).  The K-stages of
@@ -299,7 +478,32 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     const FbSeg S1 = J.seg[nseg > 1 ? 1 : 0];
     double2 ra[NA], rb[NA];
     double rd[ND];
-    auto fetch_seg = [&](const FbSeg& S, int k0) __attribute__((always_inline)) {
+    // lane offsets of the fast-path loads, per segment
+    const int xq = tid % GB_X, kq0 = tid / GB_X;
+    const bool xfull = x0 + GB_X <= g.nx, nfull = n0 + BN <= Ncol;
+    const uint32_t la0 = seg_lane_a(S0, x0 + xq, g.nx, g.lda, kq0), la1 = seg_lane_a(S1, x0 + xq, g.nx, g.lda, kq0);
+    const uint32_t ld0 = (uint32_t)((tid % GB_K + S0.ldd * (tid / GB_K)) * (int64_t)sizeof(double));
+    const uint32_t ld1 = (uint32_t)((tid % GB_K + S1.ldd * (tid / GB_K)) * (int64_t)sizeof(double));
+    const bool plain0 = seg_plain(S0), plain1 = seg_plain(S1);
+    auto fetch_seg = [&](const FbSeg& S, int k0, uint32_t la, uint32_t ldo) __attribute__((always_inline)) {
+        if (xfull && nfull && k0 + GB_K <= K) {
+            // offsets within one segment's operand stay below 2^32 bytes (checked on the host)
+            const uint32_t row = (uint32_t)(g.lda * sizeof(double2));
+            const __amdgpu_buffer_rsrc_t rs_a = ch_rsrc(S.a0);
+#pragma unroll
+            for (int i = 0; i < NA; i++) ra[i] = ldb_c(rs_a, la, (uint32_t)(k0 + (GB_THREADS / GB_X) * i) * row, double2{});
+            if (S.a1) {
+                const __amdgpu_buffer_rsrc_t rs_b = ch_rsrc(S.a1);
+#pragma unroll
+                for (int i = 0; i < NA; i++) rb[i] = ldb_c(rs_b, la, (uint32_t)(k0 + (GB_THREADS / GB_X) * i) * row, double2{});
+            }
+            const uint32_t col = (uint32_t)(S.ldd * sizeof(double));
+            const __amdgpu_buffer_rsrc_t rs_d = ch_rsrc(S.D);
+#pragma unroll
+            for (int i = 0; i < ND; i++)
+                rd[i] = ldb_r(rs_d, ldo, (uint32_t)(k0 * sizeof(double)) + (uint32_t)(n0 + (GB_THREADS / GB_K) * i) * col, double{});
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
@@ -328,10 +532,10 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     };
     auto fetch = [&](int t) __attribute__((always_inline)) {
         const int k0 = (t % stages_per_seg) * GB_K;
-        if (t < stages_per_seg) fetch_seg(S0, k0);
-        else fetch_seg(S1, k0);
+        if (t < stages_per_seg) fetch_seg(S0, k0, la0, ld0);
+        else fetch_seg(S1, k0, la1, ld1);
     };
-    auto stash_seg = [&](const FbSeg& S, int buf) __attribute__((always_inline)) {
+    auto stash_seg = [&](const FbSeg& S, int buf, bool plain) __attribute__((always_inline)) {
         double* A = As + buf * A_STAGE;
         double* D = Ds + buf * D_STAGE;
 #pragma unroll
@@ -339,7 +543,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
             const int e = tid + GB_THREADS * i;
             const int xl = e % GB_X, kk = e / GB_X;
             const int c = xl & 7, slot = (xl >> 3) * 16 + (c & 3) + 8 * (c >> 2);
-            const double2 v = seg_finish(S, ra[i], rb[i]);
+            const double2 v = plain ? ra[i] : seg_finish(S, ra[i], rb[i]);
             A[kk * LDA + slot] = v.x;
             A[kk * LDA + slot + 4] = v.y;
         }
@@ -350,8 +554,8 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
         }
     };
     auto stash = [&](int buf, int t) __attribute__((always_inline)) {
-        if (t < stages_per_seg) stash_seg(S0, buf);
-        else stash_seg(S1, buf);
+        if (t < stages_per_seg) stash_seg(S0, buf, plain0);
+        else stash_seg(S1, buf, plain1);
     };
     // K-stages [tb, te) into one accumulator set; the prefetch runs across the
     // boundary between the two segments of a two-accumulator job
@@ -378,34 +582,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     } else {
         pipe(acc0, 0, nstages);
     }
-    // ---- epilogue: lane holds complex outputs at x = x0 + wr*32 + i*8 + lane/16 + 4h, h = 0,1
-    for (int o = 0; o < J.nout; o++) {
-        const FbOut& O = J.out[o];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < WN; j++) {
-                const int64_t ng = n0 + wc * 16 * WN + j * 16 + (lane & 15);
-                if (ng >= Ncol) continue;
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int64_t xg = x0 + wr * 32 + i * 8 + (lane >> 4) + 4 * h;
-                    if (xg >= g.nx) continue;
-                    double2 v = cmul(O.alpha0, cmk(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]));
-                    if constexpr (NACC == 2) v = cadd(v, cmul(O.alpha1, cmk(acc1[i][j][2 * h], acc1[i][j][2 * h + 1])));
-                    const int64_t off = xg + g.ldo * ng;
-                    if (O.diag) v = cadd(v, crs(g.kx[xg], cmul(O.gamma, gld2(O.diag, off))));
-                    if (O.scale) v = crs(gld1(O.scale, off), v);
-                    if (O.poiss) {
-                        // maxwell_solvers.f90:153-158: J += (v + (g_np1 - g_n)*dt_inv)*w2_inv
-                        const double2 d = crs(g.dt_inv, csub(gld2(O.poiss_np1, off), gld2(O.poiss_n, off)));
-                        gst2(O.p, off, cadd(gld2(O.p, off), crs(gld1(O.poiss_w2, off), cadd(v, d))));
-                    } else {
-                        gst2(O.p, off, J.beta ? cadd(gld2(O.p, off), v) : v);
-                    }
-                }
-            }
-    }
+    gemm_epilogue<double, NACC, WN>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
 }
 
 template <int NACC, int WN>
@@ -569,7 +746,32 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     const Seg S1 = J.seg[nseg > 1 ? 1 : 0];
     float2 ra[NA], rb[NA];
     float rd[ND];
-    auto fetch_seg = [&](const Seg& S, int k0) __attribute__((always_inline)) {
+    // lane offsets of the fast-path loads, per segment
+    const int xq = tid % GB_X, kq0 = tid / GB_X;
+    const bool xfull = x0 + GB_X <= g.nx, nfull = n0 + BN <= Ncol;
+    const uint32_t la0 = seg_lane_a(S0, x0 + xq, g.nx, g.lda, kq0), la1 = seg_lane_a(S1, x0 + xq, g.nx, g.lda, kq0);
+    const uint32_t ld0 = (uint32_t)((tid % GBF_K + S0.ldd * (tid / GBF_K)) * (int64_t)sizeof(float));
+    const uint32_t ld1 = (uint32_t)((tid % GBF_K + S1.ldd * (tid / GBF_K)) * (int64_t)sizeof(float));
+    const bool plain0 = seg_plain(S0), plain1 = seg_plain(S1);
+    auto fetch_seg = [&](const Seg& S, int k0, uint32_t la, uint32_t ldo) __attribute__((always_inline)) {
+        if (xfull && nfull && k0 + GBF_K <= K) {
+            // offsets within one segment's operand stay below 2^32 bytes (checked on the host)
+            const uint32_t row = (uint32_t)(g.lda * sizeof(float2));
+            const __amdgpu_buffer_rsrc_t rs_a = ch_rsrc(S.a0);
+#pragma unroll
+            for (int i = 0; i < NA; i++) ra[i] = ldb_c(rs_a, la, (uint32_t)(k0 + (GB_THREADS / GB_X) * i) * row, float2{});
+            if (S.a1) {
+                const __amdgpu_buffer_rsrc_t rs_b = ch_rsrc(S.a1);
+#pragma unroll
+                for (int i = 0; i < NA; i++) rb[i] = ldb_c(rs_b, la, (uint32_t)(k0 + (GB_THREADS / GB_X) * i) * row, float2{});
+            }
+            const uint32_t col = (uint32_t)(S.ldd * sizeof(float));
+            const __amdgpu_buffer_rsrc_t rs_d = ch_rsrc(S.D);
+#pragma unroll
+            for (int i = 0; i < ND; i++)
+                rd[i] = ldb_r(rs_d, ldo, (uint32_t)(k0 * sizeof(float)) + (uint32_t)(n0 + (GB_THREADS / GBF_K) * i) * col, float{});
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
@@ -597,17 +799,17 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     };
     auto fetch = [&](int t) __attribute__((always_inline)) {
         const int k0 = (t % stages_per_seg) * GBF_K;
-        if (t < stages_per_seg) fetch_seg(S0, k0);
-        else fetch_seg(S1, k0);
+        if (t < stages_per_seg) fetch_seg(S0, k0, la0, ld0);
+        else fetch_seg(S1, k0, la1, ld1);
     };
-    auto stash_seg = [&](const Seg& S, int buf) __attribute__((always_inline)) {
+    auto stash_seg = [&](const Seg& S, int buf, bool plain) __attribute__((always_inline)) {
         float* A = As + buf * A_STAGE;
         float* D = Ds + buf * D_STAGE;
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
             const int xl = e % GB_X, kk = e / GB_X;
-            const float2 v = seg_finish(S, ra[i], rb[i]);
+            const float2 v = plain ? ra[i] : seg_finish(S, ra[i], rb[i]);
             *reinterpret_cast<float2*>(A + kk * LDA + 2 * xl) = v;
         }
 #pragma unroll
@@ -617,8 +819,8 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
         }
     };
     auto stash = [&](int buf, int t) __attribute__((always_inline)) {
-        if (t < stages_per_seg) stash_seg(S0, buf);
-        else stash_seg(S1, buf);
+        if (t < stages_per_seg) stash_seg(S0, buf, plain0);
+        else stash_seg(S1, buf, plain1);
     };
     auto pipe = [&](flt4 (&acc)[4][WN], int tb, int te) __attribute__((always_inline)) {
         for (int t = tb; t < te; t++) {
@@ -643,34 +845,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     } else {
         pipe(acc0, 0, nstages);
     }
-    // ---- epilogue: lane holds complex outputs at x = x0 + wr*32 + i*8 + 2*(lane/16) + h, h = 0,1
-    for (int o = 0; o < J.nout; o++) {
-        const FbOutT<float>& O = J.out[o];
-        const float2 al0 = to_f(O.alpha0), al1 = to_f(O.alpha1), gam = to_f(O.gamma);
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < WN; j++) {
-                const int64_t ng = n0 + wc * 16 * WN + j * 16 + (lane & 15);
-                if (ng >= Ncol) continue;
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int64_t xg = x0 + wr * 32 + i * 8 + 2 * (lane >> 4) + h;
-                    if (xg >= g.nx) continue;
-                    float2 v = cmul(al0, cmkf(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]));
-                    if constexpr (NACC == 2) v = cadd(v, cmul(al1, cmkf(acc1[i][j][2 * h], acc1[i][j][2 * h + 1])));
-                    const int64_t off = xg + g.ldo * ng;
-                    if (O.diag) v = cadd(v, crs((float)g.kx[xg], cmul(gam, gld2(O.diag, off))));
-                    if (O.scale) v = crs(gld1(O.scale, off), v);
-                    if (O.poiss) {
-                        const float2 d = crs((float)g.dt_inv, csub(gld2(O.poiss_np1, off), gld2(O.poiss_n, off)));
-                        gst2(O.p, off, cadd(gld2(O.p, off), crs(gld1(O.poiss_w2, off), cadd(v, d))));
-                    } else {
-                        gst2(O.p, off, J.beta ? cadd(gld2(O.p, off), v) : v);
-                    }
-                }
-            }
-    }
+    gemm_epilogue<float, NACC, WN>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
 }
 
 template <int NACC, int WN>

@@ -810,11 +810,12 @@ static int32_t push_deposit_impl(double* coord, double* coord_half, const R* mom
     KTimer kt(a.J ? KF_DEP_CURR : KF_DEP_DENS);
     const int64_t sr = nx + 1, sm = sr * (nr + 1);
     dim3 grid((unsigned)((nx + 1 + CMB_T - 1) / CMB_T), (unsigned)(nr + 1), 2u);
-    // small launches (up to 512 blocks, 131072 particles): every combine
-    // workgroup scans all descriptors (DEP_SLOT_R x nb / CMB_T rounds of
-    // coalesced L2 reads) instead of sorting them into buckets, which takes a
-    // dozen launches
-    const bool scan_all = nb <= 512;
+    // small launches: every combine workgroup scans all descriptors
+    // (DEP_SLOT_R x nb / CMB_T rounds of coalesced L2 reads, 64 B per
+    // descriptor and workgroup) instead of sorting them into buckets, which
+    // takes a dozen dependent launches of about 5 us each; up to 512 blocks
+    // (131072 particles), or while the scans read at most 128 MiB in all
+    const bool scan_all = nb <= 512 || (int64_t)nb * grid.x * grid.y * grid.z <= ((int64_t)1 << 21);
 #define CMB_LAUNCH(N, IDX, BOUNDS)                                                                                \
     hipLaunchKernelGGL((k_dep_combine<N, R>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR, a.desc, IDX, \
                        BOUNDS, nb, nrb, sr, sm, nx + 1)
