@@ -198,16 +198,17 @@ template int32_t make_phase<float>(float2*, const double*, double, double, int, 
 
 // ================================================================== FB GEMM
 
-// Block tile: GB_X complex x (2*GB_X real rows) by 32*WN output columns; K
-// staged GB_K at a time through LDS.  Four waves in a 2 x 2 arrangement, each
-// owning 64 real rows x 16*WN columns as 4 x WN tiles of v_mfma_f64_16x16x4f64.
+// Block tile: GX complex x (2*GX real rows; GX = GB_X, or GB_X/2 for the
+// smallest launches) by 32*WN output columns; K staged GB_K at a time through
+// LDS.  Four waves in a 2 x 2 arrangement, each owning GX real rows x 16*WN
+// columns as (GX/16) x WN tiles of v_mfma_f64_16x16x4f64.
 // Lane layout of that instruction on gfx950 (checked on the device): lane l
 // supplies A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4i][l%16],
 // i = 0..3.  Each 16-row MFMA tile covers 8 complex x; complex c (0..7) puts
 // its real part in row (c%4) + 8*(c/4) and its imaginary part 4 rows below, so
 // that a lane's items (0,1) and (2,3) are the (re, im) of x = c and x = c + 4
 // with c = l/16.
-constexpr int GB_X = 64, GB_ROWS = 2 * GB_X, GB_K = 16, GB_THREADS = 256;
+constexpr int GB_X = 64, GB_K = 16, GB_THREADS = 256;
 // LDS row padding in doubles: rows 16 doubles (128 B) apart modulo 256 B put the
 // two rows that a half-wave reads in one ds_read_b64 on disjoint bank halves.
 constexpr int GB_PAD = 16;
@@ -242,32 +243,35 @@ __device__ __forceinline__ float2 seg_finish(const FbSegT<float>& s, float2 v0, 
     return v;
 }
 
-// One K-stage (GB_K = 16) of a wave's 64 x (16*WN) tile: 4 x WN MFMA tiles.
-template <int WN>
-__device__ __forceinline__ void mfma_stage(dbl4 (&acc)[4][WN], const double* __restrict__ As,
+// One K-stage (GB_K = 16) of a wave's (16*NI) x (16*WN) tile: NI x WN MFMA
+// tiles (NI = 4 for 64-complex x tiles, 2 for 32-complex ones).
+template <int WN, int NI>
+__device__ __forceinline__ void mfma_stage(dbl4 (&acc)[NI][WN], const double* __restrict__ As,
                                            const double* __restrict__ Ds, int lane, int wr, int wc) {
-    constexpr int LDA = GB_ROWS + GB_PAD, LDD = 32 * WN + GB_PAD;
+    constexpr int LDA = 32 * NI + GB_PAD, LDD = 32 * WN + GB_PAD;
 #pragma unroll
     for (int kq = 0; kq < GB_K / 4; kq++) {
         const int kk = kq * 4 + (lane >> 4);
-        double a[4], b[WN];
+        double a[NI], b[WN];
 #pragma unroll
-        for (int i = 0; i < 4; i++) a[i] = As[kk * LDA + wr * 64 + i * 16 + (lane & 15)];
+        for (int i = 0; i < NI; i++) a[i] = As[kk * LDA + wr * 16 * NI + i * 16 + (lane & 15)];
 #pragma unroll
         for (int j = 0; j < WN; j++) b[j] = Ds[kk * LDD + wc * 16 * WN + j * 16 + (lane & 15)];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NI; i++)
 #pragma unroll
             for (int j = 0; j < WN; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
 }
 
-// LDS bytes of one k_fb_gemm<NACC, WN> block: A and D stages, double-buffered.
-constexpr size_t gb_lds_bytes(int WN) { return 2 * (size_t)GB_K * ((GB_ROWS + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(double); }
+// LDS bytes of one k_fb_gemm<NACC, WN, GX> block: A and D stages, double-buffered.
+constexpr size_t gb_lds_bytes(int WN, int GX) {
+    return 2 * (size_t)GB_K * ((2 * GX + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(double);
+}
 
 // Fast-path loads of the K-stages.  With the thread layout of the fetch
 // (element e = tid + GB_THREADS*i), a thread's A elements of a stage are
-// (x0 + tid%GB_X, k0 + tid/GB_X + (GB_THREADS/GB_X)*i) and its D elements
+// (x0 + tid%GX, k0 + tid/GX + (GB_THREADS/GX)*i) and its D elements
 // (k0 + tid%KD, n0 + tid/KD + (GB_THREADS/KD)*i): a per-lane byte offset that
 // is the same for every stage plus a uniform one.  They are issued as buffer
 // loads (segment base in a buffer resource, lane offset in a VGPR, uniform
@@ -323,23 +327,23 @@ constexpr int gemm_eb(int rbytes, int nacc, int wn) {
 // before the batch's stores: one memory round trip per batch instead of
 // several per element.  Every element is read and written by its own lane
 // only, so the results are those of one element at a time.
-template <typename R, int NACC, int WN, typename ACC>
-__device__ __forceinline__ void gemm_epilogue_serial(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[4][WN],
-                                                     const ACC (&acc1)[4][NACC == 2 ? WN : 1], int Ncol, int n0,
+template <typename R, int NACC, int WN, int NI, typename ACC>
+__device__ __forceinline__ void gemm_epilogue_serial(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[NI][WN],
+                                                     const ACC (&acc1)[NI][NACC == 2 ? WN : 1], int Ncol, int n0,
                                                      int x0, int lane, int wr, int wc);
 
-template <typename R, int NACC, int WN, typename ACC>
-__device__ __forceinline__ void gemm_epilogue(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[4][WN],
-                                              const ACC (&acc1)[4][NACC == 2 ? WN : 1], int Ncol, int n0, int x0,
+template <typename R, int NACC, int WN, int NI, typename ACC>
+__device__ __forceinline__ void gemm_epilogue(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[NI][WN],
+                                              const ACC (&acc1)[NI][NACC == 2 ? WN : 1], int Ncol, int n0, int x0,
                                               int lane, int wr, int wc) {
     if constexpr (WN == 4) {
         // the 128-column tiles of large grids have no registers to spare for a
         // batch; their launches have enough blocks to hide the latency
-        gemm_epilogue_serial<R, NACC, WN>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
+        gemm_epilogue_serial<R, NACC, WN, NI>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
         return;
     }
     typedef cplx_t<R> C;
-    constexpr int NE = 4 * WN * 2, EB = gemm_eb((int)sizeof(R), NACC, WN);
+    constexpr int NE = NI * WN * 2, EB0 = gemm_eb((int)sizeof(R), NACC, WN), EB = EB0 < NE ? EB0 : NE;
     static_assert(NE % EB == 0, "epilogue batch");
     const int nout = J.nout;
     const bool beta = J.beta != 0;
@@ -362,7 +366,7 @@ __device__ __forceinline__ void gemm_epilogue(const FbJobT<R>& J, const FbGemmAr
             for (int e = 0; e < EB; e++) {
                 const int q = b + e, i = q / (2 * WN), j = (q / 2) % WN, h = q % 2;
                 const int64_t ng = n0 + wc * 16 * WN + j * 16 + (lane & 15);
-                const int xg = x0 + wr * 32 + i * 8 + (sizeof(R) == 8 ? (lane >> 4) + 4 * h : 2 * (lane >> 4) + h);
+                const int xg = x0 + wr * 8 * NI + i * 8 + (sizeof(R) == 8 ? (lane >> 4) + 4 * h : 2 * (lane >> 4) + h);
                 ok[e] = ng < Ncol && xg < g.nx;
                 off[e] = ok[e] ? xg + g.ldo * ng : 0;
                 xs[e] = ok[e] ? xg : 0;
@@ -410,22 +414,22 @@ __device__ __forceinline__ void gemm_epilogue(const FbJobT<R>& J, const FbGemmAr
 }
 
 // One element at a time, in the order of the batched form.
-template <typename R, int NACC, int WN, typename ACC>
-__device__ __forceinline__ void gemm_epilogue_serial(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[4][WN],
-                                                     const ACC (&acc1)[4][NACC == 2 ? WN : 1], int Ncol, int n0,
+template <typename R, int NACC, int WN, int NI, typename ACC>
+__device__ __forceinline__ void gemm_epilogue_serial(const FbJobT<R>& J, const FbGemmArgs& g, const ACC (&acc0)[NI][WN],
+                                                     const ACC (&acc1)[NI][NACC == 2 ? WN : 1], int Ncol, int n0,
                                                      int x0, int lane, int wr, int wc) {
     typedef cplx_t<R> C;
     for (int o = 0; o < J.nout; o++) {
         const FbOutT<R>& O = J.out[o];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NI; i++)
 #pragma unroll
             for (int j = 0; j < WN; j++) {
                 const int64_t ng = n0 + wc * 16 * WN + j * 16 + (lane & 15);
                 if (ng >= Ncol) continue;
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    const int64_t xg = x0 + wr * 32 + i * 8 + (sizeof(R) == 8 ? (lane >> 4) + 4 * h : 2 * (lane >> 4) + h);
+                    const int64_t xg = x0 + wr * 8 * NI + i * 8 + (sizeof(R) == 8 ? (lane >> 4) + 4 * h : 2 * (lane >> 4) + h);
                     if (xg >= g.nx) continue;
                     C v = cmul(cconv<R>(O.alpha0), cmake<R>(acc0[i][j][2 * h], acc0[i][j][2 * h + 1]));
                     if constexpr (NACC == 2)
@@ -450,28 +454,28 @@ __device__ __forceinline__ void gemm_epilogue_serial(const FbJobT<R>& J, const F
 // all segments form one pipeline: the global loads of stage t+1 go to
 // registers while the MFMAs of stage t run, then land in the other LDS
 // buffer, with one barrier per stage.
-template <int NACC, int WN>
+template <int NACC, int WN, int GX>
 __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restrict__ jobs, FbGemmArgs g) {
-    constexpr int BN = 32 * WN, LDA = GB_ROWS + GB_PAD, LDD = BN + GB_PAD;
+    constexpr int NI = GX / 16, BN = 32 * WN, LDA = 2 * GX + GB_PAD, LDD = BN + GB_PAD;
     constexpr int A_STAGE = GB_K * LDA, D_STAGE = GB_K * LDD;
     const FbJob& J = jobs[blockIdx.z];
     const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
     const int n0 = blockIdx.y * BN;
     if (n0 >= Ncol) return;
-    const int x0 = blockIdx.x * GB_X;
+    const int x0 = blockIdx.x * GX;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
 
     extern __shared__ double gb_smem[];
     double* const As = gb_smem;                  // [2][GB_K][LDA]
     double* const Ds = gb_smem + 2 * A_STAGE;    // [2][GB_K][LDD]
 
-    dbl4 acc0[4][WN], acc1[4][NACC == 2 ? WN : 1];
+    dbl4 acc0[NI][WN], acc1[NI][NACC == 2 ? WN : 1];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < NI; i++)
 #pragma unroll
         for (int j = 0; j < WN; j++) acc0[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-    constexpr int NA = (GB_K * GB_X) / GB_THREADS, ND = (GB_K * BN) / GB_THREADS;
+    constexpr int NA = (GB_K * GX) / GB_THREADS, ND = (GB_K * BN) / GB_THREADS;
     const int stages_per_seg = (K + GB_K - 1) / GB_K;
     const int nstages = stages_per_seg * nseg;
     const FbSeg S0 = J.seg[0];
@@ -479,23 +483,24 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     double2 ra[NA], rb[NA];
     double rd[ND];
     // lane offsets of the fast-path loads, per segment
-    const int xq = tid % GB_X, kq0 = tid / GB_X;
-    const bool xfull = x0 + GB_X <= g.nx, nfull = n0 + BN <= Ncol;
+    const int xq = tid % GX, kq0 = tid / GX;
+    const bool xfull = x0 + GX <= g.nx, nfull = n0 + BN <= Ncol;
     const uint32_t la0 = seg_lane_a(S0, x0 + xq, g.nx, g.lda, kq0), la1 = seg_lane_a(S1, x0 + xq, g.nx, g.lda, kq0);
     const uint32_t ld0 = (uint32_t)((tid % GB_K + S0.ldd * (tid / GB_K)) * (int64_t)sizeof(double));
     const uint32_t ld1 = (uint32_t)((tid % GB_K + S1.ldd * (tid / GB_K)) * (int64_t)sizeof(double));
     const bool plain0 = seg_plain(S0), plain1 = seg_plain(S1);
     auto fetch_seg = [&](const FbSeg& S, int k0, uint32_t la, uint32_t ldo) __attribute__((always_inline)) {
-        if (xfull && nfull && k0 + GB_K <= K) {
+        // (not in the 128-column FP64 tiles, which have no registers left for it)
+        if (WN < 4 && xfull && nfull && k0 + GB_K <= K) {
             // offsets within one segment's operand stay below 2^32 bytes (checked on the host)
             const uint32_t row = (uint32_t)(g.lda * sizeof(double2));
             const __amdgpu_buffer_rsrc_t rs_a = ch_rsrc(S.a0);
 #pragma unroll
-            for (int i = 0; i < NA; i++) ra[i] = ldb_c(rs_a, la, (uint32_t)(k0 + (GB_THREADS / GB_X) * i) * row, double2{});
+            for (int i = 0; i < NA; i++) ra[i] = ldb_c(rs_a, la, (uint32_t)(k0 + (GB_THREADS / GX) * i) * row, double2{});
             if (S.a1) {
                 const __amdgpu_buffer_rsrc_t rs_b = ch_rsrc(S.a1);
 #pragma unroll
-                for (int i = 0; i < NA; i++) rb[i] = ldb_c(rs_b, la, (uint32_t)(k0 + (GB_THREADS / GB_X) * i) * row, double2{});
+                for (int i = 0; i < NA; i++) rb[i] = ldb_c(rs_b, la, (uint32_t)(k0 + (GB_THREADS / GX) * i) * row, double2{});
             }
             const uint32_t col = (uint32_t)(S.ldd * sizeof(double));
             const __amdgpu_buffer_rsrc_t rs_d = ch_rsrc(S.D);
@@ -507,7 +512,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
-            const int xl = e % GB_X, kk = e / GB_X;
+            const int xl = e % GX, kk = e / GX;
             const int64_t xg = x0 + xl;
             const bool ok = xg < g.nx && k0 + kk < K;
             const int64_t off = seg_off(S, xg, k0 + kk, g.nx, g.lda);
@@ -517,7 +522,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
 #pragma unroll
             for (int i = 0; i < NA; i++) {
                 const int e = tid + GB_THREADS * i;
-                const int xl = e % GB_X, kk = e / GB_X;
+                const int xl = e % GX, kk = e / GX;
                 const int64_t xg = x0 + xl;
                 const bool ok = xg < g.nx && k0 + kk < K;
                 rb[i] = ok ? gld2(S.a1, seg_off(S, xg, k0 + kk, g.nx, g.lda)) : cmk(0.0, 0.0);
@@ -541,7 +546,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
-            const int xl = e % GB_X, kk = e / GB_X;
+            const int xl = e % GX, kk = e / GX;
             const int c = xl & 7, slot = (xl >> 3) * 16 + (c & 3) + 8 * (c >> 2);
             const double2 v = plain ? ra[i] : seg_finish(S, ra[i], rb[i]);
             A[kk * LDA + slot] = v.x;
@@ -559,10 +564,10 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     };
     // K-stages [tb, te) into one accumulator set; the prefetch runs across the
     // boundary between the two segments of a two-accumulator job
-    auto pipe = [&](dbl4 (&acc)[4][WN], int tb, int te) __attribute__((always_inline)) {
+    auto pipe = [&](dbl4 (&acc)[NI][WN], int tb, int te) __attribute__((always_inline)) {
         for (int t = tb; t < te; t++) {
             if (t + 1 < nstages) fetch(t + 1);
-            mfma_stage<WN>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
+            mfma_stage<WN, NI>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
             if (t + 1 < nstages) stash((t + 1) & 1, t + 1);
             __syncthreads();
         }
@@ -575,27 +580,27 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     if constexpr (NACC == 2) {
         pipe(acc0, 0, stages_per_seg);
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NI; i++)
 #pragma unroll
             for (int j = 0; j < WN; j++) acc1[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
         pipe(acc1, stages_per_seg, nstages);
     } else {
         pipe(acc0, 0, nstages);
     }
-    gemm_epilogue<double, NACC, WN>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
+    gemm_epilogue<double, NACC, WN, NI>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
 }
 
-template <int NACC, int WN>
+template <int NACC, int WN, int GX = GB_X>
 static int32_t launch_gemm_variant(const FbJob* dj, size_t njobs, int maxN, const FbGemmArgs& g) {
     constexpr int BN = 32 * WN;
     static bool attr = false;
     if (!attr) {
-        CH_HIP(hipFuncSetAttribute((const void*)k_fb_gemm<NACC, WN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)gb_lds_bytes(WN)));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fb_gemm<NACC, WN, GX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)gb_lds_bytes(WN, GX)));
         attr = true;
     }
-    dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
-    hipLaunchKernelGGL((k_fb_gemm<NACC, WN>), grid, GB_THREADS, gb_lds_bytes(WN), stream(), dj, g);
+    dim3 grid((unsigned)((g.nx + GX - 1) / GX), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
+    hipLaunchKernelGGL((k_fb_gemm<NACC, WN, GX>), grid, GB_THREADS, gb_lds_bytes(WN, GX), stream(), dj, g);
     CH_LAUNCH_CHECK("k_fb_gemm");
     return ST_OK;
 }
@@ -667,12 +672,17 @@ int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs&
         const int64_t xb = (g.nx + GB_X - 1) / GB_X;
         const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
         const bool narrow = xb * ((maxN + 63) / 64) * (int64_t)js.size() < 256;
+        // and 32-complex x tiles of 32 columns when even those leave CUs
+        // without two blocks: twice the blocks, half the MFMA chain per wave
+        const bool half = narrow && xb * ((maxN + 31) / 32) * (int64_t)js.size() < 512;
         int32_t st;
         if (pass == 0) st = wide ? launch_gemm_variant<1, 4>(dj, js.size(), maxN, g)
+                            : half ? launch_gemm_variant<1, 1, GB_X / 2>(dj, js.size(), maxN, g)
                             : narrow ? launch_gemm_variant<1, 1>(dj, js.size(), maxN, g)
                                      : launch_gemm_variant<1, 2>(dj, js.size(), maxN, g);
-        else st = narrow ? launch_gemm_variant<2, 1>(dj, js.size(), maxN, g)
-                         : launch_gemm_variant<2, 2>(dj, js.size(), maxN, g);
+        else st = half ? launch_gemm_variant<2, 1, GB_X / 2>(dj, js.size(), maxN, g)
+                  : narrow ? launch_gemm_variant<2, 1>(dj, js.size(), maxN, g)
+                           : launch_gemm_variant<2, 2>(dj, js.size(), maxN, g);
         CH_TRY(st);
     }
     return ST_OK;
@@ -693,53 +703,53 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 // the global-load latency of a stage and gain from half as many stages.
 constexpr int gbf_k(int nacc, int wn) { return wn == 1 ? 32 : 16; }
 
-template <int WN, int GBF_K>
-__device__ __forceinline__ void mfma_stage_f32(flt4 (&acc)[4][WN], const float* __restrict__ As,
+template <int WN, int GBF_K, int NI>
+__device__ __forceinline__ void mfma_stage_f32(flt4 (&acc)[NI][WN], const float* __restrict__ As,
                                                const float* __restrict__ Ds, int lane, int wr, int wc) {
-    constexpr int LDA = GB_ROWS + GB_PAD, LDD = 32 * WN + GB_PAD;
+    constexpr int LDA = 32 * NI + GB_PAD, LDD = 32 * WN + GB_PAD;
 #pragma unroll
     for (int kq = 0; kq < GBF_K / 4; kq++) {
         const int kk = kq * 4 + (lane >> 4);
-        float a[4], b[WN];
+        float a[NI], b[WN];
 #pragma unroll
-        for (int i = 0; i < 4; i++) a[i] = As[kk * LDA + wr * 64 + i * 16 + (lane & 15)];
+        for (int i = 0; i < NI; i++) a[i] = As[kk * LDA + wr * 16 * NI + i * 16 + (lane & 15)];
 #pragma unroll
         for (int j = 0; j < WN; j++) b[j] = Ds[kk * LDD + wc * 16 * WN + j * 16 + (lane & 15)];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NI; i++)
 #pragma unroll
             for (int j = 0; j < WN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
 }
 
-constexpr size_t gb_lds_bytes_f32(int WN, int GBF_K) {
-    return 2 * (size_t)GBF_K * ((GB_ROWS + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(float);
+constexpr size_t gb_lds_bytes_f32(int WN, int GBF_K, int GX) {
+    return 2 * (size_t)GBF_K * ((2 * GX + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(float);
 }
 
-template <int NACC, int WN>
+template <int NACC, int WN, int GX>
 __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<float>* __restrict__ jobs, FbGemmArgs g) {
     typedef FbSegT<float> Seg;
     constexpr int GBF_K = gbf_k(NACC, WN);
-    constexpr int BN = 32 * WN, LDA = GB_ROWS + GB_PAD, LDD = BN + GB_PAD;
+    constexpr int NI = GX / 16, BN = 32 * WN, LDA = 2 * GX + GB_PAD, LDD = BN + GB_PAD;
     constexpr int A_STAGE = GBF_K * LDA, D_STAGE = GBF_K * LDD;
     const FbJobT<float>& J = jobs[blockIdx.z];
     const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
     const int n0 = blockIdx.y * BN;
     if (n0 >= Ncol) return;
-    const int x0 = blockIdx.x * GB_X;
+    const int x0 = blockIdx.x * GX;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
 
     extern __shared__ __align__(16) unsigned char gbf_smem[];
     float* const As = reinterpret_cast<float*>(gbf_smem);  // [2][GBF_K][LDA]
     float* const Ds = As + 2 * A_STAGE;                       // [2][GBF_K][LDD]
 
-    flt4 acc0[4][WN], acc1[4][NACC == 2 ? WN : 1];
+    flt4 acc0[NI][WN], acc1[NI][NACC == 2 ? WN : 1];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < NI; i++)
 #pragma unroll
         for (int j = 0; j < WN; j++) acc0[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
 
-    constexpr int NA = (GBF_K * GB_X) / GB_THREADS, ND = (GBF_K * BN) / GB_THREADS;
+    constexpr int NA = (GBF_K * GX) / GB_THREADS, ND = (GBF_K * BN) / GB_THREADS;
     const int stages_per_seg = (K + GBF_K - 1) / GBF_K;
     const int nstages = stages_per_seg * nseg;
     const Seg S0 = J.seg[0];
@@ -747,8 +757,8 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     float2 ra[NA], rb[NA];
     float rd[ND];
     // lane offsets of the fast-path loads, per segment
-    const int xq = tid % GB_X, kq0 = tid / GB_X;
-    const bool xfull = x0 + GB_X <= g.nx, nfull = n0 + BN <= Ncol;
+    const int xq = tid % GX, kq0 = tid / GX;
+    const bool xfull = x0 + GX <= g.nx, nfull = n0 + BN <= Ncol;
     const uint32_t la0 = seg_lane_a(S0, x0 + xq, g.nx, g.lda, kq0), la1 = seg_lane_a(S1, x0 + xq, g.nx, g.lda, kq0);
     const uint32_t ld0 = (uint32_t)((tid % GBF_K + S0.ldd * (tid / GBF_K)) * (int64_t)sizeof(float));
     const uint32_t ld1 = (uint32_t)((tid % GBF_K + S1.ldd * (tid / GBF_K)) * (int64_t)sizeof(float));
@@ -759,11 +769,11 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
             const uint32_t row = (uint32_t)(g.lda * sizeof(float2));
             const __amdgpu_buffer_rsrc_t rs_a = ch_rsrc(S.a0);
 #pragma unroll
-            for (int i = 0; i < NA; i++) ra[i] = ldb_c(rs_a, la, (uint32_t)(k0 + (GB_THREADS / GB_X) * i) * row, float2{});
+            for (int i = 0; i < NA; i++) ra[i] = ldb_c(rs_a, la, (uint32_t)(k0 + (GB_THREADS / GX) * i) * row, float2{});
             if (S.a1) {
                 const __amdgpu_buffer_rsrc_t rs_b = ch_rsrc(S.a1);
 #pragma unroll
-                for (int i = 0; i < NA; i++) rb[i] = ldb_c(rs_b, la, (uint32_t)(k0 + (GB_THREADS / GB_X) * i) * row, float2{});
+                for (int i = 0; i < NA; i++) rb[i] = ldb_c(rs_b, la, (uint32_t)(k0 + (GB_THREADS / GX) * i) * row, float2{});
             }
             const uint32_t col = (uint32_t)(S.ldd * sizeof(float));
             const __amdgpu_buffer_rsrc_t rs_d = ch_rsrc(S.D);
@@ -775,7 +785,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
-            const int xl = e % GB_X, kk = e / GB_X;
+            const int xl = e % GX, kk = e / GX;
             const int64_t xg = x0 + xl;
             const bool ok = xg < g.nx && k0 + kk < K;
             ra[i] = ok ? gld2(S.a0, seg_off(S, xg, k0 + kk, g.nx, g.lda)) : cmkf(0.f, 0.f);
@@ -784,7 +794,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
 #pragma unroll
             for (int i = 0; i < NA; i++) {
                 const int e = tid + GB_THREADS * i;
-                const int xl = e % GB_X, kk = e / GB_X;
+                const int xl = e % GX, kk = e / GX;
                 const int64_t xg = x0 + xl;
                 const bool ok = xg < g.nx && k0 + kk < K;
                 rb[i] = ok ? gld2(S.a1, seg_off(S, xg, k0 + kk, g.nx, g.lda)) : cmkf(0.f, 0.f);
@@ -808,7 +818,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + GB_THREADS * i;
-            const int xl = e % GB_X, kk = e / GB_X;
+            const int xl = e % GX, kk = e / GX;
             const float2 v = plain ? ra[i] : seg_finish(S, ra[i], rb[i]);
             *reinterpret_cast<float2*>(A + kk * LDA + 2 * xl) = v;
         }
@@ -822,10 +832,10 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
         if (t < stages_per_seg) stash_seg(S0, buf, plain0);
         else stash_seg(S1, buf, plain1);
     };
-    auto pipe = [&](flt4 (&acc)[4][WN], int tb, int te) __attribute__((always_inline)) {
+    auto pipe = [&](flt4 (&acc)[NI][WN], int tb, int te) __attribute__((always_inline)) {
         for (int t = tb; t < te; t++) {
             if (t + 1 < nstages) fetch(t + 1);
-            mfma_stage_f32<WN, GBF_K>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
+            mfma_stage_f32<WN, GBF_K, NI>(acc, As + (t & 1) * A_STAGE, Ds + (t & 1) * D_STAGE, lane, wr, wc);
             if (t + 1 < nstages) stash((t + 1) & 1, t + 1);
             __syncthreads();
         }
@@ -838,27 +848,28 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     if constexpr (NACC == 2) {
         pipe(acc0, 0, stages_per_seg);
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NI; i++)
 #pragma unroll
             for (int j = 0; j < WN; j++) acc1[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
         pipe(acc1, stages_per_seg, nstages);
     } else {
         pipe(acc0, 0, nstages);
     }
-    gemm_epilogue<float, NACC, WN>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
+    gemm_epilogue<float, NACC, WN, NI>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
 }
 
-template <int NACC, int WN>
+template <int NACC, int WN, int GX = GB_X>
 static int32_t launch_gemm_variant_f32(const FbJobT<float>* dj, size_t njobs, int maxN, const FbGemmArgs& g) {
     constexpr int BN = 32 * WN;
+    constexpr size_t lds = gb_lds_bytes_f32(WN, gbf_k(NACC, WN), GX);
     static bool attr = false;
     if (!attr) {
-        CH_HIP(hipFuncSetAttribute((const void*)k_fb_gemm_f32<NACC, WN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)gb_lds_bytes_f32(WN, gbf_k(NACC, WN))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fb_gemm_f32<NACC, WN, GX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
         attr = true;
     }
-    dim3 grid((unsigned)((g.nx + GB_X - 1) / GB_X), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
-    hipLaunchKernelGGL((k_fb_gemm_f32<NACC, WN>), grid, GB_THREADS, gb_lds_bytes_f32(WN, gbf_k(NACC, WN)), stream(), dj, g);
+    dim3 grid((unsigned)((g.nx + GX - 1) / GX), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
+    hipLaunchKernelGGL((k_fb_gemm_f32<NACC, WN, GX>), grid, GB_THREADS, lds, stream(), dj, g);
     CH_LAUNCH_CHECK("k_fb_gemm_f32");
     return ST_OK;
 }
@@ -884,11 +895,16 @@ int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGe
         const int64_t xb = (g.nx + GB_X - 1) / GB_X;
         const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
         const bool narrow = xb * ((maxN + 63) / 64) * (int64_t)js.size() < 256;
+        // and 32-complex x tiles of 32 columns when even those leave CUs
+        // without two blocks: twice the blocks, half the MFMA chain per wave
+        const bool half = narrow && xb * ((maxN + 31) / 32) * (int64_t)js.size() < 512;
         int32_t st;
         if (pass == 0) st = wide ? launch_gemm_variant_f32<1, 4>(dj, js.size(), maxN, g)
+                            : half ? launch_gemm_variant_f32<1, 1, GB_X / 2>(dj, js.size(), maxN, g)
                             : narrow ? launch_gemm_variant_f32<1, 1>(dj, js.size(), maxN, g)
                                      : launch_gemm_variant_f32<1, 2>(dj, js.size(), maxN, g);
         else st = wide ? launch_gemm_variant_f32<2, 4>(dj, js.size(), maxN, g)
+                  : half ? launch_gemm_variant_f32<2, 1, GB_X / 2>(dj, js.size(), maxN, g)
                   : narrow ? launch_gemm_variant_f32<2, 1>(dj, js.size(), maxN, g)
                            : launch_gemm_variant_f32<2, 2>(dj, js.size(), maxN, g);
         CH_TRY(st);
