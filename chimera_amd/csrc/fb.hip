@@ -198,8 +198,8 @@ template int32_t make_phase<float>(float2*, const double*, double, double, int, 
 
 // ================================================================== FB GEMM
 
-// Block tile: GX complex x (2*GX real rows; GX = GB_X, or GB_X/2 for the
-// smallest launches) by 32*WN output columns; K staged GB_K at a time through
+// Block tile: GX complex x (2*GX real rows; GX = GB_X, or GB_X/2 or GB_X/4
+// for the smallest launches) by 32*WN output columns; K staged GB_K at a time through
 // LDS.  Four waves in a 2 x 2 arrangement, each owning GX real rows x 16*WN
 // columns as (GX/16) x WN tiles of v_mfma_f64_16x16x4f64.
 // Lane layout of that instruction on gfx950 (checked on the device): lane l
@@ -672,15 +672,19 @@ int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs&
         const int64_t xb = (g.nx + GB_X - 1) / GB_X;
         const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
         const bool narrow = xb * ((maxN + 63) / 64) * (int64_t)js.size() < 256;
-        // and 32-complex x tiles of 32 columns when even those leave CUs
-        // without two blocks: twice the blocks, half the MFMA chain per wave
-        const bool half = narrow && xb * ((maxN + 31) / 32) * (int64_t)js.size() < 512;
+        // and 32- or 16-complex x tiles of 32 columns when even those leave
+        // CUs without two blocks: two or four times the blocks, a half or a
+        // quarter of the dependent MFMA chain per wave
+        const int64_t nb32 = xb * ((maxN + 31) / 32) * (int64_t)js.size();
+        const bool half = narrow && nb32 < 512, quarter = narrow && nb32 < 256;
         int32_t st;
         if (pass == 0) st = wide ? launch_gemm_variant<1, 4>(dj, js.size(), maxN, g)
+                            : quarter ? launch_gemm_variant<1, 1, GB_X / 4>(dj, js.size(), maxN, g)
                             : half ? launch_gemm_variant<1, 1, GB_X / 2>(dj, js.size(), maxN, g)
                             : narrow ? launch_gemm_variant<1, 1>(dj, js.size(), maxN, g)
                                      : launch_gemm_variant<1, 2>(dj, js.size(), maxN, g);
-        else st = half ? launch_gemm_variant<2, 1, GB_X / 2>(dj, js.size(), maxN, g)
+        else st = quarter ? launch_gemm_variant<2, 1, GB_X / 4>(dj, js.size(), maxN, g)
+                  : half ? launch_gemm_variant<2, 1, GB_X / 2>(dj, js.size(), maxN, g)
                   : narrow ? launch_gemm_variant<2, 1>(dj, js.size(), maxN, g)
                            : launch_gemm_variant<2, 2>(dj, js.size(), maxN, g);
         CH_TRY(st);
@@ -895,15 +899,19 @@ int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGe
         const int64_t xb = (g.nx + GB_X - 1) / GB_X;
         const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
         const bool narrow = xb * ((maxN + 63) / 64) * (int64_t)js.size() < 256;
-        // and 32-complex x tiles of 32 columns when even those leave CUs
-        // without two blocks: twice the blocks, half the MFMA chain per wave
-        const bool half = narrow && xb * ((maxN + 31) / 32) * (int64_t)js.size() < 512;
+        // and 32- or 16-complex x tiles of 32 columns when even those leave
+        // CUs without two blocks: two or four times the blocks, a half or a
+        // quarter of the dependent MFMA chain per wave
+        const int64_t nb32 = xb * ((maxN + 31) / 32) * (int64_t)js.size();
+        const bool half = narrow && nb32 < 512, quarter = narrow && nb32 < 256;
         int32_t st;
         if (pass == 0) st = wide ? launch_gemm_variant_f32<1, 4>(dj, js.size(), maxN, g)
+                            : quarter ? launch_gemm_variant_f32<1, 1, GB_X / 4>(dj, js.size(), maxN, g)
                             : half ? launch_gemm_variant_f32<1, 1, GB_X / 2>(dj, js.size(), maxN, g)
                             : narrow ? launch_gemm_variant_f32<1, 1>(dj, js.size(), maxN, g)
                                      : launch_gemm_variant_f32<1, 2>(dj, js.size(), maxN, g);
         else st = wide ? launch_gemm_variant_f32<2, 4>(dj, js.size(), maxN, g)
+                  : quarter ? launch_gemm_variant_f32<2, 1, GB_X / 4>(dj, js.size(), maxN, g)
                   : half ? launch_gemm_variant_f32<2, 1, GB_X / 2>(dj, js.size(), maxN, g)
                   : narrow ? launch_gemm_variant_f32<2, 1>(dj, js.size(), maxN, g)
                            : launch_gemm_variant_f32<2, 2>(dj, js.size(), maxN, g);
