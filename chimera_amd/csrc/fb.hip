@@ -68,20 +68,68 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
     const int N = a.N;
     const int lpb = TOT / N;
     const int64_t line0 = (int64_t)blockIdx.x * lpb;
-    // ---- load (with the per-x pre-multiplier)
-    for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
-        const int64_t line = line0 + i / N;
-        const int x = i % N;
-        C v = cmake<R>(0, 0);
-        if (line < a.nlines) {
-            const int64_t r = line % a.n_inner, q = line / a.n_inner;
-            v = a.in[r * a.in_r + q * a.in_q + x];
-            if (a.pre) v = cmul(v, a.pre[x]);
+    const C* tw = a.tw;
+    const C* sph = nullptr;
+    if constexpr (EPT == 8) {
+        // lines of up to 2048 points: the loads go to registers first; where
+        // the LDS has room the twiddle table is staged there (the passes read
+        // it there instead of waiting on memory once per pass) and the phase
+        // table formed there while the loads are in flight
+        C vin[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+            const int i = threadIdx.x + FFT_THREADS * j;
+            const int64_t line = line0 + i / N;
+            vin[j] = cmake<R>(0, 0);
+            if (line < a.nlines) {
+                const int64_t r = line % a.n_inner, q = line / a.n_inner;
+                vin[j] = a.in[r * a.in_r + q * a.in_q + i % N];
+            }
         }
-        sbuf[i] = v;
+        C* const stw = sbuf + TOT;
+        if (a.tw_lds) {
+            for (int i = threadIdx.x; i < N; i += FFT_THREADS) stw[i] = a.tw[i];
+            tw = stw;
+        }
+        if (a.ph_at) {
+            // the table of k_phase, formed here: one launch fewer per transform
+            C* const p = stw + N;
+            const double lx = a.ph_leftX_dev ? *a.ph_leftX_dev : a.ph_leftX;
+            for (int i = threadIdx.x; i < N; i += FFT_THREADS) {
+                double sn, cs;
+                sincos(lx * a.ph_kx[i], &sn, &cs);
+                p[i] = cmake<R>((R)cs, (R)(a.ph_s * sn));
+            }
+            sph = p;
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+            const int i = threadIdx.x + FFT_THREADS * j;
+            const int x = i % N;
+            C v = vin[j];
+            if (line0 + i / N < a.nlines) {
+                if (a.pre) v = cmul(v, a.pre[x]);
+                else if (a.ph_at == 1) v = cmul(v, sph[x]);
+            }
+            sbuf[i] = v;
+        }
+    } else {
+        // ---- load (with the per-x pre-multiplier)
+        for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+            const int64_t line = line0 + i / N;
+            const int x = i % N;
+            C v = cmake<R>(0, 0);
+            if (line < a.nlines) {
+                const int64_t r = line % a.n_inner, q = line / a.n_inner;
+                v = a.in[r * a.in_r + q * a.in_q + x];
+                if (a.pre) v = cmul(v, a.pre[x]);
+            }
+            sbuf[i] = v;
+        }
     }
     __syncthreads();
-    fft_lds<EPT, R>(sbuf, N, a.log2N, a.sign, a.tw);
+    fft_lds<EPT, R>(sbuf, N, a.log2N, a.sign, tw);
     if (a.filtr) {
         // fb_io.f90:263: multiply the first nxfilt points, transform back
         for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
@@ -89,7 +137,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
             if (x < a.nxfilt) sbuf[i] = crs(a.filtr[x], sbuf[i]);
         }
         __syncthreads();
-        fft_lds<EPT, R>(sbuf, N, a.log2N, -a.sign, a.tw);
+        fft_lds<EPT, R>(sbuf, N, a.log2N, -a.sign, tw);
     }
     // ---- store (with post-multiplier, real scale table, mode normalisation)
     for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
@@ -99,6 +147,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
         const int64_t r = line % a.n_inner, q = line / a.n_inner;
         C v = sbuf[i];
         if (a.post) v = cmul(v, a.post[x]);
+        else if (a.ph_at == 2) v = cmul(v, sph[x]);
         if (a.scale) v = crs(a.scale[x + (int64_t)N * r], v);
         int m = 0;
         if (a.ghost) {
@@ -131,12 +180,23 @@ int32_t launch_fft(FftArgsT<R> a) {
     const int tot = ept * FFT_THREADS;
     const int lpb = tot / N;
     const unsigned grid = (unsigned)((a.nlines + lpb - 1) / lpb);
-    const size_t shmem = (size_t)tot * sizeof(C);
+    // LDS of an EPT = 8 workgroup kept within 48 KiB (three per CU): the
+    // twiddles and the phase table are staged there only when they fit
+    a.tw_lds = ept == 8 && (size_t)(tot + N) * sizeof(C) <= 48 * 1024;
+    if (a.ph_at && !(a.tw_lds && (size_t)(tot + 2 * N) * sizeof(C) <= 48 * 1024)) {
+        // no LDS room for the phase table in the transform: k_phase forms it
+        CH_TRY(make_phase<R>(a.ph_table, a.ph_kx, a.ph_leftX, a.ph_s, N, a.ph_leftX_dev));
+        if (a.ph_at == 1) a.pre = a.ph_table;
+        else a.post = a.ph_table;
+        a.ph_at = 0;
+    }
+    const size_t shmem = (size_t)(tot + (a.tw_lds ? N : 0) + (a.ph_at ? N : 0)) * sizeof(C);
     static bool attr_done = false;
     if (!attr_done) {
-        // complex128 lines of 4096 and 8192 points use 64 and 128 KiB of LDS per workgroup
+        // complex128 lines of 4096 and 8192 points use 64 and 128 KiB of LDS per
+        // workgroup; EPT = 8 at most 64 KiB (2048 complex128 points and their twiddles)
         CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<8, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(8 * FFT_THREADS * sizeof(C))));
+                                   (int)(64 * 1024)));
         CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<16, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(16 * FFT_THREADS * sizeof(C))));
         CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<32, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -548,7 +608,8 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
             const int e = tid + GB_THREADS * i;
             const int xl = e % GX, kk = e / GX;
             const int c = xl & 7, slot = (xl >> 3) * 16 + (c & 3) + 8 * (c >> 2);
-            const double2 v = plain ? ra[i] : seg_finish(S, ra[i], rb[i]);
+            // (the 128-column tiles always scale: the branch costs them a spill)
+            const double2 v = (WN < 4 && plain) ? ra[i] : seg_finish(S, ra[i], rb[i]);
             A[kk * LDA + slot] = v.x;
             A[kk * LDA + slot + 4] = v.y;
         }
@@ -650,6 +711,21 @@ static void merge_jobs(std::vector<FbJobT<R>>& two, const std::vector<FbJobT<R>>
     }
 }
 
+// Tile of a launch: the first of 128-, 64- and 32-column tiles of 64-complex x
+// and 32-column tiles of 32- and 16-complex x that makes at least
+// GEMM_MIN_BLOCKS blocks (two per CU), else the last.  Larger tiles re-read
+// less of A and D; smaller ones put waves on every SIMD when the grid is small.
+constexpr int64_t GEMM_MIN_BLOCKS = 512;
+enum { GT_W4 = 0, GT_W2, GT_W1, GT_H, GT_Q };
+static int gemm_tile(int nx, int maxN, size_t njobs, bool has_w4) {
+    const int64_t xb = (nx + GB_X - 1) / GB_X, nj = (int64_t)njobs, c32 = (maxN + 31) / 32;
+    const int64_t nb[GT_Q] = {xb * ((maxN + 127) / 128) * nj, xb * ((maxN + 63) / 64) * nj, xb * c32 * nj,
+                              ((nx + GB_X / 2 - 1) / (GB_X / 2)) * c32 * nj};
+    for (int t = has_w4 ? GT_W4 : GT_W2; t < GT_Q; t++)
+        if (nb[t] >= GEMM_MIN_BLOCKS) return t;
+    return GT_Q;
+}
+
 template <>
 int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family) {
     if (jobs.empty()) return ST_OK;
@@ -667,26 +743,24 @@ int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs&
         for (const auto& j : js) maxN = j.Ncol > maxN ? j.Ncol : maxN;
         const FbJob* dj = static_cast<const FbJob*>(device_table(js.data(), js.size() * sizeof(FbJob)));
         if (!dj) return ST_ENOMEM;
-        // 128-column tiles when they still give every CU two blocks, 32-column
-        // tiles when 64-column ones would leave CUs without a block (small grids)
-        const int64_t xb = (g.nx + GB_X - 1) / GB_X;
-        const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
-        const bool narrow = xb * ((maxN + 63) / 64) * (int64_t)js.size() < 256;
-        // and 32- or 16-complex x tiles of 32 columns when even those leave
-        // CUs without two blocks: two or four times the blocks, a half or a
-        // quarter of the dependent MFMA chain per wave
-        const int64_t nb32 = xb * ((maxN + 31) / 32) * (int64_t)js.size();
-        const bool half = narrow && nb32 < 512, quarter = narrow && nb32 < 256;
         int32_t st;
-        if (pass == 0) st = wide ? launch_gemm_variant<1, 4>(dj, js.size(), maxN, g)
-                            : quarter ? launch_gemm_variant<1, 1, GB_X / 4>(dj, js.size(), maxN, g)
-                            : half ? launch_gemm_variant<1, 1, GB_X / 2>(dj, js.size(), maxN, g)
-                            : narrow ? launch_gemm_variant<1, 1>(dj, js.size(), maxN, g)
-                                     : launch_gemm_variant<1, 2>(dj, js.size(), maxN, g);
-        else st = quarter ? launch_gemm_variant<2, 1, GB_X / 4>(dj, js.size(), maxN, g)
-                  : half ? launch_gemm_variant<2, 1, GB_X / 2>(dj, js.size(), maxN, g)
-                  : narrow ? launch_gemm_variant<2, 1>(dj, js.size(), maxN, g)
-                           : launch_gemm_variant<2, 2>(dj, js.size(), maxN, g);
+        const size_t nj = js.size();
+        if (pass == 0) {
+            switch (gemm_tile(g.nx, maxN, nj, true)) {
+                case GT_W4: st = launch_gemm_variant<1, 4>(dj, nj, maxN, g); break;
+                case GT_W2: st = launch_gemm_variant<1, 2>(dj, nj, maxN, g); break;
+                case GT_W1: st = launch_gemm_variant<1, 1>(dj, nj, maxN, g); break;
+                case GT_H: st = launch_gemm_variant<1, 1, GB_X / 2>(dj, nj, maxN, g); break;
+                default: st = launch_gemm_variant<1, 1, GB_X / 4>(dj, nj, maxN, g); break;
+            }
+        } else {
+            switch (gemm_tile(g.nx, maxN, nj, false)) {
+                case GT_W2: st = launch_gemm_variant<2, 2>(dj, nj, maxN, g); break;
+                case GT_W1: st = launch_gemm_variant<2, 1>(dj, nj, maxN, g); break;
+                case GT_H: st = launch_gemm_variant<2, 1, GB_X / 2>(dj, nj, maxN, g); break;
+                default: st = launch_gemm_variant<2, 1, GB_X / 4>(dj, nj, maxN, g); break;
+            }
+        }
         CH_TRY(st);
     }
     return ST_OK;
@@ -896,25 +970,20 @@ int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGe
         const FbJobT<float>* dj =
             static_cast<const FbJobT<float>*>(device_table(js.data(), js.size() * sizeof(FbJobT<float>)));
         if (!dj) return ST_ENOMEM;
-        const int64_t xb = (g.nx + GB_X - 1) / GB_X;
-        const bool wide = xb * ((maxN + 127) / 128) * (int64_t)js.size() >= 512;
-        const bool narrow = xb * ((maxN + 63) / 64) * (int64_t)js.size() < 256;
-        // and 32- or 16-complex x tiles of 32 columns when even those leave
-        // CUs without two blocks: two or four times the blocks, a half or a
-        // quarter of the dependent MFMA chain per wave
-        const int64_t nb32 = xb * ((maxN + 31) / 32) * (int64_t)js.size();
-        const bool half = narrow && nb32 < 512, quarter = narrow && nb32 < 256;
         int32_t st;
-        if (pass == 0) st = wide ? launch_gemm_variant_f32<1, 4>(dj, js.size(), maxN, g)
-                            : quarter ? launch_gemm_variant_f32<1, 1, GB_X / 4>(dj, js.size(), maxN, g)
-                            : half ? launch_gemm_variant_f32<1, 1, GB_X / 2>(dj, js.size(), maxN, g)
-                            : narrow ? launch_gemm_variant_f32<1, 1>(dj, js.size(), maxN, g)
-                                     : launch_gemm_variant_f32<1, 2>(dj, js.size(), maxN, g);
-        else st = wide ? launch_gemm_variant_f32<2, 4>(dj, js.size(), maxN, g)
-                  : quarter ? launch_gemm_variant_f32<2, 1, GB_X / 4>(dj, js.size(), maxN, g)
-                  : half ? launch_gemm_variant_f32<2, 1, GB_X / 2>(dj, js.size(), maxN, g)
-                  : narrow ? launch_gemm_variant_f32<2, 1>(dj, js.size(), maxN, g)
-                           : launch_gemm_variant_f32<2, 2>(dj, js.size(), maxN, g);
+        const size_t nj = js.size();
+        switch (gemm_tile(g.nx, maxN, nj, true)) {
+            case GT_W4: st = pass == 0 ? launch_gemm_variant_f32<1, 4>(dj, nj, maxN, g)
+                                       : launch_gemm_variant_f32<2, 4>(dj, nj, maxN, g); break;
+            case GT_W2: st = pass == 0 ? launch_gemm_variant_f32<1, 2>(dj, nj, maxN, g)
+                                       : launch_gemm_variant_f32<2, 2>(dj, nj, maxN, g); break;
+            case GT_W1: st = pass == 0 ? launch_gemm_variant_f32<1, 1>(dj, nj, maxN, g)
+                                       : launch_gemm_variant_f32<2, 1>(dj, nj, maxN, g); break;
+            case GT_H: st = pass == 0 ? launch_gemm_variant_f32<1, 1, GB_X / 2>(dj, nj, maxN, g)
+                                      : launch_gemm_variant_f32<2, 1, GB_X / 2>(dj, nj, maxN, g); break;
+            default: st = pass == 0 ? launch_gemm_variant_f32<1, 1, GB_X / 4>(dj, nj, maxN, g)
+                                    : launch_gemm_variant_f32<2, 1, GB_X / 4>(dj, nj, maxN, g); break;
+        }
         CH_TRY(st);
     }
     return ST_OK;
@@ -1124,9 +1193,14 @@ int32_t fb_forward_t(cplx_t<R>* out, cplx_t<R>* in, double leftX, const double* 
     g.ldo = nkx;
     g.kx = kx_dev;
     CH_TRY(launch_fb_gemm<R>(jobs, g, KF_DHT));
-    CH_TRY(make_phase<R>(ph, kx_dev, leftX, -1.0, nkx, leftX_dev));
     FftArgsT<R> a;
     memset(&a, 0, sizeof a);
+    a.ph_at = 2;
+    a.ph_s = -1.0;
+    a.ph_leftX = leftX;
+    a.ph_kx = kx_dev;
+    a.ph_leftX_dev = leftX_dev;
+    a.ph_table = ph;
     a.in = tmp;
     a.out = out;
     a.nlines = (int64_t)nkr * nmodes * ncomp;
@@ -1135,7 +1209,6 @@ int32_t fb_forward_t(cplx_t<R>* out, cplx_t<R>* in, double leftX, const double* 
     a.in_q = (int64_t)nkx * nkr * nmodes;
     a.out_r = nkx;
     a.out_q = (int64_t)nkx * nkr * nmodes;
-    a.post = ph;
     a.scale = scale_tab;
     a.sign = -1;
     a.N = nkx;
@@ -1164,9 +1237,14 @@ int32_t fb_inverse_t(cplx_t<R>* eb, const cplx_t<R>* e_fb, const cplx_t<R>* b_fb
     g.ldo = nkx;
     g.kx = kx_dev;
     CH_TRY(launch_fb_gemm<R>(jobs, g, KF_DHT));
-    CH_TRY(make_phase<R>(ph, kx_dev, leftX, +1.0, nkx, leftX_dev));
     FftArgsT<R> a;
     memset(&a, 0, sizeof a);
+    a.ph_at = 1;
+    a.ph_s = +1.0;
+    a.ph_leftX = leftX;
+    a.ph_kx = kx_dev;
+    a.ph_leftX_dev = leftX_dev;
+    a.ph_table = ph;
     a.in = tmp;
     a.out = eb + nkx;  // row 1
     a.nlines = (int64_t)nr * nmodes * 6;
@@ -1175,7 +1253,6 @@ int32_t fb_inverse_t(cplx_t<R>* eb, const cplx_t<R>* e_fb, const cplx_t<R>* b_fb
     a.in_q = (int64_t)nkx * nr;
     a.out_r = nkx;
     a.out_q = (int64_t)nkx * (nr + 1);
-    a.pre = ph;
     a.sign = +1;
     a.ghost = ghost;
     a.nmodes = nmodes;

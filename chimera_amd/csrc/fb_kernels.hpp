@@ -29,6 +29,16 @@ struct FftArgsT {
     int sign;              // -1 FFTW_FORWARD, +1 FFTW_BACKWARD
     int N, log2N;
     const C* tw;
+    // per-x phase exp(ph_s*i*kx*leftX) (leftX from ph_leftX_dev when set) as
+    // the pre (ph_at = 1) or post (ph_at = 2) multiplier: formed inside the
+    // kernel for lines of up to 2048 points when its LDS has room, else by
+    // k_phase into ph_table
+    int ph_at;
+    int tw_lds;            // set by launch_fft: twiddle table staged in LDS (EPT = 8)
+    double ph_s, ph_leftX;
+    const double* ph_kx;
+    const double* ph_leftX_dev;
+    C* ph_table;
 };
 typedef FftArgsT<double> FftArgs;
 
