@@ -159,7 +159,12 @@ def test_particle_bookkeeping(gpu, port):
         gpu.align_data_scl(w.copy(), np.array([0, 1, 10 ** 7]))
 
 
-@pytest.mark.parametrize("nkx,nr,nkr,M", [(64, 16, 16, 3), (512, 64, 64, 2), (8, 5, 7, 1), (4096, 12, 12, 1)])
+# 2048x128: 32- and 64-column GEMM tiles of 64-complex x, and 2048-point
+# complex128 lines whose twiddles stay in memory; 1024x64: twiddles staged in
+# LDS and the phase formed by its own launch; 512x64: both in LDS; 8192: one
+# line per workgroup (EPT = 32)
+@pytest.mark.parametrize("nkx,nr,nkr,M", [(64, 16, 16, 3), (512, 64, 64, 2), (8, 5, 7, 1), (4096, 12, 12, 1),
+                                          (2048, 128, 128, 2), (1024, 64, 64, 3), (8192, 16, 16, 1)])
 def test_fb_transforms(gpu, port, nkx, nr, nkr, M):
     say(f"case: fb_vec_in/fb_scl_in/fb_eb_out nkx={nkx} nr={nr} nkr={nkr} M={M}")
     r = rng(700 + nkx)
