@@ -337,7 +337,8 @@ constexpr size_t gb_lds_bytes(int WN, int GX) {
 // loads (segment base in a buffer resource, lane offset in a VGPR, uniform
 // offset in an SGPR) instead of a 64-bit address computation per element.
 // Used when the whole stage is in range (full x tile, full column tile, no K
-// remainder); the other stages take the guarded form.
+// remainder) and the host has checked that every offset of the launch fits 31
+// bits (FbGemmArgs::buf_ok); the other stages take the guarded form.
 typedef float ch_vf2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t ch_rsrc(const void* p) {
     // raw buffer (stride 0), the whole 32-bit offset range, gfx9 word 3
@@ -551,8 +552,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     const bool plain0 = seg_plain(S0), plain1 = seg_plain(S1);
     auto fetch_seg = [&](const FbSeg& S, int k0, uint32_t la, uint32_t ldo) __attribute__((always_inline)) {
         // (not in the 128-column FP64 tiles, which have no registers left for it)
-        if (WN < 4 && xfull && nfull && k0 + GB_K <= K) {
-            // offsets within one segment's operand stay below 2^32 bytes (checked on the host)
+        if (WN < 4 && g.buf_ok && xfull && nfull && k0 + GB_K <= K) {
             const uint32_t row = (uint32_t)(g.lda * sizeof(double2));
             const __amdgpu_buffer_rsrc_t rs_a = ch_rsrc(S.a0);
 #pragma unroll
@@ -711,6 +711,19 @@ static void merge_jobs(std::vector<FbJobT<R>>& two, const std::vector<FbJobT<R>>
     }
 }
 
+// Whether every byte offset the fast-path buffer loads form (A: K rows of lda
+// complex values; D: Ncol columns of ldd reals) stays below 2^31.
+template <typename R>
+static int gemm_offsets_fit(const std::vector<FbJobT<R>>& jobs, const FbGemmArgs& g) {
+    const int64_t lim = (int64_t)1 << 31;
+    for (const auto& j : jobs)
+        for (int s = 0; s < j.nseg; s++) {
+            if (((int64_t)j.K * g.lda + g.nx) * (int64_t)sizeof(cplx_t<R>) >= lim) return 0;
+            if (((int64_t)j.K + j.seg[s].ldd * (int64_t)j.Ncol) * (int64_t)sizeof(R) >= lim) return 0;
+        }
+    return 1;
+}
+
 // Tile of a launch: the first of 128-, 64- and 32-column tiles of 64-complex x
 // and 32-column tiles of 32- and 16-complex x that makes at least
 // GEMM_MIN_BLOCKS blocks (two per CU), else the last.  Larger tiles re-read
@@ -727,9 +740,11 @@ static int gemm_tile(int nx, int maxN, size_t njobs, bool has_w4) {
 }
 
 template <>
-int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs& g, int family) {
+int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs& g_in, int family) {
     if (jobs.empty()) return ST_OK;
     KTimer kt(family);
+    FbGemmArgs g = g_in;
+    g.buf_ok = gemm_offsets_fit(jobs, g);
     std::vector<FbJob> one, two;
     CH_TRY(split_jobs<double>(jobs, one, two));
     if (!two.empty() && !one.empty() && gemm_narrow(g.nx, jobs)) {
@@ -842,8 +857,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     const uint32_t ld1 = (uint32_t)((tid % GBF_K + S1.ldd * (tid / GBF_K)) * (int64_t)sizeof(float));
     const bool plain0 = seg_plain(S0), plain1 = seg_plain(S1);
     auto fetch_seg = [&](const Seg& S, int k0, uint32_t la, uint32_t ldo) __attribute__((always_inline)) {
-        if (xfull && nfull && k0 + GBF_K <= K) {
-            // offsets within one segment's operand stay below 2^32 bytes (checked on the host)
+        if (g.buf_ok && xfull && nfull && k0 + GBF_K <= K) {
             const uint32_t row = (uint32_t)(g.lda * sizeof(float2));
             const __amdgpu_buffer_rsrc_t rs_a = ch_rsrc(S.a0);
 #pragma unroll
@@ -953,9 +967,11 @@ static int32_t launch_gemm_variant_f32(const FbJobT<float>* dj, size_t njobs, in
 }
 
 template <>
-int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGemmArgs& g, int family) {
+int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGemmArgs& g_in, int family) {
     if (jobs.empty()) return ST_OK;
     KTimer kt(family);
+    FbGemmArgs g = g_in;
+    g.buf_ok = gemm_offsets_fit(jobs, g);
     std::vector<FbJobT<float>> one, two;
     CH_TRY(split_jobs<float>(jobs, one, two));
     if (!two.empty() && !one.empty() && gemm_narrow(g.nx, jobs)) {

@@ -84,6 +84,7 @@ struct FbGemmArgs {
     int64_t lda, ldo;
     const double* kx;
     double dt_inv;
+    int buf_ok;            // set by launch_fb_gemm: every operand offset fits the 31 bits of a buffer load
 };
 
 enum { FB_ROT = 0, FB_GRAD = 1, FB_DIV = 2, FB_GRADDIV = 3 };
