@@ -1,0 +1,119 @@
+// diag.hip -- on-device beam moments for the envelopes of diagnostics.py
+// (get_beam_envelops, diagnostics.py:177-207), so that a served simulation
+// can return its beam envelope without copying the particles to the host.
+//
+// The 13 weighted sums the envelope formulas need,
+//   S = [sum w, sum x w, sum y w, sum z w, sum x^2 w, sum y^2 w, sum z^2 w,
+//        sum px^2 w, sum py^2 w, sum pz^2 w, sum x px w, sum y py w, sum z pz w]
+// (x, y, z the half-step positions; products formed as the reference writes
+// them, x**2*w and x*px*w), are accumulated in FP64 in two launches: a
+// grid-stride pass that leaves one partial sum per workgroup, then one
+// workgroup that adds the partials.  Both reductions run in a fixed order for
+// a given particle count, so the sums are the same bits on every call.
+#include <algorithm>
+
+#include "common.hpp"
+#include "../../include/chimera_hip.h"
+
+namespace chimera {
+
+constexpr int MOM_T = 256, MOM_NS = 13, MOM_MAXB = 512;
+
+// Tree sum of v over the workgroup into sh[0]; every thread must call it.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = MOM_T / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename U>
+__global__ void __launch_bounds__(MOM_T) k_moments_part(const double* __restrict__ xh, const U* __restrict__ u,
+                                                        const U* __restrict__ w, int64_t ld, int np,
+                                                        double* __restrict__ part) {
+    __shared__ double sh[MOM_T];
+    double s[MOM_NS];
+#pragma unroll
+    for (int k = 0; k < MOM_NS; k++) s[k] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * MOM_T + threadIdx.x; i < np; i += (int64_t)gridDim.x * MOM_T) {
+        const double wi = (double)w[i];
+        const double x = xh[i], y = xh[ld + i], z = xh[2 * ld + i];
+        const double px = (double)u[i], py = (double)u[ld + i], pz = (double)u[2 * ld + i];
+        s[0] += wi;
+        s[1] += x * wi;
+        s[2] += y * wi;
+        s[3] += z * wi;
+        s[4] += x * x * wi;
+        s[5] += y * y * wi;
+        s[6] += z * z * wi;
+        s[7] += px * px * wi;
+        s[8] += py * py * wi;
+        s[9] += pz * pz * wi;
+        s[10] += x * px * wi;
+        s[11] += y * py * wi;
+        s[12] += z * pz * wi;
+    }
+#pragma unroll
+    for (int k = 0; k < MOM_NS; k++) {
+        const double t = block_sum(s[k], sh);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * MOM_NS + k] = t;
+    }
+}
+
+__global__ void __launch_bounds__(MOM_T) k_moments_final(const double* __restrict__ part, int nb,
+                                                         double* __restrict__ out) {
+    __shared__ double sh[MOM_T];
+    for (int k = 0; k < MOM_NS; k++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nb; b += MOM_T) v += part[(int64_t)b * MOM_NS + k];
+        const double t = block_sum(v, sh);
+        if (threadIdx.x == 0) out[k] = t;
+    }
+}
+
+template <typename U>
+static int32_t beam_moments_impl(const double* coord_half, const U* momenta, const U* wghts, int64_t ld, int32_t np,
+                                 double* sums) {
+    if (np < 0 || ld < np) return fail(ST_EINVAL, "beam_moments: np=%d ld=%lld", np, (long long)ld);
+    if (sums == nullptr) return fail(ST_EINVAL, "beam_moments: null output");
+    if (np > 0 && (!is_device_ptr(coord_half) || !is_device_ptr(momenta) || !is_device_ptr(wghts)))
+        return fail(ST_EINVAL, "beam_moments: the particle arrays are device pointers");
+    Stage st;
+    double* d_out = st.out(sums, MOM_NS);
+    CH_TRY(st.status());
+    if (np == 0) {
+        CH_TRY(fill_bytes(d_out, 0, MOM_NS * sizeof(double)));
+        return st.finish();
+    }
+    const int nb = (int)std::min<int64_t>(MOM_MAXB, (np + MOM_T - 1) / MOM_T);
+    double* part = static_cast<double*>(scratch(12, (size_t)nb * MOM_NS * sizeof(double)));
+    if (!part) return ST_ENOMEM;
+    hipLaunchKernelGGL((k_moments_part<U>), nb, MOM_T, 0, stream(), coord_half, momenta, wghts, ld, np, part);
+    CH_LAUNCH_CHECK("k_moments_part");
+    hipLaunchKernelGGL(k_moments_final, 1, MOM_T, 0, stream(), part, nb, d_out);
+    CH_LAUNCH_CHECK("k_moments_final");
+    return st.finish();
+}
+
+}  // namespace chimera
+
+using namespace chimera;
+
+extern "C" {
+
+int32_t chimera_beam_moments(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
+                             int32_t np, double* sums) {
+    return beam_moments_impl<double>(coord_half, momenta, wghts, ld, np, sums);
+}
+
+int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
+                                   int32_t np, double* sums) {
+    return beam_moments_impl<float>(coord_half, momenta, wghts, ld, np, sums);
+}
+
+}  // extern "C"

@@ -356,6 +356,21 @@ int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const dou
                           int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp);
 
 /* ======================================================================
+ * Diagnostics on the device (the reference computes them on the host from
+ * the particle arrays).
+ * ====================================================================== */
+/* The 13 weighted sums behind the beam envelopes of get_beam_envelops
+ * (diagnostics.py:177-207): sums[0..12] = sum w, sum x w, sum y w, sum z w,
+ * sum x^2 w, sum y^2 w, sum z^2 w, sum px^2 w, sum py^2 w, sum pz^2 w,
+ * sum x px w, sum y py w, sum z pz w over the np particles (x, y, z the rows
+ * of coord_half, px, py, pz those of momenta, component stride ld).  The
+ * particle arrays are in device memory; sums in device memory (the call is
+ * asynchronous) or host memory (the call waits).  FP64 sums in an order fixed
+ * by np, so repeated calls give the same bits. */
+int32_t chimera_beam_moments(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
+                             int32_t np, double* sums);
+
+/* ======================================================================
  * Mixed-precision resident tier (no counterpart in the reference, which is
  * FP64 throughout).  Particle positions stay FP64; momenta and weights are
  * float32; every grid and FB-space array is complex64 (interleaved float
@@ -376,6 +391,8 @@ int32_t chimera_soa_gather_push_mixed(const double* coord, float* momenta, const
                                       double push_dt, const double* undul_params, const double* frame);
 int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* momenta, float* wghts, int64_t ld,
                                   int32_t np, const double* lims, int32_t test_half, int32_t* np_out);
+int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
+                                   int32_t np, double* sums);
 int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double* kx, const float* In,
                               const float* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
                               int32_t fold, const double* frame);

@@ -1,0 +1,97 @@
+"""Beam envelopes (get_beam_envelops, diagnostics.py:177-207) from sums
+reduced on the device.
+
+CPU: the oracle's restatement is pinned by a two-particle closed form and by
+numpy's weighted average and covariance; the host formula that turns the 13
+sums into the envelope is checked against the oracle on random beams.
+GPU: the device sums of a bunch and of an lpa_cfg0 run, in FP64 and mixed
+precision, against the oracle on the same particles copied to the host
+(1e-9 relative: the sums differ in order only, and the rms and emittance
+formulas subtract), the same bits on a second call, zeros for no particles,
+and a refusal of host particle arrays.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import diagnostics_ref as D
+
+
+def test_oracle_envelope_closed_form():
+    x = np.array([[1.0, 3.0], [0.0, 0.0], [2.0, 2.0]])
+    u = np.array([[2.0, 2.0], [1.0, -1.0], [0.0, 4.0]])
+    w = np.array([1.0, 1.0])
+    env = D.beam_envelope(x, u, w)
+    assert np.allclose(env[0], [2.0, 0.0, 2.0])
+    assert np.allclose(env[1], [1.0, 0.0, 0.0])
+    # x: sqrt(<x^2><px^2> - <x px>^2) = sqrt(5*4 - 4^2); y: 0; z: sqrt(4*8 - 4^2)
+    assert np.allclose(env[2], [2.0, 0.0, 4.0])
+
+
+def test_oracle_envelope_against_numpy():
+    r = np.random.default_rng(11)
+    x, u = r.normal(size=(3, 5000)) * [[3.0], [0.5], [0.7]] + [[-4.0], [0.1], [0.0]], r.normal(size=(3, 5000))
+    w = r.uniform(0.1, 2.0, 5000)
+    env = D.beam_envelope(x, u, w)
+    for c in range(3):
+        assert np.isclose(env[0, c], np.average(x[c], weights=w), rtol=1e-12)
+        assert np.isclose(env[1, c], np.sqrt(np.cov(x[c], aweights=w, bias=True)), rtol=1e-10)
+
+
+def test_envelope_from_sums_matches_oracle():
+    from chimera_amd.diagnostics import Diagnostics
+    r = np.random.default_rng(12)
+    x, u, w = r.normal(size=(3, 3000)) - 2.0, r.normal(size=(3, 3000)) + 5.0, r.uniform(0.5, 1.5, 3000)
+    env = Diagnostics.envelope_from_sums(D.moment_sums(x, u, w))
+    assert np.allclose(env, D.beam_envelope(x, u, w), rtol=1e-12, atol=0)
+
+
+def _runs(precision):
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    bunch = W.with_precision(W.beam(Nx=256, Nkr=48, gamma=150.0, sig_x=2.0, sig_r=1.0, emittance=0.5, n_target=5e4,
+                                    K=2.0, lambda_u=150.0, steps=6), precision)
+    runs = [W.build_beam(G, bunch, 3), W.build(G, W.with_precision(W.lpa_cfg0(), precision), 5)]
+    for run in runs:
+        for i in range(1, 5):
+            run.make_step(i)
+    return runs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_envelopes_match_oracle(precision):
+    from chimera_amd.diagnostics import Diagnostics
+    for run in _runs(precision):
+        dg = Diagnostics(run)
+        envs = dg.get_beam_envelops()
+        live = [sp for sp in run.Particles if "Still" not in sp.Args["Features"]]
+        assert len(envs) == len(live) >= 1
+        for env, sp in zip(envs, live):
+            xh = sp.host("coords_halfstep").astype(np.float64)
+            u, w = sp.host("momenta").astype(np.float64), sp.host("weights").astype(np.float64)
+            ref = D.beam_envelope(xh, u, w)
+            # means relative to |mean| + rms (a centred beam has means near 0)
+            scale = np.abs(ref) + np.r_[[ref[1]], np.zeros((2, 3))]
+            err = np.max(np.abs(env - ref) / np.maximum(scale, 1e-300))
+            print(f"case: {precision} {sp.np} particles, envelope rel err {err:.2e}", flush=True)
+            assert err < 1e-9, (env, ref)
+            s1, s2 = dg.beam_moment_sums(sp), dg.beam_moment_sums(sp)
+            assert np.array_equal(s1, s2)
+            # each sum against the sum of its terms' magnitudes (order of summation only)
+            mag = D.moment_sums(np.abs(xh), np.abs(u), np.abs(w))
+            assert np.all(np.abs(s1 - D.moment_sums(xh, u, w)) <= 1e-12 * mag), (s1, D.moment_sums(xh, u, w))
+
+
+@pytest.mark.gpu
+def test_moments_edge_cases():
+    from chimera_amd._lib import lib
+    from chimera_amd.device import DeviceArray
+    L = lib()
+    out = np.full(13, 7.0)
+    x, u, w = DeviceArray((8, 3), np.float64), DeviceArray((8, 3), np.float64), DeviceArray((8,), np.float64)
+    assert L.chimera_beam_moments(x.ptr, u.ptr, w.ptr, 8, 0, out.ctypes.data) == 0
+    assert not np.any(out)
+    host = np.zeros(24)
+    assert L.chimera_beam_moments(host.ctypes.data, u.ptr, w.ptr, 8, 8, out.ctypes.data) != 0
+    assert L.chimera_beam_moments(x.ptr, u.ptr, w.ptr, 4, 8, out.ctypes.data) != 0
