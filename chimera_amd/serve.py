@@ -13,8 +13,9 @@ steps recorded into graphs and replayed) and schedules them:
   free HBM (less a reserve); it is built (plasma generation, tables, initial
   half step) on one of a few builder threads, then advanced a few steps at a time in turn
   with every other active simulation, so that the work of requests that
-  arrived at different times overlaps on the device.  Each simulation's
-  arrays and context are freed as soon as it completes.
+  arrived at different times overlaps on the device.  A finished
+  simulation's result, the beam envelopes of its pushed species, is reduced
+  on the device into Request.result; then its arrays and context are freed.
 """
 from __future__ import annotations
 
@@ -25,6 +26,7 @@ import time
 import numpy as np
 
 from .batch import Simulation, free_device_bytes
+from .diagnostics import Diagnostics
 
 GIB = 1 << 30
 
@@ -64,6 +66,7 @@ class Request:
         self.sim = None
         self.t_admit = self.t_built = self.t_done = None
         self.particle_steps = 0
+        self.result = None             # beam envelopes of the finished run (diagnostics.get_beam_envelops)
 
     def build(self, graphs=True):
         import chimera_amd as G
@@ -205,6 +208,8 @@ class Server:
             if s.finished_issuing:
                 if s.done():
                     r.t_done = s.t_done
+                    with s.ctx:
+                        r.result = Diagnostics(s.run).get_beam_envelops()
                     if self.on_finish is not None:
                         self.on_finish(r)
                     s.close()

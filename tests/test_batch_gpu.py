@@ -195,6 +195,7 @@ def test_served_results_match_alone(precision):
         server.close()
     assert sorted(served) == sorted(alone)
     assert all(r.t_done >= r.t_arrival_host for r in server.finished)
+    assert all(r.result is not None and len(r.result) == 1 and r.result[0].shape == (3, 3) for r in server.finished)
     for name in alone:
         for k in ("EB", "EG_fb", "momenta", "coords"):
             e = rel_l2(served[name][k], alone[name][k])
