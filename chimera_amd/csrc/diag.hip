@@ -1,6 +1,7 @@
-// diag.hip -- on-device beam moments for the envelopes of diagnostics.py
-// (get_beam_envelops, diagnostics.py:177-207), so that a served simulation
-// can return its beam envelope without copying the particles to the host.
+// diag.hip -- diagnostics on the device: the beam moments behind the envelopes
+// of get_beam_envelops (diagnostics.py:177-207), so that a served simulation
+// can return its beam envelope without copying the particles to the host, and
+// the EM energy per kx of nrg_out (diagnostics.py:109-122).
 //
 // The 13 weighted sums the envelope formulas need,
 //   S = [sum w, sum x w, sum y w, sum z w, sum x^2 w, sum y^2 w, sum z^2 w,
@@ -100,6 +101,48 @@ static int32_t beam_moments_impl(const double* coord_half, const U* momenta, con
     return st.finish();
 }
 
+// ---------------------------------------------------------------- EM energy per kx
+//
+// nrg_out (diagnostics.py:109-122): out[x] = sum over k < nkr, m < nmodes of
+// EnergyFact[x,k,m] * (|E_x|^2 + |E_y|^2 + |E_z|^2)[x,k,m] from EG_fb(nkx,
+// nkr, nmodes, 6) (the first three components are E), summed in FP64 by one
+// thread per kx in (k, m) order: the same bits on every call.
+template <typename C>
+__global__ void k_field_energy(const C* __restrict__ eg, const double* __restrict__ ef, int nkx, int nkr, int nmodes,
+                               double* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nkx) return;
+    const int64_t cs = (int64_t)nkx * nkr * nmodes;   // component stride
+    double acc = 0.0;
+    for (int m = 0; m < nmodes; m++)
+        for (int k = 0; k < nkr; k++) {
+            const int64_t o = x + (int64_t)nkx * (k + (int64_t)nkr * m);
+            double e2 = 0.0;
+            for (int c = 0; c < 3; c++) {
+                const C v = eg[o + c * cs];
+                e2 += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            }
+            acc += e2 * ef[o];
+        }
+    out[x] = acc;
+}
+
+template <typename C>
+static int32_t field_energy_impl(const C* eg_fb, const double* energy_fact, int32_t nkx, int32_t nkr, int32_t nmodes,
+                                 double* out) {
+    if (nkx < 1 || nkr < 1 || nmodes < 1) return fail(ST_EINVAL, "field_energy: bad sizes");
+    if (out == nullptr) return fail(ST_EINVAL, "field_energy: null output");
+    if (!is_device_ptr(eg_fb) || !is_device_ptr(energy_fact))
+        return fail(ST_EINVAL, "field_energy: EG_fb and EnergyFact are device pointers");
+    Stage st;
+    double* d_out = st.out(out, (size_t)nkx);
+    CH_TRY(st.status());
+    hipLaunchKernelGGL((k_field_energy<C>), blocks_for(nkx, 256), 256, 0, stream(), eg_fb, energy_fact, nkx, nkr,
+                       nmodes, d_out);
+    CH_LAUNCH_CHECK("k_field_energy");
+    return st.finish();
+}
+
 }  // namespace chimera
 
 using namespace chimera;
@@ -114,6 +157,16 @@ int32_t chimera_beam_moments(const double* coord_half, const double* momenta, co
 int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
                                    int32_t np, double* sums) {
     return beam_moments_impl<float>(coord_half, momenta, wghts, ld, np, sums);
+}
+
+int32_t chimera_field_energy(const double* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr, int32_t nmodes,
+                             double* out) {
+    return field_energy_impl<double2>(reinterpret_cast<const double2*>(EG_fb), energy_fact, nkx, nkr, nmodes, out);
+}
+
+int32_t chimera_field_energy_c8(const float* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr,
+                                int32_t nmodes, double* out) {
+    return field_energy_impl<float2>(reinterpret_cast<const float2*>(EG_fb), energy_fact, nkx, nkr, nmodes, out);
 }
 
 }  // extern "C"

@@ -1,17 +1,23 @@
 """Diagnostics computed on the device, with the interface of the reference's
-Diagnostics class (diagnostics.py:25-207) for what a served simulation
-returns: the beam envelopes of get_beam_envelops, formed on the host from 13
-weighted sums that chimera_beam_moments reduces on the GPU, so that no
-particle array leaves the device.
+Diagnostics class (diagnostics.py:25-207):
 
-The reference's dump diagnostics (fields, FB fields, particles, density, EM
-energy, power; diagnostics.py:33-150), which write files, are not provided.
+* get_beam_envelops: the beam envelopes, formed on the host from 13 weighted
+  sums that chimera_beam_moments reduces on the GPU, so that no particle
+  array leaves the device;
+* nrg_out: the EM energy per kx (chimera_field_energy), returned ('Return'
+  in the diagnostic's Features) or appended to nrg_<solver>.txt as the
+  reference does; do_diags runs the 'EnergyEM' entries of the list.
+
+The other dump diagnostics of the reference (fields, FB fields, particles,
+density, power; diagnostics.py:46-150) are not provided.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
-from .device import call
+from .device import DeviceArray, call
 
 
 class Diagnostics:
@@ -19,6 +25,57 @@ class Diagnostics:
         self.Chimera = Chimera
         self.diags = diags
         self.out_folder = out_folder
+        if out_folder is not None:
+            # created if absent (the reference deletes and recreates it)
+            os.makedirs(out_folder, exist_ok=True)
+
+    def do_diags(self, i):
+        """diagnostics.py:34-46 for the types provided here ('EnergyEM')."""
+        for diag in self.diags:
+            if np.mod(i, diag["Step"]) != 0:
+                continue
+            diag.setdefault("Features", {})
+            if diag["Type"] == "EnergyEM":
+                self.nrg_out(diag)
+            else:
+                raise NotImplementedError(f"diagnostic type {diag['Type']!r} is not provided on the device")
+
+    @staticmethod
+    def energy_fact_table(A):
+        """EnergyFact of solvers.py:142-149 from a solver's Args (host array)."""
+        from scipy.constants import c, e, epsilon_0, m_e
+        from scipy.special import jn
+        tmp_idx = (np.abs(A["idxM"]) + 1)[None, None, :]
+        EF = 0.5e-2 * (m_e * c ** 2 / e) ** 2 * (4 * np.pi * epsilon_0) * A["lengthR"] ** 2 / A["dkx"] \
+            * jn(tmp_idx, A["kr_g"] * A["lengthR"]) ** 2
+        if "KxShift" not in A:
+            EF *= 0.5
+        return np.asfortranarray(EF)
+
+    @classmethod
+    def energy_fact(cls, solver):
+        """EnergyFact kept on the device with the solver."""
+        if "EnergyFact" not in solver.dev:
+            solver.dev["EnergyFact"] = DeviceArray.from_numpy(cls.energy_fact_table(solver.Args), np.float64)
+        return solver.dev["EnergyFact"]
+
+    def nrg_out(self, diag):
+        """diagnostics.py:109-122."""
+        ret = []
+        for jj, sol in enumerate(self.Chimera.Solvers):
+            Nx, Nkr, M = sol.n_fb
+            dat = np.zeros(Nx)
+            ef = self.energy_fact(sol)
+            call("chimera_field_energy_c8" if sol.mixed else "chimera_field_energy", sol.Data["EG_fb"].ptr, ef.ptr,
+                 Nx, Nkr, M, dat.ctypes.data)
+            dat = np.r_[dat[dat.shape[0] // 2 + 1:], dat[:dat.shape[0] // 2 + 1]]
+            if "Return" in diag["Features"]:
+                ret.append(dat.copy())
+            else:
+                with open(os.path.join(self.out_folder, f"nrg_{jj}.txt"), mode="a") as fout:
+                    np.savetxt(fout, dat[None, :])
+        if "Return" in diag["Features"]:
+            return ret
 
     @staticmethod
     def beam_moment_sums(species):

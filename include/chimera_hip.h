@@ -370,6 +370,14 @@ int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const dou
 int32_t chimera_beam_moments(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
                              int32_t np, double* sums);
 
+/* EM energy per kx of nrg_out (diagnostics.py:109-122), before its reordering
+ * of the kx axis: out[x] = sum_{k,m} EnergyFact[x,k,m] * sum_{c<3} |EG_fb[x,k,m,c]|^2
+ * with EG_fb(nkx,nkr,nmodes,6) and EnergyFact(nkx,nkr,nmodes) (solvers.py:142-149)
+ * in device memory; out (nkx doubles) in device memory (asynchronous) or host
+ * memory (the call waits).  FP64 sums in a fixed order. */
+int32_t chimera_field_energy(const double* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr, int32_t nmodes,
+                             double* out);
+
 /* ======================================================================
  * Mixed-precision resident tier (no counterpart in the reference, which is
  * FP64 throughout).  Particle positions stay FP64; momenta and weights are
@@ -393,6 +401,8 @@ int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* mome
                                   int32_t np, const double* lims, int32_t test_half, int32_t* np_out);
 int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
                                    int32_t np, double* sums);
+int32_t chimera_field_energy_c8(const float* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr,
+                                int32_t nmodes, double* out);
 int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double* kx, const float* In,
                               const float* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
                               int32_t fold, const double* frame);

@@ -1,6 +1,8 @@
-"""CPU restatement of the beam envelopes of the reference's diagnostics
-(get_beam_envelops, diagnostics.py:177-207), on host arrays in the
-reference's layout (coords_halfstep and momenta as (3, np), weights (np,)).
+"""CPU restatement of two of the reference's diagnostics on host arrays in
+its layouts: the beam envelopes of get_beam_envelops (diagnostics.py:177-207;
+coords_halfstep and momenta as (3, np), weights (np,)) and the EM energy per
+kx of nrg_out (diagnostics.py:109-122) with the EnergyFact of
+solvers.py:142-149.
 
 TEST INFRASTRUCTURE ONLY: used by tests/ as the checker; the product never
 imports it.
@@ -36,3 +38,21 @@ def moment_sums(coords_halfstep, momenta, weights):
     return np.array([w.sum(), (x * w).sum(), (y * w).sum(), (z * w).sum(), (x ** 2 * w).sum(), (y ** 2 * w).sum(),
                      (z ** 2 * w).sum(), (px ** 2 * w).sum(), (py ** 2 * w).sum(), (pz ** 2 * w).sum(),
                      (x * px * w).sum(), (y * py * w).sum(), (z * pz * w).sum()])
+
+
+def energy_fact(idxM, kr_g, lengthR, dkx, kx_shift=False):
+    """The energy integration factor of solvers.py:142-149."""
+    from scipy.constants import c, e, epsilon_0, m_e
+    from scipy.special import jn
+    tmp_idx = (np.abs(idxM) + 1)[None, None, :]
+    EF = 0.5e-2 * (m_e * c ** 2 / e) ** 2 * (4 * np.pi * epsilon_0) * lengthR ** 2 / dkx \
+        * jn(tmp_idx, kr_g * lengthR) ** 2
+    if not kx_shift:
+        EF *= 0.5
+    return EF
+
+
+def nrg_out(EG_fb, EnergyFact):
+    """diagnostics.py:113-116: EM energy per kx, the kx axis reordered."""
+    dat = ((abs(EG_fb[:, :, :, :3]) ** 2).sum(-1) * EnergyFact).sum(-1).sum(-1)
+    return np.r_[dat[dat.shape[0] // 2 + 1:], dat[:dat.shape[0] // 2 + 1]]

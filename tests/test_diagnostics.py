@@ -95,3 +95,68 @@ def test_moments_edge_cases():
     host = np.zeros(24)
     assert L.chimera_beam_moments(host.ctypes.data, u.ptr, w.ptr, 8, 8, out.ctypes.data) != 0
     assert L.chimera_beam_moments(x.ptr, u.ptr, w.ptr, 4, 8, out.ctypes.data) != 0
+
+
+# ------------------------------------------------------------------ EM energy per kx (nrg_out)
+
+def test_oracle_nrg_out_single_element():
+    N, K, M = 8, 3, 2
+    EF = np.random.default_rng(3).uniform(1.0, 2.0, (N, K, M))
+    for x0 in range(N):
+        EG = np.zeros((N, K, M, 6), complex)
+        EG[x0, 1, 1, 2] = 3.0 - 4.0j           # |.|^2 = 25, an E component
+        EG[x0, 2, 0, 4] = 100.0                 # a G component: no contribution
+        dat = D.nrg_out(EG, EF)
+        pos = x0 - (N // 2 + 1) if x0 >= N // 2 + 1 else x0 + N - (N // 2 + 1)
+        expect = np.zeros(N)
+        expect[pos] = 25.0 * EF[x0, 1, 1]
+        assert np.allclose(dat, expect, rtol=1e-15, atol=0)
+
+
+def test_oracle_energy_fact_bessel_normalisation():
+    """EnergyFact / prefactor = J_{|m|+1}(j_{m,k})^2 = 2 * int_0^1 J_m(j_{m,k} rho)^2 rho d rho:
+    the index |m|+1 and the argument kr*lengthR of solvers.py:145-147."""
+    from scipy.integrate import quad
+    from scipy.special import jn, jn_zeros
+    R, dkx, nk = 3.0, 0.25, 4
+    idxM = np.arange(3)
+    kr_g = np.stack([jn_zeros(m, nk) / R for m in idxM], -1)[None, :, :]
+    EF = D.energy_fact(idxM, kr_g, R, dkx)
+    from scipy.constants import c, e, epsilon_0, m_e
+    pref = 0.5 * 0.5e-2 * (m_e * c ** 2 / e) ** 2 * (4 * np.pi * epsilon_0) * R ** 2 / dkx
+    for m in idxM:
+        for k, z in enumerate(jn_zeros(m, nk)):
+            integral = quad(lambda rho: jn(m, z * rho) ** 2 * rho, 0.0, 1.0, epsabs=1e-14, epsrel=1e-12)[0]
+            assert np.isclose(EF[0, k, m] / pref, 2.0 * integral, rtol=1e-9)
+
+
+def test_energy_fact_table_matches_oracle():
+    from chimera_amd import operators as ops
+    from chimera_amd.diagnostics import Diagnostics
+    Nx, Nkr, Nko, R = 64, 16, 2, 4.8
+    kx = 2 * np.pi * np.fft.fftfreq(Nx, 0.1)
+    Nkr, Rg, R = ops.r_grid(R, 0.3)
+    ext0 = ops.spectral_operators(Nx, Nkr, Nko, R, kx, Rg, ext=0)
+    A = {"idxM": np.arange(Nko + 1), "lengthR": R, "dkx": (kx[1] - kx[0]) / (2 * np.pi), "kr_g": ext0["kr_g"]}
+    assert np.array_equal(Diagnostics.energy_fact_table(A), D.energy_fact(A["idxM"], A["kr_g"], R, A["dkx"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_nrg_out_matches_oracle(precision, tmp_path):
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    from chimera_amd.diagnostics import Diagnostics
+    run = W.build(G, W.with_precision(W.lpa_cfg0(), precision), 5)
+    for i in range(1, 4):
+        run.make_step(i)
+    dg = Diagnostics(run, diags=({"Type": "EnergyEM", "Step": 1},), out_folder=str(tmp_path))
+    dat = dg.nrg_out({"Features": {"Return": True}})[0]
+    sol = run.Solvers[0]
+    ref = D.nrg_out(sol.Data["EG_fb"].numpy().astype(np.complex128),
+                    D.energy_fact(sol.Args["idxM"], sol.Args["kr_g"], sol.Args["lengthR"], sol.Args["dkx"]))
+    err = np.max(np.abs(dat - ref)) / np.max(np.abs(ref))
+    print(f"case: nrg_out lpa_cfg0 {precision}: {err:.2e}", flush=True)
+    assert err < 1e-12
+    dg.do_diags(1)                                      # appends one line to nrg_0.txt
+    assert np.allclose(np.loadtxt(tmp_path / "nrg_0.txt"), dat, rtol=1e-15, atol=0)
