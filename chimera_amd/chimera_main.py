@@ -18,6 +18,9 @@ its make_halfstep therefore sees electrons without ions).
 """
 from __future__ import annotations
 
+import os
+import time
+
 import numpy as np
 
 from .device import call
@@ -389,3 +392,85 @@ class ChimeraRun:
             a = self.Solvers[0].Args if len(self.Solvers) > 0 else self.Particles[0].Args
             SimDom = np.asfortranarray([a["leftX"], a["rightX"], 0, a["Rgrid"].max() ** 2])
             species.chunk_and_damp(SimDom=SimDom, position="cntr")
+
+    # ------------------------------------------------------------------ snapshots (chimera_main.py:323-410)
+    @staticmethod
+    def _snap_value(v):
+        """The array a snapshot stores for one Data/Args value, or None for the
+        values the reference skips (containers) and those an archive cannot hold
+        without pickling (functions, objects)."""
+        from .device import DeviceArray
+        if isinstance(v, DeviceArray):
+            return v.numpy()
+        if isinstance(v, (list, tuple, set, dict)) or callable(v) or v is None:
+            return None
+        a = np.asarray(v)
+        return None if a.dtype == object else a
+
+    def drop_snap(self, fname="./snap_", verbose=False):
+        """chimera_main.py:323-366: every solver's and species' Data arrays and
+        plain Args under the reference's names ('solver<i>/Data/<key>',
+        'specie<i>/Args/<key>', 'NumSolvers', 'NumSpecies', 'NumThreads'),
+        copied from the device.  The container is a NumPy .npz archive (h5py,
+        which the reference writes with, is not a dependency here); species
+        arrays are stored in the reference's layout, (3, np) and (np,), so only
+        live particles are kept.  Returns the file name."""
+        fname += time.ctime().replace(" ", "_").replace(":", "-") + ".npz"
+        out = {}
+        for i, solver in enumerate(self.Solvers):
+            for grp, d in (("Data", solver.Data), ("Args", solver.Args)):
+                for key, v in d.items():
+                    a = self._snap_value(v)
+                    if a is not None:
+                        if verbose:
+                            print(f"solver{i}/{grp}/{key}")
+                        out[f"solver{i}/{grp}/{key}"] = a
+        out["NumSolvers"] = np.array(len(self.Solvers))
+        for i, sp in enumerate(self.Particles):
+            for key in sp.Data:
+                out[f"specie{i}/Data/{key}"] = sp.host(key)
+            for key, v in sp.Args.items():
+                a = self._snap_value(v)
+                if a is not None:
+                    out[f"specie{i}/Args/{key}"] = a
+        out["NumSpecies"] = np.array(len(self.Particles))
+        out["NumThreads"] = np.array(int(os.environ.get("OMP_NUM_THREADS", "1")))
+        np.savez(fname, **out)
+        return fname
+
+    def read_snap(self, fname="./test.npz", verbose=False):
+        """chimera_main.py:368-410: load a snapshot of drop_snap into this run's
+        solvers and species (same configuration), key by key of their own Data
+        and Args; device arrays are overwritten in place, so recorded steps keep
+        their buffers.  A device window edge restarts from the restored leftX."""
+        from .device import DeviceArray
+        with np.load(fname, allow_pickle=False) as f:
+            for i in range(int(f["NumSolvers"])):
+                solver = self.Solvers[i]
+                for key, v in list(solver.Data.items()):
+                    k = f"solver{i}/Data/{key}"
+                    if self._snap_value(v) is None or k not in f:
+                        continue
+                    if verbose:
+                        print(k)
+                    if isinstance(v, DeviceArray):
+                        v.upload(f[k])
+                    else:
+                        solver.Data[key] = f[k]
+                for key, v in list(solver.Args.items()):
+                    k = f"solver{i}/Args/{key}"
+                    if self._snap_value(v) is not None and k in f:
+                        a = f[k]
+                        solver.Args[key] = a[()] if a.ndim == 0 else a
+                if solver.frame is not None:
+                    x = float(solver.Args["leftX"])
+                    solver.frame.upload(np.array([x, x]))
+            for i in range(int(f["NumSpecies"])):
+                sp = self.Particles[i]
+                arrays = {key: f[f"specie{i}/Data/{key}"] for key in ("coords", "coords_halfstep", "momenta", "weights")}
+                sp.restore_particles(**arrays)
+                for key, v in list(sp.Args.items()):
+                    k = f"specie{i}/Args/{key}"
+                    if self._snap_value(v) is not None and k in f:
+                        a = f[k]
+                        sp.Args[key] = a[()] if a.ndim == 0 else a

@@ -169,6 +169,21 @@ class Specie:
         D["weights"].write(o, weights)
         self.np += n
 
+    def restore_particles(self, coords, coords_halfstep, momenta, weights):
+        """Replace the live particles by host arrays in the reference's layout
+        ((3, np) and (np,)), the half-step positions included (read_snap)."""
+        n = weights.shape[-1]
+        if n > self.cap:
+            self.np = 0
+            self._alloc(int(n * 1.25) + 1024)
+        D, cap = self.Data, self.cap
+        for c in range(3):
+            D["coords"].write(c * cap, coords[c])
+            D["coords_halfstep"].write(c * cap, coords_halfstep[c])
+            D["momenta"].write(c * cap, momenta[c])
+        D["weights"].write(0, weights)
+        self.np = n
+
     def denoise(self, WaveNums2Kill):
         """Mirror particles in anti-phase (species.py:322-349)."""
         if self.mixed:
