@@ -72,6 +72,8 @@ class Diagnostics:
             if "Return" in diag["Features"]:
                 ret.append(dat.copy())
             else:
+                if self.out_folder is None:
+                    raise ValueError("nrg_out writes nrg_<n>.txt into out_folder: give one, or 'Return' in Features")
                 with open(os.path.join(self.out_folder, f"nrg_{jj}.txt"), mode="a") as fout:
                     np.savetxt(fout, dat[None, :])
         if "Return" in diag["Features"]:
