@@ -1,7 +1,8 @@
 // diag.hip -- diagnostics on the device: the beam moments behind the envelopes
 // of get_beam_envelops (diagnostics.py:177-207), so that a served simulation
-// can return its beam envelope without copying the particles to the host, and
-// the EM energy per kx of nrg_out (diagnostics.py:109-122).
+// can return its beam envelope without copying the particles to the host, the
+// EM energy per kx of nrg_out (diagnostics.py:109-122) and the laser power
+// per x of pwr_out (diagnostics.py:124-149).
 //
 // The 13 weighted sums the envelope formulas need,
 //   S = [sum w, sum x w, sum y w, sum z w, sum x^2 w, sum y^2 w, sum z^2 w,
@@ -143,6 +144,45 @@ static int32_t field_energy_impl(const C* eg_fb, const double* energy_fact, int3
     return st.finish();
 }
 
+// ---------------------------------------------------------------- laser power per x
+//
+// pwr_out (diagnostics.py:124-149) after fb_vec_out: out[x] = sum over rows
+// r = 0..nr of Rgrid[r] * sum_m sum_{c<3} |vec[x,r,m,c]|^2 for vec(nkx, 0:nr,
+// nmodes, >= 3); one thread per x, (r, m, c) order.
+template <typename C>
+__global__ void k_line_power(const C* __restrict__ vec, const double* __restrict__ rgrid, int nkx, int nr, int nmodes,
+                             double* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nkx) return;
+    const int64_t rs = nkx, ms = (int64_t)nkx * (nr + 1), cs = ms * nmodes;
+    double acc = 0.0;
+    for (int r = 0; r <= nr; r++) {
+        double a2 = 0.0;
+        for (int m = 0; m < nmodes; m++)
+            for (int c = 0; c < 3; c++) {
+                const C v = vec[x + r * rs + m * ms + c * cs];
+                a2 += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            }
+        acc += rgrid[r] * a2;
+    }
+    out[x] = acc;
+}
+
+template <typename C>
+static int32_t line_power_impl(const C* vec, const double* rgrid, int32_t nkx, int32_t nr, int32_t nmodes,
+                               double* out) {
+    if (nkx < 1 || nr < 0 || nmodes < 1) return fail(ST_EINVAL, "line_power: bad sizes");
+    if (out == nullptr) return fail(ST_EINVAL, "line_power: null output");
+    if (!is_device_ptr(vec) || !is_device_ptr(rgrid))
+        return fail(ST_EINVAL, "line_power: vec and Rgrid are device pointers");
+    Stage st;
+    double* d_out = st.out(out, (size_t)nkx);
+    CH_TRY(st.status());
+    hipLaunchKernelGGL((k_line_power<C>), blocks_for(nkx, 256), 256, 0, stream(), vec, rgrid, nkx, nr, nmodes, d_out);
+    CH_LAUNCH_CHECK("k_line_power");
+    return st.finish();
+}
+
 }  // namespace chimera
 
 using namespace chimera;
@@ -167,6 +207,16 @@ int32_t chimera_field_energy(const double* EG_fb, const double* energy_fact, int
 int32_t chimera_field_energy_c8(const float* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr,
                                 int32_t nmodes, double* out) {
     return field_energy_impl<float2>(reinterpret_cast<const float2*>(EG_fb), energy_fact, nkx, nkr, nmodes, out);
+}
+
+int32_t chimera_line_power(const double* vec, const double* Rgrid, int32_t nkx, int32_t nr, int32_t nmodes,
+                           double* out) {
+    return line_power_impl<double2>(reinterpret_cast<const double2*>(vec), Rgrid, nkx, nr, nmodes, out);
+}
+
+int32_t chimera_line_power_c8(const float* vec, const double* Rgrid, int32_t nkx, int32_t nr, int32_t nmodes,
+                              double* out) {
+    return line_power_impl<float2>(reinterpret_cast<const float2*>(vec), Rgrid, nkx, nr, nmodes, out);
 }
 
 }  // extern "C"

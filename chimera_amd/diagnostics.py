@@ -6,10 +6,14 @@ Diagnostics class (diagnostics.py:25-207):
   array leaves the device;
 * nrg_out: the EM energy per kx (chimera_field_energy), returned ('Return'
   in the diagnostic's Features) or appended to nrg_<solver>.txt as the
-  reference does; do_diags runs the 'EnergyEM' entries of the list.
+  reference does;
+* pwr_out: the laser power per x (the inverse transform of fb_vec_out, then
+  chimera_line_power), returned or appended to pwrX_<solver>.txt; the 'Spot'
+  intensity profile is not provided;
+* do_diags runs the 'EnergyEM' and 'Power' entries of the list.
 
-The other dump diagnostics of the reference (fields, FB fields, particles,
-density, power; diagnostics.py:46-150) are not provided.
+The dump diagnostics of the reference (fields, FB fields, particles, density;
+diagnostics.py:46-108) are not provided.
 """
 from __future__ import annotations
 
@@ -18,6 +22,8 @@ import os
 import numpy as np
 
 from .device import DeviceArray, call
+
+PwrFctr = 0.5 * 0.511e6 * 1.6022e-19 / 2.818e-13 * 2.9979e10      # diagnostics.py:24
 
 
 class Diagnostics:
@@ -37,6 +43,8 @@ class Diagnostics:
             diag.setdefault("Features", {})
             if diag["Type"] == "EnergyEM":
                 self.nrg_out(diag)
+            elif diag["Type"] == "Power":
+                self.pwr_out(diag)
             else:
                 raise NotImplementedError(f"diagnostic type {diag['Type']!r} is not provided on the device")
 
@@ -109,3 +117,44 @@ class Diagnostics:
         xyz_rms = np.sqrt(S[4:7] / W - xyz_0 ** 2)
         xyz_emmit = np.sqrt(S[4:7] / W * S[7:10] / W - S[10:13] ** 2 / W ** 2)
         return np.array([xyz_0, xyz_rms, xyz_emmit])
+
+    @staticmethod
+    def _table(solver, key, host, dtype):
+        if key not in solver.dev:
+            solver.dev[key] = DeviceArray.from_numpy(np.asfortranarray(host), dtype)
+        return solver.dev[key]
+
+    def pwr_out(self, diag):
+        """diagnostics.py:124-149 without the 'Spot' profile: the field of
+        fb_vec_out (fb_io.f90:100-140; the E part of the inverse transform with
+        the full radial grid, ghost row zero) reduced on the device to
+        PwrFctr*2*dr * sum_r Rgrid[r] sum_m sum_c |E|^2 per x."""
+        if "Spot" in diag["Features"]:
+            raise NotImplementedError("the 'Spot' intensity profile (intens_profo) is not provided on the device")
+        ret = []
+        for jj, sol in enumerate(self.Chimera.Solvers):
+            A = sol.Args
+            Nx, Nkr, M = sol.n_fb
+            nr = A["OutFull"].shape[1]
+            out_full = self._table(sol, "OutFull", A["OutFull"], sol.rt)
+            rgrid = self._table(sol, "RgridFull", A["RgridFull"], np.float64)
+            vec = sol.dev.get("PwrVec")
+            if vec is None:
+                vec = sol.dev["PwrVec"] = DeviceArray((Nx, nr + 1, M, 6), sol.ct)
+            EG = sol.Data["EG_fb"]
+            # E in both halves of the transform's input: the first three output components are fb_vec_out
+            call(sol.fn("chimera_fb_inverse"), vec.ptr, EG.ptr, EG.ptr, float(A["leftX"]), sol.dev["kx"].ptr,
+                 out_full.ptr, Nx, nr, M, Nkr, 1, None)
+            dat = np.zeros(Nx)
+            call("chimera_line_power_c8" if sol.mixed else "chimera_line_power", vec.ptr, rgrid.ptr, Nx, nr, M,
+                 dat.ctypes.data)
+            dat = PwrFctr * 2 * A["dr"] * dat
+            if "Return" in diag["Features"]:
+                ret.append(dat)
+            else:
+                if self.out_folder is None:
+                    raise ValueError("pwr_out writes pwrX_<n>.txt into out_folder: give one, or 'Return' in Features")
+                with open(os.path.join(self.out_folder, f"pwrX_{jj}.txt"), mode="a") as fout:
+                    np.savetxt(fout, dat[None])
+        if "Return" in diag["Features"]:
+            return ret

@@ -378,6 +378,14 @@ int32_t chimera_beam_moments(const double* coord_half, const double* momenta, co
 int32_t chimera_field_energy(const double* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr, int32_t nmodes,
                              double* out);
 
+/* Laser power per x of pwr_out (diagnostics.py:124-149), before its factor
+ * PwrFctr*2*dr: out[x] = sum_{r=0..nr} Rgrid[r] * sum_m sum_{c<3} |vec[x,r,m,c]|^2
+ * for vec(nkx,0:nr,nmodes,>=3) in device memory (the field of fb_vec_out,
+ * fb_io.f90:100-140, or of chimera_fb_inverse with ghost = 1) and Rgrid(0:nr)
+ * in device memory; out (nkx doubles) in device or host memory. */
+int32_t chimera_line_power(const double* vec, const double* Rgrid, int32_t nkx, int32_t nr, int32_t nmodes,
+                           double* out);
+
 /* ======================================================================
  * Mixed-precision resident tier (no counterpart in the reference, which is
  * FP64 throughout).  Particle positions stay FP64; momenta and weights are
@@ -403,6 +411,8 @@ int32_t chimera_beam_moments_mixed(const double* coord_half, const float* moment
                                    int32_t np, double* sums);
 int32_t chimera_field_energy_c8(const float* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr,
                                 int32_t nmodes, double* out);
+int32_t chimera_line_power_c8(const float* vec, const double* Rgrid, int32_t nkx, int32_t nr, int32_t nmodes,
+                              double* out);
 int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double* kx, const float* In,
                               const float* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
                               int32_t fold, const double* frame);

@@ -1,8 +1,9 @@
-"""CPU restatement of two of the reference's diagnostics on host arrays in
+"""CPU restatement of three of the reference's diagnostics on host arrays in
 its layouts: the beam envelopes of get_beam_envelops (diagnostics.py:177-207;
-coords_halfstep and momenta as (3, np), weights (np,)) and the EM energy per
-kx of nrg_out (diagnostics.py:109-122) with the EnergyFact of
-solvers.py:142-149.
+coords_halfstep and momenta as (3, np), weights (np,)), the EM energy per kx
+of nrg_out (diagnostics.py:109-122) with the EnergyFact of
+solvers.py:142-149, and the laser power of pwr_out (diagnostics.py:124-149)
+through the C port's inverse transform.
 
 TEST INFRASTRUCTURE ONLY: used by tests/ as the checker; the product never
 imports it.
@@ -56,3 +57,19 @@ def nrg_out(EG_fb, EnergyFact):
     """diagnostics.py:113-116: EM energy per kx, the kx axis reordered."""
     dat = ((abs(EG_fb[:, :, :, :3]) ** 2).sum(-1) * EnergyFact).sum(-1).sum(-1)
     return np.r_[dat[dat.shape[0] // 2 + 1:], dat[:dat.shape[0] // 2 + 1]]
+
+
+PwrFctr = 0.5 * 0.511e6 * 1.6022e-19 / 2.818e-13 * 2.9979e10      # diagnostics.py:24
+
+
+def pwr_out(port, EG_fb, leftX, kx, OutFull, RgridFull, dr):
+    """diagnostics.py:128-147 (no 'Spot'): fb_vec_out through the oracle port's
+    fb_eb_out (its first three components are fb_vec_out, fb_io.f90:100-140
+    and :182-228), then PwrFctr*2*dr*((|dat|^2).sum(-1).sum(-1)*Rgrid).sum(-1)."""
+    nkx, nkr, M = EG_fb.shape[:3]
+    nr = OutFull.shape[1]
+    e = np.asfortranarray(EG_fb[..., :3])
+    eb = port.fb_eb_out(np.zeros((nkx, nr + 1, M, 6), complex, order="F"), np.asfortranarray(EG_fb), e, leftX, kx,
+                        OutFull)
+    dat = eb[..., :3]
+    return PwrFctr * 2 * dr * ((np.abs(dat) ** 2).sum(-1).sum(-1) * RgridFull[None, :]).sum(-1)

@@ -160,3 +160,43 @@ def test_device_nrg_out_matches_oracle(precision, tmp_path):
     assert err < 1e-12
     dg.do_diags(1)                                      # appends one line to nrg_0.txt
     assert np.allclose(np.loadtxt(tmp_path / "nrg_0.txt"), dat, rtol=1e-15, atol=0)
+
+
+# ------------------------------------------------------------------ laser power per x (pwr_out)
+
+def test_oracle_pwr_out_single_mode(port):
+    """A field made of one Out row: fb_vec_out puts Out[k0, r, m0] * c * e^{i kx0 leftX} (a
+    single kx, unnormalised backward FFT) on every x of row r, so the power is
+    PwrFctr*2*dr * sum_r Rgrid[r] * |Out[k0,r,m0]|^2 * |c|^2 on every x."""
+    nkx, nkr, M, dr = 16, 6, 2, 0.3
+    r = np.random.default_rng(4)
+    Out = np.asfortranarray(r.normal(size=(nkr, nkr, M)))
+    Rg = dr * (np.arange(nkr + 1) - 0.5)
+    kx = 2 * np.pi * np.fft.fftfreq(nkx, 0.1)
+    EG = np.zeros((nkx, nkr, M, 6), complex, order="F")
+    EG[3, 2, 1, 1] = 2.0 - 1.0j
+    p = D.pwr_out(port, EG, -1.3, kx, Out, Rg, dr)
+    expect = D.PwrFctr * 2 * dr * (Rg[1:] * Out[2, :, 1] ** 2).sum() * 5.0
+    assert np.allclose(p, expect, rtol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_pwr_out_matches_oracle(precision, port, tmp_path):
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    from chimera_amd.diagnostics import Diagnostics
+    run = W.build(G, W.with_precision(W.lpa_cfg0(), precision), 5)
+    for i in range(1, 4):
+        run.make_step(i)
+    dg = Diagnostics(run, diags=({"Type": "Power", "Step": 1},), out_folder=str(tmp_path))
+    p = dg.pwr_out({"Features": {"Return": True}})[0]
+    sol = run.Solvers[0]
+    A = sol.Args
+    ref = D.pwr_out(port, sol.Data["EG_fb"].numpy().astype(np.complex128), float(A["leftX"]), A["kx"],
+                    A["OutFull"].astype(sol.rt).astype(np.float64), A["RgridFull"], A["dr"])
+    err = np.max(np.abs(p - ref)) / np.max(np.abs(ref))
+    print(f"case: pwr_out lpa_cfg0 {precision}: {err:.2e}", flush=True)
+    assert err < (1e-12 if precision == "fp64" else 2e-6)
+    dg.do_diags(1)
+    assert np.allclose(np.loadtxt(tmp_path / "pwrX_0.txt"), p, rtol=1e-15, atol=0)
