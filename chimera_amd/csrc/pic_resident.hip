@@ -814,8 +814,10 @@ static int32_t push_deposit_impl(double* coord, double* coord_half, const R* mom
     // (DEP_SLOT_R x nb / CMB_T rounds of coalesced L2 reads, 64 B per
     // descriptor and workgroup) instead of sorting them into buckets, which
     // takes a dozen dependent launches of about 5 us each; up to 512 blocks
-    // (131072 particles), or while the scans read at most 128 MiB in all
-    const bool scan_all = nb <= 512 || (int64_t)nb * grid.x * grid.y * grid.z <= ((int64_t)1 << 21);
+    // (131072 particles), or while the scans read at most 512 MiB in all (L2
+    // reads: on a 2048 x 256 bunch of 300 k particles, 352 MiB of them take
+    // less time than the sort)
+    const bool scan_all = nb <= 512 || (int64_t)nb * grid.x * grid.y * grid.z <= ((int64_t)1 << 23);
 #define CMB_LAUNCH(N, IDX, BOUNDS)                                                                                \
     hipLaunchKernelGGL((k_dep_combine<N, R>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR, a.desc, IDX, \
                        BOUNDS, nb, nrb, sr, sm, nx + 1)
