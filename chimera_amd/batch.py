@@ -14,7 +14,8 @@ Recording: steps 1 and 2 run eagerly (they size the scratch memory and upload
 the job tables of both arrangements of the swapped density-gradient buffers);
 steps 3 and 4 are recorded while they are issued and then launched; from step
 5 on, step i replays the graph of its parity after the host-side bookkeeping
-of make_step (window moves, buffer swap).  A run whose steps differ from one
+of make_step (window moves, buffer swap).  A run resumed from a snapshot after
+step s does the same from step s + 1.  A run whose steps differ from one
 another (plasma injection, absorption, re-sorting, a window that does not move
 every step; ChimeraRun.graph_blockers) is stepped eagerly on its context.
 """
@@ -71,9 +72,10 @@ class Simulation:
     more steps and returns at once; done() says whether everything queued has
     finished; result() waits and returns the run."""
 
-    def __init__(self, build, steps, graphs=True, name=""):
+    def __init__(self, build, steps, graphs=True, name="", start=0):
         self.name = name
         self.steps = int(steps)
+        self.start = int(start)         # steps already done (a run resumed from a snapshot)
         self.ctx = Context()
         self.t_admit = time.perf_counter()
         with self.ctx:
@@ -82,7 +84,7 @@ class Simulation:
             if self.graphs_on:
                 self.run.enable_device_frame()
         self.graphs = {}
-        self.istep = 0
+        self.istep = self.start
         self.t_done = None
 
     @property
@@ -91,7 +93,7 @@ class Simulation:
 
     def _step(self, i):
         L = lib()
-        if not self.graphs_on or i <= 2:
+        if not self.graphs_on or i <= self.start + 2:
             self.run.make_step(i)
             return
         parity = i % 2

@@ -404,7 +404,7 @@ class ChimeraRun:
             return v.numpy()
         if isinstance(v, (list, tuple, set, dict)) or callable(v) or v is None:
             return None
-        a = np.asarray(v)
+        a = np.array(v)                       # a copy: the snapshot must not follow the run
         return None if a.dtype == object else a
 
     def drop_snap(self, fname="./snap_", verbose=False):
@@ -416,6 +416,11 @@ class ChimeraRun:
         arrays are stored in the reference's layout, (3, np) and (np,), so only
         live particles are kept.  Returns the file name."""
         fname += time.ctime().replace(" ", "_").replace(":", "-") + ".npz"
+        np.savez(fname, **self.snapshot(verbose))
+        return fname
+
+    def snapshot(self, verbose=False):
+        """The arrays of drop_snap, in host memory (name -> array)."""
         out = {}
         for i, solver in enumerate(self.Solvers):
             for grp, d in (("Data", solver.Data), ("Args", solver.Args)):
@@ -435,42 +440,49 @@ class ChimeraRun:
                     out[f"specie{i}/Args/{key}"] = a
         out["NumSpecies"] = np.array(len(self.Particles))
         out["NumThreads"] = np.array(int(os.environ.get("OMP_NUM_THREADS", "1")))
-        np.savez(fname, **out)
-        return fname
+        return out
 
     def read_snap(self, fname="./test.npz", verbose=False):
-        """chimera_main.py:368-410: load a snapshot of drop_snap into this run's
-        solvers and species (same configuration), key by key of their own Data
-        and Args; device arrays are overwritten in place, so recorded steps keep
-        their buffers.  A device window edge restarts from the restored leftX."""
+        """chimera_main.py:368-410: load a snapshot of drop_snap (a file name, or
+        the dict of snapshot()) into this run's solvers and species (same
+        configuration), key by key of their own Data and Args; device arrays are
+        overwritten in place, so recorded steps keep their buffers.  A device
+        window edge restarts from the restored leftX."""
+        if isinstance(fname, dict):
+            self._load_snap(fname, verbose)
+        else:
+            with np.load(fname, allow_pickle=False) as f:
+                self._load_snap(f, verbose)
+
+    def _load_snap(self, f, verbose):
+        """Load from a mapping of snapshot names to arrays (an npz file or a dict)."""
         from .device import DeviceArray
-        with np.load(fname, allow_pickle=False) as f:
-            for i in range(int(f["NumSolvers"])):
-                solver = self.Solvers[i]
-                for key, v in list(solver.Data.items()):
-                    k = f"solver{i}/Data/{key}"
-                    if self._snap_value(v) is None or k not in f:
-                        continue
-                    if verbose:
-                        print(k)
-                    if isinstance(v, DeviceArray):
-                        v.upload(f[k])
-                    else:
-                        solver.Data[key] = f[k]
-                for key, v in list(solver.Args.items()):
-                    k = f"solver{i}/Args/{key}"
-                    if self._snap_value(v) is not None and k in f:
-                        a = f[k]
-                        solver.Args[key] = a[()] if a.ndim == 0 else a
-                if solver.frame is not None:
-                    x = float(solver.Args["leftX"])
-                    solver.frame.upload(np.array([x, x]))
-            for i in range(int(f["NumSpecies"])):
-                sp = self.Particles[i]
-                arrays = {key: f[f"specie{i}/Data/{key}"] for key in ("coords", "coords_halfstep", "momenta", "weights")}
-                sp.restore_particles(**arrays)
-                for key, v in list(sp.Args.items()):
-                    k = f"specie{i}/Args/{key}"
-                    if self._snap_value(v) is not None and k in f:
-                        a = f[k]
-                        sp.Args[key] = a[()] if a.ndim == 0 else a
+        for i in range(int(f["NumSolvers"])):
+            solver = self.Solvers[i]
+            for key, v in list(solver.Data.items()):
+                k = f"solver{i}/Data/{key}"
+                if self._snap_value(v) is None or k not in f:
+                    continue
+                if verbose:
+                    print(k)
+                if isinstance(v, DeviceArray):
+                    v.upload(f[k])
+                else:
+                    solver.Data[key] = f[k]
+            for key, v in list(solver.Args.items()):
+                k = f"solver{i}/Args/{key}"
+                if self._snap_value(v) is not None and k in f:
+                    a = f[k]
+                    solver.Args[key] = a[()] if a.ndim == 0 else a
+            if solver.frame is not None:
+                x = float(solver.Args["leftX"])
+                solver.frame.upload(np.array([x, x]))
+        for i in range(int(f["NumSpecies"])):
+            sp = self.Particles[i]
+            arrays = {key: f[f"specie{i}/Data/{key}"] for key in ("coords", "coords_halfstep", "momenta", "weights")}
+            sp.restore_particles(**arrays)
+            for key, v in list(sp.Args.items()):
+                k = f"specie{i}/Args/{key}"
+                if self._snap_value(v) is not None and k in f:
+                    a = f[k]
+                    sp.Args[key] = a[()] if a.ndim == 0 else a

@@ -16,6 +16,10 @@ steps recorded into graphs and replayed) and schedules them:
   arrived at different times overlaps on the device.  A finished
   simulation's result, the beam envelopes of its pushed species, is reduced
   on the device into Request.result; then its arrays and context are freed.
+  With evict=True a request that does not fit may evict the active request
+  with the most steps left: its state goes to host memory (the arrays of
+  ChimeraRun.snapshot), its device memory is freed, and it queues again to
+  resume from there.
 """
 from __future__ import annotations
 
@@ -67,6 +71,7 @@ class Request:
         self.t_admit = self.t_built = self.t_done = None
         self.particle_steps = 0
         self.result = None             # beam envelopes of the finished run (diagnostics.get_beam_envelops)
+        self.snap, self.istep0, self.evictions = None, 0, 0     # state of an evicted run
 
     def build(self, graphs=True):
         import chimera_amd as G
@@ -75,6 +80,14 @@ class Request:
             fn = lambda: W.build_beam(G, self.spec, self.seed, own_rng=True)
         else:
             fn = lambda: W.build(G, self.spec, self.seed, own_rng=True)
+        if self.snap is not None:
+            def resume(make=fn, snap=self.snap):
+                run = make()
+                run.read_snap(snap)
+                return run
+            sim = Simulation(resume, self.steps, graphs=graphs, name=self.name, start=self.istep0)
+            self.snap = None
+            return sim
         return Simulation(fn, self.steps, graphs=graphs, name=self.name)
 
 
@@ -138,8 +151,13 @@ class Server:
     chunk once the previous one has finished)."""
 
     def __init__(self, slots=64, reserve=16 * GIB, chunk=4, graphs=True, builder_thread=True, log=None,
-                 on_finish=None, builders=4):
+                 on_finish=None, builders=4, evict=False, hbm=None, max_evictions=2, evict_after=16):
         self.slots, self.reserve, self.chunk, self.graphs = slots, reserve, chunk, graphs
+        # evict: a request that does not fit may evict the active request with the most
+        # steps left once it has run evict_after steps since its admission (its state to
+        # host memory, resumed later from there; at most max_evictions times); hbm: device
+        # bytes the server may use (default: whatever the device has free)
+        self.evict, self.hbm, self.max_evictions, self.evict_after = evict, hbm, max_evictions, evict_after
         self.on_finish = on_finish     # called with each finished request before its simulation is freed
         self.log = log
         self.pending = []          # arrived, not admitted (FIFO)
@@ -175,11 +193,45 @@ class Server:
         r.t_arrival_host = now
         self.pending.append(r)
 
-    def _admit(self, now):
+    def _free(self):
         free, _ = free_device_bytes()
+        if self.hbm is not None:
+            free = min(free, self.hbm - sum(a.bytes for a in self.active))
+        return free
+
+    def _evict_for(self, r, free):
+        """Evict the active request with the most steps left whose memory makes
+        room for r, if it has run evict_after steps since it was admitted and
+        has not been evicted too often; returns whether one was evicted."""
+        need = r.bytes + self.reserve + self.reserved - free
+        cands = [a for a in self.active if a.sim is not None and not a.sim.finished_issuing
+                 and a.sim.istep >= a.istep0 + max(1, self.evict_after) and a.evictions < self.max_evictions
+                 and a.bytes >= need]
+        if not cands:
+            return False
+        v = max(cands, key=lambda a: a.sim.steps - a.sim.istep)
+        s = v.sim
+        s.ctx.synchronize()
+        with s.ctx:
+            v.snap = s.run.snapshot()
+        v.istep0 = s.istep
+        v.evictions += 1
+        s.close()
+        v.sim = None
+        self.active.remove(v)
+        self.pending.append(v)
+        if self.log:
+            self.log(f"  evicted {v.name} after step {v.istep0} for {r.name}")
+        return True
+
+    def _admit(self, now):
+        free = self._free()
         while self.pending and len(self.active) + self.building < self.slots:
             r = self.pending[0]
             if r.bytes + self.reserve + self.reserved > free:
+                if self.evict and self._evict_for(r, free):
+                    free = self._free()
+                    continue
                 break
             self.pending.pop(0)
             r.t_admit = now

@@ -201,3 +201,41 @@ def test_served_results_match_alone(precision):
             e = rel_l2(served[name][k], alone[name][k])
             assert e <= TOL[precision], (name, k, e)
     say("  all requests match their runs alone")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_evicted_requests_resume(precision):
+    """With device memory for one request at a time (an emulated budget) and
+    eviction on, later arrivals evict the running request, whose state goes to
+    host memory and resumes from there; every request still gives the result
+    it gives alone."""
+    from chimera_amd import serve as S
+    # one grid and particle count (the same memory estimate), three bunches
+    specs = [small_beam(Nx=256, Nkr=40, n_target=3e4, gamma=gm, steps=st) for gm, st in ((80.0, 90), (300.0, 80), (60.0, 70))]
+    say(f"case: 3 beams {precision}, memory for one at a time, eviction on, vs alone")
+    # all arrive at once: the first runs, the others wait for memory and evict it
+    reqs = [S.Request(f"e{k}", sp, 70 + k, sp["steps"], precision, t_arrival=0.0) for k, sp in enumerate(specs)]
+    alone = {}
+    for r in reqs:
+        sim = r.build()
+        sim.advance(r.steps)
+        alone[r.name] = snap(sim.result())
+        sim.close()
+    served = {}
+    assert len({r.bytes for r in reqs}) == 1
+    budget = int(1.5 * reqs[0].bytes)             # one request at a time
+    server = S.Server(slots=3, reserve=0, chunk=2, evict=True, hbm=budget, log=say,
+                      on_finish=lambda r: served.__setitem__(r.name, snap(r.sim.run)))
+    try:
+        S.serve(reqs, server)
+    finally:
+        server.close()
+    assert sorted(served) == sorted(alone)
+    n_evict = sum(r.evictions for r in server.finished)
+    say(f"  {n_evict} evictions")
+    assert n_evict >= 1
+    for name in alone:
+        for k in ("EB", "EG_fb", "momenta", "coords"):
+            e = rel_l2(served[name][k], alone[name][k])
+            assert e <= TOL[precision], (name, k, e)
+    say("  every request matches its run alone")
