@@ -184,6 +184,18 @@ class Specie:
         D["weights"].write(0, weights)
         self.np = n
 
+    def beam_focus(self, x_foc):
+        """species.py:473-486: move a relativistic bunch back along its ballistic
+        drift so that it focuses x_foc downstream (a set-up step, done on host
+        copies with the reference's NumPy arithmetic and written back)."""
+        coords, momenta = self.host("coords"), self.host("momenta").astype(np.float64)
+        weights = self.host("weights").astype(np.float64)
+        gg = (1 + momenta ** 2).sum(0) ** 0.5
+        pzmean = (momenta[0] * weights).sum() / weights.sum()
+        coords[0] = coords[0] - x_foc * (momenta[0] - pzmean) / pzmean / gg ** 2
+        coords[1:3] = coords[1:3] - momenta[1:3] / pzmean * x_foc
+        self.restore_particles(coords, coords, self.host("momenta"), self.host("weights"))
+
     def denoise(self, WaveNums2Kill):
         """Mirror particles in anti-phase (species.py:322-349)."""
         if self.mixed:
