@@ -77,6 +77,35 @@ __global__ void k_poiss_corr(cplx_t<R>* __restrict__ J, const cplx_t<R>* __restr
     }
 }
 
+// maxwell_solvers.f90:166-197: j += (grad div j + DT(x) grad rho_n) w2_inv, DT complex per x
+template <typename R>
+__global__ void k_poiss_corr_stat(cplx_t<R>* __restrict__ J, const cplx_t<R>* __restrict__ GD,
+                                  const cplx_t<R>* __restrict__ Gn, const double2* __restrict__ DT,
+                                  const R* __restrict__ w2, int nkx, int64_t n) {
+    typedef cplx_t<R> C_;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const C_ dt = cconv<R>(DT[e % nkx]);
+    const R w = w2[e];
+#pragma unroll
+    for (int l = 0; l < 3; l++) J[e + l * n] = cadd(J[e + l * n], crs(w, cadd(GD[e + l * n], cmul(dt, Gn[e + l * n]))));
+}
+
+// maxwell_solvers.f90:199-226: EG *= exp(fact kx), fact = -0.5 i dt beta0: the
+// real part of fact kx is zero, so the factor is (cos, sin) of a*kx with
+// a = (-0.5 dt) beta0, formed in FP64
+template <typename R>
+__global__ void k_field_drift(cplx_t<R>* __restrict__ EG, const double* __restrict__ kx, double a, int nkx, int64_t n) {
+    typedef cplx_t<R> C_;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double sn, cs;
+    sincos(a * kx[e % nkx], &sn, &cs);
+    const C_ p = cmake<R>((R)cs, (R)sn);
+#pragma unroll
+    for (int l = 0; l < 6; l++) EG[e + l * n] = cmul(EG[e + l * n], p);
+}
+
 // maxwell_solvers.f90:228-272
 template <typename R>
 __global__ void k_mult(cplx_t<R>* __restrict__ v, const R* __restrict__ A, int64_t n, int ncomp) {
@@ -131,6 +160,23 @@ int32_t poiss_corr(cplx_t<R>* J, const cplx_t<R>* GD, const cplx_t<R>* Gn, const
     KTimer kt(KF_PSATD);
     hipLaunchKernelGGL((k_poiss_corr<R>), blocks_for(n, 256), 256, 0, stream(), J, GD, Gn, Gp, dt_inv, w2, n);
     CH_LAUNCH_CHECK("k_poiss_corr");
+    return ST_OK;
+}
+
+template <typename R>
+int32_t poiss_corr_stat(cplx_t<R>* J, const cplx_t<R>* GD, const cplx_t<R>* Gn, const double2* DT, const R* w2,
+                        int nkx, int64_t n) {
+    if (n <= 0) return ST_OK;
+    hipLaunchKernelGGL((k_poiss_corr_stat<R>), blocks_for(n, 256), 256, 0, stream(), J, GD, Gn, DT, w2, nkx, n);
+    CH_LAUNCH_CHECK("k_poiss_corr_stat");
+    return ST_OK;
+}
+
+template <typename R>
+int32_t field_drift(cplx_t<R>* EG, const double* kx, double beta0, double dt, int nkx, int64_t n) {
+    if (n <= 0) return ST_OK;
+    hipLaunchKernelGGL((k_field_drift<R>), blocks_for(n, 256), 256, 0, stream(), EG, kx, (-0.5 * dt) * beta0, nkx, n);
+    CH_LAUNCH_CHECK("k_field_drift");
     return ST_OK;
 }
 
@@ -257,6 +303,51 @@ static int32_t add_entry(double* v_, const double* A_, int32_t nkx, int32_t nkr,
     CH_TRY(st.status());
     CH_TRY(add_arr<double>(v, A, n));
     return st.finish();
+}
+
+static int32_t dev_only(const char* name, std::initializer_list<const void*> ps);
+
+int32_t chimera_poiss_corr_stat(double* j_fb, const double* grad_div_j_fb, const double* grad_rho_n_fb, const double* DT,
+                                const double* w2_inv, int32_t nkx, int32_t nkr, int32_t nkO) {
+    SIZES_OK(nkx, nkr, nkO);
+    const int64_t n = nel(nkx, nkr, nkO);
+    Stage st;
+    double2* J = reinterpret_cast<double2*>(st.inout(j_fb, 2 * (size_t)n * 3));
+    const double2* GD = reinterpret_cast<const double2*>(st.in(grad_div_j_fb, 2 * (size_t)n * 3));
+    const double2* Gn = reinterpret_cast<const double2*>(st.in(grad_rho_n_fb, 2 * (size_t)n * 3));
+    const double2* D = reinterpret_cast<const double2*>(st.in(DT, 2 * (size_t)nkx));
+    const double* w = st.in(w2_inv, (size_t)n);
+    CH_TRY(st.status());
+    CH_TRY(poiss_corr_stat<double>(J, GD, Gn, D, w, nkx, n));
+    return st.finish();
+}
+
+int32_t chimera_field_drift(double* EG_fb, const double* kx, double beta0, double dt, int32_t nkx, int32_t nkr,
+                            int32_t nkO) {
+    SIZES_OK(nkx, nkr, nkO);
+    const int64_t n = nel(nkx, nkr, nkO);
+    Stage st;
+    double2* EG = reinterpret_cast<double2*>(st.inout(EG_fb, 2 * (size_t)n * 6));
+    const double* k = st.in(kx, (size_t)nkx);
+    CH_TRY(st.status());
+    CH_TRY(field_drift<double>(EG, k, beta0, dt, nkx, n));
+    return st.finish();
+}
+
+int32_t chimera_poiss_corr_stat_c8(float* j_fb, const float* grad_div_j_fb, const float* grad_rho_n_fb,
+                                   const double* DT, const float* w2_inv, int32_t nkx, int32_t nkr, int32_t nkO) {
+    SIZES_OK(nkx, nkr, nkO);
+    CH_TRY(dev_only("poiss_corr_stat_c8", {j_fb, grad_div_j_fb, grad_rho_n_fb, DT, w2_inv}));
+    return poiss_corr_stat<float>(reinterpret_cast<float2*>(j_fb), reinterpret_cast<const float2*>(grad_div_j_fb),
+                                  reinterpret_cast<const float2*>(grad_rho_n_fb), reinterpret_cast<const double2*>(DT),
+                                  w2_inv, nkx, nel(nkx, nkr, nkO));
+}
+
+int32_t chimera_field_drift_c8(float* EG_fb, const double* kx, double beta0, double dt, int32_t nkx, int32_t nkr,
+                               int32_t nkO) {
+    SIZES_OK(nkx, nkr, nkO);
+    CH_TRY(dev_only("field_drift_c8", {EG_fb, kx}));
+    return field_drift<float>(reinterpret_cast<float2*>(EG_fb), kx, beta0, dt, nkx, nel(nkx, nkr, nkO));
 }
 
 int32_t chimera_omp_mult_vec(double* vec_fb, const double* A, int32_t nkx, int32_t nkr, int32_t nkO) {

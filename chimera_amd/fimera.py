@@ -25,7 +25,7 @@ __all__ = [
     "align_data_scl", "dep_curr", "dep_dens", "dep_curr_chnk", "dep_dens_chnk", "proj_fld", "eb_correction",
     "fb_vec_in", "fb_scl_in", "fb_eb_out", "fb_filtr", "fb_rot", "fb_grad", "fb_div", "fb_graddiv",
     "maxwell_push_with_spchrg", "maxwell_push_wo_spchrg", "maxwell_init_push", "poiss_corr", "omp_mult_vec",
-    "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic",
+    "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic", "poiss_corr_stat", "field_drift",
 ]
 
 
@@ -256,6 +256,22 @@ def maxwell_init_push(EG_fb, j_fb, grad_rho_n_fb, C1, C2):
     EG, j, gn = _arr(EG_fb, np.complex128), _arr(j_fb, np.complex128), _arr(grad_rho_n_fb, np.complex128)
     a, b = _arr(C1, np.complex128), _arr(C2, np.complex128)
     _call("chimera_maxwell_init_push", _p(EG), _p(j), _p(gn), _p(a), _p(b), *EG.shape[:3])
+    return EG
+
+
+def poiss_corr_stat(j_fb, grad_div_j_fb, grad_rho_n_fb, DT, w2_inv):
+    """maxwell_solvers.f90:166-197 -> j_fb"""
+    j = _arr(j_fb, np.complex128)
+    gd, gn = _arr(grad_div_j_fb, np.complex128), _arr(grad_rho_n_fb, np.complex128)
+    dt, w = _arr(DT, np.complex128), _arr(w2_inv, np.float64)
+    _call("chimera_poiss_corr_stat", _p(j), _p(gd), _p(gn), _p(dt), _p(w), *j.shape[:3])
+    return j
+
+
+def field_drift(EG_fb, kx, beta0, dt):
+    """maxwell_solvers.f90:199-226 -> EG_fb"""
+    EG, k = _arr(EG_fb, np.complex128), _arr(kx, np.float64)
+    _call("chimera_field_drift", _p(EG), _p(k), float(beta0), float(dt), *EG.shape[:3])
     return EG
 
 

@@ -224,6 +224,14 @@ int32_t chimera_maxwell_init_push(double* EG_fb, const double* j_fb, const doubl
 int32_t chimera_poiss_corr(double* j_fb, const double* grad_div_j_fb, const double* grad_rho_n_fb,
                            const double* grad_rho_np1_fb, double dt_inv, const double* w2_inv, int32_t nkx,
                            int32_t nkr, int32_t nkO);
+/* j += (grad div j + DT grad rho_n) w2_inv with DT(nkx) complex (the static
+ * kick's Poisson correction).  Replaces poiss_corr_stat, maxwell_solvers.f90:166-197. */
+int32_t chimera_poiss_corr_stat(double* j_fb, const double* grad_div_j_fb, const double* grad_rho_n_fb, const double* DT,
+                                const double* w2_inv, int32_t nkx, int32_t nkr, int32_t nkO);
+/* EG_fb(nkx,nkr,nkO,6) *= exp(-0.5 i dt beta0 kx) (free drift over one step).
+ * Replaces field_drift, maxwell_solvers.f90:199-226. */
+int32_t chimera_field_drift(double* EG_fb, const double* kx, double beta0, double dt, int32_t nkx, int32_t nkr,
+                            int32_t nkO);
 /* Elementwise helpers.  Replace omp_mult_vec/scl, omp_add_vec/scl, maxwell_solvers.f90:228-318. */
 int32_t chimera_omp_mult_vec(double* vec_fb, const double* A, int32_t nkx, int32_t nkr, int32_t nkO);
 int32_t chimera_omp_mult_scl(double* scl_fb, const double* A, int32_t nkx, int32_t nkr, int32_t nkO);
@@ -428,6 +436,10 @@ int32_t chimera_fb_op_c8(int32_t op, float* out, const float* in, const float* D
                          const double* kx, int32_t nkx, int32_t nkr, int32_t nkO);
 int32_t chimera_fb_filter_c8(float* vec, double leftX, const double* kx, const float* filtr, int32_t nxfilt,
                              int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp);
+int32_t chimera_poiss_corr_stat_c8(float* j_fb, const float* grad_div_j_fb, const float* grad_rho_n_fb,
+                                   const double* DT, const float* w2_inv, int32_t nkx, int32_t nkr, int32_t nkO);
+int32_t chimera_field_drift_c8(float* EG_fb, const double* kx, double beta0, double dt, int32_t nkx, int32_t nkr,
+                               int32_t nkO);
 int32_t chimera_maxwell_push_with_spchrg_c8(float* EG_fb, const float* j_fb, const float* grad_rho_n_fb,
                                             const float* grad_rho_np1_fb, const float* C1, const float* C2,
                                             int32_t nkx, int32_t nkr, int32_t nkO);

@@ -816,6 +816,34 @@ void ref_poiss_corr(double *j_, const double *gdj_, const double *gn_, const dou
         }
 }
 
+/* Poisson correction of the static kick.  maxwell_solvers.f90:166-197.  DT complex (nkx). */
+void ref_poiss_corr_stat(double *j_, const double *gdj_, const double *gn_, const double *DT_, const double *w2_inv,
+                         int32_t nkx, int32_t nkr, int32_t nkO)
+{
+    cplx *J = (cplx *)j_;
+    const cplx *GD = (const cplx *)gdj_, *Gn = (const cplx *)gn_, *DT = (const cplx *)DT_;
+    const int64_t n = (int64_t)nkx * nkr * nkO;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < n; e++)
+        for (int l = 0; l < 3; l++)
+            J[e + l * n] = cadd(J[e + l * n], crs(w2_inv[e], cadd(GD[e + l * n], cmul(DT[e % nkx], Gn[e + l * n]))));
+}
+
+/* Free drift over one step.  maxwell_solvers.f90:199-226: PROPGTR = exp(fact*kx) with
+ * fact = -0.5*i*dt*beta0, whose real part is zero: (cos, sin) of ((-0.5*dt)*beta0)*kx. */
+void ref_field_drift(double *EG_, const double *kx, double beta0, double dt, int32_t nkx, int32_t nkr, int32_t nkO)
+{
+    cplx *EG = (cplx *)EG_;
+    const int64_t n = (int64_t)nkx * nkr * nkO;
+    const double a = (-0.5 * dt) * beta0;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < n; e++) {
+        const double ph = a * kx[e % nkx];
+        const cplx p = cmk(cos(ph), sin(ph));
+        for (int l = 0; l < 6; l++) EG[e + l * n] = cmul(EG[e + l * n], p);
+    }
+}
+
 /* Elementwise helpers.  maxwell_solvers.f90:228-318.  ncomp = 3 (vec) or 1 (scl). */
 void ref_omp_mult(double *v_, const double *A, int32_t nkx, int32_t nkr, int32_t nkO, int32_t ncomp)
 {

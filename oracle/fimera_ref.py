@@ -280,6 +280,18 @@ class Fimera:
         self._call("poiss_corr", j, gd, gn, gp, float(dt_inv), w2, j.shape[0], j.shape[1], j.shape[2])
         return j
 
+    def poiss_corr_stat(self, j_fb, grad_div_j_fb, grad_rho_n_fb, DT, w2_inv):
+        j = _f(j_fb, np.complex128)
+        gd, gn, dt = _f(grad_div_j_fb, np.complex128), _f(grad_rho_n_fb, np.complex128), _f(DT, np.complex128)
+        w2 = _f(w2_inv, np.float64)
+        self._call("poiss_corr_stat", j, gd, gn, dt, w2, j.shape[0], j.shape[1], j.shape[2])
+        return j
+
+    def field_drift(self, EG_fb, kx, beta0, dt):
+        EG, kx = _f(EG_fb, np.complex128), _f(kx, np.float64)
+        self._call("field_drift", EG, kx, float(beta0), float(dt), EG.shape[0], EG.shape[1], EG.shape[2])
+        return EG
+
     def _elem(self, name, v, A, ncomp, adtype):
         v, A = _f(v, np.complex128), _f(A, adtype)
         if self._fortran:

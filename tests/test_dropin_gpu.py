@@ -248,6 +248,36 @@ def test_maxwell(gpu, port):
     assert rel_max(gpu.omp_add_scl(s.copy(order="F"), s), port.omp_add_scl(s.copy(order="F"), s)) < TOL_ELEM
 
 
+def test_static_kick(gpu, port):
+    """poiss_corr_stat and field_drift (the static-kick branch of the field
+    update): drop-in FP64 to 1e-14 (FMA contraction; the device sin/cos of the
+    drift phase are within an ulp of the host's), and the `_c8` entry points on
+    device arrays to 1e-6 (float32 storage of fields and w2_inv; DT and kx stay
+    FP64)."""
+    from chimera_amd._lib import check, lib
+    from chimera_amd.device import DeviceArray
+    say("case: poiss_corr_stat / field_drift, 64x16x3, drop-in and _c8")
+    r = rng(1010)
+    sh = (64, 16, 3)
+    EG, j, gn, gd = C(r, *sh, 6), C(r, *sh, 3), C(r, *sh, 3), C(r, *sh, 3)
+    DT, w2 = C(r, sh[0]), F(r, *sh)
+    kx = 2 * np.pi * np.fft.fftfreq(sh[0], 0.07)
+    ref_j = port.poiss_corr_stat(j.copy(order="F"), gd, gn, DT, w2)
+    ref_e = port.field_drift(EG.copy(order="F"), kx, 0.999, 0.035)
+    assert rel_max(gpu.poiss_corr_stat(j.copy(order="F"), gd, gn, DT, w2), ref_j) < TOL_ELEM
+    assert rel_max(gpu.field_drift(EG.copy(order="F"), kx, 0.999, 0.035), ref_e) < TOL_ELEM
+    L = lib()
+    dev = lambda a, t: DeviceArray.from_numpy(np.asfortranarray(a).astype(t))
+    d_j, d_gd, d_gn, d_w2 = dev(j, np.complex64), dev(gd, np.complex64), dev(gn, np.complex64), dev(w2, np.float32)
+    d_DT, d_EG, d_kx = dev(DT, np.complex128), dev(EG, np.complex64), dev(kx, np.float64)
+    check("chimera_poiss_corr_stat_c8",
+          L.chimera_poiss_corr_stat_c8(d_j.ptr, d_gd.ptr, d_gn.ptr, d_DT.ptr, d_w2.ptr, *sh))
+    check("chimera_field_drift_c8", L.chimera_field_drift_c8(d_EG.ptr, d_kx.ptr, 0.999, 0.035, *sh))
+    check("chimera_synchronize", L.chimera_synchronize())
+    assert rel_max(d_j.numpy().astype(np.complex128), ref_j) < 1e-6
+    assert rel_max(d_EG.numpy().astype(np.complex128), ref_e) < 1e-6
+
+
 def test_particles_outside_the_grid(gpu, port):
     """Particles before leftX, on or past the last x node and at Rgrid[nr]
     are skipped by deposition and gather on both sides."""

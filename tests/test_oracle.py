@@ -81,6 +81,12 @@ def test_maxwell_match_reference(port, reference):
     w2 = F(r, *sh)
     assert np.array_equal(port.poiss_corr(j.copy(order="F"), gd, gn, gp, 20.0, w2),
                           reference.poiss_corr(j.copy(order="F"), gd, gn, gp, 20.0, w2))
+    DT = C(r, sh[0])
+    assert np.array_equal(port.poiss_corr_stat(j.copy(order="F"), gd, gn, DT, w2),
+                          reference.poiss_corr_stat(j.copy(order="F"), gd, gn, DT, w2))
+    kx = 2 * np.pi * np.fft.fftfreq(sh[0], 0.07)
+    assert np.array_equal(port.field_drift(EG.copy(order="F"), kx, 0.999, 0.035),
+                          reference.field_drift(EG.copy(order="F"), kx, 0.999, 0.035))
     for name, A in (("omp_mult_vec", w2), ("omp_add_vec", gn)):
         assert np.array_equal(getattr(port, name)(j.copy(order="F"), A), getattr(reference, name)(j.copy(order="F"), A))
     s = C(r, *sh)
