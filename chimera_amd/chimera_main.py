@@ -102,7 +102,12 @@ class ChimeraRun:
                     solver.fb_curr_in()
             self.frame_act(istep, "stage2")
             for solver in self.Solvers:
-                if "SpaceCharge" in solver.Args["Features"]:
+                if "StaticKick" in solver.Args["Features"]:
+                    # the static density is that of the half-step positions, at the stage-2 edge
+                    self.dep_dens(solver, "coords_halfstep")
+                    solver.fb_dens_in()
+                    solver.FBGradDens()
+                elif "SpaceCharge" in solver.Args["Features"]:
                     D = solver.Data
                     D["gradRho_fb_prv"], D["gradRho_fb_nxt"] = D["gradRho_fb_nxt"], D["gradRho_fb_prv"]
                     solver.fb_dens_in()
@@ -143,7 +148,7 @@ class ChimeraRun:
         solver = self.Solvers[0]
         A, D = solver.Args, solver.Data
         curr = None if "NoCurrent" in A["Features"] else D["J"]
-        dens = D["Rho"] if "SpaceCharge" in A["Features"] else None
+        dens = D["Rho"] if "SpaceCharge" in A["Features"] and "StaticKick" not in A["Features"] else None
         if curr is not None:
             curr.zero_()
         if dens is not None:
@@ -171,12 +176,14 @@ class ChimeraRun:
             solver.fb_curr_in()
 
     def project_density(self):
-        for solver in self.Solvers:
-            if "SpaceCharge" not in solver.Args["Features"]:
+        for solver in self.Solvers:                                        # chimera_main.py:103-114
+            static = "StaticKick" in solver.Args["Features"]
+            if "SpaceCharge" not in solver.Args["Features"] and not static:
                 continue
             D = solver.Data
-            D["gradRho_fb_prv"], D["gradRho_fb_nxt"] = D["gradRho_fb_nxt"], D["gradRho_fb_prv"]
-            self.dep_dens(solver)
+            if not static:
+                D["gradRho_fb_prv"], D["gradRho_fb_nxt"] = D["gradRho_fb_nxt"], D["gradRho_fb_prv"]
+            self.dep_dens(solver, "coords_halfstep" if static else "coords")
             solver.fb_dens_in()
             solver.FBGradDens()
 
@@ -220,8 +227,16 @@ class ChimeraRun:
     # ------------------------------------------------------------------ fields
     def update_fields(self):
         for solver in self.Solvers:                                        # chimera_main.py:116-128
-            solver.poiss_corr()
-            solver.maxwell_solver()
+            if "StaticKick" in solver.Args["Features"]:
+                solver.Data["EG_fb"].zero_()
+                for species in self.Particles:
+                    px0 = species.momentum_mean()
+                    solver.poiss_corr_stat(px0)
+                    solver.maxwell_solver_stat(px0)
+                    solver.field_drift(px0)
+            else:
+                solver.poiss_corr()
+                solver.maxwell_solver()
 
     def project_fields(self):
         for solver in self.Solvers:                                        # chimera_main.py:130-151
@@ -256,7 +271,8 @@ class ChimeraRun:
         if "AbsorbLayer" not in wind or wind["AbsorbLayer"] <= 0:
             return
         for solver in self.Solvers:
-            solver.damp_field()
+            if "StaticKick" not in solver.Args["Features"]:
+                solver.damp_field()
 
     def damp_plasma(self, wind):
         if "AbsorbLayer" not in wind:
@@ -281,6 +297,8 @@ class ChimeraRun:
     def postframe_corr(self, wind):
         # chimera_main.py:277-284: the condition is always true in the reference
         for solver in self.Solvers:
+            if "StaticKick" in solver.Args["Features"]:
+                continue
             if "SpaceCharge" in solver.Args["Features"]:
                 if "StillAsBackground" in solver.Args["Features"]:
                     self.dep_bg(solver)
@@ -330,6 +348,8 @@ class ChimeraRun:
                 out.append("plasma injection or absorption changes the particle count")
             if self.Solvers and "StillAsBackground" in self.Solvers[0].Args["Features"]:
                 out.append("the still background is re-deposited after each move")
+        if self.Solvers and "StaticKick" in self.Solvers[0].Args["Features"]:
+            out.append("StaticKick reads the mean momentum back to the host on every step")
         return out
 
     def enable_device_frame(self):

@@ -102,6 +102,61 @@ static int32_t beam_moments_impl(const double* coord_half, const U* momenta, con
     return st.finish();
 }
 
+// ---------------------------------------------------------------- weighted mean of px
+//
+// PXmean of the StaticKick field update (chimera_main.py:120-121):
+// sum(momenta[0]*weights) / sum(weights), both sums in FP64 by the same
+// two-launch fixed-order reduction as the moments.  No particles: 0/0 = NaN,
+// as the reference's NumPy quotient gives.
+template <typename U>
+__global__ void __launch_bounds__(MOM_T) k_pxw_part(const U* __restrict__ px, const U* __restrict__ w, int np,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[MOM_T];
+    double a = 0.0, b = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * MOM_T + threadIdx.x; i < np; i += (int64_t)gridDim.x * MOM_T) {
+        const double wi = (double)w[i];
+        a += (double)px[i] * wi;
+        b += wi;
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) {
+        part[2 * (int64_t)blockIdx.x] = a;
+        part[2 * (int64_t)blockIdx.x + 1] = b;
+    }
+}
+
+__global__ void __launch_bounds__(MOM_T) k_pxw_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    __shared__ double sh[MOM_T];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += MOM_T) {
+        a += part[2 * (int64_t)k];
+        b += part[2 * (int64_t)k + 1];
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) out[0] = a / b;
+}
+
+template <typename U>
+static int32_t momentum_mean_impl(const U* momenta, const U* wghts, int32_t np, double* px0) {
+    if (np < 0) return fail(ST_EINVAL, "momentum_mean: np=%d", np);
+    if (px0 == nullptr) return fail(ST_EINVAL, "momentum_mean: null output");
+    if (np > 0 && (!is_device_ptr(momenta) || !is_device_ptr(wghts)))
+        return fail(ST_EINVAL, "momentum_mean: momenta and weights are device pointers");
+    Stage st;
+    double* d_out = st.out(px0, 1);
+    CH_TRY(st.status());
+    const int nb = (int)std::min<int64_t>(MOM_MAXB, std::max<int64_t>(1, (np + MOM_T - 1) / MOM_T));
+    double* part = static_cast<double*>(scratch(12, (size_t)nb * 2 * sizeof(double)));
+    if (!part) return ST_ENOMEM;
+    hipLaunchKernelGGL((k_pxw_part<U>), nb, MOM_T, 0, stream(), momenta, wghts, np, part);
+    CH_LAUNCH_CHECK("k_pxw_part");
+    hipLaunchKernelGGL(k_pxw_final, 1, MOM_T, 0, stream(), part, nb, d_out);
+    CH_LAUNCH_CHECK("k_pxw_final");
+    return st.finish();
+}
+
 // ---------------------------------------------------------------- EM energy per kx
 //
 // nrg_out (diagnostics.py:109-122): out[x] = sum over k < nkr, m < nmodes of
@@ -197,6 +252,14 @@ int32_t chimera_beam_moments(const double* coord_half, const double* momenta, co
 int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
                                    int32_t np, double* sums) {
     return beam_moments_impl<float>(coord_half, momenta, wghts, ld, np, sums);
+}
+
+int32_t chimera_momentum_mean(const double* momenta, const double* wghts, int32_t np, double* px0) {
+    return momentum_mean_impl<double>(momenta, wghts, np, px0);
+}
+
+int32_t chimera_momentum_mean_mixed(const float* momenta, const float* wghts, int32_t np, double* px0) {
+    return momentum_mean_impl<float>(momenta, wghts, np, px0);
 }
 
 int32_t chimera_field_energy(const double* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr, int32_t nmodes,

@@ -4,9 +4,9 @@ the reference's Solver (solvers.py:26-225); the field arrays in ``Data`` are
 DeviceArrays of the reference's shapes, and every method runs libchimera_hip.
 
 Supported: the plain (non-KxShift) solver with or without SpaceCharge and
-StillAsBackground, Rcut, NoPoissonCorrection, Xchunked grids, CoPropagative.
-The envelope (KxShift) and StaticKick paths are not part of the PIC step and
-raise NotImplementedError.
+StillAsBackground, Rcut, NoPoissonCorrection, StaticKick, Xchunked grids,
+CoPropagative.  The envelope (KxShift) path is not part of the PIC step and
+raises NotImplementedError.
 
 Extra key 'Precision': 'fp64' (default, the reference's arithmetic) or
 'mixed': every field and FB-space array complex64, the Hankel, D+- and PSATD
@@ -33,8 +33,6 @@ class Solver:
             A["TimeInterval"] = (0.0, np.inf)        # solvers.py:48-49 (key kept as the reference sets it)
         if "KxShift" in A:
             raise NotImplementedError("the envelope (KxShift) solver is outside the PIC-step path")
-        if "StaticKick" in A["Features"]:
-            raise NotImplementedError("StaticKick is outside the PIC-step path")
         A.setdefault("Precision", "fp64")
         if A["Precision"] not in ("fp64", "mixed"):
             raise ValueError(f"Precision {A['Precision']!r}: 'fp64' or 'mixed'")
@@ -119,7 +117,7 @@ class Solver:
         D["scl_fb"] = DeviceArray(fb, ct)
         D["J_fb"] = DeviceArray(fb + (3,), ct)
         D["B_fb"] = DeviceArray(fb + (3,), ct)
-        if "SpaceCharge" in A["Features"]:
+        if "SpaceCharge" in A["Features"] or "StaticKick" in A["Features"]:   # solvers.py:201-207
             D["Rho"] = DeviceArray(sp, ct)
             D["Rho_fb"] = DeviceArray(fb, ct)
             D["gradRho_fb_nxt"] = DeviceArray(fb + (3,), ct)
@@ -190,15 +188,34 @@ class Solver:
                 call(self.fn("chimera_omp_add_vec"), D["J_fb"].ptr, D["vec_fb"].ptr, Nx, Nkr, M)
 
     def maxwell_solver_stat(self, px0):
-        """solvers.py:333-362 (complex tables built on the host, once)."""
+        """solvers.py:333-362 (complex tables built on the host for each px0)."""
         A, D = self.Args, self.Data
-        if "SpaceCharge" not in A["Features"]:
+        if "SpaceCharge" not in A["Features"] and "StaticKick" not in A["Features"]:
             return
         C1, C2 = ops.static_tables(A["kx_g"], A["w"], px0)
         c1, c2 = DeviceArray.from_numpy(C1, self.ct), DeviceArray.from_numpy(C2, self.ct)
         Nx, Nkr, M = self.n_fb
         call(self.fn("chimera_maxwell_init_push"), D["EG_fb"].ptr, D["J_fb"].ptr, D["gradRho_fb_nxt"].ptr, c1.ptr, c2.ptr, Nx,
              Nkr, M)
+
+    def poiss_corr_stat(self, px0):
+        """solvers.py:364-387: J_fb += (graddiv J_fb + DT gradRho_fb_nxt) PoissFact,
+        DT = -i beta0 kx (nkx values, FP64 in both precisions)."""
+        A, D = self.Args, self.Data
+        if "NoPoissonCorrection" in A["Features"]:
+            return
+        DT = DeviceArray.from_numpy(-1.0j * px0 / np.sqrt(1 + px0 ** 2) * A["kx"], np.complex128)
+        D["vec_fb"].copy_(D["J_fb"])
+        self.FBGradDiv()
+        Nx, Nkr, M = self.n_fb
+        call(self.fn("chimera_poiss_corr_stat"), D["J_fb"].ptr, D["vec_fb"].ptr, D["gradRho_fb_nxt"].ptr, DT.ptr,
+             self.dev["PoissFact"].ptr, Nx, Nkr, M)
+
+    def field_drift(self, px0):
+        """solvers.py:389-405: EG_fb *= exp(-0.5 i dt beta0 kx)."""
+        Nx, Nkr, M = self.n_fb
+        call(self.fn("chimera_field_drift"), self.Data["EG_fb"].ptr, self.dev["kx"].ptr,
+             float(px0 / np.sqrt(1 + px0 ** 2)), float(self.Args["TimeStep"]), Nx, Nkr, M)
 
     # ------------------------------------------------------------------ FB transforms (solvers.py:407-465)
     def fb_curr_in(self):

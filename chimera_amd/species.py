@@ -184,6 +184,15 @@ class Specie:
         D["weights"].write(0, weights)
         self.np = n
 
+    def momentum_mean(self):
+        """sum(momenta[0] * weights) / sum(weights) on the device (the PXmean of
+        the StaticKick field update, chimera_main.py:120-121); NaN without
+        particles, as the reference's quotient gives."""
+        out = np.zeros(1)
+        call("chimera_momentum_mean_mixed" if self.mixed else "chimera_momentum_mean", self.ptr("momenta"),
+             self.ptr("weights"), self.np, out.ctypes.data)
+        return float(out[0])
+
     def beam_focus(self, x_foc):
         """species.py:473-486: move a relativistic bunch back along its ballistic
         drift so that it focuses x_foc downstream (a set-up step, done on host

@@ -377,6 +377,12 @@ int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const dou
  * by np, so repeated calls give the same bits. */
 int32_t chimera_beam_moments(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
                              int32_t np, double* sums);
+/* px0 = sum(px w) / sum(w) over the np particles (px the first row of
+ * momenta), the mean momentum of the StaticKick field update, FP64 sums in an
+ * order fixed by np; NaN for np = 0.  Replaces the PXmean expression of
+ * update_fields, chimera_main.py:120-121.  Device particle arrays; px0 in
+ * device or host memory, as for chimera_beam_moments. */
+int32_t chimera_momentum_mean(const double* momenta, const double* wghts, int32_t np, double* px0);
 
 /* EM energy per kx of nrg_out (diagnostics.py:109-122), before its reordering
  * of the kx axis: out[x] = sum_{k,m} EnergyFact[x,k,m] * sum_{c<3} |EG_fb[x,k,m,c]|^2
@@ -417,6 +423,7 @@ int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* mome
                                   int32_t np, const double* lims, int32_t test_half, int32_t* np_out);
 int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
                                    int32_t np, double* sums);
+int32_t chimera_momentum_mean_mixed(const float* momenta, const float* wghts, int32_t np, double* px0);
 int32_t chimera_field_energy_c8(const float* EG_fb, const double* energy_fact, int32_t nkx, int32_t nkr,
                                 int32_t nmodes, double* out);
 int32_t chimera_line_power_c8(const float* vec, const double* Rgrid, int32_t nkx, int32_t nr, int32_t nmodes,

@@ -6,7 +6,7 @@ TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py as the checker; the product never imports it.
 
 Covered: FixedCell species (Still or pushed, analytic-undulator device),
-SpaceCharge / StillAsBackground / NoPoissonCorrection solvers, Rcut, moving
+SpaceCharge / StillAsBackground / NoPoissonCorrection / StaticKick solvers, Rcut, moving
 frames with AbsorbLayer / AddPlasma / IonsOnTop / Staged, and the
 'PreDepositBackground' run feature (background deposited before the initial
 half step).  Arrays and call order follow solvers.py, species.py and
@@ -78,7 +78,7 @@ class Solver:
         z = lambda s: np.zeros(s, dtype=complex, order="F")
         D = self.Data = {"EB": z(sp + (6,)), "J": z(sp + (3,)), "scl_spc": z(sp), "EG_fb": z(fb + (6,)),
                          "vec_fb": z(fb + (3,)), "scl_fb": z(fb), "J_fb": z(fb + (3,)), "B_fb": z(fb + (3,))}
-        if "SpaceCharge" in A["Features"]:
+        if "SpaceCharge" in A["Features"] or "StaticKick" in A["Features"]:   # solvers.py:201-207
             D["Rho"], D["Rho_fb"] = z(sp), z(fb)
             D["gradRho_fb_nxt"], D["gradRho_fb_prv"] = z(fb + (3,)), z(fb + (3,))
         if "StillAsBackground" in A["Features"]:
@@ -146,7 +146,7 @@ class Solver:
 
     def maxwell_solver_stat(self, px0):                                   # solvers.py:333-362
         A, D = self.Args, self.Data
-        if "SpaceCharge" not in A["Features"]:
+        if "SpaceCharge" not in A["Features"] and "StaticKick" not in A["Features"]:
             return
         beta0 = px0 / np.sqrt(1 + px0 ** 2)
         w = A["w"]
@@ -156,6 +156,19 @@ class Solver:
         C1[..., 0], C1[..., 1] = 1.0j * k / (w ** 2 - k ** 2), -1.0 / (w ** 2 - k ** 2)
         C2[..., 0], C2[..., 1] = w ** 2 / (w ** 2 - k ** 2), 1.0j * k / (w ** 2 - k ** 2)
         D["EG_fb"] = self.c.maxwell_init_push(D["EG_fb"], D["J_fb"], D["gradRho_fb_nxt"], C1, C2)
+
+    def poiss_corr_stat(self, px0):                                       # solvers.py:364-387
+        A, D = self.Args, self.Data
+        if "NoPoissonCorrection" in A["Features"]:
+            return
+        DT = -1.0j * px0 / np.sqrt(1 + px0 ** 2) * A["kx"]
+        D["vec_fb"][:] = D["J_fb"]
+        self.FBGradDiv()
+        D["J_fb"] = self.c.poiss_corr_stat(D["J_fb"], D["vec_fb"], D["gradRho_fb_nxt"], DT, A["PoissFact"])
+
+    def field_drift(self, px0):                                           # solvers.py:389-405
+        D = self.Data
+        D["EG_fb"] = self.c.field_drift(D["EG_fb"], self.Args["kx"], px0 / np.sqrt(1 + px0 ** 2), self.Args["TimeStep"])
 
     def fb_curr_in(self):                                                 # solvers.py:407-420
         A, D = self.Args, self.Data
@@ -383,9 +396,7 @@ class ChimeraRun:
         self.project_current()
         self.frame_act(istep, "stage2")
         self.project_density()
-        for solver in self.Solvers:
-            solver.poiss_corr()
-            solver.maxwell_solver()
+        self.update_fields()
         self.project_fields()
         for sp in self.Particles:
             sp.make_device(istep)
@@ -403,11 +414,25 @@ class ChimeraRun:
                                         solver.Args["leftX"], *solver.Args["DepProj"])
             solver.fb_curr_in()
 
-    def project_density(self):
+    def update_fields(self):                                              # chimera_main.py:116-128
         for solver in self.Solvers:
-            if "SpaceCharge" not in solver.Args["Features"]:
+            if "StaticKick" in solver.Args["Features"]:
+                solver.Data["EG_fb"][:] = 0.0
+                for sp in self.Particles:
+                    PXmean = (sp.Data["momenta"][0] * sp.Data["weights"]).sum() / sp.Data["weights"].sum()
+                    solver.poiss_corr_stat(PXmean)
+                    solver.maxwell_solver_stat(PXmean)
+                    solver.field_drift(PXmean)
+            else:
+                solver.poiss_corr()
+                solver.maxwell_solver()
+
+    def project_density(self):                                            # chimera_main.py:103-114
+        for solver in self.Solvers:
+            if "SpaceCharge" not in solver.Args["Features"] and "StaticKick" not in solver.Args["Features"]:
                 continue
-            solver.Data["gradRho_fb_prv"][:] = solver.Data["gradRho_fb_nxt"]
+            if "StaticKick" not in solver.Args["Features"]:
+                solver.Data["gradRho_fb_prv"][:] = solver.Data["gradRho_fb_nxt"]
             self.dep_dens(solver)
             solver.fb_dens_in()
             solver.FBGradDens()
@@ -417,10 +442,11 @@ class ChimeraRun:
         D["Rho"][:] = 0.0
         if "StillAsBackground" in solver.Args["Features"]:
             D["Rho"] += D["BckGrndRho"]
+        component = "coords_halfstep" if "StaticKick" in solver.Args["Features"] else "coords"
         for sp in self.Particles:
             if sp.Data["coords"].shape[1] == 0 or "Still" in sp.Args["Features"]:
                 continue
-            D["Rho"] = _ch().dep_dens(sp.Data["coords"], sp.Data["weights"], D["Rho"], solver.Args["leftX"],
+            D["Rho"] = _ch().dep_dens(sp.Data[component], sp.Data["weights"], D["Rho"], solver.Args["leftX"],
                                       *solver.Args["DepProj"])
 
     def dep_bg(self, solver):                                             # chimera_main.py:220-248
@@ -453,7 +479,8 @@ class ChimeraRun:
             if act == "stage1":
                 if "AbsorbLayer" in wind and wind["AbsorbLayer"] > 0:
                     for solver in self.Solvers:
-                        solver.damp_field()
+                        if "StaticKick" not in solver.Args["Features"]:
+                            solver.damp_field()
                 self.move_frame(wind)
                 if "AddPlasma" in wind:
                     xs = int(wind["shiftX"] / wind["TimeStep"]) + 1
@@ -470,6 +497,8 @@ class ChimeraRun:
                     for sp in self.Particles:
                         sp.chunk_and_damp(wind=wind)
                 for solver in self.Solvers:
+                    if "StaticKick" in solver.Args["Features"]:
+                        continue
                     if "SpaceCharge" in solver.Args["Features"]:
                         if "StillAsBackground" in solver.Args["Features"]:
                             self.dep_bg(solver)
