@@ -427,6 +427,23 @@ int32_t chimera_undul_analytic(const double* coord, double* Fld, double t, const
     return st.finish();
 }
 
+int32_t chimera_undul_analytic_taper(const double* coord, double* Fld, double t, const double* params, int32_t np) {
+    (void)t;
+    if (np < 0 || params == nullptr) return fail(ST_EINVAL, "undul_analytic_taper: bad arguments");
+    if (np == 0) return ST_OK;
+    double hp[5];
+    if (is_device_ptr(params)) CH_HIP(hipMemcpy(hp, params, sizeof hp, hipMemcpyDeviceToHost));
+    else memcpy(hp, params, sizeof hp);
+    Stage st;
+    const double* x = st.in(coord, 3 * (size_t)np);
+    double* F = st.inout(Fld, 6 * (size_t)np);
+    CH_TRY(st.status());
+    const UndulParams up = make_undul_taper(hp);
+    hipLaunchKernelGGL(k_undul_analytic, blocks_for(np, 256), 256, 0, stream(), x, AOS3, F, AOS6, up, np);
+    CH_LAUNCH_CHECK("k_undul_analytic");
+    return st.finish();
+}
+
 int32_t chimera_genparts(double* coord, int32_t* indPart, const double* Xgrid, const double* Rgrid,
                          const double* RandPackO, const double* PackX, const double* PackR, const double* PackO,
                          int32_t np, int32_t nx, int32_t nr, int32_t PPC) {

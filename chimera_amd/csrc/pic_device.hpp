@@ -124,7 +124,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + k;
 }
 
-struct UndulParams { double a0, lambda, X0, Lx, ku; };
+struct UndulParams { double a0, lambda, X0, Lx, ku, taper; int tapered; };
 
 inline UndulParams make_undul(const double p[4]) {
     UndulParams u;
@@ -133,11 +133,22 @@ inline UndulParams make_undul(const double p[4]) {
     u.X0 = p[2];
     u.Lx = p[3];
     u.ku = 2.0 * M_PI / p[1];
+    u.taper = 0.0;
+    u.tapered = 0;
+    return u;
+}
+
+// params = (a0, lambda, X0, Lx, taper) of the linearly tapered undulator.
+inline UndulParams make_undul_taper(const double p[5]) {
+    UndulParams u = make_undul(p);
+    u.taper = p[4];
+    u.tapered = 1;
     return u;
 }
 
 // Analytic planar undulator (devices.f90:162-203), evaluated in FP64 and added
-// to the gathered fields.
+// to the gathered fields; tapered, the amplitude is scaled by
+// 1 + taper (x - X0 - Lx/2) / (Lx/2) (devices.f90:117-160).
 template <typename R>
 __device__ __forceinline__ void undul_field(const UndulParams& up, double x, double y, R f[6]) {
     double ampl;
@@ -145,6 +156,7 @@ __device__ __forceinline__ void undul_field(const UndulParams& up, double x, dou
     else if (x > up.X0 && x < up.X0 + up.lambda) ampl = (x - up.X0) / up.lambda;
     else if (x > up.X0 + up.Lx - up.lambda && x < up.X0 + up.Lx) ampl = (up.X0 + up.Lx - x) / up.lambda;
     else ampl = 1.0;
+    if (up.tapered) ampl = ampl * (1.0 + up.taper * (x - up.X0 - 0.5 * up.Lx) / (0.5 * up.Lx));
     ampl = ampl * up.a0;
     f[4] += (R)(ampl * sin(up.ku * (x - up.X0)) * cosh(up.ku * y));
     f[3] += (R)(ampl * cos(up.ku * (x - up.X0)) * sinh(up.ku * y));

@@ -243,6 +243,10 @@ int32_t chimera_omp_add_scl(double* scl_fb, const double* A, int32_t nkx, int32_
 /* Analytic planar undulator, params = (a0, lambda, X0, Lx); Fld(6,np) += field.
  * Replaces Undul_Analytic, devices.f90:162-203. */
 int32_t chimera_undul_analytic(const double* coord, double* Fld, double t, const double* params, int32_t np);
+/* Linearly tapered analytic undulator, params = (a0, lambda, X0, Lx, taper): the
+ * amplitude of chimera_undul_analytic times 1 + taper (x - X0 - Lx/2) / (Lx/2).
+ * Replaces Undul_Analytic_Taper, devices.f90:117-160. */
+int32_t chimera_undul_analytic_taper(const double* coord, double* Fld, double t, const double* params, int32_t np);
 
 /* ================================================================ resident tier
  *

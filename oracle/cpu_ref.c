@@ -868,6 +868,29 @@ void ref_omp_add(double *v_, const double *A_, int32_t nkx, int32_t nkr, int32_t
 /* ------------------------------------------------------------------------- */
 
 /* Analytic planar undulator.  devices.f90:162-203.  params = (a0, lambda, X0, Lx). */
+/* Linearly tapered analytic undulator.  devices.f90:117-160: the amplitude of
+ * Undul_Analytic times 1 + taper (x - X0 - 0.5 Lx) / (0.5 Lx). */
+void ref_undul_analytic_taper(const double *coord, double *Fld, double t, const double *params, int32_t np)
+{
+    (void)t;
+    const double pi = 4.0 * atan(1.0);
+    const double a0 = params[0], lambda = params[1], X0 = params[2], Lx = params[3], taper = params[4];
+    const double ku = 2.0 * pi / lambda;
+#pragma omp parallel for schedule(static)
+    for (int64_t ip = 0; ip < np; ip++) {
+        const double x = coord[3 * ip], y = coord[3 * ip + 1];
+        double ampl;
+        if (x <= X0 || x >= X0 + Lx) ampl = 0.0;
+        else if (x > X0 && x < X0 + lambda) ampl = (x - X0) / lambda;
+        else if (x > X0 + Lx - lambda && x < X0 + Lx) ampl = (X0 + Lx - x) / lambda;
+        else ampl = 1.0;
+        ampl = ampl * (1.0 + taper * (x - X0 - 0.5 * Lx) / (0.5 * Lx));
+        ampl = ampl * a0;
+        Fld[6 * ip + 4] = Fld[6 * ip + 4] + ampl * sin(ku * (x - X0)) * cosh(ku * y);
+        Fld[6 * ip + 3] = Fld[6 * ip + 3] + ampl * cos(ku * (x - X0)) * sinh(ku * y);
+    }
+}
+
 void ref_undul_analytic(const double *coord, double *Fld, double t, const double *params, int32_t np)
 {
     (void)t;

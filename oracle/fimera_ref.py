@@ -318,3 +318,8 @@ class Fimera:
         coord, Fld, params = _f(coord, np.float64), _f(Fld, np.float64), _f(params, np.float64)
         self._call("undul_analytic", coord, Fld, float(t), params, coord.shape[1])
         return Fld
+
+    def undul_analytic_taper(self, coord, Fld, t, params):
+        coord, Fld, params = _f(coord, np.float64), _f(Fld, np.float64), _f(params, np.float64)
+        self._call("undul_analytic_taper", coord, Fld, float(t), params, coord.shape[1])
+        return Fld

@@ -124,6 +124,11 @@ def test_undulator(gpu, port):
     a = gpu.undul_analytic(coord, ft.copy(order="F"), 0.0, p)
     b = port.undul_analytic(coord, ft.copy(order="F"), 0.0, p)
     assert rel_max(a, b) < 1e-13
+    say("case: undul_analytic_taper")
+    pt = np.array([1.7, 0.9, -2.0, 3.5, 0.3])
+    a = gpu.undul_analytic_taper(coord, ft.copy(order="F"), 0.0, pt)
+    b = port.undul_analytic_taper(coord, ft.copy(order="F"), 0.0, pt)
+    assert rel_max(a, b) < 1e-13
 
 
 def test_genparts(gpu, port):

@@ -101,6 +101,13 @@ def test_particle_tools_match_reference(port, reference):
     ft = F(rng(41), 6, coord.shape[1])
     assert np.array_equal(port.undul_analytic(coord, ft.copy(order="F"), 0.0, p),
                           reference.undul_analytic(coord, ft.copy(order="F"), 0.0, p))
+    pt = np.array([1.7, 0.9, -2.0, 3.5, 0.3])
+    a = port.undul_analytic_taper(coord, ft.copy(order="F"), 0.0, pt)
+    assert np.array_equal(a, reference.undul_analytic_taper(coord, ft.copy(order="F"), 0.0, pt))
+    assert not np.array_equal(a, port.undul_analytic(coord, ft.copy(order="F"), 0.0, pt[:4]))
+    # no taper: the untapered undulator, bit for bit
+    assert np.array_equal(port.undul_analytic_taper(coord, ft.copy(order="F"), 0.0, np.append(p, 0.0)),
+                          port.undul_analytic(coord, ft.copy(order="F"), 0.0, p))
     Xg = g.leftX + g.dx * np.arange(g.nx)
     RP = rng(42).random((g.nx, g.nkr + 1))
     pX, pR, pO = np.mgrid[1:2:2j, 1:2:2j, 1:4:4j]
