@@ -176,6 +176,67 @@ __global__ void k_undul_analytic(const double* __restrict__ coord, PStride sx, d
     F[sf.at(ip, 3)] += f[3];
 }
 
+// devices.f90:18-62 (TAP = 0) and 64-115 (TAP = 1): the field map a0(2, nx)
+// (two rows: the amplitudes behind Fld(5) and Fld(4)) on nodes Xleft + (i-1) dx,
+// i = 1..nx, interpolated with quadratic weights around node ix; particles
+// outside [Xleft + dx, Xleft + (nx - 1) dx] see no field.  A node index outside
+// 1..nx contributes zero (the reference reads a0(:, 0), before the map, for
+// x < Xleft + 1.5 dx).
+template <int TAP>
+__global__ void k_undul_mapped(const double* __restrict__ coord, double* __restrict__ F,
+                               const double* __restrict__ a0, int nx, double Xleft, double dx, double dx_inv,
+                               double Xright, double ku, double Lx, double taper, double x_shift, int np) {
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    const double xp = coord[3 * (int64_t)ip];
+    if (xp < Xleft + dx || xp > Xright - dx) return;
+    const double yp = coord[3 * (int64_t)ip + 1];
+    const double fx = floor((xp - Xleft) * dx_inv + 0.5);
+    if (!(fx >= 0.0 && fx <= (double)nx + 1.0)) return;      // NaN or off the map: no node in range
+    const int ix = (int)fx;
+    const double ddx = (xp - Xleft) * dx_inv - ix;
+    const double s[3] = {0.5 * ((0.5 - ddx) * (0.5 - ddx)), 0.75 - ddx * ddx, 0.5 * ((0.5 + ddx) * (0.5 + ddx))};
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int j = ix - 1 + k;                               // 1-based node
+        const double v1 = (j >= 1 && j <= nx) ? a0[2 * (int64_t)(j - 1)] : 0.0;
+        const double v2 = (j >= 1 && j <= nx) ? a0[2 * (int64_t)(j - 1) + 1] : 0.0;
+        b1 = b1 + s[k] * v1;
+        b2 = b2 + s[k] * v2;
+    }
+    if (TAP) {
+        const double amp = 1.0 + taper * (xp - x_shift - Xleft - 0.5 * Lx) / (0.5 * Lx);
+        F[6 * (int64_t)ip + 4] += amp * b1 * cosh(ku * yp);
+        F[6 * (int64_t)ip + 3] += amp * b2 * sinh(ku * yp);
+    } else {
+        F[6 * (int64_t)ip + 4] += b1 * cosh(ku * yp);
+        F[6 * (int64_t)ip + 3] += b2 * sinh(ku * yp);
+    }
+}
+
+template <int TAP>
+static int32_t undul_mapped_impl(const double* coord, double* Fld, const double* a0, const double* params, int32_t np,
+                                 int32_t nx) {
+    constexpr int NP = TAP ? 5 : 3;
+    if (np < 0 || nx < 1 || params == nullptr || a0 == nullptr) return fail(ST_EINVAL, "undul_mapped: bad arguments");
+    if (np == 0) return ST_OK;
+    double hp[NP];
+    if (is_device_ptr(params)) CH_HIP(hipMemcpy(hp, params, sizeof hp, hipMemcpyDeviceToHost));
+    else memcpy(hp, params, sizeof hp);
+    const double lambda = hp[0], Xleft = hp[1], dx = hp[2];
+    const double Lx = TAP ? hp[3] : 0.0, taper = TAP ? hp[4] : 0.0;
+    Stage st;
+    const double* x = st.in(coord, 3 * (size_t)np);
+    const double* a = st.in(a0, 2 * (size_t)nx);
+    double* F = st.inout(Fld, 6 * (size_t)np);
+    CH_TRY(st.status());
+    hipLaunchKernelGGL((k_undul_mapped<TAP>), blocks_for(np, 256), 256, 0, stream(), x, F, a, nx, Xleft, dx, 1.0 / dx,
+                       Xleft + nx * dx, 2.0 * M_PI / lambda, Lx, taper, 0.5 * (nx * dx - Lx), np);
+    CH_LAUNCH_CHECK("k_undul_mapped");
+    return st.finish();
+}
+
 // ----------------------------------------------------------------- lattice
 
 // particle_tools.f90:84-128.  One thread per (ix, ir, ip) lattice slot.
@@ -442,6 +503,18 @@ int32_t chimera_undul_analytic_taper(const double* coord, double* Fld, double t,
     hipLaunchKernelGGL(k_undul_analytic, blocks_for(np, 256), 256, 0, stream(), x, AOS3, F, AOS6, up, np);
     CH_LAUNCH_CHECK("k_undul_analytic");
     return st.finish();
+}
+
+int32_t chimera_undul_mapped(const double* coord, double* Fld, double t, const double* a0, const double* params,
+                             int32_t np, int32_t nx) {
+    (void)t;
+    return undul_mapped_impl<0>(coord, Fld, a0, params, np, nx);
+}
+
+int32_t chimera_undul_mapped_tap(const double* coord, double* Fld, double t, const double* a0, const double* params,
+                                 int32_t np, int32_t nx) {
+    (void)t;
+    return undul_mapped_impl<1>(coord, Fld, a0, params, np, nx);
 }
 
 int32_t chimera_genparts(double* coord, int32_t* indPart, const double* Xgrid, const double* Rgrid,

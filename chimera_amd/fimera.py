@@ -25,7 +25,7 @@ __all__ = [
     "align_data_scl", "dep_curr", "dep_dens", "dep_curr_chnk", "dep_dens_chnk", "proj_fld", "eb_correction",
     "fb_vec_in", "fb_scl_in", "fb_eb_out", "fb_filtr", "fb_rot", "fb_grad", "fb_div", "fb_graddiv",
     "maxwell_push_with_spchrg", "maxwell_push_wo_spchrg", "maxwell_init_push", "poiss_corr", "omp_mult_vec",
-    "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic", "undul_analytic_taper", "poiss_corr_stat", "field_drift",
+    "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic", "undul_analytic_taper", "undul_mapped", "undul_mapped_tap", "poiss_corr_stat", "field_drift",
 ]
 
 
@@ -316,6 +316,22 @@ def undul_analytic(coord, Fld, t, params):
     """devices.f90:162-203 -> Fld"""
     x, F, p = _arr(coord, np.float64), _arr(Fld, np.float64), _arr(params, np.float64)
     _call("chimera_undul_analytic", _p(x), _p(F), float(t), _p(p), x.shape[1])
+    return F
+
+
+def undul_mapped(coord, Fld, t, a0, params):
+    """devices.f90:18-62 -> Fld"""
+    x, F, a, p = (_arr(v, np.float64) for v in (coord, Fld, a0, params))
+    _call("chimera_undul_mapped", _p(x), _p(F), float(t), _p(a), _p(p), x.shape[1], a.shape[1])
+    return F
+
+
+def undul_mapped_tap(coord, Fld, t, a0, params):
+    """devices.f90:64-115 -> Fld"""
+    x, F, a, p = (_arr(v, np.float64) for v in (coord, Fld, a0, params))
+    if p.size < 5:
+        raise ValueError("undul_mapped_tap: params = (lambda, Xleft, dx, Lx, taper)")
+    _call("chimera_undul_mapped_tap", _p(x), _p(F), float(t), _p(a), _p(p), x.shape[1], a.shape[1])
     return F
 
 

@@ -247,6 +247,17 @@ int32_t chimera_undul_analytic(const double* coord, double* Fld, double t, const
  * amplitude of chimera_undul_analytic times 1 + taper (x - X0 - Lx/2) / (Lx/2).
  * Replaces Undul_Analytic_Taper, devices.f90:117-160. */
 int32_t chimera_undul_analytic_taper(const double* coord, double* Fld, double t, const double* params, int32_t np);
+/* Mapped undulator: a0(2,nx) on the nodes Xleft + i dx (i = 0..nx-1), params =
+ * (lambda, Xleft, dx); Fld(5) += (quadratic interpolation of a0(1,:)) cosh(ku y),
+ * Fld(4) += (that of a0(2,:)) sinh(ku y) for x in [Xleft + dx, Xleft + (nx-1) dx].
+ * A node outside the map contributes zero.  Replaces Undul_Mapped, devices.f90:18-62. */
+int32_t chimera_undul_mapped(const double* coord, double* Fld, double t, const double* a0, const double* params,
+                             int32_t np, int32_t nx);
+/* The same, tapered: params = (lambda, Xleft, dx, Lx, taper), amplitude times
+ * 1 + taper (x - x_shift - Xleft - Lx/2) / (Lx/2), x_shift = (nx dx - Lx)/2.
+ * Replaces Undul_Mapped_Tap, devices.f90:64-115. */
+int32_t chimera_undul_mapped_tap(const double* coord, double* Fld, double t, const double* a0, const double* params,
+                                 int32_t np, int32_t nx);
 
 /* ================================================================ resident tier
  *

@@ -891,6 +891,59 @@ void ref_undul_analytic_taper(const double *coord, double *Fld, double t, const 
     }
 }
 
+/* Mapped undulator.  devices.f90:18-62 (tap = 0, params = lambda, Xleft, dx) and
+ * 64-115 (tap = 1, params add Lx, taper).  a0(2,nx) Fortran order, nodes 1..nx;
+ * a node outside 1..nx contributes zero (the reference reads before the map there). */
+static void undul_mapped(const double *coord, double *Fld, const double *a0, const double *params, int32_t np,
+                         int32_t nx, int tap)
+{
+    const double pi = 4.0 * atan(1.0);
+    const double lambda = params[0], Xleft = params[1], dx = params[2];
+    const double Lx = tap ? params[3] : 0.0, taper = tap ? params[4] : 0.0;
+    const double ku = 2.0 * pi / lambda, dx_inv = 1.0 / dx, Xright = Xleft + nx * dx;
+    const double x_shift = 0.5 * (nx * dx - Lx);
+#pragma omp parallel for schedule(static)
+    for (int64_t ip = 0; ip < np; ip++) {
+        const double xp = coord[3 * ip];
+        if (xp < Xleft + dx || xp > Xright - dx) continue;
+        const double yp = coord[3 * ip + 1];
+        const double fx = floor((xp - Xleft) * dx_inv + 0.5);
+        if (!(fx >= 0.0 && fx <= (double)nx + 1.0)) continue;
+        const int ix = (int)fx;
+        const double ddx = (xp - Xleft) * dx_inv - ix;
+        const double S0[3] = {0.5 * ((0.5 - ddx) * (0.5 - ddx)), 0.75 - ddx * ddx, 0.5 * ((0.5 + ddx) * (0.5 + ddx))};
+        double b1 = 0.0, b2 = 0.0;
+        for (int k = 0; k < 3; k++) {
+            const int j = ix - 1 + k;
+            const int in = j >= 1 && j <= nx;
+            b1 = b1 + S0[k] * (in ? a0[2 * (int64_t)(j - 1)] : 0.0);
+            b2 = b2 + S0[k] * (in ? a0[2 * (int64_t)(j - 1) + 1] : 0.0);
+        }
+        if (tap) {
+            const double amp = 1.0 + taper * (xp - x_shift - Xleft - 0.5 * Lx) / (0.5 * Lx);
+            Fld[6 * ip + 4] = Fld[6 * ip + 4] + amp * b1 * cosh(ku * yp);
+            Fld[6 * ip + 3] = Fld[6 * ip + 3] + amp * b2 * sinh(ku * yp);
+        } else {
+            Fld[6 * ip + 4] = Fld[6 * ip + 4] + b1 * cosh(ku * yp);
+            Fld[6 * ip + 3] = Fld[6 * ip + 3] + b2 * sinh(ku * yp);
+        }
+    }
+}
+
+void ref_undul_mapped(const double *coord, double *Fld, double t, const double *a0, const double *params, int32_t np,
+                      int32_t nx)
+{
+    (void)t;
+    undul_mapped(coord, Fld, a0, params, np, nx, 0);
+}
+
+void ref_undul_mapped_tap(const double *coord, double *Fld, double t, const double *a0, const double *params,
+                          int32_t np, int32_t nx)
+{
+    (void)t;
+    undul_mapped(coord, Fld, a0, params, np, nx, 1);
+}
+
 void ref_undul_analytic(const double *coord, double *Fld, double t, const double *params, int32_t np)
 {
     (void)t;

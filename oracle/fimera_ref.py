@@ -319,6 +319,16 @@ class Fimera:
         self._call("undul_analytic", coord, Fld, float(t), params, coord.shape[1])
         return Fld
 
+    def undul_mapped(self, coord, Fld, t, a0, params):
+        coord, Fld, a0, params = (_f(a, np.float64) for a in (coord, Fld, a0, params))
+        self._call("undul_mapped", coord, Fld, float(t), a0, params, coord.shape[1], a0.shape[1])
+        return Fld
+
+    def undul_mapped_tap(self, coord, Fld, t, a0, params):
+        coord, Fld, a0, params = (_f(a, np.float64) for a in (coord, Fld, a0, params))
+        self._call("undul_mapped_tap", coord, Fld, float(t), a0, params, coord.shape[1], a0.shape[1])
+        return Fld
+
     def undul_analytic_taper(self, coord, Fld, t, params):
         coord, Fld, params = _f(coord, np.float64), _f(Fld, np.float64), _f(params, np.float64)
         self._call("undul_analytic_taper", coord, Fld, float(t), params, coord.shape[1])

@@ -129,6 +129,15 @@ def test_undulator(gpu, port):
     a = gpu.undul_analytic_taper(coord, ft.copy(order="F"), 0.0, pt)
     b = port.undul_analytic_taper(coord, ft.copy(order="F"), 0.0, pt)
     assert rel_max(a, b) < 1e-13
+    for name, params in (("undul_mapped", [0.9, -1.2, 0.1]), ("undul_mapped_tap", [0.9, -1.2, 0.1, 3.1, 0.25])):
+        say(f"case: {name}, 40-node map, particles over and around it")
+        r = rng(402)
+        x = r.uniform(-1.5, 3.1, 4000)
+        c = np.asfortranarray(np.vstack([x, r.normal(0, 0.3, x.size), r.normal(0, 0.3, x.size)]))
+        a0, f0 = F(r, 2, 40), F(r, 6, x.size)
+        a = getattr(gpu, name)(c, f0.copy(order="F"), 0.0, a0, np.array(params))
+        b = getattr(port, name)(c, f0.copy(order="F"), 0.0, a0, np.array(params))
+        assert rel_max(a, b) < 1e-13
 
 
 def test_genparts(gpu, port):

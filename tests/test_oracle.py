@@ -94,6 +94,54 @@ def test_maxwell_match_reference(port, reference):
     assert np.array_equal(port.omp_add_scl(s.copy(order="F"), s), reference.omp_add_scl(s.copy(order="F"), s))
 
 
+def _mapped_case(seed, tap):
+    """Particles over and around a field map of 40 nodes (x0 = -1.2, dx = 0.1),
+    none in [x0 + dx, x0 + 1.5 dx) where the reference reads the node before
+    the map; a seeded map; params of Undul_Mapped(_Tap)."""
+    r = rng(seed)
+    nx, x0, dx = 40, -1.2, 0.1
+    x = r.uniform(x0 - 0.3, x0 + (nx + 3) * dx, 3000)
+    x = x[(x < x0 + dx) | (x >= x0 + 1.5 * dx)]
+    coord = np.asfortranarray(np.vstack([x, r.normal(0, 0.3, x.size), r.normal(0, 0.3, x.size)]))
+    a0 = F(r, 2, nx)
+    params = np.array([0.9, x0, dx, 3.1, 0.25] if tap else [0.9, x0, dx])
+    return coord, a0, params, F(r, 6, x.size)
+
+
+@pytest.mark.parametrize("tap", [0, 1])
+def test_undul_mapped_matches_reference(port, reference, tap):
+    name = "undul_mapped_tap" if tap else "undul_mapped"
+    coord, a0, params, ft = _mapped_case(60 + tap, tap)
+    a = getattr(port, name)(coord, ft.copy(order="F"), 0.0, a0, params)
+    b = getattr(reference, name)(coord, ft.copy(order="F"), 0.0, a0, params)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, ft)
+
+
+def test_undul_mapped_closed_form(port):
+    """A constant map c: the quadratic weights sum to one, so inside the map
+    Fld(5) gains c cosh(ku y) and Fld(4) gains c sinh(ku y); outside, nothing.
+    Tapered, the gain is scaled by the linear taper."""
+    r = rng(62)
+    nx, x0, dx, lam = 50, 0.0, 0.2, 1.3
+    x = r.uniform(-1.0, nx * dx + 1.0, 2000)
+    y = r.normal(0, 0.4, x.size)
+    coord = np.asfortranarray(np.vstack([x, y, 0 * x]))
+    a0 = np.asfortranarray(np.vstack([np.full(nx, 2.5), np.full(nx, -0.7)]))
+    z = np.zeros((6, x.size), order="F")
+    inside = (x >= x0 + 1.5 * dx) & (x <= x0 + (nx - 1) * dx)
+    outside = (x < x0 + dx) | (x > x0 + (nx - 1) * dx)
+    ku = 2 * np.pi / lam
+    f = port.undul_mapped(coord, z.copy(order="F"), 0.0, a0, np.array([lam, x0, dx]))
+    np.testing.assert_allclose(f[4, inside], 2.5 * np.cosh(ku * y[inside]), rtol=1e-14)
+    np.testing.assert_allclose(f[3, inside], -0.7 * np.sinh(ku * y[inside]), rtol=1e-13, atol=1e-15)
+    assert not f[:, outside].any()
+    Lx, taper = 8.0, 0.3
+    amp = 1 + taper * (x - 0.5 * (nx * dx - Lx) - x0 - 0.5 * Lx) / (0.5 * Lx)
+    f = port.undul_mapped_tap(coord, z.copy(order="F"), 0.0, a0, np.array([lam, x0, dx, Lx, taper]))
+    np.testing.assert_allclose(f[4, inside], amp[inside] * 2.5 * np.cosh(ku * y[inside]), rtol=1e-13)
+
+
 def test_particle_tools_match_reference(port, reference):
     g = Grid(nx=30, nkr=10, nko=1)
     coord, mom, w = particles(g, 2500, seed=40)
