@@ -237,6 +237,44 @@ static int32_t undul_mapped_impl(const double* coord, double* Fld, const double*
     return st.finish();
 }
 
+// devices.f90:205-252: a plane wave of amplitude a0 and wavelength lambda at
+// angle theta to x, phase phi0, with linear ramps of length `ramp` at both ends of
+// [X0, X0 + Lx]; adds to Fld(3), Fld(4), Fld(5) (0-based 2, 3, 4).
+__global__ void k_planewave(const double* __restrict__ coord, double* __restrict__ F, double t, double a0,
+                            double X0, double Lx, double ramp, double sinth, double costh, double k0, double phi0,
+                            int np) {
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    const double x = coord[3 * (int64_t)ip], y = coord[3 * (int64_t)ip + 1];
+    double ampl;
+    if (x <= X0 || x >= X0 + Lx) ampl = 0.0;
+    else if (x > X0 && x < X0 + ramp) ampl = (x - X0) / ramp;
+    else if (x > X0 + Lx - ramp && x < X0 + Lx) ampl = (X0 + Lx - x) / ramp;
+    else ampl = 1.0;
+    ampl = a0 * ampl * sin(k0 * (x * costh + y * sinth - t) + phi0);
+    double* f = F + 6 * (int64_t)ip;
+    f[2] = f[2] + ampl;
+    f[3] = f[3] + ampl * sinth;
+    f[4] = f[4] - ampl * costh;
+}
+
+// devices.f90:254-297: a Gaussian pulse centred at (x0 + axis t, y0, z0) with
+// rms-like lengths (Lx, Ly, Lz) and carrier sin(2 pi xp / lambda); adds E to
+// Fld(3) and -axis E to Fld(5).
+__global__ void k_gaussbeam(const double* __restrict__ coord, double* __restrict__ F, double time, double a0,
+                            double lambda, double axis, double x0, double y0, double z0, double Lx2_inv,
+                            double Ly2_inv, double Lz2_inv, int np) {
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    const double* c = coord + 3 * (int64_t)ip;
+    const double xp = c[0] - x0 - axis * time, yp = c[1] - y0, zp = c[2] - z0;
+    const double e = a0 * exp(-(xp * xp * Lx2_inv) - yp * yp * Ly2_inv - zp * zp * Lz2_inv) *
+                     sin(2.0 * M_PI / lambda * xp);
+    double* f = F + 6 * (int64_t)ip;
+    f[2] = f[2] + e;
+    f[4] = f[4] - axis * e;
+}
+
 // ----------------------------------------------------------------- lattice
 
 // particle_tools.f90:84-128.  One thread per (ix, ir, ip) lattice slot.
@@ -515,6 +553,38 @@ int32_t chimera_undul_mapped_tap(const double* coord, double* Fld, double t, con
                                  int32_t np, int32_t nx) {
     (void)t;
     return undul_mapped_impl<1>(coord, Fld, a0, params, np, nx);
+}
+
+int32_t chimera_planewave(const double* coord, double* Fld, double t, const double* params, int32_t np) {
+    if (np < 0 || params == nullptr) return fail(ST_EINVAL, "planewave: bad arguments");
+    if (np == 0) return ST_OK;
+    double p[7];
+    if (is_device_ptr(params)) CH_HIP(hipMemcpy(p, params, sizeof p, hipMemcpyDeviceToHost));
+    else memcpy(p, params, sizeof p);
+    Stage st;
+    const double* x = st.in(coord, 3 * (size_t)np);
+    double* F = st.inout(Fld, 6 * (size_t)np);
+    CH_TRY(st.status());
+    hipLaunchKernelGGL(k_planewave, blocks_for(np, 256), 256, 0, stream(), x, F, t, p[0], p[2], p[3], p[4], sin(p[5]),
+                       cos(p[5]), 2.0 * M_PI / p[1], p[6], np);
+    CH_LAUNCH_CHECK("k_planewave");
+    return st.finish();
+}
+
+int32_t chimera_gaussbeam(const double* coord, double* Fld, double time, double a0, const double* params, int32_t np) {
+    if (np < 0 || params == nullptr) return fail(ST_EINVAL, "gaussbeam: bad arguments");
+    if (np == 0) return ST_OK;
+    double p[8];
+    if (is_device_ptr(params)) CH_HIP(hipMemcpy(p, params, sizeof p, hipMemcpyDeviceToHost));
+    else memcpy(p, params, sizeof p);
+    Stage st;
+    const double* x = st.in(coord, 3 * (size_t)np);
+    double* F = st.inout(Fld, 6 * (size_t)np);
+    CH_TRY(st.status());
+    hipLaunchKernelGGL(k_gaussbeam, blocks_for(np, 256), 256, 0, stream(), x, F, time, a0, p[0], p[1], p[2], p[3], p[4],
+                       1.0 / (p[5] * p[5]), 1.0 / (p[6] * p[6]), 1.0 / (p[7] * p[7]), np);
+    CH_LAUNCH_CHECK("k_gaussbeam");
+    return st.finish();
 }
 
 int32_t chimera_genparts(double* coord, int32_t* indPart, const double* Xgrid, const double* Rgrid,

@@ -25,7 +25,7 @@ __all__ = [
     "align_data_scl", "dep_curr", "dep_dens", "dep_curr_chnk", "dep_dens_chnk", "proj_fld", "eb_correction",
     "fb_vec_in", "fb_scl_in", "fb_eb_out", "fb_filtr", "fb_rot", "fb_grad", "fb_div", "fb_graddiv",
     "maxwell_push_with_spchrg", "maxwell_push_wo_spchrg", "maxwell_init_push", "poiss_corr", "omp_mult_vec",
-    "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic", "undul_analytic_taper", "undul_mapped", "undul_mapped_tap", "poiss_corr_stat", "field_drift",
+    "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic", "undul_analytic_taper", "undul_mapped", "undul_mapped_tap", "planewave", "gaussbeam", "poiss_corr_stat", "field_drift",
 ]
 
 
@@ -332,6 +332,24 @@ def undul_mapped_tap(coord, Fld, t, a0, params):
     if p.size < 5:
         raise ValueError("undul_mapped_tap: params = (lambda, Xleft, dx, Lx, taper)")
     _call("chimera_undul_mapped_tap", _p(x), _p(F), float(t), _p(a), _p(p), x.shape[1], a.shape[1])
+    return F
+
+
+def planewave(coord, Fld, t, params):
+    """devices.f90:205-252 -> Fld"""
+    x, F, p = _arr(coord, np.float64), _arr(Fld, np.float64), _arr(params, np.float64)
+    if p.size < 7:
+        raise ValueError("planewave: params = (a0, lambda, X0, Lx, ramp, theta, phi0)")
+    _call("chimera_planewave", _p(x), _p(F), float(t), _p(p), x.shape[1])
+    return F
+
+
+def gaussbeam(coord, Fld, time, a0, params):
+    """devices.f90:254-297 -> Fld"""
+    x, F, p = _arr(coord, np.float64), _arr(Fld, np.float64), _arr(params, np.float64)
+    if p.size < 8:
+        raise ValueError("gaussbeam: params = (lambda, axis, x0, y0, z0, Lx, Ly, Lz)")
+    _call("chimera_gaussbeam", _p(x), _p(F), float(time), float(a0), _p(p), x.shape[1])
     return F
 
 

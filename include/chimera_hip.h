@@ -258,6 +258,16 @@ int32_t chimera_undul_mapped(const double* coord, double* Fld, double t, const d
  * Replaces Undul_Mapped_Tap, devices.f90:64-115. */
 int32_t chimera_undul_mapped_tap(const double* coord, double* Fld, double t, const double* a0, const double* params,
                                  int32_t np, int32_t nx);
+/* Plane wave, params = (a0, lambda, X0, Lx, ramp, theta, phi0): with the ramped
+ * amplitude A(x), E = a0 A sin(k0 (x cos theta + y sin theta - t) + phi0);
+ * Fld(3) += E, Fld(4) += E sin theta, Fld(5) -= E cos theta.
+ * Replaces PlaneWave, devices.f90:205-252. */
+int32_t chimera_planewave(const double* coord, double* Fld, double t, const double* params, int32_t np);
+/* Gaussian beam, params = (lambda, axis, x0, y0, z0, Lx, Ly, Lz):
+ * E = a0 exp(-xp^2/Lx^2 - yp^2/Ly^2 - zp^2/Lz^2) sin(2 pi xp / lambda) with
+ * xp = x - x0 - axis time, yp = y - y0, zp = z - z0; Fld(3) += E, Fld(5) -= axis E.
+ * Replaces GaussBeam, devices.f90:254-297. */
+int32_t chimera_gaussbeam(const double* coord, double* Fld, double time, double a0, const double* params, int32_t np);
 
 /* ================================================================ resident tier
  *

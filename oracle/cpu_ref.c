@@ -944,6 +944,44 @@ void ref_undul_mapped_tap(const double *coord, double *Fld, double t, const doub
     undul_mapped(coord, Fld, a0, params, np, nx, 1);
 }
 
+/* Plane wave.  devices.f90:205-252. */
+void ref_planewave(const double *coord, double *Fld, double t, const double *params, int32_t np)
+{
+    const double pi = 4.0 * atan(1.0);
+    const double a0 = params[0], lambda = params[1], X0 = params[2], Lx = params[3], ramp = params[4];
+    const double theta = params[5], phi0 = params[6];
+    const double sinth = sin(theta), costh = cos(theta), k0 = 2.0 * pi / lambda;
+#pragma omp parallel for schedule(static)
+    for (int64_t ip = 0; ip < np; ip++) {
+        const double x = coord[3 * ip], y = coord[3 * ip + 1];
+        double ampl;
+        if (x <= X0 || x >= X0 + Lx) ampl = 0.0;
+        else if (x > X0 && x < X0 + ramp) ampl = (x - X0) / ramp;
+        else if (x > X0 + Lx - ramp && x < X0 + Lx) ampl = (X0 + Lx - x) / ramp;
+        else ampl = 1.0;
+        ampl = a0 * ampl * sin(k0 * (x * costh + y * sinth - t) + phi0);
+        Fld[6 * ip + 2] = Fld[6 * ip + 2] + ampl;
+        Fld[6 * ip + 3] = Fld[6 * ip + 3] + ampl * sinth;
+        Fld[6 * ip + 4] = Fld[6 * ip + 4] - ampl * costh;
+    }
+}
+
+/* Gaussian beam.  devices.f90:254-297. */
+void ref_gaussbeam(const double *coord, double *Fld, double time, double a0, const double *params, int32_t np)
+{
+    const double pi = 4.0 * atan(1.0);
+    const double lambda = params[0], axis = params[1], x0 = params[2], y0 = params[3], z0 = params[4];
+    const double Lx2_inv = 1.0 / (params[5] * params[5]), Ly2_inv = 1.0 / (params[6] * params[6]);
+    const double Lz2_inv = 1.0 / (params[7] * params[7]);
+#pragma omp parallel for schedule(static)
+    for (int64_t ip = 0; ip < np; ip++) {
+        const double xp = coord[3 * ip] - x0 - axis * time, yp = coord[3 * ip + 1] - y0, zp = coord[3 * ip + 2] - z0;
+        const double e = a0 * exp(-(xp * xp * Lx2_inv) - yp * yp * Ly2_inv - zp * zp * Lz2_inv) * sin(2.0 * pi / lambda * xp);
+        Fld[6 * ip + 2] = Fld[6 * ip + 2] + e;
+        Fld[6 * ip + 4] = Fld[6 * ip + 4] - axis * e;
+    }
+}
+
 void ref_undul_analytic(const double *coord, double *Fld, double t, const double *params, int32_t np)
 {
     (void)t;

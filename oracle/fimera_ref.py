@@ -329,6 +329,16 @@ class Fimera:
         self._call("undul_mapped_tap", coord, Fld, float(t), a0, params, coord.shape[1], a0.shape[1])
         return Fld
 
+    def planewave(self, coord, Fld, t, params):
+        coord, Fld, params = _f(coord, np.float64), _f(Fld, np.float64), _f(params, np.float64)
+        self._call("planewave", coord, Fld, float(t), params, coord.shape[1])
+        return Fld
+
+    def gaussbeam(self, coord, Fld, time, a0, params):
+        coord, Fld, params = _f(coord, np.float64), _f(Fld, np.float64), _f(params, np.float64)
+        self._call("gaussbeam", coord, Fld, float(time), float(a0), params, coord.shape[1])
+        return Fld
+
     def undul_analytic_taper(self, coord, Fld, t, params):
         coord, Fld, params = _f(coord, np.float64), _f(Fld, np.float64), _f(params, np.float64)
         self._call("undul_analytic_taper", coord, Fld, float(t), params, coord.shape[1])

@@ -138,6 +138,12 @@ def test_undulator(gpu, port):
         a = getattr(gpu, name)(c, f0.copy(order="F"), 0.0, a0, np.array(params))
         b = getattr(port, name)(c, f0.copy(order="F"), 0.0, a0, np.array(params))
         assert rel_max(a, b) < 1e-13
+    say("case: planewave, gaussbeam")
+    pw = np.array([0.8, 1.3, 0.5, 7.0, 1.1, 0.35, 0.2])
+    gb = np.array([1.3, 0.9, 3.0, 0.2, -0.1, 1.5, 0.8, 1.1])
+    assert rel_max(gpu.planewave(coord, ft.copy(order="F"), 0.7, pw), port.planewave(coord, ft.copy(order="F"), 0.7, pw)) < 1e-13
+    assert rel_max(gpu.gaussbeam(coord, ft.copy(order="F"), 0.4, 1.7, gb),
+                   port.gaussbeam(coord, ft.copy(order="F"), 0.4, 1.7, gb)) < 1e-13
 
 
 def test_genparts(gpu, port):

@@ -118,6 +118,36 @@ def test_undul_mapped_matches_reference(port, reference, tap):
     assert not np.array_equal(a, ft)
 
 
+def test_wave_devices_match_reference(port, reference):
+    """PlaneWave and GaussBeam (devices.f90:205-297), and the closed forms they
+    reduce to: on the flat top of the plane wave Fld(3) gains a0 sin(phase);
+    at the centre of the Gaussian beam the field is a0 sin(0) = 0 and one
+    quarter wavelength ahead it is a0 exp(-(lambda/4)^2/Lx^2)."""
+    r = rng(63)
+    n = 3000
+    coord = np.asfortranarray(np.vstack([r.uniform(-1, 9, n), r.normal(0, 1, n), r.normal(0, 1, n)]))
+    ft = F(r, 6, n)
+    pw = np.array([0.8, 1.3, 0.5, 7.0, 1.1, 0.35, 0.2])
+    a = port.planewave(coord, ft.copy(order="F"), 0.7, pw)
+    assert np.array_equal(a, reference.planewave(coord, ft.copy(order="F"), 0.7, pw))
+    gb = np.array([1.3, 0.9, 3.0, 0.2, -0.1, 1.5, 0.8, 1.1])
+    b = port.gaussbeam(coord, ft.copy(order="F"), 0.4, 1.7, gb)
+    assert np.array_equal(b, reference.gaussbeam(coord, ft.copy(order="F"), 0.4, 1.7, gb))
+    z = np.zeros((6, n), order="F")
+    x, y = coord[0], coord[1]
+    flat = (x >= 0.5 + 1.1) & (x <= 0.5 + 7.0 - 1.1)
+    ph = 2 * np.pi / 1.3 * (x * np.cos(0.35) + y * np.sin(0.35) - 0.7) + 0.2
+    f = port.planewave(coord, z.copy(order="F"), 0.7, pw)
+    np.testing.assert_allclose(f[2, flat], 0.8 * np.sin(ph[flat]), atol=1e-13)
+    np.testing.assert_allclose(f[4, flat], -np.cos(0.35) * f[2, flat], rtol=1e-15)
+    assert not f[:, (x <= 0.5) | (x >= 7.5)].any()
+    c = np.asfortranarray(np.array([[3.0 + 0.9 * 0.4, 3.0 + 0.9 * 0.4 + 1.3 / 4], [0.2, 0.2], [-0.1, -0.1]]))
+    g = port.gaussbeam(c, np.zeros((6, 2), order="F"), 0.4, 1.7, gb)
+    assert abs(g[2, 0]) < 1e-14          # xp is zero up to the rounding of the centre
+    np.testing.assert_allclose(g[2, 1], 1.7 * np.exp(-(1.3 / 4) ** 2 / 1.5 ** 2), rtol=1e-14)
+    np.testing.assert_allclose(g[4], -0.9 * g[2], rtol=1e-15)
+
+
 def test_undul_mapped_closed_form(port):
     """A constant map c: the quadratic weights sum to one, so inside the map
     Fld(5) gains c cosh(ku y) and Fld(4) gains c sinh(ku y); outside, nothing.
