@@ -460,6 +460,16 @@ class ChimeraRun:
                     out[f"specie{i}/Args/{key}"] = a
         out["NumSpecies"] = np.array(len(self.Particles))
         out["NumThreads"] = np.array(int(os.environ.get("OMP_NUM_THREADS", "1")))
+        for i, sp in enumerate(self.Particles):
+            # a species' own random stream (own_rng; the plasma a moving window
+            # injects is drawn from it): its state, so that a resumed run draws
+            # what the uninterrupted run would.  The global NumPy stream, which
+            # the reference draws from, is not the run's to save.
+            if isinstance(sp.rng, np.random.RandomState):
+                name, keys, pos, has_gauss, gauss = sp.rng.get_state()
+                out[f"specie{i}/rng/keys"] = np.asarray(keys, dtype=np.uint32)
+                out[f"specie{i}/rng/state"] = np.array([pos, has_gauss], dtype=np.int64)
+                out[f"specie{i}/rng/gauss"] = np.array(gauss, dtype=np.float64)
         return out
 
     def read_snap(self, fname="./test.npz", verbose=False):
@@ -506,3 +516,7 @@ class ChimeraRun:
                 if self._snap_value(v) is not None and k in f:
                     a = f[k]
                     sp.Args[key] = a[()] if a.ndim == 0 else a
+            if f"specie{i}/rng/keys" in f and isinstance(sp.rng, np.random.RandomState):
+                pos, has_gauss = (int(v) for v in f[f"specie{i}/rng/state"])
+                sp.rng.set_state(("MT19937", np.asarray(f[f"specie{i}/rng/keys"], dtype=np.uint32), pos, has_gauss,
+                                  float(f[f"specie{i}/rng/gauss"])))

@@ -9,8 +9,9 @@ Fortran-contiguous with the right dtype (else a converted copy is updated) and
 returned, several outputs returned as a tuple in declaration order.
 
 The arrays are NumPy host arrays; each call stages them through device memory
-and returns when the results are back.  Resident simulations (chimera_amd.sim)
-keep their state on the device and call the same C ABI with device pointers.
+and returns when the results are back.  Resident simulations (chimera_amd's
+Solver / Specie / ChimeraRun) keep their state on the device and call the same
+C ABI with device pointers.
 """
 from __future__ import annotations
 
@@ -319,9 +320,17 @@ def undul_analytic(coord, Fld, t, params):
     return F
 
 
+def _check_map(name, a0, p, nparams, names):
+    if p.size < nparams:
+        raise ValueError(f"{name}: params = ({names})")
+    if a0.ndim != 2 or a0.shape[0] != 2:
+        raise ValueError(f"{name}: a0 must have shape (2, nx), got {a0.shape}")
+
+
 def undul_mapped(coord, Fld, t, a0, params):
     """devices.f90:18-62 -> Fld"""
     x, F, a, p = (_arr(v, np.float64) for v in (coord, Fld, a0, params))
+    _check_map("undul_mapped", a, p, 3, "lambda, Xleft, dx")
     _call("chimera_undul_mapped", _p(x), _p(F), float(t), _p(a), _p(p), x.shape[1], a.shape[1])
     return F
 
@@ -329,8 +338,7 @@ def undul_mapped(coord, Fld, t, a0, params):
 def undul_mapped_tap(coord, Fld, t, a0, params):
     """devices.f90:64-115 -> Fld"""
     x, F, a, p = (_arr(v, np.float64) for v in (coord, Fld, a0, params))
-    if p.size < 5:
-        raise ValueError("undul_mapped_tap: params = (lambda, Xleft, dx, Lx, taper)")
+    _check_map("undul_mapped_tap", a, p, 5, "lambda, Xleft, dx, Lx, taper")
     _call("chimera_undul_mapped_tap", _p(x), _p(F), float(t), _p(a), _p(p), x.shape[1], a.shape[1])
     return F
 

@@ -18,8 +18,14 @@ steps recorded into graphs and replayed) and schedules them:
   on the device into Request.result; then its arrays and context are freed.
   With evict=True a request that does not fit may evict the active request
   with the most steps left: its state goes to host memory (the arrays of
-  ChimeraRun.snapshot), its device memory is freed, and it queues again to
-  resume from there.
+  ChimeraRun.snapshot, with the request's own random stream), its device
+  memory is freed, and it queues again to resume from there.
+
+A request ends in one of three states (Request.status): "done" with its
+result; "failed" (its build raised, e.g. an allocation failure, or its result
+holds non-finite values, e.g. particles that went NaN), with the reason in
+Request.error; "cancelled" (Server.cancel).  A failed or cancelled request
+frees its slot and memory and leaves the others running.
 """
 from __future__ import annotations
 
@@ -72,6 +78,8 @@ class Request:
         self.particle_steps = 0
         self.result = None             # beam envelopes of the finished run (diagnostics.get_beam_envelops)
         self.snap, self.istep0, self.evictions = None, 0, 0     # state of an evicted run
+        self.status, self.error = "pending", None
+        self.cancel_requested = False
 
     def build(self, graphs=True):
         import chimera_amd as G
@@ -151,8 +159,11 @@ class Server:
     chunk once the previous one has finished)."""
 
     def __init__(self, slots=64, reserve=16 * GIB, chunk=4, graphs=True, builder_thread=True, log=None,
-                 on_finish=None, builders=4, evict=False, hbm=None, max_evictions=2, evict_after=16):
+                 on_finish=None, builders=4, evict=False, hbm=None, max_evictions=2, evict_after=16,
+                 free_bytes=None):
         self.slots, self.reserve, self.chunk, self.graphs = slots, reserve, chunk, graphs
+        # free_bytes: () -> free device bytes (default: the device's, chimera_mem_info)
+        self.free_bytes = free_bytes if free_bytes is not None else (lambda: free_device_bytes()[0])
         # evict: a request that does not fit may evict the active request with the most
         # steps left once it has run evict_after steps since its admission (its state to
         # host memory, resumed later from there; at most max_evictions times); hbm: device
@@ -164,9 +175,11 @@ class Server:
         self.building = 0
         self.reserved = 0          # estimated bytes of admitted requests not yet allocated
         self.active = []
-        self.finished = []
+        self.finished = []         # done, in order of completion
+        self.failed = []           # failed or cancelled
+        self._in_build = set()     # admitted, being built
         self.built = queue.Queue()
-        self.errors = []
+        self.errors = []           # (name, reason) of every failed request
         self.builder_thread = builder_thread
         if builder_thread:
             # several builders: plasma generation and table construction are
@@ -181,20 +194,60 @@ class Server:
             r = self.jobs.get()
             if r is None:
                 return
-            try:
-                r.sim = r.build(graphs=self.graphs)
-                r.t_built = time.perf_counter()
-            except Exception as e:          # reported by the main loop
-                self.errors.append((r.name, repr(e)))
-                r.sim = None
+            self._build(r)
             self.built.put(r)
+
+    def _build(self, r):
+        try:
+            r.sim = r.build(graphs=self.graphs)
+            r.t_built = time.perf_counter()
+        except Exception as e:          # the request fails; the loop goes on
+            r.sim = None
+            r.error = f"build failed: {e!r}"
 
     def submit(self, r, now):
         r.t_arrival_host = now
         self.pending.append(r)
 
+    def _fail(self, r, reason, status="failed"):
+        r.status, r.error = status, reason
+        r.t_done = time.perf_counter()
+        self.failed.append(r)
+        self.errors.append((r.name, reason))
+        if self.log:
+            self.log(f"  {r.name} {status}: {reason}")
+
+    def cancel(self, name):
+        """Cancel a request by name: a pending one leaves the queue, an active
+        one is stopped and freed (its work in flight finishes first), one being
+        built is dropped once built.  Returns whether a request was found."""
+        for r in list(self.pending):
+            if r.name == name:
+                self.pending.remove(r)
+                self._fail(r, "cancelled", "cancelled")
+                return True
+        for r in list(self.active):
+            if r.name == name:
+                self.active.remove(r)
+                if r.sim is not None:
+                    r.sim.close()
+                    r.sim = None
+                self._fail(r, "cancelled", "cancelled")
+                return True
+        r = next((r for r in self._in_build if r.name == name), None)
+        if r is not None:
+            r.cancel_requested = True
+            return True
+        return False
+
+    def result_of(self, sim):
+        """The result of a finished simulation: the beam envelopes of its pushed
+        species, reduced on the device."""
+        with sim.ctx:
+            return Diagnostics(sim.run).get_beam_envelops()
+
     def _free(self):
-        free, _ = free_device_bytes()
+        free = self.free_bytes()
         if self.hbm is not None:
             free = min(free, self.hbm - sum(a.bytes for a in self.active))
         return free
@@ -235,13 +288,14 @@ class Server:
                 break
             self.pending.pop(0)
             r.t_admit = now
+            r.status = "building"
             self.reserved += r.bytes
             self.building += 1
+            self._in_build.add(r)
             if self.builder_thread:
                 self.jobs.put(r)
             else:
-                r.sim = r.build(graphs=self.graphs)
-                r.t_built = time.perf_counter()
+                self._build(r)
                 self.built.put(r)
 
     def poll(self, now):
@@ -249,10 +303,18 @@ class Server:
         while not self.built.empty():
             r = self.built.get()
             self.building -= 1
+            self._in_build.discard(r)
             self.reserved -= r.bytes        # now allocated: free_device_bytes() accounts for it
             if r.sim is None:
+                self._fail(r, r.error or "build failed")
+                continue
+            if r.cancel_requested:
+                r.sim.close()
+                r.sim = None
+                self._fail(r, "cancelled", "cancelled")
                 continue
             r.particle_steps = r.sim.particles * r.steps
+            r.status = "running"
             self.active.append(r)
         still = []
         for r in self.active:
@@ -260,13 +322,17 @@ class Server:
             if s.finished_issuing:
                 if s.done():
                     r.t_done = s.t_done
-                    with s.ctx:
-                        r.result = Diagnostics(s.run).get_beam_envelops()
+                    r.result = self.result_of(s)
                     if self.on_finish is not None:
                         self.on_finish(r)
                     s.close()
                     r.sim = None
-                    self.finished.append(r)
+                    bad = [k for k, v in enumerate(r.result or ()) if not np.all(np.isfinite(np.asarray(v)))]
+                    if bad:
+                        self._fail(r, f"non-finite result (species {bad})")
+                    else:
+                        r.status = "done"
+                        self.finished.append(r)
                     continue
             elif s.istep == 0 or s.ctx.done():
                 s.advance(self.chunk)
@@ -334,8 +400,6 @@ def serve(requests, server, log=None, t0=None):
             last = now
             log(f"  t={now - t0:6.1f}s arrived {i} pending {len(server.pending)} building {server.building} "
                 f"active {len(server.active)} finished {len(server.finished)}")
-        if server.errors:
-            raise RuntimeError(f"building {server.errors[0][0]} failed: {server.errors[0][1]}")
         if not server.active and not server.built.qsize() and i < len(requests) and not server.pending:
             time.sleep(min(0.002, max(0.0, requests[i].t_arrival - (time.perf_counter() - t0))))
     return time.perf_counter() - t0

@@ -299,6 +299,25 @@ class Specie:
         momenta = np.vstack([mm[c] + ms[c] * rm[c] for c in range(3)]).astype("d", order="F")
         return [np.asfortranarray(coords[0:3]), momenta, coords[-1].astype("d", order="F")]
 
+    def beam_focus(self, x_foc):                                          # species.py:473-486
+        D = self.Data
+        gg = (1 + D["momenta"] ** 2).sum(0) ** 0.5
+        pzmean = (D["momenta"][0] * D["weights"]).sum() / D["weights"].sum()
+        D["coords"][0] = D["coords"][0] - x_foc * (D["momenta"][0] - pzmean) / pzmean / gg ** 2
+        D["coords"][1:3] = D["coords"][1:3] - D["momenta"][1:3] / pzmean * x_foc
+        D["coords_halfstep"][:] = D["coords"]
+
+    def denoise(self, WaveNums2Kill):                                     # species.py:322-349
+        D = self.Data
+        for k_supp in WaveNums2Kill:
+            for key in ("coords", "coords_halfstep"):
+                mirror = D[key].copy(order="F")
+                mirror[0] = mirror[0] + 0.5 / k_supp
+                D[key] = np.asfortranarray(np.concatenate((D[key], mirror), axis=1))
+            D["momenta"] = np.asfortranarray(np.concatenate((D["momenta"], D["momenta"]), axis=1))
+            D["weights"] = np.concatenate((D["weights"], D["weights"]), axis=0)
+            D["weights"] *= 0.5
+
     def add_particles(self, coords, momenta, weights):                    # species.py:217-244
         D = self.Data
         D["coords"] = np.asfortranarray(np.concatenate((D["coords"], coords), axis=1))

@@ -8,9 +8,10 @@
   linear algebra, and against closed forms (a single Bessel mode maps to one FB
   coefficient; a pure phase shift; the absorbing profile on a constant line).
 * Invariants that a dropped term, wrong sign or transposed index would break:
-  deposition conserves charge, gather is the adjoint of deposition, the FB
-  operators satisfy div(rot) = 0 and rot(grad) = 0 with the reference's own
-  operator construction, Boris conserves |u| without E and rotates by 2 atan|t|.
+  deposition conserves charge, gather is the adjoint of deposition, Boris
+  conserves |u| without E and rotates by 2 atan|t|.  The operator tables the
+  FB operators take (D+, D-) are pinned by the Bessel recurrences in
+  tests/test_oracle_driver.py.
 """
 import numpy as np
 import pytest
