@@ -78,22 +78,6 @@ def algorithmic(fam, P, Gr, Gk, Nx, Nkr, M, K, prec="fp64"):
     return None
 
 
-def mfma_rates(res, K):
-    """Achieved FLOP rate of the Hankel transforms (10 M transforms of 4 Nx Nkr^2
-    flops per step) and of the FB-space operators (4 (18 M - 3) Nx Nkr^2) against
-    the dense MFMA peak of the precision they run in."""
-    Nx, Nkr, M = res["Nx"], res["Nkr"], res["M"]
-    peak = FP64_PEAK_TFS if res["precision"] == "fp64" else FP32_PEAK_TFS
-    out = {}
-    for fam, flops in (("dht", 40.0 * M * Nx * Nkr ** 2), ("fb_ops", 4.0 * (18 * M - 3) * Nx * Nkr ** 2)):
-        ms = res["fams"][fam][0] / K
-        if ms > 0:
-            tf = flops / (ms * 1e-3) / 1e12
-            out[fam] = {"tflops": round(tf, 2), "peak": peak, "frac": round(tf / peak, 4),
-                        "flop_per_step": flops}
-    return out
-
-
 def run_gpu(args, precision):
     import chimera_amd as G
     from chimera_amd import workloads as W
@@ -109,16 +93,19 @@ def run_gpu(args, precision):
     A = s.Args
     Nx, Nkr, M = A["Nx"], A["Nkr"], A["Mtot"]
     log(f"built in {time.time() - t0:.1f}s: Nx={Nx} Nkr={Nkr} modes={M} electrons={e.np} ions={run.Particles[1].np}")
-    for i in range(1, args.warmup + 1):
+    advance = advance_steps(args, spec)
+    t0 = time.time()
+    for i in range(1, advance + args.warmup + 1):
         run.make_step(i)
     L.chimera_synchronize()
-    log(f"warmup {args.warmup} steps done")
+    log(f"advanced {advance} + warmup {args.warmup} steps in {time.time() - t0:.1f}s")
     L.chimera_prof_reset()
     L.chimera_prof_enable(1)
     psteps = 0
+    first = advance + args.warmup + 1
     L.chimera_synchronize()
     t0 = time.perf_counter()
-    for i in range(args.warmup + 1, args.warmup + args.steps + 1):
+    for i in range(first, first + args.steps):
         run.make_step(i)
         psteps += e.np
     L.chimera_synchronize()
@@ -133,35 +120,104 @@ def run_gpu(args, precision):
         if n:
             log(f"  {name:12s} {ms / args.steps:9.3f} ms/step  ({n} launches)")
     return dict(run=run, wall=wall, psteps=psteps, fams=fams, Nx=Nx, Nkr=Nkr, M=M, P=e.np, workload=args.workload,
-                Gr=Nx * A["Nr"] * M, Gk=Nx * Nkr * M, precision=precision)
+                Gr=Nx * A["Nr"] * M, Gk=Nx * Nkr * M, precision=precision, first=first, last=first + args.steps - 1,
+                run_steps=spec["steps"])
+
+
+def advance_steps(args, spec):
+    """Untimed steps before the warmup: by default the timed steps are the last
+    ones of the workload's run (its own step count), where the window has moved
+    many times and the wake has scrambled the particles."""
+    if args.advance >= 0:
+        return args.advance
+    return max(0, spec["steps"] - args.warmup - args.steps)
+
+
+def family_roofline(res, K, name):
+    """Roofline entry of one kernel family over the timed steps: algorithmic
+    bytes (HBM-bound families) or flops (the MFMA GEMMs) per launch, divided
+    by the family's average launch duration from the library's HIP events."""
+    ms, n = res["fams"][name]
+    if not n:
+        return None
+    per_launch = avg_s = None
+    if name in ("dht", "fb_ops"):
+        Nx, Nkr, M = res["Nx"], res["Nkr"], res["M"]
+        flops = 40.0 * M * Nx * Nkr ** 2 if name == "dht" else 4.0 * (18 * M - 3) * Nx * Nkr ** 2
+        peak = FP64_PEAK_TFS if res["precision"] == "fp64" else FP32_PEAK_TFS
+        per_launch = flops / (n / K)
+        avg_s = ms / n / 1e3
+        achieved = per_launch / avg_s / 1e12
+        return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4),
+                "algorithmic_flops_per_launch": int(per_launch), "launches_per_step": n / K,
+                "share_of_step": round(ms / sum(v[0] for v in res["fams"].values()), 3)}
+    alg = algorithmic(name, res["P"], res["Gr"], res["Gk"], res["Nx"], res["Nkr"], res["M"], K, res["precision"])
+    if alg is None:
+        return None
+    amount, unit, bound = alg
+    avg_s = ms / n / 1e3
+    per_launch = amount / (n / K)
+    achieved = per_launch / avg_s / 1e9
+    return {"kernel": name, "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_for(name, traffic_key(res["workload"], res["precision"])),
+            "avg_launch_ms": round(avg_s * 1e3, 4), "algorithmic_bytes_per_launch": int(per_launch),
+            "launches_per_step": n / K,
+            "share_of_step": round(ms / sum(v[0] for v in res["fams"].values()), 3)}
 
 
 def roofline(res, K):
-    ranked = sorted(((ms, name) for name, (ms, n) in res["fams"].items() if n), reverse=True)
-    for ms, name in ranked:
-        alg = algorithmic(name, res["P"], res["Gr"], res["Gk"], res["Nx"], res["Nkr"], res["M"], K, res["precision"])
-        if alg is None:
-            continue
-        amount, unit, bound = alg
-        n = res["fams"][name][1]
-        avg_s = ms / n / 1e3
-        launches_per_step = n / K
-        per_launch = amount / launches_per_step if bound == "hbm" and name == "psatd" else amount
-        achieved = per_launch / avg_s / 1e9
-        return {"kernel": name, "bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_for(name, traffic_key(res["workload"], res["precision"])),
-                "avg_launch_ms": round(avg_s * 1e3, 4), "algorithmic_bytes_per_launch": int(per_launch),
-                "share_of_step": round(ms / sum(v[0] for v in res["fams"].values()), 3)}
-    return None
+    """The dominant kernel family's roofline (the one that takes the most time
+    among those with an algorithmic figure), and every family's."""
+    every = {}
+    for name in res["fams"]:
+        r = family_roofline(res, K, name)
+        if r is not None:
+            every[name] = r
+    if not every:
+        return None, every
+    dom = max(every, key=lambda k: res["fams"][k][0])
+    return every[dom], every
 
 
-def cpu_baseline(args, P_gpu):
+def host_info():
+    """CPU model, logical CPUs and physical cores of this host (lscpu)."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = dict(line.split(":", 1) for line in txt.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        cores, sockets = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cores and sockets:
+            info["physical_cores"] = int(cores) * int(sockets)
+        info["threads_per_core"] = kv.get("Thread(s) per core")
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(args):
+    """The oracle (the reference's algorithm restated in C + OpenMP, FP64)
+    timed on this host's cores at the headline workload, with the x-chunked
+    deposition of the reference's OpenMP path (Xchunked = (T, 10), T = the
+    thread count: grid_deps_chnk.f90 deposits one chunk per thread).  Bounded
+    sample: the build and initial half step are not timed, then args.cpu_steps
+    steps are."""
     from chimera_amd import workloads as W
     from oracle import driver as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    spec = W.WORKLOADS[args.workload]()
-    log(f"CPU baseline: building the oracle at {args.workload} ({threads} OpenMP threads)")
+    spec = W.WORKLOADS[args.workload](xchunked=(threads, 10))
+    if args.cpu_slab > 1:
+        spec = W.x_slab(spec, args.cpu_slab)
+    log(f"CPU baseline: building the oracle at {args.workload} (1/{args.cpu_slab} of its x extent) with "
+        f"Xchunked=({threads},10), {threads} OpenMP threads")
     t0 = time.time()
     run = W.build(O, spec, seed=args.seed)
     log(f"CPU baseline: built + half step in {time.time() - t0:.1f}s")
@@ -171,11 +227,19 @@ def cpu_baseline(args, P_gpu):
         run.make_step(i)
     dt = time.perf_counter() - t0
     log(f"CPU baseline: {args.cpu_steps} step(s) in {dt:.1f}s")
+    host = host_info()
     return {"value": round(P * args.cpu_steps / dt, 1), "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"{args.cpu_steps} make_step of {args.workload} ({P} electrons) after the initial half step; "
-                      f"oracle/cpu_ref.c (FP64, naive DHT/operator loops, serial deposition as in grid_deps.f90) "
-                      f"driven by oracle/driver.py, OMP_NUM_THREADS={threads}"}
+            "sample": f"{args.cpu_steps} make_step of {args.workload} on 1/{args.cpu_slab} of its x extent (the "
+                      f"same radial grid, modes, plasma and laser: {P} electrons, Nx={run.Solvers[0].Args['Nx']} x "
+                      f"Nkr={run.Solvers[0].Args['Nkr']}; per-particle work as in the whole box) after the untimed "
+                      f"build and half step, {dt:.2f} s",
+            "host": host, "omp_num_threads": threads,
+            "code": "oracle/cpu_ref.c driven by oracle/driver.py: FP64; naive O(Nx Nkr^2) Hankel and operator "
+                    "loops as fb_io.f90/fb_math.f90; deposition per x-chunk, one OpenMP thread per chunk, as "
+                    "grid_deps_chnk.f90 (Xchunked=(T,10), particles binned every 11 steps); own FFT (radix-2 for "
+                    "powers of two, long-double direct sum otherwise) in place of FFTW",
+            "compiler": "gcc -O2 -fopenmp -ffp-contract=off (oracle/Makefile)"}
 
 
 def scan_bench(args):
@@ -335,11 +399,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="lpa_cfg1")
+    ap.add_argument("--workload", default="lpa_cfg2")
+    ap.add_argument("--secondary", default="lpa_cfg1", help="comma list of further workloads, each an object "
+                                                           "of the line ('' for none)")
+    ap.add_argument("--advance", type=int, default=-1, help="untimed steps before the warmup (default: the "
+                                                            "timed steps are the last of the workload's run)")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-slab", type=int, default=8, help="the CPU baseline runs 1/N of the workload's x extent")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB gibibytes device to device once (reference kernel for the HBM counters)")
     ap.add_argument("--no-scan", action="store_true", help="skip the batched bunch scan (scan object)")
@@ -377,18 +446,7 @@ def main():
         return
     K = args.steps
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
-    summaries = {}
-    for prec in precs:
-        res = run_gpu(args, prec)
-        value = res["psteps"] / res["wall"]
-        summaries[prec] = {
-            "value": round(value, 1), "ms_per_step": round(res["wall"] / K * 1e3, 3),
-            "fb_transform_ms_per_step": round((res["fams"]["dht"][0] + res["fams"]["fft"][0]) / K, 3),
-            "spectral_operator_ms_per_step": round(res["fams"]["fb_ops"][0] / K, 3),
-            "kernel_ms_per_step": {k: round(v[0] / K, 3) for k, v in res["fams"].items() if v[1]},
-            "roofline": roofline(res, K), "mfma": mfma_rates(res, K), "P": res["P"],
-            "Nx": res["Nx"], "Nkr": res["Nkr"], "M": res["M"]}
-        del res["run"]
+    summaries = bench_workload(args, args.workload, precs)
     main_prec = precs[0]
     m = summaries[main_prec]
     out = {
@@ -397,28 +455,62 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if main_prec == "fp64" else "f32+c64 (f64 positions)",
         "data": "synthetic: seeded FixedCell 2x2x4 plasma (n=0.005 n_c) + Gaussian laser, random draws from NumPy",
         "config": {"workload": args.workload, "Nx": m["Nx"], "Nkr": m["Nkr"], "modes": m["M"],
-                   "electrons": m["P"], "precision": main_prec, "moving_window": True},
+                   "electrons": m["P"], "precision": main_prec, "moving_window": True,
+                   "timed_steps": m["timed_steps"]},
         "fb_transform_ms_per_step": m["fb_transform_ms_per_step"],
         "spectral_operator_ms_per_step": m["spectral_operator_ms_per_step"],
         "kernel_ms_per_step": m["kernel_ms_per_step"],
         "roofline": m["roofline"],
-        "mfma": m["mfma"],
+        "families": m["families"],
     }
     if len(precs) == 2:
-        mx = summaries["mixed"]
-        out["mixed"] = {k: mx[k] for k in ("value", "ms_per_step", "fb_transform_ms_per_step",
-                                           "spectral_operator_ms_per_step", "kernel_ms_per_step", "roofline",
-                                           "mfma")}
-        out["mixed"]["dtype"] = "f32 momenta/weights, c64 fields, FP32 MFMA; f64 positions and tables"
-        out["mixed"]["speedup_vs_fp64"] = round(mx["value"] / m["value"], 3)
+        out["mixed"] = mixed_object(summaries)
+    if args.secondary:
+        for wl in args.secondary.split(","):
+            sec = bench_workload(args, wl, precs)
+            o = {k: sec[main_prec][k] for k in ("value", "ms_per_step", "fb_transform_ms_per_step",
+                                                "spectral_operator_ms_per_step", "kernel_ms_per_step", "roofline",
+                                                "timed_steps", "Nx", "Nkr", "M", "P")}
+            if len(precs) == 2:
+                o["mixed"] = mixed_object(sec)
+            out[wl] = o
     if not args.no_scan:
         out["scan"] = scan_bench(args)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args, m["P"])
+        out["cpu_baseline"] = cpu_baseline(args)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
+
+def bench_workload(args, workload, precs):
+    K = args.steps
+    summaries = {}
+    for prec in precs:
+        wargs = argparse.Namespace(**vars(args))
+        wargs.workload = workload
+        res = run_gpu(wargs, prec)
+        value = res["psteps"] / res["wall"]
+        head, every = roofline(res, K)
+        summaries[prec] = {
+            "value": round(value, 1), "ms_per_step": round(res["wall"] / K * 1e3, 3),
+            "fb_transform_ms_per_step": round((res["fams"]["dht"][0] + res["fams"]["fft"][0]) / K, 3),
+            "spectral_operator_ms_per_step": round(res["fams"]["fb_ops"][0] / K, 3),
+            "kernel_ms_per_step": {k: round(v[0] / K, 3) for k, v in res["fams"].items() if v[1]},
+            "roofline": head, "families": every, "P": res["P"],
+            "timed_steps": f"{res['first']}-{res['last']} of {res['run_steps']}",
+            "Nx": res["Nx"], "Nkr": res["Nkr"], "M": res["M"]}
+        del res["run"]
+    return summaries
+
+
+def mixed_object(summaries):
+    mx = summaries["mixed"]
+    o = {k: mx[k] for k in ("value", "ms_per_step", "fb_transform_ms_per_step", "spectral_operator_ms_per_step",
+                            "kernel_ms_per_step", "roofline", "families", "timed_steps")}
+    o["dtype"] = "f32 momenta/weights, c64 fields, FP32 MFMA; f64 positions and tables"
+    o["speedup_vs_fp64"] = round(mx["value"] / summaries["fp64"]["value"], 3)
+    return o
 
 if __name__ == "__main__":
     main()

@@ -13,6 +13,10 @@
 // workgroup that adds the partials.  Both reductions run in a fixed order for
 // a given particle count, so the sums are the same bits on every call.
 #include <algorithm>
+#include <cmath>
+#include <complex>
+#include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "../../include/chimera_hip.h"
@@ -238,6 +242,178 @@ static int32_t line_power_impl(const C* vec, const double* rgrid, int32_t nkx, i
     return st.finish();
 }
 
+// ---------------------------------------------------------------- phase-space dump (phs_out)
+//
+// The particles phs_out keeps (diagnostics.py:94-96): sum over the three
+// components of (1 + u_c^2) > 20^2, in the reference's order of operations
+// (each square rounded before the sum; the sum row by row).  Two launches: a
+// count per chunk of SEL_CH particles, then, from the chunk offsets the host
+// forms, an ordered scatter of the kept particles' half-step positions,
+// momenta and weight into PartsPack(7, n) (Fortran order), so the particles
+// keep their order.
+constexpr int SEL_T = 256, SEL_PER = 4, SEL_CH = SEL_T * SEL_PER;
+
+template <typename U>
+__device__ __forceinline__ bool phs_keep(const U* __restrict__ u, int64_t ld, int64_t i) {
+    const double a = (double)u[i], b = (double)u[ld + i], c = (double)u[2 * ld + i];
+    return ((1.0 + __dmul_rn(a, a)) + (1.0 + __dmul_rn(b, b))) + (1.0 + __dmul_rn(c, c)) > 400.0;
+}
+
+// exclusive scan of one int per thread over the workgroup; total in *tot
+__device__ __forceinline__ int block_excl_scan(int v, int* sh, int* tot) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < SEL_T; s <<= 1) {
+        const int t = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const int incl = sh[threadIdx.x];
+    *tot = sh[SEL_T - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+template <typename U>
+__global__ void __launch_bounds__(SEL_T) k_phs_count(const U* __restrict__ u, int64_t ld, int np,
+                                                     int* __restrict__ counts) {
+    __shared__ int sh[SEL_T];
+    const int64_t base = (int64_t)blockIdx.x * SEL_CH + (int64_t)threadIdx.x * SEL_PER;
+    int c = 0;
+    for (int j = 0; j < SEL_PER; j++)
+        if (base + j < np && phs_keep(u, ld, base + j)) c++;
+    int tot;
+    block_excl_scan(c, sh, &tot);
+    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+template <typename U>
+__global__ void __launch_bounds__(SEL_T) k_phs_scatter(const double* __restrict__ xh, const U* __restrict__ u,
+                                                       const U* __restrict__ w, int64_t ld, int np,
+                                                       const int64_t* __restrict__ offs, double* __restrict__ out) {
+    __shared__ int sh[SEL_T];
+    const int64_t base = (int64_t)blockIdx.x * SEL_CH + (int64_t)threadIdx.x * SEL_PER;
+    bool keep[SEL_PER];
+    int c = 0;
+    for (int j = 0; j < SEL_PER; j++) {
+        keep[j] = base + j < np && phs_keep(u, ld, base + j);
+        c += keep[j];
+    }
+    int tot;
+    int64_t o = offs[blockIdx.x] + block_excl_scan(c, sh, &tot);
+    for (int j = 0; j < SEL_PER; j++) {
+        if (!keep[j]) continue;
+        const int64_t i = base + j;
+        double* d = out + 7 * o;
+        d[0] = xh[i];
+        d[1] = xh[ld + i];
+        d[2] = xh[2 * ld + i];
+        d[3] = (double)u[i];
+        d[4] = (double)u[ld + i];
+        d[5] = (double)u[2 * ld + i];
+        d[6] = (double)w[i];
+        o++;
+    }
+}
+
+template <typename U>
+static int32_t phs_select_impl(const double* coord_half, const U* momenta, const U* wghts, int64_t ld, int32_t np,
+                               double* out, int64_t cap, int64_t* n_sel) {
+    if (np < 0 || ld < np || n_sel == nullptr) return fail(ST_EINVAL, "phs_select: np=%d ld=%lld", np, (long long)ld);
+    if (np > 0 && (!is_device_ptr(coord_half) || !is_device_ptr(momenta) || !is_device_ptr(wghts)))
+        return fail(ST_EINVAL, "phs_select: the particle arrays are device pointers");
+    if (out != nullptr && !is_device_ptr(out)) return fail(ST_EINVAL, "phs_select: out is device memory");
+    *n_sel = 0;
+    if (np == 0) return ST_OK;
+    const int nb = (int)((np + SEL_CH - 1) / SEL_CH);
+    int* counts = static_cast<int*>(scratch(13, (size_t)nb * sizeof(int)));
+    int64_t* offs = static_cast<int64_t*>(scratch(14, (size_t)nb * sizeof(int64_t)));
+    if (!counts || !offs) return ST_ENOMEM;
+    hipLaunchKernelGGL((k_phs_count<U>), nb, SEL_T, 0, stream(), momenta, ld, np, counts);
+    CH_LAUNCH_CHECK("k_phs_count");
+    std::vector<int> hc(nb);
+    std::vector<int64_t> ho(nb);
+    CH_HIP(hipMemcpyAsync(hc.data(), counts, (size_t)nb * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    CH_HIP(hipStreamSynchronize(stream()));
+    int64_t tot = 0;
+    for (int b = 0; b < nb; b++) {
+        ho[b] = tot;
+        tot += hc[b];
+    }
+    *n_sel = tot;
+    if (out == nullptr || tot == 0) return ST_OK;
+    if (tot > cap) return fail(ST_EINVAL, "phs_select: %lld particles kept, room for %lld", (long long)tot,
+                               (long long)cap);
+    CH_HIP(hipMemcpyAsync(offs, ho.data(), (size_t)nb * sizeof(int64_t), hipMemcpyHostToDevice, stream()));
+    hipLaunchKernelGGL((k_phs_scatter<U>), nb, SEL_T, 0, stream(), coord_half, momenta, wghts, ld, np, offs, out);
+    CH_LAUNCH_CHECK("k_phs_scatter");
+    CH_HIP(hipStreamSynchronize(stream()));     // the host offsets go out of scope
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------- azimuthal intensity profile (intens_profO)
+//
+// PWR_RO(iO, ir) = sum over l < 3 and ix of |sum_m phaseO(iO, m) Fld(ix, ir, m, l)|^2
+// (utils.f90:18-57) for Fld(nkx, 0:nr, -nkO:nkO, 3) and ir = 1..nr; the phase
+// table phaseO(iO, m) is formed on the host by the reference's recurrences.
+// One thread per (iO, ir); the sums in the reference's (l, ix) order.
+template <typename C>
+__global__ void k_intens_profo(const C* __restrict__ fld, const double2* __restrict__ ph, int NO, int nkx, int nr,
+                               int nm, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NO * nr) return;
+    const int iO = t % NO, ir = 1 + t / NO;
+    const int64_t ms = (int64_t)nkx * (nr + 1), ls = ms * nm;
+    double acc = 0.0;
+    for (int l = 0; l < 3; l++)
+        for (int ix = 0; ix < nkx; ix++) {
+            double re = 0.0, im = 0.0;
+            const C* f = fld + ix + (int64_t)nkx * ir + ls * l;
+            for (int m = 0; m < nm; m++) {
+                const double2 p = ph[iO + (int64_t)NO * m];
+                const C v = f[ms * m];
+                re += p.x * (double)v.x - p.y * (double)v.y;
+                im += p.x * (double)v.y + p.y * (double)v.x;
+            }
+            acc += re * re + im * im;
+        }
+    out[iO + (int64_t)NO * (ir - 1)] = acc;
+}
+
+template <typename C>
+static int32_t intens_profo_impl(double* pwr, const C* fld, int32_t NO, int32_t nkx, int32_t nr, int32_t nm) {
+    if (NO < 2 || nkx < 1 || nr < 1 || nm < 1 || (nm & 1) == 0)
+        return fail(ST_EINVAL, "intens_profo: NO=%d nkx=%d nr=%d and %d modes (an odd count, -nkO..nkO)", NO, nkx,
+                    nr, nm);
+    const int nkO = (nm - 1) / 2;
+    std::vector<std::complex<double>> Os(NO), tab((size_t)NO * nm);
+    const double pi = 4.0 * std::atan(1.0);
+    const std::complex<double> step(std::cos(2.0 * pi / (double)(NO - 1)), std::sin(2.0 * pi / (double)(NO - 1)));
+    Os[0] = 1.0;
+    for (int i = 1; i < NO; i++) Os[i] = Os[i - 1] * step;
+    for (int i = 0; i < NO; i++) {
+        const std::complex<double> pp = Os[i], pm = 1.0 / Os[i];
+        std::vector<std::complex<double>> po(nm);
+        po[nkO] = 1.0;
+        for (int k = 1; k <= nkO; k++) {
+            po[nkO + k] = po[nkO + k - 1] * pp;
+            po[nkO - k] = po[nkO - k + 1] * pm;
+        }
+        for (int m = 0; m < nm; m++) tab[i + (size_t)NO * m] = po[m];
+    }
+    Stage st;
+    const C* d_fld = st.in(fld, (size_t)nkx * (nr + 1) * nm * 3);
+    double* d_out = st.out(pwr, (size_t)NO * nr);
+    const double2* d_ph = static_cast<const double2*>(shared_table(tab.data(), tab.size() * sizeof(double2)));
+    CH_TRY(st.status());
+    if (!d_ph) return ST_ENOMEM;
+    hipLaunchKernelGGL((k_intens_profo<C>), blocks_for((int64_t)NO * nr, 128), 128, 0, stream(), d_fld, d_ph, NO,
+                       nkx, nr, nm, d_out);
+    CH_LAUNCH_CHECK("k_intens_profo");
+    return st.finish();
+}
+
 }  // namespace chimera
 
 using namespace chimera;
@@ -280,6 +456,25 @@ int32_t chimera_line_power(const double* vec, const double* Rgrid, int32_t nkx, 
 int32_t chimera_line_power_c8(const float* vec, const double* Rgrid, int32_t nkx, int32_t nr, int32_t nmodes,
                               double* out) {
     return line_power_impl<float2>(reinterpret_cast<const float2*>(vec), Rgrid, nkx, nr, nmodes, out);
+}
+
+int32_t chimera_phs_select(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
+                           int32_t np, double* out, int64_t cap, int64_t* n_sel) {
+    return phs_select_impl<double>(coord_half, momenta, wghts, ld, np, out, cap, n_sel);
+}
+
+int32_t chimera_phs_select_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
+                                 int32_t np, double* out, int64_t cap, int64_t* n_sel) {
+    return phs_select_impl<float>(coord_half, momenta, wghts, ld, np, out, cap, n_sel);
+}
+
+int32_t chimera_intens_profo(double* PWR_RO, const double* Fld, int32_t NO, int32_t nkx, int32_t nr, int32_t nmodes) {
+    return intens_profo_impl<double2>(PWR_RO, reinterpret_cast<const double2*>(Fld), NO, nkx, nr, nmodes);
+}
+
+int32_t chimera_intens_profo_c8(double* PWR_RO, const float* Fld, int32_t NO, int32_t nkx, int32_t nr,
+                                int32_t nmodes) {
+    return intens_profo_impl<float2>(PWR_RO, reinterpret_cast<const float2*>(Fld), NO, nkx, nr, nmodes);
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 // i*kx term of the operator.  All jobs of one operator run in one launch
 // (grid.z = job), and no two jobs write the same output.
 #include <cmath>
+#include <complex>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -57,6 +58,27 @@ const float2* fft_twiddles_f(int N) {
     const float2* d = static_cast<const float2*>(shared_table(h.data(), h.size() * sizeof(float2)));
     if (d) cache[N] = d;
     return d;
+}
+
+// Store of one transformed point: post-multiplier, real scale table, mode
+// normalisation and the ghost row (shared by both line kernels).
+template <typename R>
+__device__ __forceinline__ void fft_store(const FftArgsT<R>& a, int64_t line, int x, cplx_t<R> v) {
+    typedef cplx_t<R> C;
+    const int64_t r = line % a.n_inner, q = line / a.n_inner;
+    if (a.post) v = cmul(v, a.post[x]);
+    if (a.scale) v = crs(a.scale[x + (int64_t)a.N * r], v);
+    int m = 0;
+    if (a.ghost) {
+        m = (int)(q % a.nmodes);
+        if (a.ghost == 2) v = crs((R)(m == 0 ? 0.5 / M_PI : 1.0 / M_PI), v);
+    }
+    C* o = a.out + r * a.out_r + q * a.out_q + x;
+    *o = v;
+    if (a.ghost && r == 0) {
+        // ghost radial row below row 1: zero (fb_io.f90:199) or +-row 1 (grid_deps.f90:248-265)
+        o[-a.out_r] = a.ghost == 1 ? cmake<R>(0, 0) : (m == 0 ? v : cneg(v));
+    }
 }
 
 template <int EPT, typename R>
@@ -144,23 +166,201 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
         const int64_t line = line0 + i / N;
         if (line >= a.nlines) continue;
         const int x = i % N;
-        const int64_t r = line % a.n_inner, q = line / a.n_inner;
         C v = sbuf[i];
-        if (a.post) v = cmul(v, a.post[x]);
-        else if (a.ph_at == 2) v = cmul(v, sph[x]);
-        if (a.scale) v = crs(a.scale[x + (int64_t)N * r], v);
-        int m = 0;
-        if (a.ghost) {
-            m = (int)(q % a.nmodes);
-            if (a.ghost == 2) v = crs((R)(m == 0 ? 0.5 / M_PI : 1.0 / M_PI), v);
-        }
-        C* o = a.out + r * a.out_r + q * a.out_q + x;
-        *o = v;
-        if (a.ghost && r == 0) {
-            // ghost radial row below row 1: zero (fb_io.f90:199) or +-row 1 (grid_deps.f90:248-265)
-            o[-a.out_r] = a.ghost == 1 ? cmake<R>(0, 0) : (m == 0 ? v : cneg(v));
+        if (a.ph_at == 2) v = cmul(v, sph[x]);
+        fft_store<R>(a, line, x, v);
+    }
+}
+
+// ---- lengths that are not powers of two (Bluestein)
+// A length-N DFT of sign s is X_k = c_k * sum_n (x_n c_n) conj(c_(k-n)) with the
+// chirp c_n = exp(s i pi n^2 / N) (from nk = (n^2 + k^2 - (k-n)^2) / 2): a
+// circular convolution, of power-of-two length L >= 2N - 1, of the chirped,
+// zero-padded line with g, g[m] = conj(c_|m|) for |m| < N.  ghat = FFT_L(g) / L,
+// so that IFFT_L(FFT_L(a) * ghat) (both unnormalised) is the convolution.  The
+// tables are built in long double with the angle pi (n^2 mod 2N) / N reduced
+// exactly, then rounded to R.
+static void host_fft_ld(std::vector<std::complex<long double>>& v) {
+    // iterative radix-2, forward (exp(-2 pi i jk / L)), in place
+    const size_t L = v.size();
+    for (size_t i = 1, j = 0; i < L; i++) {
+        size_t bit = L >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(v[i], v[j]);
+    }
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (size_t len = 2; len <= L; len <<= 1) {
+        for (size_t i = 0; i < L; i += len) {
+            for (size_t k = 0; k < len / 2; k++) {
+                const long double ang = -2.0L * pi * (long double)k / (long double)len;
+                const std::complex<long double> w(cosl(ang), sinl(ang));
+                const std::complex<long double> u = v[i + k], t = w * v[i + k + len / 2];
+                v[i + k] = u + t;
+                v[i + k + len / 2] = u - t;
+            }
         }
     }
+}
+
+int bluestein_length(int N) {
+    int L = 1;
+    while (L < 2 * N - 1) L <<= 1;
+    return L;
+}
+
+template <typename R>
+static int32_t bluestein_tables(int N, int sign, const cplx_t<R>** chirp, const cplx_t<R>** ghat) {
+    typedef cplx_t<R> C;
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::pair<const C*, const C*>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(N, sign);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        const int L = bluestein_length(N);
+        const long double pi = 3.141592653589793238462643383279502884L;
+        std::vector<std::complex<long double>> c(N), g(L, std::complex<long double>(0, 0));
+        for (int n = 0; n < N; n++) {
+            const long long n2 = ((long long)n * n) % (2LL * N);
+            const long double ang = (long double)sign * pi * (long double)n2 / (long double)N;
+            c[n] = std::complex<long double>(cosl(ang), sinl(ang));
+        }
+        g[0] = std::conj(c[0]);
+        for (int m = 1; m < N; m++) g[m] = g[L - m] = std::conj(c[m]);
+        host_fft_ld(g);
+        std::vector<C> hc(N), hg(L);
+        for (int n = 0; n < N; n++) hc[n] = cmake<R>((R)c[n].real(), (R)c[n].imag());
+        for (int k = 0; k < L; k++) hg[k] = cmake<R>((R)(g[k].real() / L), (R)(g[k].imag() / L));
+        const C* dc = static_cast<const C*>(shared_table(hc.data(), hc.size() * sizeof(C)));
+        const C* dg = static_cast<const C*>(shared_table(hg.data(), hg.size() * sizeof(C)));
+        if (!dc || !dg) return ST_ENOMEM;
+        it = cache.emplace(key, std::make_pair(dc, dg)).first;
+    }
+    *chirp = it->second.first;
+    *ghat = it->second.second;
+    return ST_OK;
+}
+
+// The convolution step on the lines of buf (stride L): forward FFT, times ghat,
+// unnormalised backward FFT.
+template <int EPT, typename R>
+__device__ __forceinline__ void bluestein_conv(cplx_t<R>* buf, int L, int log2L, const cplx_t<R>* __restrict__ ghat,
+                                               const cplx_t<R>* __restrict__ tw) {
+    constexpr int TOT = EPT * FFT_THREADS;
+    fft_lds<EPT, R>(buf, L, log2L, -1, tw);
+    for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) buf[i] = cmul(buf[i], ghat[i & (L - 1)]);
+    __syncthreads();
+    fft_lds<EPT, R>(buf, L, log2L, +1, tw);
+}
+
+// Lines of length N (not a power of two, 2N - 1 <= 8192) through Bluestein:
+// TOT / L lines per block, each in L points of LDS.  Same loads, pre/post
+// multipliers, absorbing-layer pass and stores as k_fft_lines.
+template <int EPT, typename R>
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_bluestein(FftArgsT<R> a) {
+    typedef cplx_t<R> C;
+    constexpr int TOT = EPT * FFT_THREADS;
+    extern __shared__ __align__(16) unsigned char fft_smem[];
+    C* const sbuf = reinterpret_cast<C*>(fft_smem);
+    const int N = a.N, lm = a.L - 1;
+    const int64_t line0 = (int64_t)blockIdx.x * (TOT >> a.log2L);
+    for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+        const int64_t line = line0 + (i >> a.log2L);
+        const int x = i & lm;
+        C v = cmake<R>(0, 0);
+        if (x < N && line < a.nlines) {
+            const int64_t r = line % a.n_inner, q = line / a.n_inner;
+            v = a.in[r * a.in_r + q * a.in_q + x];
+            if (a.pre) v = cmul(v, a.pre[x]);
+            v = cmul(v, a.bs_chirp[x]);
+        }
+        sbuf[i] = v;
+    }
+    __syncthreads();
+    bluestein_conv<EPT, R>(sbuf, a.L, a.log2L, a.bs_ghat, a.tw);
+    const C* chirp = a.bs_chirp;
+    if (a.filtr) {
+        // fb_io.f90:263: multiply the first nxfilt points, transform back
+        for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+            const int x = i & lm;
+            C v = cmake<R>(0, 0);
+            if (x < N) {
+                v = cmul(sbuf[i], a.bs_chirp[x]);
+                if (x < a.nxfilt) v = crs(a.filtr[x], v);
+                v = cmul(v, a.bs_chirp2[x]);
+            }
+            sbuf[i] = v;
+        }
+        __syncthreads();
+        bluestein_conv<EPT, R>(sbuf, a.L, a.log2L, a.bs_ghat2, a.tw);
+        chirp = a.bs_chirp2;
+    }
+    for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+        const int64_t line = line0 + (i >> a.log2L);
+        const int x = i & lm;
+        if (x >= N || line >= a.nlines) continue;
+        fft_store<R>(a, line, x, cmul(sbuf[i], chirp[x]));
+    }
+}
+
+int32_t fft_length_check(int N) {
+    if (N < 1 || N > 8192 || ((N & (N - 1)) != 0 && bluestein_length(N) > 8192))
+        return fail(ST_EUNSUP,
+                    "FFT length %d: this build transforms powers of two up to 8192 and other lengths up to 4096 "
+                    "along x", N);
+    return ST_OK;
+}
+
+template <typename R>
+static int32_t launch_fft_bluestein(FftArgsT<R> a) {
+    typedef cplx_t<R> C;
+    const int N = a.N, L = bluestein_length(N);
+    a.L = L;
+    a.log2L = 0;
+    while ((1 << a.log2L) < L) a.log2L++;
+    if constexpr (sizeof(R) == 8) a.tw = fft_twiddles(L);
+    else a.tw = fft_twiddles_f(L);
+    if (!a.tw) return ST_ENOMEM;
+    CH_TRY(bluestein_tables<R>(N, a.sign, &a.bs_chirp, &a.bs_ghat));
+    if (a.filtr) CH_TRY(bluestein_tables<R>(N, -a.sign, &a.bs_chirp2, &a.bs_ghat2));
+    a.tw_lds = 0;
+    if (a.ph_at) {
+        CH_TRY(make_phase<R>(a.ph_table, a.ph_kx, a.ph_leftX, a.ph_s, N, a.ph_leftX_dev));
+        if (a.ph_at == 1) a.pre = a.ph_table;
+        else a.post = a.ph_table;
+        a.ph_at = 0;
+    }
+    const int ept = L <= 2048 ? 8 : L / FFT_THREADS;
+    const int tot = ept * FFT_THREADS;
+    const unsigned grid = (unsigned)((a.nlines + tot / L - 1) / (tot / L));
+    const size_t shmem = (size_t)tot * sizeof(C);
+    static bool attr_done = false;
+    if (!attr_done) {
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_bluestein<8, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(8 * FFT_THREADS * sizeof(C))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_bluestein<16, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(16 * FFT_THREADS * sizeof(C))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_bluestein<32, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(32 * FFT_THREADS * sizeof(C))));
+        attr_done = true;
+    }
+    KTimer kt(KF_FFT);
+    switch (ept) {
+        case 8:
+            hipLaunchKernelGGL((k_fft_bluestein<8, R>), grid, FFT_THREADS, shmem, stream(), a);
+            break;
+        case 16:
+            hipLaunchKernelGGL((k_fft_bluestein<16, R>), grid, FFT_THREADS, shmem, stream(), a);
+            break;
+        case 32:
+            hipLaunchKernelGGL((k_fft_bluestein<32, R>), grid, FFT_THREADS, shmem, stream(), a);
+            break;
+        default:
+            return fail(ST_EUNSUP, "FFT length %d", N);
+    }
+    CH_LAUNCH_CHECK("k_fft_bluestein");
+    return ST_OK;
 }
 
 template <typename R>
@@ -168,8 +368,8 @@ int32_t launch_fft(FftArgsT<R> a) {
     typedef cplx_t<R> C;
     if (a.nlines <= 0) return ST_OK;
     const int N = a.N;
-    if (N < 1 || (N & (N - 1)) != 0 || N > 8192)
-        return fail(ST_EUNSUP, "FFT length %d: this build transforms powers of two up to 8192 along x", N);
+    CH_TRY(fft_length_check(N));
+    if ((N & (N - 1)) != 0) return launch_fft_bluestein<R>(a);
     int l2 = 0;
     while ((1 << l2) < N) l2++;
     a.log2N = l2;
@@ -1391,18 +1591,14 @@ CH_FB_INSTANTIATE(float)
 
 using namespace chimera;
 
-static int32_t pow2_check(int nkx) {
-    if (nkx < 1 || (nkx & (nkx - 1)) != 0 || nkx > 8192)
-        return fail(ST_EUNSUP, "nkx = %d: the x transform needs a power of two <= 8192", nkx);
-    return ST_OK;
-}
-
 extern "C" {
+
+int32_t chimera_fft_length_check(int32_t n) { return fft_length_check(n); }
 
 int32_t chimera_fb_vec_in(double* vec_fb, const double* vec, double leftX, const double* kx, const double* In,
                           int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr) {
     if (nkx < 1 || nr < 1 || nkO < 1 || nkr < 1) return fail(ST_EINVAL, "fb_vec_in: bad sizes");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     Stage st;
     double2* o = reinterpret_cast<double2*>(st.out(vec_fb, 2 * (size_t)nkx * nkr * nkO * 3));
     const double2* v = reinterpret_cast<const double2*>(st.in(vec, 2 * (size_t)nkx * (nr + 1) * nkO * 3));
@@ -1416,7 +1612,7 @@ int32_t chimera_fb_vec_in(double* vec_fb, const double* vec, double leftX, const
 int32_t chimera_fb_scl_in(double* scl_fb, const double* scl, double leftX, const double* kx, const double* In,
                           int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr) {
     if (nkx < 1 || nr < 1 || nkO < 1 || nkr < 1) return fail(ST_EINVAL, "fb_scl_in: bad sizes");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     Stage st;
     double2* o = reinterpret_cast<double2*>(st.out(scl_fb, 2 * (size_t)nkx * nkr * nkO));
     const double2* v = reinterpret_cast<const double2*>(st.in(scl, 2 * (size_t)nkx * (nr + 1) * nkO));
@@ -1430,7 +1626,7 @@ int32_t chimera_fb_scl_in(double* scl_fb, const double* scl, double leftX, const
 int32_t chimera_fb_eb_out(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
                           const double* Out, int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr) {
     if (nkx < 1 || nr < 1 || nkO < 1 || nkr < 1) return fail(ST_EINVAL, "fb_eb_out: bad sizes");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     Stage st;
     double2* eb = reinterpret_cast<double2*>(st.out(eb_spc, 2 * (size_t)nkx * (nr + 1) * nkO * 6));
     const double2* e = reinterpret_cast<const double2*>(st.in(e_fb, 2 * (size_t)nkx * nkr * nkO * 6));
@@ -1448,7 +1644,7 @@ int32_t chimera_fb_filtr(double* vec, double leftX, const double* kx, const doub
         return fail(ST_EUNSUP, "fb_filtr: modefilt=%d; only 0 ('left') is defined (the reference's other modes index "
                                "the profile out of its bounds)", modefilt);
     if (nkx < 1 || nkr < 1 || nkO < 1 || nxfilt < 0) return fail(ST_EINVAL, "fb_filtr: bad sizes");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     Stage st;
     double2* v = reinterpret_cast<double2*>(st.inout(vec, 2 * (size_t)nkx * nkr * nkO * 3));
     const double* k = st.in(kx, nkx);
@@ -1511,7 +1707,7 @@ int32_t chimera_fb_forward(double* out, double* in, double leftX, const double* 
                            const double* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
                            int32_t fold, const double* frame) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ncomp < 1) return fail(ST_EINVAL, "fb_forward: bad sizes");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     if (!is_device_ptr(out) || !is_device_ptr(in) || !is_device_ptr(kx) || !is_device_ptr(In) ||
         (scale && !is_device_ptr(scale)))
         return fail(ST_EINVAL, "fb_forward: resident entry points take device pointers");
@@ -1524,7 +1720,7 @@ int32_t chimera_fb_inverse(double* eb_spc, const double* e_fb, const double* b_f
                            const double* frame) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 2)
         return fail(ST_EINVAL, "fb_inverse: bad sizes or ghost mode");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     if (!is_device_ptr(eb_spc) || !is_device_ptr(e_fb) || !is_device_ptr(b_fb) || !is_device_ptr(kx) ||
         !is_device_ptr(Out))
         return fail(ST_EINVAL, "fb_inverse: resident entry points take device pointers");
@@ -1565,7 +1761,7 @@ int32_t chimera_fb_op(int32_t op, double* out, const double* in, const double* D
 int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const double* filtr, int32_t nxfilt,
                           int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp) {
     if (nkx < 1 || nkr < 1 || nmodes < 1 || ncomp < 1 || nxfilt < 0) return fail(ST_EINVAL, "fb_filter: bad sizes");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     if (!is_device_ptr(vec) || !is_device_ptr(kx) || (nxfilt > 0 && !is_device_ptr(filtr)))
         return fail(ST_EINVAL, "fb_filter: resident entry points take device pointers");
     return fb_filter(reinterpret_cast<double2*>(vec), leftX, kx, nxfilt > 0 ? filtr : nullptr, nxfilt, nkx, nkr,
@@ -1578,7 +1774,7 @@ int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double*
                               const float* scale, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ncomp,
                               int32_t fold, const double* frame) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ncomp < 1) return fail(ST_EINVAL, "fb_forward_c8: bad sizes");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     if (!is_device_ptr(out) || !is_device_ptr(in) || !is_device_ptr(kx) || !is_device_ptr(In) ||
         (scale && !is_device_ptr(scale)))
         return fail(ST_EINVAL, "fb_forward_c8: resident entry points take device pointers");
@@ -1591,7 +1787,7 @@ int32_t chimera_fb_inverse_c8(float* eb_spc, const float* e_fb, const float* b_f
                               const double* frame) {
     if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 2)
         return fail(ST_EINVAL, "fb_inverse_c8: bad sizes or ghost mode");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     if (!is_device_ptr(eb_spc) || !is_device_ptr(e_fb) || !is_device_ptr(b_fb) || !is_device_ptr(kx) ||
         !is_device_ptr(Out))
         return fail(ST_EINVAL, "fb_inverse_c8: resident entry points take device pointers");
@@ -1633,7 +1829,7 @@ int32_t chimera_fb_op_c8(int32_t op, float* out, const float* in, const float* D
 int32_t chimera_fb_filter_c8(float* vec, double leftX, const double* kx, const float* filtr, int32_t nxfilt,
                              int32_t nkx, int32_t nkr, int32_t nmodes, int32_t ncomp) {
     if (nkx < 1 || nkr < 1 || nmodes < 1 || ncomp < 1 || nxfilt < 0) return fail(ST_EINVAL, "fb_filter_c8: bad sizes");
-    CH_TRY(pow2_check(nkx));
+    CH_TRY(fft_length_check(nkx));
     if (!is_device_ptr(vec) || !is_device_ptr(kx) || (nxfilt > 0 && !is_device_ptr(filtr)))
         return fail(ST_EINVAL, "fb_filter_c8: resident entry points take device pointers");
     return fb_filter_t<float>(reinterpret_cast<float2*>(vec), leftX, kx, nxfilt > 0 ? filtr : nullptr, nxfilt, nkx,

@@ -35,6 +35,15 @@ struct FftArgsT {
     // k_phase into ph_table
     int ph_at;
     int tw_lds;            // set by launch_fft: twiddle table staged in LDS (EPT = 8)
+    // set by launch_fft when N is not a power of two: the line is transformed
+    // as a circular convolution of power-of-two length L >= 2N-1 (Bluestein),
+    // bs_chirp/bs_ghat for the transform of sign `sign`, bs_chirp2/bs_ghat2 for
+    // the transform back of fb_filtr (sign -sign)
+    int L, log2L;
+    const C* bs_chirp;
+    const C* bs_ghat;
+    const C* bs_chirp2;
+    const C* bs_ghat2;
     double ph_s, ph_leftX;
     const double* ph_kx;
     const double* ph_leftX_dev;
@@ -90,6 +99,8 @@ struct FbGemmArgs {
 enum { FB_ROT = 0, FB_GRAD = 1, FB_DIV = 2, FB_GRADDIV = 3 };
 
 template <typename R> int32_t launch_fft(FftArgsT<R> a);
+// EUNSUP unless the x transform handles length N
+int32_t fft_length_check(int N);
 // ph[i] = exp(s*i*kx[i]*leftX), leftX read from leftX_dev on the device when that is set.
 template <typename R>
 int32_t make_phase(cplx_t<R>* ph, const double* kx, double leftX, double s, int n, const double* leftX_dev = nullptr);

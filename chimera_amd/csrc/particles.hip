@@ -105,11 +105,12 @@ __global__ void k_dep_curr(const double* __restrict__ coord, PStride sx, const d
 }
 
 // grid_deps.f90:89-147 (particle loop)
-__global__ void k_dep_dens(const double* __restrict__ coord, PStride sx, const double* __restrict__ w,
+template <typename W>
+__global__ void k_dep_dens(const double* __restrict__ coord, PStride sx, const W* __restrict__ w,
                            double2* __restrict__ dens, GridGeom g, int ip0, int ip1) {
     const int ip = ip0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= ip1) return;
-    const double wp = w[ip];
+    const double wp = (double)w[ip];
     if (wp == 0.0) return;
     PartGeom pg;
     if (!part_geom(pg, coord[sx.at(ip, 0)], coord[sx.at(ip, 1)], coord[sx.at(ip, 2)], g, -1.0)) return;
@@ -331,12 +332,31 @@ int32_t launch_dep_curr(const double* x, PStride sx, const double* u, PStride su
     return ST_OK;
 }
 
-int32_t launch_dep_dens(const double* x, PStride sx, const double* w, double2* dens, const GridGeom& g, int ip0,
-                        int ip1) {
+template <typename W>
+int32_t launch_dep_dens(const double* x, PStride sx, const W* w, double2* dens, const GridGeom& g, int ip0, int ip1) {
     if (ip1 <= ip0) return ST_OK;
     KTimer kt(KF_DEP_DENS);
-    hipLaunchKernelGGL(k_dep_dens, blocks_for(ip1 - ip0, 256), 256, 0, stream(), x, sx, w, dens, g, ip0, ip1);
+    hipLaunchKernelGGL(k_dep_dens<W>, blocks_for(ip1 - ip0, 256), 256, 0, stream(), x, sx, w, dens, g, ip0, ip1);
     CH_LAUNCH_CHECK("k_dep_dens");
+    return ST_OK;
+}
+template int32_t launch_dep_dens<double>(const double*, PStride, const double*, double2*, const GridGeom&, int, int);
+template int32_t launch_dep_dens<float>(const double*, PStride, const float*, double2*, const GridGeom&, int, int);
+
+// Density on the grid of get_dens_on_grid (species.py:400-427) after the
+// deposit and its ghost fold: each radial row ir times VGrid[ir] (zero at the
+// ghost row), in place on dens(nx1, nr1, nmodes).
+__global__ void k_scale_rows(double2* __restrict__ a, const double* __restrict__ v, int nx1, int nr1, int nlines) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx1 * nr1 * nlines) return;
+    const int ir = (int)((t / nx1) % nr1);
+    a[t] = crs(v[ir], a[t]);
+}
+
+int32_t scale_rows(double2* a, const double* v, int nx1, int nr1, int nlines) {
+    hipLaunchKernelGGL(k_scale_rows, blocks_for((int64_t)nx1 * nr1 * nlines, 256), 256, 0, stream(), a, v, nx1, nr1,
+                       nlines);
+    CH_LAUNCH_CHECK("k_scale_rows");
     return ST_OK;
 }
 
@@ -442,7 +462,8 @@ static int32_t dep_dens_impl(const double* coord, const double* wghts, double* d
     GridGeom g = make_geom(leftX, R, dx_inv, dr_inv, nx, nr, nkO);
     CH_TRY(load_rgrid_edges(g, Rgrid, R));
     if (ip1 > ip0) {
-        hipLaunchKernelGGL(k_dep_dens, blocks_for(ip1 - ip0, 256), 256, 0, stream(), x, AOS3, w, D, g, ip0, ip1);
+        hipLaunchKernelGGL(k_dep_dens<double>, blocks_for(ip1 - ip0, 256), 256, 0, stream(), x, AOS3, w, D, g, ip0,
+                           ip1);
         CH_LAUNCH_CHECK("k_dep_dens");
     }
     hipLaunchKernelGGL(k_ghost_fold<double2>, blocks_for((int64_t)(nx + 1) * (nkO + 1), 256), 256, 0, stream(), D, nx + 1,

@@ -167,7 +167,9 @@ int32_t launch_push_coords(double* x, PStride sx, const double* u, PStride su, d
                            int np);
 int32_t launch_dep_curr(const double* x, PStride sx, const double* u, PStride su, const double* w, double2* curr,
                         const GridGeom& g, int ip0, int ip1);
-int32_t launch_dep_dens(const double* x, PStride sx, const double* w, double2* dens, const GridGeom& g, int ip0,
-                        int ip1);
+template <typename W>
+int32_t launch_dep_dens(const double* x, PStride sx, const W* w, double2* dens, const GridGeom& g, int ip0, int ip1);
+// a[ix, ir, line] *= v[ir] on nlines (nx1, nr1) planes
+int32_t scale_rows(double2* a, const double* v, int nx1, int nr1, int nlines);
 
 }  // namespace chimera

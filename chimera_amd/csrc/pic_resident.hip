@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "fb_kernels.hpp"
 #include "pic_device.hpp"
 #include "sort.hpp"
 #include "../../include/chimera_hip.h"
@@ -639,6 +640,26 @@ static int32_t check_resident(int np, int64_t ld, int nkO) {
     return ST_OK;
 }
 
+// get_dens_on_grid (species.py:400-427): dens(nx+1, nr+1, nkO+1) = the
+// deposited density (grid_deps.f90:89-147, ghost fold included) times
+// VGrid[ir] per radial row; dens is overwritten.
+template <typename W>
+static int32_t dens_on_grid_impl(const double* coord, const W* wghts, int64_t ld, int32_t np, double* dens,
+                                 double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0,
+                                 double rmax, int32_t nx, int32_t nr, int32_t nkO, const double* vgrid) {
+    CH_TRY(check_resident(np, ld, nkO));
+    if (nx < 1 || nr < 1 || !dens || !vgrid || !is_device_ptr(dens) || !is_device_ptr(vgrid))
+        return fail(ST_EINVAL, "dens_on_grid: nx=%d nr=%d; dens and vgrid are device arrays", nx, nr);
+    double2* D = reinterpret_cast<double2*>(dens);
+    CH_TRY(fill_bytes(D, 0, (size_t)(nx + 1) * (nr + 1) * (nkO + 1) * sizeof(double2)));
+    if (np > 0) {
+        const GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+        CH_TRY(launch_dep_dens<W>(coord, PStride{1, ld}, wghts, D, g, 0, np));
+    }
+    CH_TRY(ghost_fold<double>(D, nx + 1, nr + 1, nkO + 1));
+    return scale_rows(D, vgrid, nx + 1, nr + 1, nkO + 1);
+}
+
 // ------------------------------------------------------------------ counting sort by cell
 //
 // key = ir*nx + ix for a particle in cell (ix, ir) of the grid, nx*nr for a
@@ -886,6 +907,21 @@ int32_t chimera_soa_dep_dens(const double* coord, const double* wghts, int64_t l
     const PStride s = {1, ld};
     const GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
     return launch_dep_dens(coord, s, wghts, reinterpret_cast<double2*>(dens), g, 0, np);
+}
+
+int32_t chimera_soa_dens_on_grid(const double* coord, const double* wghts, int64_t ld, int32_t np, double* dens,
+                                 double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0,
+                                 double rmax, int32_t nx, int32_t nr, int32_t nkO, const double* vgrid) {
+    return dens_on_grid_impl<double>(coord, wghts, ld, np, dens, leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO,
+                                     vgrid);
+}
+
+int32_t chimera_soa_dens_on_grid_mixed(const double* coord, const float* wghts, int64_t ld, int32_t np,
+                                       double* dens, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
+                                       double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
+                                       const double* vgrid) {
+    return dens_on_grid_impl<float>(coord, wghts, ld, np, dens, leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO,
+                                    vgrid);
 }
 
 int32_t chimera_soa_gather_push(const double* coord, double* momenta, const double* wghts, int64_t ld, int32_t np,

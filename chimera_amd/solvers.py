@@ -19,6 +19,7 @@ import numpy as np
 
 from . import fimera as F
 from . import operators as ops
+from ._lib import lib
 from .device import DeviceArray, call
 
 
@@ -52,6 +53,8 @@ class Solver:
         A["Nx"], A["Xgrid"] = ops.x_grid(A["leftX"], A["rightX"], A["dx"], nthrds)
         if nthrds is not None:
             A["Nxchunk"] = A["Nx"] // nthrds
+        if lib().chimera_fft_length_check(int(A["Nx"])) != 0:
+            raise ValueError(lib().chimera_last_error().decode())
         A["leftX"] = A["Xgrid"][0]
         A["kx0"] = 0.0
         A["kx_env"] = 2 * np.pi * np.fft.fftfreq(A["Nx"], A["dx"])

@@ -112,6 +112,40 @@ class Specie:
             return a[:self.np].copy()
         return np.asfortranarray(a[:self.np].T)
 
+    # ------------------------------------------------------------------ diagnostics
+    def get_dens_on_grid(self, Nko=0):
+        """species.py:400-427: the species' density on its grid, modes 0..Nko,
+        complex (Nx, Nr, Nko+1): dep_dens of the particles times
+        1/(2 pi dx dr Rgrid) per radial row (0 at Rgrid <= 0), deposited on
+        the device (FP64 sums in every precision) and returned to the host."""
+        A = self.Args
+        R = A["Rgrid"]
+        V = 2 * np.pi * A["dx"] * A["dr"] * R
+        V = (V + (R == 0)) ** -1 * (R > 0.0)
+        if getattr(self, "_dens_tabs", (None,))[0] is not R:
+            self._dens_tabs = (R, DeviceArray.from_numpy(np.asarray(R, float), np.float64),
+                               DeviceArray.from_numpy(np.asarray(V, float), np.float64))
+        _, Rdev, Vdev = self._dens_tabs
+        dens = DeviceArray((A["Nx"], A["Nr"], Nko + 1), np.complex128)
+        call("chimera_soa_dens_on_grid_mixed" if self.mixed else "chimera_soa_dens_on_grid", self.ptr("coords"),
+             self.ptr("weights"), self.ld, self.np, dens.ptr, float(A["leftX"]), Rdev.ptr, 1.0 / A["dx"],
+             1.0 / A["dr"], float(R[0]), float(R[-1]), A["Nx"] - 1, A["Nr"] - 1, int(Nko), Vdev.ptr)
+        return dens.numpy()
+
+    def phase_space(self):
+        """The PartsPack of phs_out (diagnostics.py:94-103): half-step
+        positions, momenta and weights of the particles with
+        sum_c (1 + u_c^2) > 20^2, as a (7, n) array, selected on the device."""
+        fn = "chimera_phs_select_mixed" if self.mixed else "chimera_phs_select"
+        n = ctypes.c_int64(0)
+        args = (self.ptr("coords_halfstep"), self.ptr("momenta"), self.ptr("weights"), self.ld, self.np)
+        call(fn, *args, None, 0, ctypes.addressof(n))
+        if n.value == 0:
+            return np.zeros((7, 0), order="F")
+        out = DeviceArray((7, n.value), np.float64)
+        call(fn, *args, out.ptr, n.value, ctypes.addressof(n))
+        return out.numpy()
+
     # ------------------------------------------------------------------ generation (species.py:132-244)
     def gen_parts(self, Domain=None, Xsteps=None, ProfileFunc=None):
         A = self.Args

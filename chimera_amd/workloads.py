@@ -11,6 +11,8 @@ instantiates them with any module exposing Solver / Specie / ChimeraRun.
   lpa_cfg2   the cfg1 physics at the precision-form size: Nx=8192 x Nkr=512,
              modes m=0..3, 67.0M electrons, moving window, 500 steps
   lpa_small  the cfg0 physics on a reduced grid, for quick parity runs
+  lpa_testrun  the reference's own CI scenario (doc/tests/lpa-testrun.py):
+             Nx=528 on 4 x-chunks, modes m=0,1, density ramp injected
   beam       one relativistic bunch with space charge in an undulator, in a
              window moving with it; beam_scan draws the 64 bunches of a scan
              (Nx 256-2048 x Nkr 32-256, modes m=0,1, 1e4-2e6 particles, 300 steps)
@@ -27,7 +29,10 @@ import numpy as np
 
 
 def lpa(Nx, Nkr, nko, dx=0.05, dr=0.3, a0=1.0, waist=5.0, duration=4.0, x0_from_right=12.8, dens=0.005,
-        steps=100, moving=False, absorb=0, fixed_cell=(2, 2, 4)):
+        steps=100, moving=False, absorb=0, fixed_cell=(2, 2, 4), xchunked=None):
+    """xchunked = (T, n): the reference's x-chunked layout (solver and species
+    'Xchunked'): T chunks deposited by one OpenMP thread each, particles binned
+    into chunks every n + 1 steps; Nx stays a multiple of 2T."""
     xmin = -Nx * dx
     rmax = Nkr * dr
     solver_in = {"Grid": (xmin, 0.0, rmax, dx, dr), "TimeStep": dx, "MaxAzimuthMode": nko,
@@ -35,6 +40,9 @@ def lpa(Nx, Nkr, nko, dx=0.05, dr=0.3, a0=1.0, waist=5.0, duration=4.0, x0_from_
     laser_in = {"a0": a0, "k0": 1.0, "x0": -x0_from_right, "x_foc": -x0_from_right, "Lx": duration, "LR": waist}
     electrons_in = {"Grid": (xmin, 0.0, rmax, dx, dr), "TimeStep": dx, "Density": dens, "FixedCell": fixed_cell,
                     "MomentaSpreads": (1e-3, 1e-3, 1e-3)}
+    if xchunked is not None:
+        solver_in["Xchunked"] = tuple(xchunked)
+        electrons_in["Xchunked"] = tuple(xchunked)
     ions_in = copy.deepcopy(electrons_in)
     ions_in["Charge"] = 1.0
     ions_in["Mass"] = 1886.0
@@ -65,17 +73,46 @@ def lpa_cfg1(**kw):
     return lpa(4096, 256, 2, **args)
 
 
-def lpa_cfg2():
-    return lpa(8192, 512, 3, a0=2.0, waist=8.0, duration=6.0, x0_from_right=30.0, steps=500, moving=True,
-               absorb=512)
+def lpa_cfg2(**kw):
+    args = dict(a0=2.0, waist=8.0, duration=6.0, x0_from_right=30.0, steps=500, moving=True, absorb=512)
+    args.update(kw)
+    return lpa(8192, 512, 3, **args)
 
 
-def lpa_small(Nx=128, Nkr=32, nko=1, moving=False, steps=10):
+def lpa_small(Nx=128, Nkr=32, nko=1, moving=False, steps=10, **kw):
     return lpa(Nx, Nkr, nko, a0=1.0, waist=2.0, duration=1.5, x0_from_right=2.5, steps=steps, moving=moving,
-               absorb=16 if moving else 0)
+               absorb=16 if moving else 0, **kw)
 
 
-WORKLOADS = {"lpa_cfg0": lpa_cfg0, "lpa_cfg1": lpa_cfg1, "lpa_cfg2": lpa_cfg2, "lpa_small": lpa_small}
+def lpa_testrun(steps=550):
+    """The reference's own LPA test scenario (doc/tests/lpa-testrun.py:11-63):
+    x in [-20, 1] with dx = 0.04 on 4 x-chunks (Nx = 528, not a power of two),
+    R = 16 with dr = 0.25, modes m = 0, 1; a0 = 3 laser focused at x = 45;
+    FixedCell 2x2x4 electrons (NoSorting) and still ions at n = 0.005 n_c; a
+    window moving every 10 steps with a 175-point absorbing layer and a density
+    ramp injected on the right."""
+    xgmin, xgmax, Rg, dx, dr = -20.0, 1.0, 16.0, 0.04, 0.25
+    solver_in = {"Grid": (xgmin, xgmax, Rg, dx, dr), "TimeStep": dx, "MaxAzimuthMode": 1, "Xchunked": (4, 10),
+                 "Features": ("SpaceCharge", "StillAsBackground")}
+    laser_in = {"a0": 3.0, "k0": 1.0, "x0": -14.0, "x_foc": 45.0, "Lx": 4.0, "LR": 4.0}
+    electrons_in = {"Grid": (xgmin, xgmax, Rg, dx, dr), "TimeStep": dx, "Density": 0.005, "FixedCell": (2, 2, 4),
+                    "Xchunked": (4, 10), "Features": ("NoSorting",)}
+    ions_in = copy.deepcopy(electrons_in)
+    ions_in["Charge"] = 1
+    ions_in["Mass"] = 1886
+    ions_in["Features"] += ("Still",)
+    frames = ({"TimeStep": dx, "Steps": 10, "AbsorbLayer": 175, "AddPlasma": _density_ramp,
+               "Features": ("IonsOnTop",)},)
+    return {"solver": solver_in, "laser": laser_in, "electrons": electrons_in, "ions": ions_in,
+            "frames": frames, "features": (), "steps": steps}
+
+
+def _density_ramp(x):
+    return np.interp(x, [0.0, 40.0, 50.0, 300], [0.0, 1.0, 0.5, 0.5])
+
+
+WORKLOADS = {"lpa_cfg0": lpa_cfg0, "lpa_cfg1": lpa_cfg1, "lpa_cfg2": lpa_cfg2, "lpa_small": lpa_small,
+             "lpa_testrun": lpa_testrun}
 
 
 def beam(Nx, Nkr, gamma, sig_x, sig_r, emittance, n_target, K, lambda_u, dens=1e-3, steps=300):
@@ -154,6 +191,18 @@ def build_beam(mod, spec, seed=12345, own_rng=False):
     _seed((bunch,), seed, own_rng)
     bunch.add_particles(*bunch.gen_parts(Domain=spec["domain"], ProfileFunc=profile))
     return mod.ChimeraRun({"Solvers": (solver,), "Particles": (bunch,), "MovingFrames": spec["frames"]})
+
+
+def x_slab(spec, parts):
+    """The workload on 1/parts of its x extent (the window's right edge kept):
+    the same radial grid, modes, plasma and laser, parts times fewer x nodes
+    and particles."""
+    spec = copy.deepcopy(spec)
+    for key in ("solver", "electrons", "ions"):
+        if key in spec:
+            xmin, xmax, rmax, dx, dr = spec[key]["Grid"]
+            spec[key]["Grid"] = (xmax - (xmax - xmin) / parts, xmax, rmax, dx, dr)
+    return spec
 
 
 def with_precision(spec, precision):

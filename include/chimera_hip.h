@@ -43,6 +43,12 @@ extern "C" {
 /* ---------------------------------------------------------------- runtime */
 const char* chimera_version(void);
 const char* chimera_last_error(void);
+/* CHIMERA_OK when the x transform handles lines of n points (powers of two up
+ * to 8192, other lengths up to 4096 through a power-of-two convolution), else
+ * CHIMERA_EUNSUP with chimera_last_error set.  Touches no device.  Lets a
+ * Solver refuse its grid at construction; the reference's FFTW plans any
+ * length (fb_io.f90:37-38), so it has no counterpart there. */
+int32_t chimera_fft_length_check(int32_t n);
 /* Stream (hipStream_t) every later launch goes to; NULL = the null stream. */
 int32_t chimera_set_stream(void* hip_stream);
 void* chimera_get_stream(void);
@@ -294,6 +300,19 @@ int32_t chimera_soa_dep_dens(const double* coord, const double* wghts, int64_t l
                              double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
                              int32_t nx, int32_t nr, int32_t nkO);
 
+/* get_dens_on_grid (species.py:400-427): dens(nx+1, nr+1, nkO+1) complex128 =
+ * dep_dens of the particles (ghost fold included, grid_deps.f90:89-147) times
+ * vgrid[ir] = 1/(2 pi dx dr Rgrid[ir]) (0 where Rgrid <= 0) per radial row.
+ * dens and vgrid in device memory; dens is overwritten.  The _mixed form
+ * reads float weights (the deposit and dens stay FP64). */
+int32_t chimera_soa_dens_on_grid(const double* coord, const double* wghts, int64_t ld, int32_t np, double* dens,
+                                 double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0,
+                                 double rmax, int32_t nx, int32_t nr, int32_t nkO, const double* vgrid);
+int32_t chimera_soa_dens_on_grid_mixed(const double* coord, const float* wghts, int64_t ld, int32_t np,
+                                       double* dens, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
+                                       double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
+                                       const double* vgrid);
+
 /* Window frames on the device.  The resident entry points below that take a
  * window edge (leftX) also take `frame`, a DEVICE pointer or NULL: when it is
  * not NULL the kernels read the edge from it at run time and ignore the
@@ -409,6 +428,25 @@ int32_t chimera_beam_moments(const double* coord_half, const double* momenta, co
  * device or host memory, as for chimera_beam_moments. */
 int32_t chimera_momentum_mean(const double* momenta, const double* wghts, int32_t np, double* px0);
 
+/* The particles phs_out dumps (diagnostics.py:89-107): those with
+ * sum_c (1 + u_c^2) > 20^2, in their order, as PartsPack(7, n) (rows: x, y, z
+ * at the half step, px, py, pz, w; Fortran order, FP64).  Device particle
+ * arrays of component stride ld.  out = NULL only counts: *n_sel (host) = n.
+ * Otherwise out (device, room for cap particles) receives them; the call
+ * waits for the kernels. */
+int32_t chimera_phs_select(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
+                           int32_t np, double* out, int64_t cap, int64_t* n_sel);
+int32_t chimera_phs_select_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
+                                 int32_t np, double* out, int64_t cap, int64_t* n_sel);
+/* intens_profO (utils.f90:18-57; f2py name intens_profo): PWR_RO(NO, nr) =
+ * sum over l < 3 and ix of |sum_m phaseO(m) Fld(ix, ir, m, l)|^2, ir = 1..nr,
+ * for Fld(nkx, 0:nr, -nkO:nkO, 3) (nmodes = 2 nkO + 1, odd, as the
+ * reference's dimension requires) and phaseO(m) = exp(i m theta), theta =
+ * 2 pi (iO-1)/(NO-1) formed by the reference's recurrences.  Fld and PWR_RO
+ * host or device. */
+int32_t chimera_intens_profo(double* PWR_RO, const double* Fld, int32_t NO, int32_t nkx, int32_t nr, int32_t nmodes);
+int32_t chimera_intens_profo_c8(double* PWR_RO, const float* Fld, int32_t NO, int32_t nkx, int32_t nr,
+                                int32_t nmodes);
 /* EM energy per kx of nrg_out (diagnostics.py:109-122), before its reordering
  * of the kx axis: out[x] = sum_{k,m} EnergyFact[x,k,m] * sum_{c<3} |EG_fb[x,k,m,c]|^2
  * with EG_fb(nkx,nkr,nmodes,6) and EnergyFact(nkx,nkr,nmodes) (solvers.py:142-149)

@@ -341,7 +341,7 @@ void ref_eb_correction(double *eb_, int32_t nx, int32_t nr, int32_t nkO)
 
 /* In-place length-n DFT of a, sign -1 (FFTW_FORWARD) or +1 (FFTW_BACKWARD),
  * unnormalised.  Radix-2 decimation in time for powers of two, direct O(n^2)
- * sum otherwise.  Twiddles from cos/sin of the exact angle. */
+ * sum in long double otherwise.  Twiddles from cos/sin of the exact angle. */
 static void fft_line(cplx *a, int64_t n, int sign, cplx *work)
 {
     if (n <= 1) return;
@@ -365,16 +365,31 @@ static void fft_line(cplx *a, int64_t n, int sign, cplx *work)
             }
         }
     } else {
+        /* direct sum in long double (64-bit significand) with a long double
+         * twiddle table: the result is the DFT rounded once to double, more
+         * accurate than any FFT, so that the oracle's own rounding stays well
+         * below the differences a parity test measures */
+        long double *twr = (long double *)malloc(sizeof(long double) * 2 * (size_t)n);
+        long double *twi = twr + n;
         for (int64_t k = 0; k < n; k++) {
-            double sr = 0, si = 0;
-            for (int64_t j = 0; j < n; j++) {
-                const double ang = sign * 2.0 * PI_D * (double)((j * k) % n) / (double)n;
-                const double c = cos(ang), s = sin(ang);
-                sr += a[j].re * c - a[j].im * s;
-                si += a[j].re * s + a[j].im * c;
-            }
-            work[k] = cmk(sr, si);
+            const long double ang = (long double)sign * 2.0L * 3.141592653589793238462643383279502884L *
+                                    (long double)k / (long double)n;
+            twr[k] = cosl(ang);
+            twi[k] = sinl(ang);
         }
+        for (int64_t k = 0; k < n; k++) {
+            long double sr = 0, si = 0;
+            int64_t jk = 0;
+            for (int64_t j = 0; j < n; j++) {
+                const long double c = twr[jk], s = twi[jk];
+                sr += (long double)a[j].re * c - (long double)a[j].im * s;
+                si += (long double)a[j].re * s + (long double)a[j].im * c;
+                jk += k;
+                if (jk >= n) jk -= n;
+            }
+            work[k] = cmk((double)sr, (double)si);
+        }
+        free(twr);
         memcpy(a, work, sizeof(cplx) * (size_t)n);
     }
 }

@@ -1,9 +1,12 @@
-"""CPU restatement of three of the reference's diagnostics on host arrays in
-its layouts: the beam envelopes of get_beam_envelops (diagnostics.py:177-207;
+"""CPU restatement of the reference's diagnostics on host arrays in its
+layouts: the beam envelopes of get_beam_envelops (diagnostics.py:177-207;
 coords_halfstep and momenta as (3, np), weights (np,)), the EM energy per kx
 of nrg_out (diagnostics.py:109-122) with the EnergyFact of
-solvers.py:142-149, and the laser power of pwr_out (diagnostics.py:124-149)
-through the C port's inverse transform.
+solvers.py:142-149, the laser power of pwr_out (diagnostics.py:124-149)
+through the C port's inverse transform, the density of get_dens_on_grid
+(species.py:400-427), the phase-space selection of phs_out
+(diagnostics.py:89-107) and the azimuthal profile intens_profO
+(utils.f90:18-57).
 
 TEST INFRASTRUCTURE ONLY: used by tests/ as the checker; the product never
 imports it.
@@ -73,3 +76,49 @@ def pwr_out(port, EG_fb, leftX, kx, OutFull, RgridFull, dr):
                         OutFull)
     dat = eb[..., :3]
     return PwrFctr * 2 * dr * ((np.abs(dat) ** 2).sum(-1).sum(-1) * RgridFull[None, :]).sum(-1)
+
+
+def dens_on_grid(port, coords, weights, leftX, Rgrid, dx, dr, Nx, Nko):
+    """species.py:400-427: get_dens_on_grid of a species on its grid (Nx x
+    len(Rgrid) nodes, modes 0..Nko): the oracle port's dep_dens (ghost fold
+    included) times 1/(2 pi dx dr Rgrid) per radial row, 0 where Rgrid <= 0."""
+    VGrid = 2 * np.pi * dx * dr * Rgrid
+    VGrid = (VGrid + (Rgrid == 0)) ** -1 * (Rgrid > 0.0)
+    dens = np.zeros((Nx, Rgrid.shape[0], Nko + 1), dtype="complex", order="F")
+    dens = port.dep_dens(np.asfortranarray(coords), np.asarray(weights, float), dens, leftX, Rgrid, 1.0 / dx,
+                         1 / dr) * VGrid[None, :, None]
+    return dens
+
+
+def phase_space(coords_halfstep, momenta, weights):
+    """diagnostics.py:94-103: PartsPack of the particles with
+    (1 + u**2).sum(0) > 20**2."""
+    indx = np.nonzero((1. + momenta ** 2).sum(0) > 20. ** 2)[0]
+    return np.concatenate((coords_halfstep[:, indx], momenta[:, indx], weights[indx][None, :]), axis=0)
+
+
+def intens_profo(Fld, NO):
+    """utils.f90:18-57 (f2py name intens_profo): Fld(0:nx, 0:nr, -nkO:nkO, 3);
+    PWR_RO(NO, 1:nr) = sum over l and ix of |sum_m phaseO(m) Fld(ix, ir, m, l)|^2
+    with the phases formed by the reference's recurrences (Os(iO) =
+    Os(iO-1) * e^{2 pi i/(NO-1)}; phaseO(+-k) by repeated products)."""
+    nm = Fld.shape[2]
+    if nm % 2 == 0:
+        raise ValueError("intens_profo: the mode axis is -nkO..nkO (odd length)")
+    nkO = (nm - 1) // 2
+    nr = Fld.shape[1] - 1
+    Os = np.zeros(NO, complex)
+    Os[0] = 1.0
+    phase_p = np.cos(2.0 * np.pi / (NO - 1)) + 1j * np.sin(2.0 * np.pi / (NO - 1))
+    for iO in range(1, NO):
+        Os[iO] = Os[iO - 1] * phase_p
+    PWR = np.zeros((NO, nr))
+    for iO in range(NO):
+        pp, pm = Os[iO], 1.0 / Os[iO]
+        phaseO = np.ones(nm, complex)
+        for k in range(1, nkO + 1):
+            phaseO[nkO + k] = phaseO[nkO + k - 1] * pp
+            phaseO[nkO - k] = phaseO[nkO - k + 1] * pm
+        s = np.einsum("m,xrml->xrl", phaseO, Fld[:, 1:, :, :])
+        PWR[iO] = (np.abs(s) ** 2).sum(axis=(0, 2))
+    return PWR

@@ -369,6 +369,20 @@ class Specie:
 class ChimeraRun:
     """chimera_main.py:23-318."""
 
+    def shuffle_particles(self, seed):
+        """Put every species' particles in another (seeded) order.  Not in the
+        reference: the same run with its particles shuffled gives the oracle's
+        own noise floor, the spread that the order of the floating-point sums
+        over particles alone makes in fields and moments."""
+        rng = np.random.RandomState(seed)
+        for sp in self.Particles:
+            n = sp.Data["coords"].shape[1]
+            perm = rng.permutation(n)
+            for key in ("coords", "coords_halfstep", "momenta", "weights", "EB"):
+                v = sp.Data.get(key)
+                if isinstance(v, np.ndarray) and v.shape[-1] == n:
+                    sp.Data[key] = np.asfortranarray(v[..., perm])
+
     def __init__(self, SimComps):
         self.Particles = SimComps.get("Particles", ())
         self.Solvers = SimComps.get("Solvers", ())

@@ -200,3 +200,154 @@ def test_device_pwr_out_matches_oracle(precision, port, tmp_path):
     assert err < (1e-12 if precision == "fp64" else 2e-6)
     dg.do_diags(1)
     assert np.allclose(np.loadtxt(tmp_path / "pwrX_0.txt"), p, rtol=1e-15, atol=0)
+
+
+# ------------------------------------------------------------------ dumps, density, phase space, spot
+
+def test_oracle_intens_profo_closed_forms():
+    """m = 0 alone: the same power at every angle, sum |F|^2; modes +-1 of
+    equal amplitude a: |a e^{i th} + a e^{-i th}|^2 = 4 |a|^2 cos^2 th at the
+    angles th = 2 pi (iO - 1)/(NO - 1)."""
+    r = np.random.default_rng(21)
+    nx, nr, NO = 5, 4, 13
+    F0 = np.zeros((nx, nr + 1, 3, 3), complex)
+    F0[:, :, 1, :] = r.normal(size=(nx, nr + 1, 3)) + 1j * r.normal(size=(nx, nr + 1, 3))
+    P = D.intens_profo(F0, NO)
+    expect = (np.abs(F0[:, 1:, 1, :]) ** 2).sum(axis=(0, 2))
+    assert np.allclose(P, expect[None, :], rtol=1e-13)
+    a = r.normal(size=(nx, nr + 1, 3)) + 1j * r.normal(size=(nx, nr + 1, 3))
+    F1 = np.zeros((nx, nr + 1, 3, 3), complex)
+    F1[:, :, 0, :] = a
+    F1[:, :, 2, :] = a
+    th = 2 * np.pi * np.arange(NO) / (NO - 1)
+    expect = 4 * np.cos(th)[:, None] ** 2 * (np.abs(a[:, 1:]) ** 2).sum(axis=(0, 2))[None, :]
+    assert np.allclose(D.intens_profo(F1, NO), expect, rtol=1e-12, atol=1e-12 * expect.max())
+
+
+def test_oracle_phase_space_filter():
+    """(1 + u^2) summed over the three components > 400, i.e. |u|^2 > 397."""
+    u = np.array([[19.9, 0.0, 19.93, 0.0], [0.0, 0.0, 0.0, 12.0], [0.0, 0.0, 0.0, 15.0]])
+    x = np.arange(12.0).reshape(3, 4)
+    w = np.array([1.0, 2.0, 3.0, 4.0])
+    pp = D.phase_space(x, u, w)
+    assert pp.shape == (7, 1) and pp[6, 0] == 3.0 and np.array_equal(pp[:3, 0], x[:, 2])
+    u[0, 0] = 19.95
+    assert D.phase_space(x, u, w).shape == (7, 2)
+
+
+def test_oracle_dens_on_grid_charge(port):
+    """One particle of weight w at radius r0 in the middle of a cell: the m = 0
+    density times the cell volumes 2 pi dx dr r sums to w (the ghost fold keeps
+    the charge of the rows around the axis)."""
+    dx, dr, Nx, Nr = 0.1, 0.3, 16, 8
+    Rg = dr * (np.arange(Nr) - 0.5)
+    leftX = -0.8
+    x = np.array([[leftX + 7.5 * dx], [0.0], [2.2 * dr]])
+    d = D.dens_on_grid(port, x, np.array([0.7]), leftX, Rg, dx, dr, Nx, 1)
+    vol = 2 * np.pi * dx * dr * np.where(Rg > 0, Rg, 0.0)
+    assert np.isclose((d[:, :, 0].real * vol[None, :]).sum(), 0.7, rtol=1e-13)
+
+
+def _diag_run(precision, steps=3):
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    run = W.build(G, W.with_precision(W.lpa_small(Nx=128, Nkr=32, nko=1), precision), 5)
+    for i in range(1, steps + 1):
+        run.make_step(i)
+    return run
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_density_and_dumps(precision, port, tmp_path):
+    from chimera_amd.diagnostics import Diagnostics
+    run = _diag_run(precision)
+    sol, el = run.Solvers[0], run.Particles[0]
+    diags = ({"Type": "Fields", "Step": 1}, {"Type": "FieldsFB", "Step": 2}, {"Type": "Density", "Step": 1},
+             {"Type": "Particles", "Step": 1})
+    dg = Diagnostics(run, diags=diags, out_folder=str(tmp_path))
+    A = el.Args
+    for nko in (0, 1, 2):
+        dens = el.get_dens_on_grid(nko)
+        ref = D.dens_on_grid(port, el.host("coords").astype(float), el.host("weights").astype(float),
+                             float(A["leftX"]), A["Rgrid"], A["dx"], A["dr"], A["Nx"], nko)
+        err = np.linalg.norm(dens - ref) / np.linalg.norm(ref)
+        print(f"case: get_dens_on_grid {precision} nko={nko}: {err:.2e}", flush=True)
+        assert dens.shape == ref.shape and err < 1e-13
+    f = dg.fld_out({"Features": {"Return": True}})[0]
+    assert np.array_equal(f, sol.Data["EB"].numpy()[:, 1:])
+    g = dg.fldfb_out({"Features": {"Return": True}})[0]
+    assert np.array_equal(g, sol.Data["EG_fb"].numpy())
+    dg.do_diags(1)
+    assert np.array_equal(np.load(tmp_path / "ee_0_0000001.npy"), f)
+    d1 = np.load(tmp_path / "dens_0_0000001.npy")
+    assert d1.shape == (A["Nx"], A["Nr"] - 1, 2)
+    assert (tmp_path / "phs0_0000001.npy").exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_phase_space_matches_oracle(precision):
+    """A bunch with momenta spread over the filter's edge (|u| near 20): the
+    selected particles, their order and their values."""
+    from chimera_amd.diagnostics import Diagnostics
+    run = _diag_run(precision, steps=1)
+    el = run.Particles[0]
+    r = np.random.default_rng(31)
+    u = r.normal(size=(3, el.np)) * 12.0
+    u[:, ::7] = [[19.93], [0.0], [0.0]]           # just above the edge (|u|^2 = 397.2)
+    u[:, 3::11] = [[19.92], [0.0], [0.0]]         # just below (396.8)
+    rt = el.Data["momenta"].dtype
+    buf = el.Data["momenta"].numpy()
+    buf[:el.np] = u.T.astype(rt)
+    el.Data["momenta"].upload(buf)
+    pp = Diagnostics(run).phs_out({"Features": {"Return": True}})[0]
+    ref = D.phase_space(el.host("coords_halfstep").astype(float), el.host("momenta").astype(float),
+                        el.host("weights").astype(float))
+    print(f"case: phs_out {precision}: {pp.shape[1]} of {el.np} particles", flush=True)
+    assert pp.shape == ref.shape and 0 < pp.shape[1] < el.np
+    assert np.array_equal(pp, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_intens_profo_and_spot(precision, port):
+    """intens_profO through the C ABI on a random field with modes -1..1
+    against the oracle; then the 'Spot' profile of pwr_out on a single-mode
+    run and get_spot_cartesian (the reference's griddata on it)."""
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    from chimera_amd._lib import lib
+    from chimera_amd.diagnostics import Diagnostics
+    r = np.random.default_rng(41)
+    nkx, nr, nm, NO = 48, 12, 3, 60
+    F = np.asfortranarray(r.normal(size=(nkx, nr + 1, nm, 3)) + 1j * r.normal(size=(nkx, nr + 1, nm, 3)))
+    out = np.zeros((NO, nr), order="F")
+    if precision == "fp64":
+        st = lib().chimera_intens_profo(out.ctypes.data, F.ctypes.data, NO, nkx, nr, nm)
+        ref = D.intens_profo(F, NO)
+    else:
+        F8 = np.asfortranarray(F.astype(np.complex64))
+        st = lib().chimera_intens_profo_c8(out.ctypes.data, F8.ctypes.data, NO, nkx, nr, nm)
+        ref = D.intens_profo(F8.astype(np.complex128), NO)
+    assert st == 0
+    err = np.max(np.abs(out - ref)) / np.max(np.abs(ref))
+    print(f"case: intens_profo {precision}: {err:.2e}", flush=True)
+    assert err < 1e-13
+    assert lib().chimera_intens_profo(out.ctypes.data, F.ctypes.data, NO, nkx, nr, 2) != 0   # even mode count
+    run = W.build(G, W.with_precision(W.lpa_small(Nx=128, Nkr=32, nko=0), precision), 5)
+    run.make_step(1)
+    dg = Diagnostics(run)
+    pw, prof = dg.pwr_out({"Features": {"Return": True, "Spot": True}})[0]
+    sol = run.Solvers[0]
+    A = sol.Args
+    e = np.asfortranarray(sol.Data["EG_fb"].numpy().astype(np.complex128))
+    eb = port.fb_eb_out(np.zeros((A["Nx"], A["OutFull"].shape[1] + 1, 1, 6), complex, order="F"), e,
+                        np.asfortranarray(e[..., :3]), float(A["leftX"]), A["kx"],
+                        A["OutFull"].astype(sol.rt).astype(np.float64))
+    ref = D.intens_profo(np.asfortranarray(eb[..., :3]), NO)
+    err = np.max(np.abs(prof - ref)) / np.max(np.abs(ref))
+    print(f"case: pwr_out Spot {precision}: {err:.2e}", flush=True)
+    assert err < (1e-12 if precision == "fp64" else 2e-6)
+    val, ext = dg.get_spot_cartesian(bins=(40, 40))
+    assert val.shape == (40, 40) and np.all(np.isfinite(val)) and ext[1] == A["RgridFull"].max()

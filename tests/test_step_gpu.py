@@ -70,6 +70,19 @@ def compare_snaps(g, c, tol_fields, tol_parts, what):
         assert out[key] <= tol_parts, (what, key, out[key])
 
 
+def noise_floor(c, cs, what):
+    """Print the oracle's own noise floor beside a comparison: the oracle run
+    c against the same run with its particles shuffled (cs), either runs or
+    their snapshots; fields and moments only, since the particle order
+    differs."""
+    a = c if isinstance(c, dict) else snapshot_cpu(c)
+    b = cs if isinstance(cs, dict) else snapshot_cpu(cs)
+    out = {k: rel_l2(b[k], a[k]) for k in ("EG_fb", "J_fb", "EB")}
+    out["moments"] = moments_err(moments(b["momenta"], b["weights"]), moments(a["momenta"], a["weights"]))
+    say(f"{what} oracle noise floor (shuffled particle order): " + " ".join(f"{k}={v:.2e}" for k, v in out.items()))
+    return out
+
+
 def compare(gpu_run, cpu_run, tol_fields, tol_parts, what):
     compare_snaps(snapshot_gpu(gpu_run), snapshot_cpu(cpu_run), tol_fields, tol_parts, what)
 
@@ -134,9 +147,14 @@ def test_cfg0_one_step():
     g, c = _runs(spec, seed=2024)
     assert g.Particles[0].np == 515088
     compare(g, c, 1e-12, 1e-12, "cfg0 half step")
+    from oracle import driver as O
+    cs = W.build(O, spec, seed=2024)
+    cs.shuffle_particles(1)
     g.make_step(1)
     c.make_step(1)
+    cs.make_step(1)
     compare(g, c, 1e-12, 1e-12, "cfg0 step 1")
+    noise_floor(c, cs, "cfg0 step 1")
 
 
 @pytest.mark.timeout(600)
@@ -159,11 +177,15 @@ def test_cfg0_100_steps():
         g.make_step(i)
         if i in checks:
             snaps[i] = snapshot_gpu(g)
-    say("  GPU run done; oracle run (about 0.2 s per step)")
+    say("  GPU run done; oracle run and the same run with shuffled particles (about 0.2 s per step each)")
     c = W.build(O, spec, 99)
+    c2 = W.build(O, spec, 99)
+    c2.shuffle_particles(3)
     for i in range(1, 101):
         c.make_step(i)
+        c2.make_step(i)
         if i in checks:
+            noise_floor(c, c2, f"  step {i}")
             cs = snapshot_cpu(c)
             tf = 1e-12 if i == 1 else 1e-9
             out = {k: rel_l2(snaps[i][k], cs[k]) for k in ("EG_fb", "J_fb", "EB")}
