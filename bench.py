@@ -71,8 +71,6 @@ def algorithmic(fam, P, Gr, Gk, Nx, Nkr, M, K, prec="fp64"):
         return P * (24 + 7 * s_u) + 6 * Gr * s_f, "B", "hbm"
     if fam == "dep_curr":         # fused push + J + rho: x, u, w read, x and x_half written;
         return P * (72 + 4 * s_u) + 4 * Gr * 2 * s_f, "B", "hbm"   # J (3 components) and rho read+written once
-    if fam == "dep_dens":         # x + w read; rho read+written once
-        return P * (24 + s_u) + Gr * 2 * s_f, "B", "hbm"
     if fam == "psatd":            # E,G,J,grad rho_n, grad rho_n+1 read, E,G written, 10 table reals
         return Gk * (21 * s_f + 10 * s_u), "B", "hbm"
     return None
@@ -407,6 +405,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", type=int, default=1, help="1: GEMM tile shapes chosen by timing (first use)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-slab", type=int, default=8, help="the CPU baseline runs 1/N of the workload's x extent")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -434,6 +433,9 @@ def main():
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
         return
+    if args.autotune:
+        from chimera_amd._lib import lib as _l
+        _l().chimera_set_autotune(1, None)          # tile shapes timed on first use (during the warmup)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the PIC step of one simulation runs on one MI355X and this build "
               f"has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)

@@ -112,7 +112,11 @@ class Simulation:
             g = self.graphs[parity] = h.value
         else:
             self.run.host_step(i)
-        check("chimera_graph_launch", L.chimera_graph_launch(g))
+        # like every other library call, under the device lock: no builder
+        # thread allocates, copies or captures while the graph is submitted
+        with DEVICE_LOCK:
+            st = L.chimera_graph_launch(g)
+        check("chimera_graph_launch", st)
 
     def advance(self, n=1):
         """Queue up to n more steps (fewer at the end of the run); returns how many."""

@@ -18,6 +18,7 @@ its make_halfstep therefore sees electrons without ions).
 """
 from __future__ import annotations
 
+import ctypes
 import os
 import time
 
@@ -255,7 +256,7 @@ class ChimeraRun:
             if species.np == 0 or "Still" in species.Args["Features"]:
                 continue
             A = species.Args
-            und = species.undulator_params()
+            devs = species.device_descriptors(istep)
             push_dt = float(A["push_fact"] * dt_frac * A["TimeStep"])
             frame = None
             if self.Solvers and "NoPush" not in self.Solvers[0].Args["Features"]:
@@ -263,8 +264,10 @@ class ChimeraRun:
                 EB, g, frame = solver.Data["EB"].ptr, solver.geom_args(), solver.frame_ptr(1)
             else:
                 EB, g = 0, (0.0, 0, 1.0, 1.0, 0.0, -1.0, 1, 1, 0)   # rmax < 0: nothing gathered
-            call(self._pfn("chimera_soa_gather_push"), species.ptr("coords"), species.ptr("momenta"), species.ptr("weights"),
-                 species.ld, species.np, EB, *g, push_dt, und.ctypes.data if und is not None else None, frame)
+            recs, t = devs if devs is not None else (None, 0.0)
+            call(self._pfn("chimera_soa_gather_push_dev"), species.ptr("coords"), species.ptr("momenta"),
+                 species.ptr("weights"), species.ld, species.np, EB, *g, push_dt,
+                 ctypes.addressof(recs) if recs is not None else None, len(recs) if recs is not None else 0, t, frame)
 
     # ------------------------------------------------------------------ moving frames (chimera_main.py:250-318)
     def damp_fields(self, wind):
@@ -350,6 +353,12 @@ class ChimeraRun:
                 out.append("the still background is re-deposited after each move")
         if self.Solvers and "StaticKick" in self.Solvers[0].Args["Features"]:
             out.append("StaticKick reads the mean momentum back to the host on every step")
+        for sp in self.Particles:
+            for dev in getattr(sp, "Devices", ()):
+                fn = dev[0]
+                name = (fn if isinstance(fn, str) else getattr(fn, "__name__", "")).lower()
+                if name in ("planewave", "gaussbeam"):
+                    out.append(f"the {name} device depends on the step's time")
         return out
 
     def enable_device_frame(self):

@@ -12,7 +12,11 @@
 // outputs, each a complex combination of the accumulators plus the diagonal
 // i*kx term of the operator.  All jobs of one operator run in one launch
 // (grid.z = job), and no two jobs write the same output.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <string>
 #include <complex>
 #include <map>
 #include <mutex>
@@ -939,6 +943,179 @@ static int gemm_tile(int nx, int maxN, size_t njobs, bool has_w4) {
     return GT_Q;
 }
 
+template <int NACC, int WN, int GX = GB_X>
+static int32_t launch_gemm_variant_f32(const FbJobT<float>* dj, size_t njobs, int maxN, const FbGemmArgs& g);
+template <int NACC, int WN, int GX = GB_X>
+static int32_t launch_gemm_variant_bf16(const FbJobT<float>* dj, size_t njobs, int maxN, const FbGemmArgs& g);
+
+// Form of the mixed-precision GEMMs: split-BF16 (six products, the default)
+// or FP32 MFMA; process-wide, set by chimera_set_mixed_gemm.
+static std::atomic<int> g_mixed_gemm{MIXED_GEMM_BF16X6};
+
+// One launch of a job table in tile shape `tile` (GT_*) with NACC
+// accumulators; EUNSUP for a shape the form has no kernel for.
+static int32_t gemm_tile_launch(const FbJob* dj, size_t nj, int maxN, const FbGemmArgs& g, int nacc, int tile) {
+    if (nacc == 1) {
+        switch (tile) {
+            case GT_W4: return launch_gemm_variant<1, 4>(dj, nj, maxN, g);
+            case GT_W2: return launch_gemm_variant<1, 2>(dj, nj, maxN, g);
+            case GT_W1: return launch_gemm_variant<1, 1>(dj, nj, maxN, g);
+            case GT_H: return launch_gemm_variant<1, 1, GB_X / 2>(dj, nj, maxN, g);
+            default: return launch_gemm_variant<1, 1, GB_X / 4>(dj, nj, maxN, g);
+        }
+    }
+    switch (tile) {
+        case GT_W4: return ST_EUNSUP;
+        case GT_W2: return launch_gemm_variant<2, 2>(dj, nj, maxN, g);
+        case GT_W1: return launch_gemm_variant<2, 1>(dj, nj, maxN, g);
+        case GT_H: return launch_gemm_variant<2, 1, GB_X / 2>(dj, nj, maxN, g);
+        default: return launch_gemm_variant<2, 1, GB_X / 4>(dj, nj, maxN, g);
+    }
+}
+
+static int32_t gemm_tile_launch(const FbJobT<float>* dj, size_t nj, int maxN, const FbGemmArgs& g, int nacc,
+                                int tile) {
+    if (g_mixed_gemm.load() == MIXED_GEMM_BF16X6) {
+        if (nacc == 1) {
+            switch (tile) {
+                case GT_W4: return launch_gemm_variant_bf16<1, 4>(dj, nj, maxN, g);
+                case GT_W2: return launch_gemm_variant_bf16<1, 2>(dj, nj, maxN, g);
+                case GT_W1: return launch_gemm_variant_bf16<1, 1>(dj, nj, maxN, g);
+                case GT_H: return launch_gemm_variant_bf16<1, 1, GB_X / 2>(dj, nj, maxN, g);
+                default: return launch_gemm_variant_bf16<1, 1, GB_X / 4>(dj, nj, maxN, g);
+            }
+        }
+        switch (tile) {
+            case GT_W4: return ST_EUNSUP;          // two accumulators of 128 columns: no registers left
+            case GT_W2: return launch_gemm_variant_bf16<2, 2>(dj, nj, maxN, g);
+            case GT_W1: return launch_gemm_variant_bf16<2, 1>(dj, nj, maxN, g);
+            case GT_H: return launch_gemm_variant_bf16<2, 1, GB_X / 2>(dj, nj, maxN, g);
+            default: return launch_gemm_variant_bf16<2, 1, GB_X / 4>(dj, nj, maxN, g);
+        }
+    }
+    if (nacc == 1) {
+        switch (tile) {
+            case GT_W4: return launch_gemm_variant_f32<1, 4>(dj, nj, maxN, g);
+            case GT_W2: return launch_gemm_variant_f32<1, 2>(dj, nj, maxN, g);
+            case GT_W1: return launch_gemm_variant_f32<1, 1>(dj, nj, maxN, g);
+            case GT_H: return launch_gemm_variant_f32<1, 1, GB_X / 2>(dj, nj, maxN, g);
+            default: return launch_gemm_variant_f32<1, 1, GB_X / 4>(dj, nj, maxN, g);
+        }
+    }
+    switch (tile) {
+        case GT_W4: return launch_gemm_variant_f32<2, 4>(dj, nj, maxN, g);
+        case GT_W2: return launch_gemm_variant_f32<2, 2>(dj, nj, maxN, g);
+        case GT_W1: return launch_gemm_variant_f32<2, 1>(dj, nj, maxN, g);
+        case GT_H: return launch_gemm_variant_f32<2, 1, GB_X / 2>(dj, nj, maxN, g);
+        default: return launch_gemm_variant_f32<2, 1, GB_X / 4>(dj, nj, maxN, g);
+    }
+}
+
+// ---------------------------------------------------------------- tile autotuner
+//
+// With autotuning on (chimera_set_autotune), the first launch of a GEMM shape
+// -- (element type, GEMM form, accumulators, nx, columns, jobs, K, segments,
+// buffer-load path) -- times every tile shape the form has a kernel for and
+// keeps the fastest for that shape; later launches of the shape use it.  The
+// timed launches run the real job table with every output redirected to a
+// scratch buffer (the operands are only read), three launches per shape after
+// one untimed, HIP events on the library stream; the shape with the least
+// mean time wins.  Tuned shapes can be kept in a text file
+// (chimera_set_autotune's path: one "key... tile mean_ms" line per shape,
+// read once and appended to).  Under stream capture, or when the scratch
+// cannot grow, a shape not yet tuned takes the block-count rule (gemm_tile).
+// Every tile shape sums each output over K in the same order, so the choice
+// changes the time of a launch, not its result.
+struct TuneKey {
+    int v[9];
+    bool operator<(const TuneKey& o) const { return std::lexicographical_compare(v, v + 9, o.v, o.v + 9); }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, int> g_tuned;
+static std::atomic<int> g_autotune{0};
+static std::string g_tune_path;
+static bool g_tune_loaded = false;
+
+static void tune_load_locked() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    if (g_tune_path.empty()) return;
+    FILE* f = fopen(g_tune_path.c_str(), "r");
+    if (!f) return;
+    TuneKey k;
+    int tile;
+    double ms;
+    while (fscanf(f, "%d %d %d %d %d %d %d %d %d %d %lf", &k.v[0], &k.v[1], &k.v[2], &k.v[3], &k.v[4], &k.v[5],
+                  &k.v[6], &k.v[7], &k.v[8], &tile, &ms) == 11)
+        if (tile >= GT_W4 && tile <= GT_Q) g_tuned[k] = tile;
+    fclose(f);
+}
+
+template <typename J>
+static int gemm_choose_tile(const J* dj, const std::vector<J>& js, int maxN, const FbGemmArgs& g, int nacc,
+                            int rbytes) {
+    const size_t nj = js.size();
+    const int rule = gemm_tile(g.nx, maxN, nj, rbytes == 4 || nacc == 1);
+    if (g_autotune.load() == 0) return rule;
+    int maxK = 0, nseg = 0;
+    for (const auto& j : js) {
+        maxK = j.K > maxK ? j.K : maxK;
+        nseg = j.nseg > nseg ? j.nseg : nseg;
+    }
+    const int form = rbytes == 4 ? g_mixed_gemm.load() : 2;
+    const TuneKey key = {{rbytes, form, nacc, g.nx, maxN, (int)nj, maxK, nseg, g.buf_ok}};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load_locked();
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return it->second;
+    }
+    if (capturing()) return rule;
+    // the job table with every output in one scratch buffer of ldo * maxN elements
+    std::vector<J> tj(js);
+    void* sink = scratch(20, (size_t)g.ldo * maxN * 2 * rbytes);
+    if (!sink) return rule;
+    for (auto& j : tj)
+        for (int o = 0; o < j.nout; o++) j.out[o].p = static_cast<decltype(j.out[o].p)>(sink);
+    const J* tdj = static_cast<const J*>(device_table(tj.data(), tj.size() * sizeof(J)));
+    if (!tdj) return rule;
+    (void)dj;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return rule;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return rule;
+    }
+    int best = rule;
+    float best_ms = 1e30f;
+    for (int t = GT_W4; t <= GT_Q; t++) {
+        if (gemm_tile_launch(tdj, nj, maxN, g, nacc, t) != ST_OK) continue;     // warm-up (or no such kernel)
+        (void)hipEventRecord(e0, stream());
+        bool ok = true;
+        for (int r = 0; r < 3 && ok; r++) ok = gemm_tile_launch(tdj, nj, maxN, g, nacc, t) == ST_OK;
+        (void)hipEventRecord(e1, stream());
+        if (!ok || hipEventSynchronize(e1) != hipSuccess) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = best;
+    if (!g_tune_path.empty()) {
+        if (FILE* f = fopen(g_tune_path.c_str(), "a")) {
+            for (int i = 0; i < 9; i++) fprintf(f, "%d ", key.v[i]);
+            fprintf(f, "%d %.6f\n", best, best_ms / 3.0);
+            fclose(f);
+        }
+    }
+    return best;
+}
+
 template <>
 int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs& g_in, int family) {
     if (jobs.empty()) return ST_OK;
@@ -958,25 +1135,8 @@ int32_t launch_fb_gemm<double>(const std::vector<FbJob>& jobs, const FbGemmArgs&
         for (const auto& j : js) maxN = j.Ncol > maxN ? j.Ncol : maxN;
         const FbJob* dj = static_cast<const FbJob*>(device_table(js.data(), js.size() * sizeof(FbJob)));
         if (!dj) return ST_ENOMEM;
-        int32_t st;
-        const size_t nj = js.size();
-        if (pass == 0) {
-            switch (gemm_tile(g.nx, maxN, nj, true)) {
-                case GT_W4: st = launch_gemm_variant<1, 4>(dj, nj, maxN, g); break;
-                case GT_W2: st = launch_gemm_variant<1, 2>(dj, nj, maxN, g); break;
-                case GT_W1: st = launch_gemm_variant<1, 1>(dj, nj, maxN, g); break;
-                case GT_H: st = launch_gemm_variant<1, 1, GB_X / 2>(dj, nj, maxN, g); break;
-                default: st = launch_gemm_variant<1, 1, GB_X / 4>(dj, nj, maxN, g); break;
-            }
-        } else {
-            switch (gemm_tile(g.nx, maxN, nj, false)) {
-                case GT_W2: st = launch_gemm_variant<2, 2>(dj, nj, maxN, g); break;
-                case GT_W1: st = launch_gemm_variant<2, 1>(dj, nj, maxN, g); break;
-                case GT_H: st = launch_gemm_variant<2, 1, GB_X / 2>(dj, nj, maxN, g); break;
-                default: st = launch_gemm_variant<2, 1, GB_X / 4>(dj, nj, maxN, g); break;
-            }
-        }
-        CH_TRY(st);
+        const int tile = gemm_choose_tile<FbJob>(dj, js, maxN, g, pass + 1, 8);
+        CH_TRY(gemm_tile_launch(dj, js.size(), maxN, g, pass + 1, tile));
     }
     return ST_OK;
 }
@@ -1150,7 +1310,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     gemm_epilogue<float, NACC, WN, NI>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
 }
 
-template <int NACC, int WN, int GX = GB_X>
+template <int NACC, int WN, int GX>
 static int32_t launch_gemm_variant_f32(const FbJobT<float>* dj, size_t njobs, int maxN, const FbGemmArgs& g) {
     constexpr int BN = 32 * WN;
     constexpr size_t lds = gb_lds_bytes_f32(WN, gbf_k(NACC, WN), GX);
@@ -1163,6 +1323,248 @@ static int32_t launch_gemm_variant_f32(const FbJobT<float>* dj, size_t njobs, in
     dim3 grid((unsigned)((g.nx + GX - 1) / GX), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
     hipLaunchKernelGGL((k_fb_gemm_f32<NACC, WN, GX>), grid, GB_THREADS, lds, stream(), dj, g);
     CH_LAUNCH_CHECK("k_fb_gemm_f32");
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------- split-BF16 (mixed precision)
+//
+// The FP32 GEMM of the mixed path on the BF16 matrix cores: every FP32 value
+// v of A (fields) and D (matrices) is split exactly into three BF16 pieces,
+// v = h + m + l (h = RNE(v) to 8 significant bits, m = RNE(v - h), l = v - h - m;
+// the differences are exact in FP32), and
+//   A.D ~= Ah.Dh + Ah.Dm + Am.Dh + Ah.Dl + Am.Dm + Al.Dh
+// in six v_mfma_f32_16x16x32_bf16 products with FP32 accumulation.  The
+// dropped products (Am.Dl, Al.Dm, Al.Dl) are below 2^-24 of A.D, so the
+// result has the accuracy of the FP32 kernel; three products (dropping the
+// 2^-16 terms) would leave ~1e-5 relative error per transform.
+//
+// Lane layout of v_mfma_f32_16x16x32_bf16 on gfx950 (checked on the device):
+// lane l supplies A[l%16][8*(l/16) + j] and B[8*(l/16) + j][l%16], j = 0..7,
+// and holds D[4*(l/16) + i][l%16], i = 0..3 (the output layout of the FP32
+// kernel, so both share the epilogue).  A rows stay in the FP32 kernel's
+// order (re, im of x interleaved).  K is staged 32 at a time; the pieces are
+// kept in LDS with K contiguous per row (A) and per column (D), so that each
+// operand is one 16-byte LDS read, rows 80 bytes apart (16-lane groups of a
+// 16-byte read then hit disjoint banks).  One LDS buffer: the next stage is
+// loaded into registers while the MFMAs of this one run.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int GBB_K = 32, GBB_RS = GBB_K + 8;          // K per stage; LDS row stride in BF16
+
+__device__ __forceinline__ uint32_t bf16_rne_bits(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+}
+// v = h + m + l exactly (finite v of normal magnitude); pieces as BF16 bit patterns
+__device__ __forceinline__ void split3_bf16(float v, uint16_t& h, uint16_t& m, uint16_t& l) {
+    const uint32_t hb = bf16_rne_bits(v);
+    const float r1 = v - __uint_as_float(hb);
+    const uint32_t mb = bf16_rne_bits(r1);
+    const float r2 = r1 - __uint_as_float(mb);
+    h = (uint16_t)(hb >> 16);
+    m = (uint16_t)(mb >> 16);
+    l = (uint16_t)(bf16_rne_bits(r2) >> 16);
+}
+
+constexpr size_t gb_lds_bytes_bf16(int WN, int GX) {
+    return 3 * (size_t)(2 * GX + 32 * WN) * GBB_RS * sizeof(uint16_t);
+}
+
+template <int NACC, int WN, int GX>
+__global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_bf16(const FbJobT<float>* __restrict__ jobs, FbGemmArgs g) {
+    typedef FbSegT<float> Seg;
+    constexpr int NI = GX / 16, BN = 32 * WN, ROWS = 2 * GX;
+    constexpr int KA = GBB_K * GX / GB_THREADS;          // consecutive k per thread in the A fetch
+    constexpr int DC = (BN * 4 + GB_THREADS - 1) / GB_THREADS;   // D columns per thread (8 k each)
+    const FbJobT<float>& J = jobs[blockIdx.z];
+    const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
+    const int n0 = blockIdx.y * BN;
+    if (n0 >= Ncol) return;
+    const int x0 = blockIdx.x * GX;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
+
+    extern __shared__ __align__(16) unsigned char gbb_smem[];
+    uint16_t* const As = reinterpret_cast<uint16_t*>(gbb_smem);     // [3][ROWS][GBB_RS]
+    uint16_t* const Ds = As + 3 * ROWS * GBB_RS;                     // [3][BN][GBB_RS]
+
+    flt4 acc0[NI][WN], acc1[NI][NACC == 2 ? WN : 1];
+#pragma unroll
+    for (int i = 0; i < NI; i++)
+#pragma unroll
+        for (int j = 0; j < WN; j++) acc0[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+
+    const int stages_per_seg = (K + GBB_K - 1) / GBB_K;
+    const int nstages = stages_per_seg * nseg;
+    const Seg S0 = J.seg[0];
+    const Seg S1 = J.seg[nseg > 1 ? 1 : 0];
+    // A fetch: x = x0 + tid % GX, k = k0 + (tid / GX) * KA + q, q < KA
+    const int xl = tid % GX, kb = (tid / GX) * KA;
+    // D fetch: column n0 + (tid / 4) + (GB_THREADS / 4) * c, k = k0 + 8 * (tid % 4) + q, q < 8
+    const int dk = 8 * (tid % 4), dn = tid / 4;
+    float2 ra[KA], rb[KA];
+    float rd[DC][8];
+    const bool xfull = x0 + GX <= g.nx, nfull = n0 + BN <= Ncol;
+    const uint32_t la0 = seg_lane_a(S0, x0 + xl, g.nx, g.lda, kb), la1 = seg_lane_a(S1, x0 + xl, g.nx, g.lda, kb);
+    const bool plain0 = seg_plain(S0), plain1 = seg_plain(S1);
+    auto fetch_seg = [&](const Seg& S, int k0, uint32_t la) __attribute__((always_inline)) {
+        if (g.buf_ok && xfull && nfull && k0 + GBB_K <= K) {
+            const uint32_t row = (uint32_t)(g.lda * sizeof(float2));
+            const __amdgpu_buffer_rsrc_t rs_a = ch_rsrc(S.a0);
+#pragma unroll
+            for (int q = 0; q < KA; q++) ra[q] = ldb_c(rs_a, la, (uint32_t)(k0 + q) * row, float2{});
+            if (S.a1) {
+                const __amdgpu_buffer_rsrc_t rs_b = ch_rsrc(S.a1);
+#pragma unroll
+                for (int q = 0; q < KA; q++) rb[q] = ldb_c(rs_b, la, (uint32_t)(k0 + q) * row, float2{});
+            }
+            const __amdgpu_buffer_rsrc_t rs_d = ch_rsrc(S.D);
+#pragma unroll
+            for (int c = 0; c < DC; c++) {
+                const int nl = dn + (GB_THREADS / 4) * c;
+                const uint32_t off = (uint32_t)((k0 + dk + S.ldd * (int64_t)(n0 + (nl < BN ? nl : 0))) * sizeof(float));
+#pragma unroll
+                for (int q = 0; q < 8; q++) rd[c][q] = ldb_r(rs_d, off, (uint32_t)(q * sizeof(float)), float{});
+            }
+            return;
+        }
+        const int64_t xg = x0 + xl;
+#pragma unroll
+        for (int q = 0; q < KA; q++) {
+            const bool ok = xg < g.nx && k0 + kb + q < K;
+            ra[q] = ok ? gld2(S.a0, seg_off(S, xg, k0 + kb + q, g.nx, g.lda)) : cmkf(0.f, 0.f);
+        }
+        if (S.a1) {
+#pragma unroll
+            for (int q = 0; q < KA; q++) {
+                const bool ok = xg < g.nx && k0 + kb + q < K;
+                rb[q] = ok ? gld2(S.a1, seg_off(S, xg, k0 + kb + q, g.nx, g.lda)) : cmkf(0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < DC; c++) {
+            const int nl = dn + (GB_THREADS / 4) * c;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int kk = k0 + dk + q;
+                rd[c][q] = (nl < BN && kk < K && n0 + nl < Ncol) ? gld1(S.D, (int64_t)kk + S.ldd * (int64_t)(n0 + nl))
+                                                                  : 0.f;
+            }
+        }
+    };
+    auto fetch = [&](int t) __attribute__((always_inline)) {
+        const int k0 = (t % stages_per_seg) * GBB_K;
+        if (t < stages_per_seg) fetch_seg(S0, k0, la0);
+        else fetch_seg(S1, k0, la1);
+    };
+    // split and store: A rows 2*xl (re) and 2*xl + 1 (im), k = kb..kb+KA-1; D columns, k = dk..dk+7
+    auto stash_seg = [&](const Seg& S, bool plain) __attribute__((always_inline)) {
+        uint16_t hr[KA], mr[KA], lr[KA], hi[KA], mi[KA], li[KA];
+#pragma unroll
+        for (int q = 0; q < KA; q++) {
+            const float2 v = plain ? ra[q] : seg_finish(S, ra[q], rb[q]);
+            split3_bf16(v.x, hr[q], mr[q], lr[q]);
+            split3_bf16(v.y, hi[q], mi[q], li[q]);
+        }
+        uint16_t* const a_re = As + (2 * xl) * GBB_RS + kb;
+        uint16_t* const a_im = a_re + GBB_RS;
+        constexpr int PA = ROWS * GBB_RS;
+#pragma unroll
+        for (int q = 0; q < KA; q++) {
+            a_re[q] = hr[q];
+            a_re[PA + q] = mr[q];
+            a_re[2 * PA + q] = lr[q];
+            a_im[q] = hi[q];
+            a_im[PA + q] = mi[q];
+            a_im[2 * PA + q] = li[q];
+        }
+        constexpr int PD = BN * GBB_RS;
+#pragma unroll
+        for (int c = 0; c < DC; c++) {
+            const int nl = dn + (GB_THREADS / 4) * c;
+            if (nl >= BN) continue;
+            uint16_t* const d = Ds + nl * GBB_RS + dk;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                uint16_t h, m, l;
+                split3_bf16(rd[c][q], h, m, l);
+                d[q] = h;
+                d[PD + q] = m;
+                d[2 * PD + q] = l;
+            }
+        }
+    };
+    auto stash = [&](int t) __attribute__((always_inline)) {
+        if (t < stages_per_seg) stash_seg(S0, plain0);
+        else stash_seg(S1, plain1);
+    };
+    auto mfma_stage = [&](flt4 (&acc)[NI][WN]) __attribute__((always_inline)) {
+        constexpr int PA = ROWS * GBB_RS, PD = BN * GBB_RS;
+        const int ko = 8 * (lane >> 4);
+        bf16x8 ah[NI], am[NI], al[NI];
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const uint16_t* a = As + (wr * GX + 16 * i + (lane & 15)) * GBB_RS + ko;
+            ah[i] = *reinterpret_cast<const bf16x8*>(a);
+            am[i] = *reinterpret_cast<const bf16x8*>(a + PA);
+            al[i] = *reinterpret_cast<const bf16x8*>(a + 2 * PA);
+        }
+#pragma unroll
+        for (int j = 0; j < WN; j++) {
+            const uint16_t* d = Ds + (wc * 16 * WN + 16 * j + (lane & 15)) * GBB_RS + ko;
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(d);
+            const bf16x8 bm = *reinterpret_cast<const bf16x8*>(d + PD);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(d + 2 * PD);
+#pragma unroll
+            for (int i = 0; i < NI; i++) {
+                flt4 c = acc[i][j];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[i], bm, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[i], bh, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bm, c, 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, c, 0, 0, 0);
+            }
+        }
+    };
+    auto pipe = [&](flt4 (&acc)[NI][WN], int tb, int te) __attribute__((always_inline)) {
+        for (int t = tb; t < te; t++) {
+            if (t + 1 < nstages) fetch(t + 1);
+            mfma_stage(acc);
+            __syncthreads();
+            if (t + 1 < nstages) stash(t + 1);
+            __syncthreads();
+        }
+    };
+    if (nstages > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    if constexpr (NACC == 2) {
+        pipe(acc0, 0, stages_per_seg);
+#pragma unroll
+        for (int i = 0; i < NI; i++)
+#pragma unroll
+            for (int j = 0; j < WN; j++) acc1[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+        pipe(acc1, stages_per_seg, nstages);
+    } else {
+        pipe(acc0, 0, nstages);
+    }
+    gemm_epilogue<float, NACC, WN, NI>(J, g, acc0, acc1, Ncol, n0, x0, lane, wr, wc);
+}
+
+template <int NACC, int WN, int GX>
+static int32_t launch_gemm_variant_bf16(const FbJobT<float>* dj, size_t njobs, int maxN, const FbGemmArgs& g) {
+    constexpr int BN = 32 * WN;
+    constexpr size_t lds = gb_lds_bytes_bf16(WN, GX);
+    static bool attr = false;
+    if (!attr) {
+        CH_HIP(hipFuncSetAttribute((const void*)k_fb_gemm_bf16<NACC, WN, GX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        attr = true;
+    }
+    dim3 grid((unsigned)((g.nx + GX - 1) / GX), (unsigned)((maxN + BN - 1) / BN), (unsigned)njobs);
+    hipLaunchKernelGGL((k_fb_gemm_bf16<NACC, WN, GX>), grid, GB_THREADS, lds, stream(), dj, g);
+    CH_LAUNCH_CHECK("k_fb_gemm_bf16");
     return ST_OK;
 }
 
@@ -1186,21 +1588,9 @@ int32_t launch_fb_gemm<float>(const std::vector<FbJobT<float>>& jobs, const FbGe
         const FbJobT<float>* dj =
             static_cast<const FbJobT<float>*>(device_table(js.data(), js.size() * sizeof(FbJobT<float>)));
         if (!dj) return ST_ENOMEM;
-        int32_t st;
-        const size_t nj = js.size();
-        switch (gemm_tile(g.nx, maxN, nj, true)) {
-            case GT_W4: st = pass == 0 ? launch_gemm_variant_f32<1, 4>(dj, nj, maxN, g)
-                                       : launch_gemm_variant_f32<2, 4>(dj, nj, maxN, g); break;
-            case GT_W2: st = pass == 0 ? launch_gemm_variant_f32<1, 2>(dj, nj, maxN, g)
-                                       : launch_gemm_variant_f32<2, 2>(dj, nj, maxN, g); break;
-            case GT_W1: st = pass == 0 ? launch_gemm_variant_f32<1, 1>(dj, nj, maxN, g)
-                                       : launch_gemm_variant_f32<2, 1>(dj, nj, maxN, g); break;
-            case GT_H: st = pass == 0 ? launch_gemm_variant_f32<1, 1, GB_X / 2>(dj, nj, maxN, g)
-                                      : launch_gemm_variant_f32<2, 1, GB_X / 2>(dj, nj, maxN, g); break;
-            default: st = pass == 0 ? launch_gemm_variant_f32<1, 1, GB_X / 4>(dj, nj, maxN, g)
-                                    : launch_gemm_variant_f32<2, 1, GB_X / 4>(dj, nj, maxN, g); break;
-        }
-        CH_TRY(st);
+        int tile = gemm_choose_tile<FbJobT<float>>(dj, js, maxN, g, pass + 1, 4);
+        if (tile == GT_W4 && pass == 1 && g_mixed_gemm.load() == MIXED_GEMM_BF16X6) tile = GT_W2;
+        CH_TRY(gemm_tile_launch(dj, js.size(), maxN, g, pass + 1, tile));
     }
     return ST_OK;
 }
@@ -1594,6 +1984,31 @@ using namespace chimera;
 extern "C" {
 
 int32_t chimera_fft_length_check(int32_t n) { return fft_length_check(n); }
+
+int32_t chimera_set_mixed_gemm(int32_t form) {
+    if (form != MIXED_GEMM_BF16X6 && form != MIXED_GEMM_FP32)
+        return fail(ST_EINVAL, "chimera_set_mixed_gemm: form %d (0 = split-BF16, 1 = FP32 MFMA)", form);
+    g_mixed_gemm.store(form);
+    return ST_OK;
+}
+
+int32_t chimera_get_mixed_gemm(void) { return g_mixed_gemm.load(); }
+
+int32_t chimera_set_autotune(int32_t on, const char* path) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_autotune.store(on ? 1 : 0);
+    const std::string p = path ? path : "";
+    if (p != g_tune_path) {
+        g_tune_path = p;
+        g_tune_loaded = false;
+    }
+    return ST_OK;
+}
+
+int32_t chimera_autotune_entries(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return (int32_t)g_tuned.size();
+}
 
 int32_t chimera_fb_vec_in(double* vec_fb, const double* vec, double leftX, const double* kx, const double* In,
                           int32_t nkx, int32_t nr, int32_t nkO, int32_t nkr) {

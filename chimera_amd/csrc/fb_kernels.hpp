@@ -97,6 +97,8 @@ struct FbGemmArgs {
 };
 
 enum { FB_ROT = 0, FB_GRAD = 1, FB_DIV = 2, FB_GRADDIV = 3 };
+// form of the complex64 GEMMs (chimera_set_mixed_gemm)
+enum { MIXED_GEMM_BF16X6 = 0, MIXED_GEMM_FP32 = 1 };
 
 template <typename R> int32_t launch_fft(FftArgsT<R> a);
 // EUNSUP unless the x transform handles length N

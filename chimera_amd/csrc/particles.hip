@@ -166,114 +166,100 @@ __global__ void k_eb_correction(double2* __restrict__ eb, int nx1, int nr1, int 
 
 // ----------------------------------------------------------------- devices
 
-// devices.f90:162-203.  params = (a0, lambda, X0, Lx).
-__global__ void k_undul_analytic(const double* __restrict__ coord, PStride sx, double* __restrict__ F, PStride sf,
-                                 UndulParams up, int np) {
+// f2py layouts of the drop-in entry points: coord (3, np), Fld (6, np)
+static const PStride AOS3 = {3, 1}, AOS6 = {6, 1};
+
+// Every device entry point (devices.f90): one thread per particle adds the
+// device's field (the shared ext_device_field) to the particle's row of Fld.
+__global__ void k_ext_device(const double* __restrict__ coord, PStride sx, double* __restrict__ F, PStride sf,
+                             ExtDevice d, double t, int np) {
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= np) return;
-    double f[6] = {0, 0, 0, 0, 0, 0};
-    undul_field(up, coord[sx.at(ip, 0)], coord[sx.at(ip, 1)], f);
-    F[sf.at(ip, 4)] += f[4];
-    F[sf.at(ip, 3)] += f[3];
+    ExtDevices D;
+    D.d[0] = d;
+    D.n = 1;
+    D.t = t;
+    double f[6];
+    for (int c = 0; c < 6; c++) f[c] = F[sf.at(ip, c)];
+    ext_fields<double>(D, coord[sx.at(ip, 0)], coord[sx.at(ip, 1)], coord[sx.at(ip, 2)], f);
+    for (int c = 2; c < 5; c++) F[sf.at(ip, c)] = f[c];
 }
 
-// devices.f90:18-62 (TAP = 0) and 64-115 (TAP = 1): the field map a0(2, nx)
-// (two rows: the amplitudes behind Fld(5) and Fld(4)) on nodes Xleft + (i-1) dx,
-// i = 1..nx, interpolated with quadratic weights around node ix; particles
-// outside [Xleft + dx, Xleft + (nx - 1) dx] see no field.  A node index outside
-// 1..nx contributes zero (the reference reads a0(:, 0), before the map, for
-// x < Xleft + 1.5 dx).
-template <int TAP>
-__global__ void k_undul_mapped(const double* __restrict__ coord, double* __restrict__ F,
-                               const double* __restrict__ a0, int nx, double Xleft, double dx, double dx_inv,
-                               double Xright, double ku, double Lx, double taper, double x_shift, int np) {
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ip >= np) return;
-    const double xp = coord[3 * (int64_t)ip];
-    if (xp < Xleft + dx || xp > Xright - dx) return;
-    const double yp = coord[3 * (int64_t)ip + 1];
-    const double fx = floor((xp - Xleft) * dx_inv + 0.5);
-    if (!(fx >= 0.0 && fx <= (double)nx + 1.0)) return;      // NaN or off the map: no node in range
-    const int ix = (int)fx;
-    const double ddx = (xp - Xleft) * dx_inv - ix;
-    const double s[3] = {0.5 * ((0.5 - ddx) * (0.5 - ddx)), 0.75 - ddx * ddx, 0.5 * ((0.5 + ddx) * (0.5 + ddx))};
-    double b1 = 0.0, b2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const int j = ix - 1 + k;                               // 1-based node
-        const double v1 = (j >= 1 && j <= nx) ? a0[2 * (int64_t)(j - 1)] : 0.0;
-        const double v2 = (j >= 1 && j <= nx) ? a0[2 * (int64_t)(j - 1) + 1] : 0.0;
-        b1 = b1 + s[k] * v1;
-        b2 = b2 + s[k] * v2;
+int32_t make_ext_device(ExtDevice& d, int kind, const double* p, int np, const double* map, int nmap, double a0) {
+    memset(&d, 0, sizeof d);
+    d.kind = kind;
+    static const int need[6] = {4, 5, 3, 5, 7, 8};
+    if (kind < 0 || kind > DEV_GAUSSBEAM) return fail(ST_EINVAL, "external device kind %d", kind);
+    if (p == nullptr || np < need[kind])
+        return fail(ST_EINVAL, "external device kind %d: %d parameters, %d needed", kind, np, need[kind]);
+    switch (kind) {
+        case DEV_UNDUL:
+        case DEV_UNDUL_TAPER:
+            d.p[0] = p[0];
+            d.p[1] = p[1];
+            d.p[2] = p[2];
+            d.p[3] = p[3];
+            d.p[4] = 2.0 * M_PI / p[1];
+            d.p[5] = kind == DEV_UNDUL_TAPER ? p[4] : 0.0;
+            break;
+        case DEV_MAPPED:
+        case DEV_MAPPED_TAP: {
+            if (map == nullptr || nmap < 1) return fail(ST_EINVAL, "mapped undulator: no map");
+            const double lambda = p[0], Xleft = p[1], dx = p[2];
+            const double Lx = kind == DEV_MAPPED_TAP ? p[3] : 0.0, taper = kind == DEV_MAPPED_TAP ? p[4] : 0.0;
+            d.p[0] = Xleft;
+            d.p[1] = dx;
+            d.p[2] = 1.0 / dx;
+            d.p[3] = Xleft + nmap * dx;
+            d.p[4] = 2.0 * M_PI / lambda;
+            d.p[5] = Lx;
+            d.p[6] = taper;
+            d.p[7] = 0.5 * (nmap * dx - Lx);
+            d.map = map;
+            d.nmap = nmap;
+            break;
+        }
+        case DEV_PLANEWAVE:
+            // params = (a0, lambda, X0, Lx, ramp, theta, phi0)
+            d.p[0] = p[0];
+            d.p[1] = p[2];
+            d.p[2] = p[3];
+            d.p[3] = p[4];
+            d.p[4] = sin(p[5]);
+            d.p[5] = cos(p[5]);
+            d.p[6] = 2.0 * M_PI / p[1];
+            d.p[7] = p[6];
+            break;
+        default:
+            // params = (lambda, axis, x0, y0, z0, Lx, Ly, Lz); amplitude a0
+            d.p[0] = a0;
+            for (int k = 0; k < 5; k++) d.p[1 + k] = p[k];
+            d.p[6] = 1.0 / (p[5] * p[5]);
+            d.p[7] = 1.0 / (p[6] * p[6]);
+            d.p[8] = 1.0 / (p[7] * p[7]);
+            break;
     }
-    if (TAP) {
-        const double amp = 1.0 + taper * (xp - x_shift - Xleft - 0.5 * Lx) / (0.5 * Lx);
-        F[6 * (int64_t)ip + 4] += amp * b1 * cosh(ku * yp);
-        F[6 * (int64_t)ip + 3] += amp * b2 * sinh(ku * yp);
-    } else {
-        F[6 * (int64_t)ip + 4] += b1 * cosh(ku * yp);
-        F[6 * (int64_t)ip + 3] += b2 * sinh(ku * yp);
-    }
+    return ST_OK;
 }
 
-template <int TAP>
-static int32_t undul_mapped_impl(const double* coord, double* Fld, const double* a0, const double* params, int32_t np,
-                                 int32_t nx) {
-    constexpr int NP = TAP ? 5 : 3;
-    if (np < 0 || nx < 1 || params == nullptr || a0 == nullptr) return fail(ST_EINVAL, "undul_mapped: bad arguments");
+// A drop-in device call: Fld(6, np) += the device's field at coord(3, np).
+static int32_t ext_device_call(int kind, const double* coord, double* Fld, double t, const double* params, int npar,
+                               const double* a0map, int nmap, double a0, int32_t np) {
+    if (np < 0 || params == nullptr) return fail(ST_EINVAL, "device kind %d: bad arguments", kind);
     if (np == 0) return ST_OK;
-    double hp[NP];
-    if (is_device_ptr(params)) CH_HIP(hipMemcpy(hp, params, sizeof hp, hipMemcpyDeviceToHost));
-    else memcpy(hp, params, sizeof hp);
-    const double lambda = hp[0], Xleft = hp[1], dx = hp[2];
-    const double Lx = TAP ? hp[3] : 0.0, taper = TAP ? hp[4] : 0.0;
+    double hp[8];
+    if (is_device_ptr(params)) CH_HIP(hipMemcpy(hp, params, npar * sizeof(double), hipMemcpyDeviceToHost));
+    else memcpy(hp, params, npar * sizeof(double));
     Stage st;
     const double* x = st.in(coord, 3 * (size_t)np);
-    const double* a = st.in(a0, 2 * (size_t)nx);
+    const double* m = a0map ? st.in(a0map, 2 * (size_t)nmap) : nullptr;
     double* F = st.inout(Fld, 6 * (size_t)np);
     CH_TRY(st.status());
-    hipLaunchKernelGGL((k_undul_mapped<TAP>), blocks_for(np, 256), 256, 0, stream(), x, F, a, nx, Xleft, dx, 1.0 / dx,
-                       Xleft + nx * dx, 2.0 * M_PI / lambda, Lx, taper, 0.5 * (nx * dx - Lx), np);
-    CH_LAUNCH_CHECK("k_undul_mapped");
+    ExtDevice d;
+    CH_TRY(make_ext_device(d, kind, hp, npar, m, nmap, a0));
+    hipLaunchKernelGGL(k_ext_device, blocks_for(np, 256), 256, 0, stream(), x, AOS3, F, AOS6, d, t, np);
+    CH_LAUNCH_CHECK("k_ext_device");
     return st.finish();
-}
-
-// devices.f90:205-252: a plane wave of amplitude a0 and wavelength lambda at
-// angle theta to x, phase phi0, with linear ramps of length `ramp` at both ends of
-// [X0, X0 + Lx]; adds to Fld(3), Fld(4), Fld(5) (0-based 2, 3, 4).
-__global__ void k_planewave(const double* __restrict__ coord, double* __restrict__ F, double t, double a0,
-                            double X0, double Lx, double ramp, double sinth, double costh, double k0, double phi0,
-                            int np) {
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ip >= np) return;
-    const double x = coord[3 * (int64_t)ip], y = coord[3 * (int64_t)ip + 1];
-    double ampl;
-    if (x <= X0 || x >= X0 + Lx) ampl = 0.0;
-    else if (x > X0 && x < X0 + ramp) ampl = (x - X0) / ramp;
-    else if (x > X0 + Lx - ramp && x < X0 + Lx) ampl = (X0 + Lx - x) / ramp;
-    else ampl = 1.0;
-    ampl = a0 * ampl * sin(k0 * (x * costh + y * sinth - t) + phi0);
-    double* f = F + 6 * (int64_t)ip;
-    f[2] = f[2] + ampl;
-    f[3] = f[3] + ampl * sinth;
-    f[4] = f[4] - ampl * costh;
-}
-
-// devices.f90:254-297: a Gaussian pulse centred at (x0 + axis t, y0, z0) with
-// rms-like lengths (Lx, Ly, Lz) and carrier sin(2 pi xp / lambda); adds E to
-// Fld(3) and -axis E to Fld(5).
-__global__ void k_gaussbeam(const double* __restrict__ coord, double* __restrict__ F, double time, double a0,
-                            double lambda, double axis, double x0, double y0, double z0, double Lx2_inv,
-                            double Ly2_inv, double Lz2_inv, int np) {
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ip >= np) return;
-    const double* c = coord + 3 * (int64_t)ip;
-    const double xp = c[0] - x0 - axis * time, yp = c[1] - y0, zp = c[2] - z0;
-    const double e = a0 * exp(-(xp * xp * Lx2_inv) - yp * yp * Ly2_inv - zp * zp * Lz2_inv) *
-                     sin(2.0 * M_PI / lambda * xp);
-    double* f = F + 6 * (int64_t)ip;
-    f[2] = f[2] + e;
-    f[4] = f[4] - axis * e;
 }
 
 // ----------------------------------------------------------------- lattice
@@ -370,7 +356,6 @@ static int32_t check_modes(int nkO) {
 
 using namespace chimera;
 
-static const PStride AOS3 = {3, 1}, AOS6 = {6, 1};
 
 extern "C" {
 
@@ -531,81 +516,31 @@ int32_t chimera_eb_correction(double* eb_spc, int32_t nx, int32_t nr, int32_t nk
 }
 
 int32_t chimera_undul_analytic(const double* coord, double* Fld, double t, const double* params, int32_t np) {
-    (void)t;
-    if (np < 0 || params == nullptr) return fail(ST_EINVAL, "undul_analytic: bad arguments");
-    if (np == 0) return ST_OK;
-    double hp[4];
-    if (is_device_ptr(params)) CH_HIP(hipMemcpy(hp, params, sizeof hp, hipMemcpyDeviceToHost));
-    else memcpy(hp, params, sizeof hp);
-    Stage st;
-    const double* x = st.in(coord, 3 * (size_t)np);
-    double* F = st.inout(Fld, 6 * (size_t)np);
-    CH_TRY(st.status());
-    const UndulParams up = make_undul(hp);
-    hipLaunchKernelGGL(k_undul_analytic, blocks_for(np, 256), 256, 0, stream(), x, AOS3, F, AOS6, up, np);
-    CH_LAUNCH_CHECK("k_undul_analytic");
-    return st.finish();
+    return ext_device_call(DEV_UNDUL, coord, Fld, t, params, 4, nullptr, 0, 0.0, np);
 }
 
 int32_t chimera_undul_analytic_taper(const double* coord, double* Fld, double t, const double* params, int32_t np) {
-    (void)t;
-    if (np < 0 || params == nullptr) return fail(ST_EINVAL, "undul_analytic_taper: bad arguments");
-    if (np == 0) return ST_OK;
-    double hp[5];
-    if (is_device_ptr(params)) CH_HIP(hipMemcpy(hp, params, sizeof hp, hipMemcpyDeviceToHost));
-    else memcpy(hp, params, sizeof hp);
-    Stage st;
-    const double* x = st.in(coord, 3 * (size_t)np);
-    double* F = st.inout(Fld, 6 * (size_t)np);
-    CH_TRY(st.status());
-    const UndulParams up = make_undul_taper(hp);
-    hipLaunchKernelGGL(k_undul_analytic, blocks_for(np, 256), 256, 0, stream(), x, AOS3, F, AOS6, up, np);
-    CH_LAUNCH_CHECK("k_undul_analytic");
-    return st.finish();
+    return ext_device_call(DEV_UNDUL_TAPER, coord, Fld, t, params, 5, nullptr, 0, 0.0, np);
 }
 
 int32_t chimera_undul_mapped(const double* coord, double* Fld, double t, const double* a0, const double* params,
                              int32_t np, int32_t nx) {
-    (void)t;
-    return undul_mapped_impl<0>(coord, Fld, a0, params, np, nx);
+    if (nx < 1 || a0 == nullptr) return fail(ST_EINVAL, "undul_mapped: bad arguments");
+    return ext_device_call(DEV_MAPPED, coord, Fld, t, params, 3, a0, nx, 0.0, np);
 }
 
 int32_t chimera_undul_mapped_tap(const double* coord, double* Fld, double t, const double* a0, const double* params,
                                  int32_t np, int32_t nx) {
-    (void)t;
-    return undul_mapped_impl<1>(coord, Fld, a0, params, np, nx);
+    if (nx < 1 || a0 == nullptr) return fail(ST_EINVAL, "undul_mapped_tap: bad arguments");
+    return ext_device_call(DEV_MAPPED_TAP, coord, Fld, t, params, 5, a0, nx, 0.0, np);
 }
 
 int32_t chimera_planewave(const double* coord, double* Fld, double t, const double* params, int32_t np) {
-    if (np < 0 || params == nullptr) return fail(ST_EINVAL, "planewave: bad arguments");
-    if (np == 0) return ST_OK;
-    double p[7];
-    if (is_device_ptr(params)) CH_HIP(hipMemcpy(p, params, sizeof p, hipMemcpyDeviceToHost));
-    else memcpy(p, params, sizeof p);
-    Stage st;
-    const double* x = st.in(coord, 3 * (size_t)np);
-    double* F = st.inout(Fld, 6 * (size_t)np);
-    CH_TRY(st.status());
-    hipLaunchKernelGGL(k_planewave, blocks_for(np, 256), 256, 0, stream(), x, F, t, p[0], p[2], p[3], p[4], sin(p[5]),
-                       cos(p[5]), 2.0 * M_PI / p[1], p[6], np);
-    CH_LAUNCH_CHECK("k_planewave");
-    return st.finish();
+    return ext_device_call(DEV_PLANEWAVE, coord, Fld, t, params, 7, nullptr, 0, 0.0, np);
 }
 
 int32_t chimera_gaussbeam(const double* coord, double* Fld, double time, double a0, const double* params, int32_t np) {
-    if (np < 0 || params == nullptr) return fail(ST_EINVAL, "gaussbeam: bad arguments");
-    if (np == 0) return ST_OK;
-    double p[8];
-    if (is_device_ptr(params)) CH_HIP(hipMemcpy(p, params, sizeof p, hipMemcpyDeviceToHost));
-    else memcpy(p, params, sizeof p);
-    Stage st;
-    const double* x = st.in(coord, 3 * (size_t)np);
-    double* F = st.inout(Fld, 6 * (size_t)np);
-    CH_TRY(st.status());
-    hipLaunchKernelGGL(k_gaussbeam, blocks_for(np, 256), 256, 0, stream(), x, F, time, a0, p[0], p[1], p[2], p[3], p[4],
-                       1.0 / (p[5] * p[5]), 1.0 / (p[6] * p[6]), 1.0 / (p[7] * p[7]), np);
-    CH_LAUNCH_CHECK("k_gaussbeam");
-    return st.finish();
+    return ext_device_call(DEV_GAUSSBEAM, coord, Fld, time, params, 8, nullptr, 0, a0, np);
 }
 
 int32_t chimera_genparts(double* coord, int32_t* indPart, const double* Xgrid, const double* Rgrid,

@@ -124,43 +124,141 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + k;
 }
 
-struct UndulParams { double a0, lambda, X0, Lx, ku, taper; int tapered; };
+// ---------------------------------------------------------------- external-field devices
+//
+// The devices of devices.f90 evaluated at a particle, each adding its field to
+// f (Ex, Ey, Ez, Bx, By, Bz) the way the reference's subroutine adds it to
+// Fld; the parameters are prepared on the host (make_ext_device below).
+enum { DEV_UNDUL = 0, DEV_UNDUL_TAPER = 1, DEV_MAPPED = 2, DEV_MAPPED_TAP = 3, DEV_PLANEWAVE = 4,
+       DEV_GAUSSBEAM = 5 };
+struct ExtDevice {
+    int kind, nmap;
+    const double* map;    // DEV_MAPPED*: a0(2, nmap) in device memory
+    double p[10];
+};
+constexpr int MAX_EXT_DEVICES = 4;
+struct ExtDevices {
+    ExtDevice d[MAX_EXT_DEVICES];
+    int n;
+    double t;             // time of the devices that depend on it (i_step * TimeStep)
+};
 
-inline UndulParams make_undul(const double p[4]) {
-    UndulParams u;
-    u.a0 = p[0];
-    u.lambda = p[1];
-    u.X0 = p[2];
-    u.Lx = p[3];
-    u.ku = 2.0 * M_PI / p[1];
-    u.taper = 0.0;
-    u.tapered = 0;
-    return u;
-}
-
-// params = (a0, lambda, X0, Lx, taper) of the linearly tapered undulator.
-inline UndulParams make_undul_taper(const double p[5]) {
-    UndulParams u = make_undul(p);
-    u.taper = p[4];
-    u.tapered = 1;
-    return u;
-}
-
-// Analytic planar undulator (devices.f90:162-203), evaluated in FP64 and added
-// to the gathered fields; tapered, the amplitude is scaled by
-// 1 + taper (x - X0 - Lx/2) / (Lx/2) (devices.f90:117-160).
-template <typename R>
-__device__ __forceinline__ void undul_field(const UndulParams& up, double x, double y, R f[6]) {
+// devices.f90:162-203 (analytic) and :117-160 (tapered): p = a0, lambda, X0, Lx, ku, taper
+__device__ __forceinline__ void ext_undul(const double* p, bool tapered, double x, double y, double df[6]) {
+    const double a0 = p[0], lambda = p[1], X0 = p[2], Lx = p[3], ku = p[4];
     double ampl;
-    if (x <= up.X0 || x >= up.X0 + up.Lx) ampl = 0.0;
-    else if (x > up.X0 && x < up.X0 + up.lambda) ampl = (x - up.X0) / up.lambda;
-    else if (x > up.X0 + up.Lx - up.lambda && x < up.X0 + up.Lx) ampl = (up.X0 + up.Lx - x) / up.lambda;
+    if (x <= X0 || x >= X0 + Lx) ampl = 0.0;
+    else if (x > X0 && x < X0 + lambda) ampl = (x - X0) / lambda;
+    else if (x > X0 + Lx - lambda && x < X0 + Lx) ampl = (X0 + Lx - x) / lambda;
     else ampl = 1.0;
-    if (up.tapered) ampl = ampl * (1.0 + up.taper * (x - up.X0 - 0.5 * up.Lx) / (0.5 * up.Lx));
-    ampl = ampl * up.a0;
-    f[4] += (R)(ampl * sin(up.ku * (x - up.X0)) * cosh(up.ku * y));
-    f[3] += (R)(ampl * cos(up.ku * (x - up.X0)) * sinh(up.ku * y));
+    if (tapered) ampl = ampl * (1.0 + p[5] * (x - X0 - 0.5 * Lx) / (0.5 * Lx));
+    ampl = ampl * a0;
+    df[4] = ampl * sin(ku * (x - X0)) * cosh(ku * y);
+    df[3] = ampl * cos(ku * (x - X0)) * sinh(ku * y);
 }
+
+// devices.f90:18-62 (tap = false) and :64-115 (tap = true): the field map a0(2, nx)
+// on nodes Xleft + (i-1) dx, i = 1..nx, with quadratic weights around node ix;
+// p = Xleft, dx, dx_inv, Xright, ku, Lx, taper, x_shift.  Particles outside
+// [Xleft + dx, Xright - dx] see no field; a node outside 1..nx contributes zero.
+__device__ __forceinline__ void ext_mapped(const double* p, const double* __restrict__ a0, int nx, bool tap, double xp,
+                                           double yp, double df[6]) {
+    const double Xleft = p[0], dx = p[1], dx_inv = p[2], Xright = p[3], ku = p[4];
+    if (xp < Xleft + dx || xp > Xright - dx) return;
+    const double fx = floor((xp - Xleft) * dx_inv + 0.5);
+    if (!(fx >= 0.0 && fx <= (double)nx + 1.0)) return;       // NaN or off the map: no node in range
+    const int ix = (int)fx;
+    const double ddx = (xp - Xleft) * dx_inv - ix;
+    const double s[3] = {0.5 * ((0.5 - ddx) * (0.5 - ddx)), 0.75 - ddx * ddx, 0.5 * ((0.5 + ddx) * (0.5 + ddx))};
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int j = ix - 1 + k;                                // 1-based node
+        const double v1 = (j >= 1 && j <= nx) ? a0[2 * (int64_t)(j - 1)] : 0.0;
+        const double v2 = (j >= 1 && j <= nx) ? a0[2 * (int64_t)(j - 1) + 1] : 0.0;
+        b1 = b1 + s[k] * v1;
+        b2 = b2 + s[k] * v2;
+    }
+    if (tap) {
+        const double Lx = p[5], taper = p[6], x_shift = p[7];
+        const double amp = 1.0 + taper * (xp - x_shift - Xleft - 0.5 * Lx) / (0.5 * Lx);
+        df[4] = amp * b1 * cosh(ku * yp);
+        df[3] = amp * b2 * sinh(ku * yp);
+    } else {
+        df[4] = b1 * cosh(ku * yp);
+        df[3] = b2 * sinh(ku * yp);
+    }
+}
+
+// devices.f90:205-252: p = a0, X0, Lx, ramp, sin(theta), cos(theta), k0, phi0
+__device__ __forceinline__ void ext_planewave(const double* p, double t, double x, double y, double df[6]) {
+    const double a0 = p[0], X0 = p[1], Lx = p[2], ramp = p[3], sinth = p[4], costh = p[5], k0 = p[6], phi0 = p[7];
+    double ampl;
+    if (x <= X0 || x >= X0 + Lx) ampl = 0.0;
+    else if (x > X0 && x < X0 + ramp) ampl = (x - X0) / ramp;
+    else if (x > X0 + Lx - ramp && x < X0 + Lx) ampl = (X0 + Lx - x) / ramp;
+    else ampl = 1.0;
+    ampl = a0 * ampl * sin(k0 * (x * costh + y * sinth - t) + phi0);
+    df[2] = ampl;
+    df[3] = ampl * sinth;
+    df[4] = -(ampl * costh);
+}
+
+// devices.f90:254-297: p = a0, lambda, axis, x0, y0, z0, 1/Lx^2, 1/Ly^2, 1/Lz^2
+__device__ __forceinline__ void ext_gaussbeam(const double* p, double t, double x, double y, double z, double df[6]) {
+    const double a0 = p[0], lambda = p[1], axis = p[2];
+    const double xp = x - p[3] - axis * t, yp = y - p[4], zp = z - p[5];
+    const double e = a0 * exp(-(xp * xp * p[6]) - yp * yp * p[7] - zp * zp * p[8]) * sin(2.0 * M_PI / lambda * xp);
+    df[2] = e;
+    df[4] = -(axis * e);
+}
+
+// One device's contribution at (x, y, z); df is zeroed here, so f + df equals
+// the reference's f = f + term for every component a device writes.
+__device__ __forceinline__ void ext_device_field(const ExtDevice& d, double t, double x, double y, double z,
+                                                 double df[6]) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) df[c] = 0.0;
+    switch (d.kind) {
+        case DEV_UNDUL: ext_undul(d.p, false, x, y, df); break;
+        case DEV_UNDUL_TAPER: ext_undul(d.p, true, x, y, df); break;
+        case DEV_MAPPED: ext_mapped(d.p, d.map, d.nmap, false, x, y, df); break;
+        case DEV_MAPPED_TAP: ext_mapped(d.p, d.map, d.nmap, true, x, y, df); break;
+        case DEV_PLANEWAVE: ext_planewave(d.p, t, x, y, df); break;
+        default: ext_gaussbeam(d.p, t, x, y, z, df); break;
+    }
+}
+
+// Every device of D in order, added to the particle's field f (FP64 terms,
+// added in the precision of f).
+template <typename R>
+__device__ __forceinline__ void ext_fields(const ExtDevices& D, double x, double y, double z, R f[6]) {
+    for (int k = 0; k < D.n; k++) {
+        double df[6];
+        ext_device_field(D.d[k], D.t, x, y, z, df);
+        switch (D.d[k].kind) {
+            case DEV_PLANEWAVE:
+                f[2] += (R)df[2];
+                f[3] += (R)df[3];
+                f[4] += (R)df[4];
+                break;
+            case DEV_GAUSSBEAM:
+                f[2] += (R)df[2];
+                f[4] += (R)df[4];
+                break;
+            default:
+                f[4] += (R)df[4];
+                f[3] += (R)df[3];
+                break;
+        }
+    }
+}
+
+// Host: one device from its kind, the reference's parameter array and, for the
+// mapped undulators, the device copy of the map (a0 rows of nmap nodes); the
+// Gaussian beam's scalar amplitude is passed as a0.  EINVAL on a short array.
+int32_t make_ext_device(ExtDevice& d, int kind, const double* params, int nparams, const double* map, int nmap,
+                        double a0);
 
 // Launchers of the particle kernels (particles.hip).
 int32_t launch_push_coords(double* x, PStride sx, const double* u, PStride su, double* xc, PStride sc, double dt,

@@ -63,7 +63,7 @@ template <int NMOD, typename R>
 __global__ void __launch_bounds__(GP_T) k_gather_push(const double* __restrict__ coord, PStride sx, R* __restrict__ u,
                                                       PStride su, const R* __restrict__ w,
                                                       const cplx_t<R>* __restrict__ Fld, GridGeom g_in, double push_dt,
-                                                      UndulParams up, int n_undul, int np) {
+                                                      ExtDevices devs, int np) {
     typedef cplx_t<R> C;
     constexpr int NC = 6 * NMOD;   // (component, mode) planes
     __shared__ C tile[GP_LDS_BYTES / sizeof(C)];
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(GP_T) k_gather_push(const double* __restrict__
             }
         }
     }
-    if (n_undul) undul_field<R>(up, xp, yp, f);
+    if (devs.n) ext_fields<R>(devs, xp, yp, zp, f);
     R m[3] = {u[su.at(ip, 0)], u[su.at(ip, 1)], u[su.at(ip, 2)]};
     boris<R>(m, f, (R)(0.5 * push_dt));
     for (int c = 0; c < 3; c++) u[su.at(ip, c)] = m[c];
@@ -723,26 +723,43 @@ __global__ void k_gather_particles(const int32_t* __restrict__ idx, int n, const
 
 using namespace chimera;
 
+// The device list of a resident gather-push: none, or the one analytic
+// undulator of undul_params (host, 4 doubles), or the described devices.
+static int32_t undul_devices(ExtDevices& devs, const double* undul_params) {
+    memset(&devs, 0, sizeof devs);
+    if (!undul_params) return ST_OK;
+    devs.n = 1;
+    return make_ext_device(devs.d[0], DEV_UNDUL, undul_params, 4, nullptr, 0, 0.0);
+}
+
+static int32_t ext_devices(ExtDevices& devs, const chimera_device_t* d, int32_t ndev, double t) {
+    memset(&devs, 0, sizeof devs);
+    if (ndev < 0 || ndev > MAX_EXT_DEVICES || (ndev > 0 && d == nullptr))
+        return fail(ST_EINVAL, "gather_push: %d devices (at most %d per species)", ndev, MAX_EXT_DEVICES);
+    for (int k = 0; k < ndev; k++) {
+        if ((d[k].kind == DEV_MAPPED || d[k].kind == DEV_MAPPED_TAP) && !is_device_ptr(d[k].map))
+            return fail(ST_EINVAL, "gather_push: the map of a mapped undulator is device memory");
+        CH_TRY(make_ext_device(devs.d[k], d[k].kind, d[k].params, 8, d[k].map, d[k].nmap, d[k].a0));
+    }
+    devs.n = ndev;
+    devs.t = t;
+    return ST_OK;
+}
+
 template <typename R>
 static int32_t gather_push_impl(const double* coord, R* momenta, const R* wghts, int64_t ld, int32_t np,
                                 const cplx_t<R>* Fld, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
                                 double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
-                                const double* undul_params, const double* frame) {
+                                const ExtDevices& devs, const double* frame) {
     CH_TRY(check_resident(np, ld, nkO));
     if (np == 0) return ST_OK;
     const PStride s = {1, ld};
     GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
     g.leftX_dev = frame;
-    UndulParams up = {0, 1, 0, 0, 0};
-    int nu = 0;
-    if (undul_params) {
-        up = make_undul(undul_params);
-        nu = 1;
-    }
     KTimer kt(KF_GATHER_PUSH);
 #define GATHER_LAUNCH(N)                                                                                      \
     hipLaunchKernelGGL((k_gather_push<N, R>), blocks_for(np, GP_T), GP_T, 0, stream(), coord, s, momenta, s, wghts, \
-                       Fld, g, push_dt, up, nu, np)
+                       Fld, g, push_dt, devs, np)
     CH_MODE_SWITCH(nkO, GATHER_LAUNCH)
 #undef GATHER_LAUNCH
     CH_LAUNCH_CHECK("k_gather_push");
@@ -928,16 +945,42 @@ int32_t chimera_soa_gather_push(const double* coord, double* momenta, const doub
                                 const double* Fld, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
                                 double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
                                 const double* undul_params, const double* frame) {
+    ExtDevices devs;
+    CH_TRY(undul_devices(devs, undul_params));
     return gather_push_impl<double>(coord, momenta, wghts, ld, np, reinterpret_cast<const double2*>(Fld), leftX, Rgrid,
-                                    dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, undul_params, frame);
+                                    dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, devs, frame);
 }
 
 int32_t chimera_soa_gather_push_mixed(const double* coord, float* momenta, const float* wghts, int64_t ld, int32_t np,
                                       const float* Fld, double leftX, const double* Rgrid, double dx_inv,
                                       double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
                                       double push_dt, const double* undul_params, const double* frame) {
+    ExtDevices devs;
+    CH_TRY(undul_devices(devs, undul_params));
     return gather_push_impl<float>(coord, momenta, wghts, ld, np, reinterpret_cast<const float2*>(Fld), leftX, Rgrid,
-                                   dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, undul_params, frame);
+                                   dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, devs, frame);
+}
+
+int32_t chimera_soa_gather_push_dev(const double* coord, double* momenta, const double* wghts, int64_t ld,
+                                    int32_t np, const double* Fld, double leftX, const double* Rgrid, double dx_inv,
+                                    double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
+                                    double push_dt, const chimera_device_t* devices, int32_t ndev, double t,
+                                    const double* frame) {
+    ExtDevices devs;
+    CH_TRY(ext_devices(devs, devices, ndev, t));
+    return gather_push_impl<double>(coord, momenta, wghts, ld, np, reinterpret_cast<const double2*>(Fld), leftX, Rgrid,
+                                    dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, devs, frame);
+}
+
+int32_t chimera_soa_gather_push_dev_mixed(const double* coord, float* momenta, const float* wghts, int64_t ld,
+                                          int32_t np, const float* Fld, double leftX, const double* Rgrid,
+                                          double dx_inv, double dr_inv, double r0, double rmax, int32_t nx, int32_t nr,
+                                          int32_t nkO, double push_dt, const chimera_device_t* devices, int32_t ndev,
+                                          double t, const double* frame) {
+    ExtDevices devs;
+    CH_TRY(ext_devices(devs, devices, ndev, t));
+    return gather_push_impl<float>(coord, momenta, wghts, ld, np, reinterpret_cast<const float2*>(Fld), leftX, Rgrid,
+                                   dx_inv, dr_inv, r0, rmax, nx, nr, nkO, push_dt, devs, frame);
 }
 
 // Advance of a device-side window frame, as move_frame forms the new edge

@@ -64,18 +64,20 @@ class Diagnostics:
 
     # ------------------------------------------------------------------ dumps (diagnostics.py:50-107)
     def _dump(self, diag, name, dat, ret):
+        """Return dat (appended to ret) or save it as out_folder/name, name a
+        pattern with {istr} for the zero-padded step of the last do_diags."""
         if "Return" in diag["Features"]:
             ret.append(dat)
             return
         if self.out_folder is None:
             raise ValueError(f"{name} is written into out_folder: give one, or 'Return' in Features")
-        np.save(os.path.join(self.out_folder, name), dat)
+        np.save(os.path.join(self.out_folder, name.format(istr=getattr(self, "istr", "0000000"))), dat)
 
     def fld_out(self, diag):
         """diagnostics.py:50-59: EB without its ghost row, per solver, as ee_<n>_<step>.npy."""
         ret = []
         for jj, sol in enumerate(self.Chimera.Solvers):
-            self._dump(diag, f"ee_{jj}_{self.istr}.npy", sol.Data["EB"].numpy()[:, 1:], ret)
+            self._dump(diag, f"ee_{jj}_{{istr}}.npy", sol.Data["EB"].numpy()[:, 1:], ret)
         if "Return" in diag["Features"]:
             return ret
 
@@ -84,7 +86,7 @@ class Diagnostics:
         files ee_<n>_<step>.npy too)."""
         ret = []
         for jj, sol in enumerate(self.Chimera.Solvers):
-            self._dump(diag, f"ee_{jj}_{self.istr}.npy", sol.Data["EG_fb"].numpy(), ret)
+            self._dump(diag, f"ee_{jj}_{{istr}}.npy", sol.Data["EG_fb"].numpy(), ret)
         if "Return" in diag["Features"]:
             return ret
 
@@ -102,7 +104,7 @@ class Diagnostics:
         for jj, sp in enumerate(self.Chimera.Particles):
             if "Still" in sp.Args["Features"]:
                 continue
-            self._dump(diag, f"dens_{jj}_{self.istr}.npy", sp.get_dens_on_grid(modes)[:, 1:], ret)
+            self._dump(diag, f"dens_{jj}_{{istr}}.npy", sp.get_dens_on_grid(modes)[:, 1:], ret)
         if "Return" in diag["Features"]:
             return ret
 
@@ -114,7 +116,7 @@ class Diagnostics:
         for jj, sp in enumerate(self.Chimera.Particles):
             if "Still" in sp.Args["Features"]:
                 continue
-            self._dump(diag, f"phs{jj}_{self.istr}.npy", sp.phase_space(), ret)
+            self._dump(diag, f"phs{jj}_{{istr}}.npy", sp.phase_space(), ret)
         if "Return" in diag["Features"]:
             return ret
 

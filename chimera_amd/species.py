@@ -22,6 +22,12 @@ from . import fimera as F
 from .device import DeviceArray, call, copy_elems
 
 
+class DeviceRecord(ctypes.Structure):
+    """chimera_device_t of include/chimera_hip.h."""
+    _fields_ = [("kind", ctypes.c_int32), ("nmap", ctypes.c_int32), ("map", ctypes.c_void_p), ("a0", ctypes.c_double),
+                ("params", ctypes.c_double * 8)]
+
+
 class Specie:
     def __init__(self, specie_in):
         self.Args = specie_in
@@ -309,15 +315,48 @@ class Specie:
         call("chimera_soa_push_coords", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"), self.ld,
              self.np, float(dt_frac * self.Args["TimeStep"]))
 
-    def undulator_params(self):
-        """The analytic-undulator device of this species (params on the host), or None."""
+    # the devices of devices.f90 by their f2py names: (kind, parameter count, has a field map, has an amplitude)
+    DEVICE_KINDS = {"undul_analytic": (0, 4, False, False), "undul_analytic_taper": (1, 5, False, False),
+                    "undul_mapped": (2, 3, True, False), "undul_mapped_tap": (3, 5, True, False),
+                    "planewave": (4, 7, False, False), "gaussbeam": (5, 8, False, True)}
+
+    def device_descriptors(self, i_step):
+        """The species' Devices (species.py:258-277: entries (device, *args),
+        the device an f2py name or a function of that name, args as the
+        reference passes them) as chimera_device_t records for the fused
+        gather-push at time i_step * TimeStep, or None without devices.  Field
+        maps are uploaded once and kept with the species."""
         if not self.Devices:
             return None
-        if len(self.Devices) > 1:
-            raise NotImplementedError("one analytic undulator per species is supported")
-        dev = self.Devices[0]
-        fn = dev[0]
-        name = fn if isinstance(fn, str) else getattr(fn, "__name__", "")
-        if name.lower() != "undul_analytic":
-            raise NotImplementedError(f"device {name!r}: only the analytic undulator (devices.f90:162-203) is fused")
-        return np.ascontiguousarray(np.asarray(dev[1], dtype=np.float64)[:4])
+        if len(self.Devices) > 4:
+            raise NotImplementedError("at most 4 devices per species are fused into the push")
+        recs = (DeviceRecord * len(self.Devices))()
+        keep = getattr(self, "_dev_maps", None)
+        if keep is None:
+            keep = self._dev_maps = {}
+        for k, dev in enumerate(self.Devices):
+            fn = dev[0]
+            name = (fn if isinstance(fn, str) else getattr(fn, "__name__", "")).lower()
+            if name not in self.DEVICE_KINDS:
+                raise NotImplementedError(f"device {name!r}: the fused push provides {sorted(self.DEVICE_KINDS)}")
+            kind, npar, has_map, has_amp = self.DEVICE_KINDS[name]
+            args = list(dev[1:])
+            rec = recs[k]
+            rec.kind = kind
+            if has_map:
+                src = args.pop(0)
+                a0 = np.asarray(src, dtype=np.float64)
+                if a0.ndim != 2 or a0.shape[0] != 2:
+                    raise ValueError(f"{name}: the map a0 has shape (2, nx), not {a0.shape}")
+                if k not in keep or keep[k][0] is not src:
+                    keep[k] = (src, DeviceArray.from_numpy(np.asfortranarray(a0)))
+                rec.map = keep[k][1].ptr
+                rec.nmap = a0.shape[1]
+            if has_amp:
+                rec.a0 = float(args.pop(0))
+            params = np.asarray(args[0], dtype=np.float64).ravel()
+            if params.size < npar:
+                raise ValueError(f"{name}: {params.size} parameters, {npar} needed")
+            for i in range(npar):
+                rec.params[i] = float(params[i])
+        return recs, i_step * self.Args["TimeStep"]

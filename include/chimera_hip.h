@@ -49,6 +49,21 @@ const char* chimera_last_error(void);
  * Solver refuse its grid at construction; the reference's FFTW plans any
  * length (fb_io.f90:37-38), so it has no counterpart there. */
 int32_t chimera_fft_length_check(int32_t n);
+/* Form of the mixed-precision (complex64) Hankel and FB-operator GEMMs, for the
+ * whole process: 0 = split-BF16, each FP32 operand split exactly into three
+ * BF16 pieces and six BF16 MFMA products accumulated in FP32 (default);
+ * 1 = FP32 MFMA.  chimera_get_mixed_gemm returns the current form.  No
+ * counterpart in the reference, which has no mixed precision. */
+int32_t chimera_set_mixed_gemm(int32_t form);
+int32_t chimera_get_mixed_gemm(void);
+/* Tile autotuner of the Hankel and FB-operator GEMMs (no counterpart in the
+ * reference).  on = 1: the first launch of each GEMM shape (element type,
+ * form, accumulators, Nx, columns, jobs, K, segments) times every tile shape
+ * and keeps the fastest; on = 0 (default): tiles by a block-count rule.  path
+ * (may be NULL) names a text file of tuned shapes, read once and appended to.
+ * chimera_autotune_entries returns the number of shapes tuned or loaded. */
+int32_t chimera_set_autotune(int32_t on, const char* path);
+int32_t chimera_autotune_entries(void);
 /* Stream (hipStream_t) every later launch goes to; NULL = the null stream. */
 int32_t chimera_set_stream(void* hip_stream);
 void* chimera_get_stream(void);
@@ -349,6 +364,29 @@ int32_t chimera_soa_gather_push(const double* coord, double* momenta, const doub
                                 double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO, double push_dt,
                                 const double* undul_params, const double* frame);
 
+/* An external-field device of devices.f90, as a species' Devices entry
+ * (species.py:258-277) describes it: kind 0 undul_analytic (params a0, lambda,
+ * X0, Lx), 1 undul_analytic_taper (+ taper), 2 undul_mapped (lambda, Xleft,
+ * dx; map), 3 undul_mapped_tap (+ Lx, taper), 4 planewave (a0, lambda, X0,
+ * Lx, ramp, theta, phi0), 5 gaussbeam (lambda, axis, x0, y0, z0, Lx, Ly, Lz;
+ * amplitude a0).  map: the field map a0(2, nmap) in DEVICE memory. */
+typedef struct {
+    int32_t kind;
+    int32_t nmap;
+    const double* map;
+    double a0;
+    double params[8];
+} chimera_device_t;
+/* chimera_soa_gather_push with up to 4 devices (HOST array of ndev
+ * descriptors) added in order to the gathered field at time t (i_step *
+ * TimeStep), before the push.  = make_field + proj_fld + make_device +
+ * push_velocs (species.py:246-298, devices.f90:18-297). */
+int32_t chimera_soa_gather_push_dev(const double* coord, double* momenta, const double* wghts, int64_t ld,
+                                    int32_t np, const double* Fld, double leftX, const double* Rgrid, double dx_inv,
+                                    double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
+                                    double push_dt, const chimera_device_t* devices, int32_t ndev, double t,
+                                    const double* frame);
+
 /* Stable removal of the particles outside lims (HOST double[4]: x range,
  * r^2 range), tested on coord (test_half=0) or coord_half (1); *np_out (HOST)
  * receives the new count.  = sortpartsout + align_data_* (species.py:385-398). */
@@ -482,6 +520,11 @@ int32_t chimera_soa_gather_push_mixed(const double* coord, float* momenta, const
                                       const float* Fld, double leftX, const double* Rgrid, double dx_inv,
                                       double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
                                       double push_dt, const double* undul_params, const double* frame);
+int32_t chimera_soa_gather_push_dev_mixed(const double* coord, float* momenta, const float* wghts, int64_t ld,
+                                          int32_t np, const float* Fld, double leftX, const double* Rgrid,
+                                          double dx_inv, double dr_inv, double r0, double rmax, int32_t nx, int32_t nr,
+                                          int32_t nkO, double push_dt, const chimera_device_t* devices, int32_t ndev,
+                                          double t, const double* frame);
 int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* momenta, float* wghts, int64_t ld,
                                   int32_t np, const double* lims, int32_t test_half, int32_t* np_out);
 int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,

@@ -333,9 +333,12 @@ class Specie:
     def make_device(self, i_step=0):
         if "Still" in self.Args["Features"]:
             return
-        for dev in self.Devices:
-            self.Data["EB"] = _ch().undul_analytic(self.Data["coords"], self.Data["EB"], i_step * self.Args["TimeStep"],
-                                                   np.asarray(dev[1], dtype=float))
+        for dev in self.Devices:                                            # species.py:273-277
+            fn = dev[0]
+            name = (fn if isinstance(fn, str) else getattr(fn, "__name__", "")).lower()
+            args = [np.asarray(a, dtype=float) if np.ndim(a) else float(a) for a in dev[1:]]
+            self.Data["EB"] = getattr(_ch(), name)(self.Data["coords"], self.Data["EB"],
+                                                   i_step * self.Args["TimeStep"], *args)
 
     def push_velocs(self, dt_frac=1.0):
         if self.Data["coords"].shape[-1] == 0 or "Still" in self.Args["Features"]:
