@@ -22,8 +22,17 @@
 #include "../../include/chimera_hip.h"
 
 namespace chimera {
-
 constexpr int MOM_T = 256, MOM_NS = 13, MOM_MAXB = 512;
+}
+
+// Scratch of chimera_beam_moments_noalloc: the per-workgroup partial sums of
+// the first reduction pass, MOM_MAXB rows of MOM_NS doubles (53,248 bytes), in
+// every GPU's code object for the life of the process.  Host code never uses
+// the name itself (on the host it is only a stand-in for the symbol): the
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[chimera::MOM_MAXB * chimera::MOM_NS];
+
+namespace chimera {
 
 // Tree sum of v over the workgroup into sh[0]; every thread must call it.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -104,6 +113,55 @@ static int32_t beam_moments_impl(const double* coord_half, const U* momenta, con
     hipLaunchKernelGGL(k_moments_final, 1, MOM_T, 0, stream(), part, nb, d_out);
     CH_LAUNCH_CHECK("k_moments_final");
     return st.finish();
+}
+
+// ---------------------------------------------------------------- no-alloc beam moments
+//
+// The same two launches as beam_moments_impl, with the partial sums in the
+// file-scope array instead of the library's grown scratch memory and the sums
+// written straight to device memory: nothing is allocated.  The array's
+// device address is looked up once per device (with that device current) and
+// kept; every call on a device shares its array, so calls that overlap on one
+// GPU race.  Every partial row a call reads it has written itself first, so
+// nothing needs clearing.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+static int32_t noalloc_scratch(double** out) {
+    static std::mutex mu;
+    static double* ptr[NOALLOC_MAX_DEVICES] = {};
+    int dev = -1;
+    CH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(ST_EUNSUP, "beam_moments_noalloc: device %d", dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(ST_EHIP, "beam_moments_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        ptr[dev] = static_cast<double*>(p);
+    }
+    *out = ptr[dev];
+    return ST_OK;
+}
+
+static int32_t beam_moments_noalloc_impl(const double* coord_half, const double* momenta, const double* wghts,
+                                         int64_t ld, int32_t np, double* sums) {
+    if (np < 0 || ld < np) return fail(ST_EINVAL, "beam_moments_noalloc: np=%d ld=%lld", np, (long long)ld);
+    if (sums == nullptr || !is_device_ptr(sums))
+        return fail(ST_EINVAL, "beam_moments_noalloc: sums must be device memory (13 doubles)");
+    if (np > 0 && (!is_device_ptr(coord_half) || !is_device_ptr(momenta) || !is_device_ptr(wghts)))
+        return fail(ST_EINVAL, "beam_moments_noalloc: the particle arrays are device pointers");
+    double* part = nullptr;
+    CH_TRY(noalloc_scratch(&part));
+    const int nb = (int)std::min<int64_t>(MOM_MAXB, std::max<int64_t>(1, (np + MOM_T - 1) / MOM_T));
+    if ((size_t)nb * MOM_NS * sizeof(double) > sizeof(::scratch))
+        return fail(ST_EINVAL, "beam_moments_noalloc: %d partial rows exceed the %d-row scratch", nb, MOM_MAXB);
+    // np = 0: the grid-stride pass runs no iteration and writes zero partials
+    hipLaunchKernelGGL((k_moments_part<double>), nb, MOM_T, 0, stream(), coord_half, momenta, wghts, ld, np, part);
+    CH_LAUNCH_CHECK("k_moments_part");
+    hipLaunchKernelGGL(k_moments_final, 1, MOM_T, 0, stream(), part, nb, sums);
+    CH_LAUNCH_CHECK("k_moments_final");
+    return ST_OK;
 }
 
 // ---------------------------------------------------------------- weighted mean of px
@@ -428,6 +486,11 @@ int32_t chimera_beam_moments(const double* coord_half, const double* momenta, co
 int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
                                    int32_t np, double* sums) {
     return beam_moments_impl<float>(coord_half, momenta, wghts, ld, np, sums);
+}
+
+int32_t chimera_beam_moments_noalloc(const double* coord_half, const double* momenta, const double* wghts,
+                                     int64_t ld, int32_t np, double* sums) {
+    return beam_moments_noalloc_impl(coord_half, momenta, wghts, ld, np, sums);
 }
 
 int32_t chimera_momentum_mean(const double* momenta, const double* wghts, int32_t np, double* px0) {

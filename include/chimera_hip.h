@@ -459,6 +459,16 @@ int32_t chimera_fb_filter(double* vec, double leftX, const double* kx, const dou
  * by np, so repeated calls give the same bits. */
 int32_t chimera_beam_moments(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
                              int32_t np, double* sums);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  chimera_beam_moments with the same kernels and the same
+ * bits, but its partial sums live in a fixed 53,248-byte device array of the
+ * library (512 workgroups x 13 doubles, any np) and sums must be device
+ * memory (13 doubles): nothing is allocated, staged or freed.  Asynchronous on
+ * the library stream.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU race. */
+int32_t chimera_beam_moments_noalloc(const double* coord_half, const double* momenta, const double* wghts,
+                                     int64_t ld, int32_t np, double* sums);
 /* px0 = sum(px w) / sum(w) over the np particles (px the first row of
  * momenta), the mean momentum of the StaticKick field update, FP64 sums in an
  * order fixed by np; NaN for np = 0.  Replaces the PXmean expression of
