@@ -1963,6 +1963,140 @@ int32_t fb_graddiv_poisson_t(cplx_t<R>* j, const cplx_t<R>* gn, const cplx_t<R>*
 }
 
 
+// ---------------------------------------------------------------- envelope (KxShift) operators
+//
+// fb_math_env.f90: fields carry modes -nkO..nkO (mode m at index m + nkO), the
+// coupling matrices D+-(nkr, nkr_loc, -nkO-1:nkO+1) (mode m at m + nkO + 1);
+// mode m couples to m-1 and m+1 only where those exist on the grid (no
+// reflected mode), and the curl drops the x term of the m-1 coupling as the
+// reference does (fb_math_env.f90:141-146: the sum is reset before use).
+
+// rot: loc_x = -D+ (i v_y - v_z)[m+1]; loc_y = -i kx v_z + i S+ + i S-;
+// loc_z = i kx v_y - S+ + S-, S+ = D+ v_x[m+1], S- = D- v_x[m-1]
+template <typename R>
+void build_rot_env_jobs(std::vector<FbJobT<R>>& jobs, FbViewW<R> loc, FbView<R> v, const R* Dp, const R* Dm, int nkr,
+                        int nkr_loc, int nkO, const R* scale_tab) {
+    auto D = [&](const R* B, int m) { return B + (int64_t)nkr * nkr_loc * (m + nkO + 1); };
+    auto X = [&](int m) { return m + nkO; };
+    for (int m = -nkO; m <= nkO; m++) {
+        FbJobT<R> a = job0<R>();
+        if (m < nkO) a.seg[a.nseg++] = seg<R>(v.at(X(m + 1), 1), I1, v.at(X(m + 1), 2), MONE, D(Dp, m), nkr, 0, 0);
+        a.out[0] = outp<R>(loc.at(X(m), 0), MONE, ZERO);
+        a.nout = 1;
+        a.K = nkr;
+        a.Ncol = nkr_loc;
+        FbJobT<R> b = job0<R>();
+        if (m < nkO) b.seg[b.nseg++] = seg<R>(v.at(X(m + 1), 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 0);
+        if (m > -nkO) b.seg[b.nseg++] = seg<R>(v.at(X(m - 1), 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 1);
+        b.out[0] = outp<R>(loc.at(X(m), 1), I1, I1, v.at(X(m), 2), MI);
+        b.out[1] = outp<R>(loc.at(X(m), 2), MONE, ONE, v.at(X(m), 1), I1);
+        b.nout = 2;
+        b.K = nkr;
+        b.Ncol = nkr_loc;
+        if (scale_tab) {
+            const R* sc = scale_tab + (int64_t)v.nkx * nkr_loc * X(m);
+            a.out[0].scale = sc;
+            b.out[0].scale = sc;
+            b.out[1].scale = sc;
+        }
+        jobs.push_back(a);
+        jobs.push_back(b);
+    }
+}
+
+// grad: loc_x = i kx s[m]; loc_y = -V + W; loc_z = i V + i W, V = D- s[m-1], W = D+ s[m+1]
+// (modes mlo..mhi of s, output modes -nkO..nkO; s of the second half of
+// grad-div carries modes -nkO-1..nkO+1 and then every coupling exists)
+template <typename R>
+void build_grad_env_jobs(std::vector<FbJobT<R>>& jobs, FbViewW<R> loc, FbView<R> s, int s_lo, int s_hi, const R* Dp,
+                         const R* Dm, int nkr, int nkr_loc, int K, int nkO, const PoissArgs<R>* poiss = nullptr) {
+    auto D = [&](const R* B, int m) { return B + (int64_t)nkr * nkr_loc * (m + nkO + 1); };
+    for (int m = -nkO; m <= nkO; m++) {
+        FbJobT<R> j = job0<R>();
+        if (m - 1 >= s_lo) j.seg[j.nseg++] = seg<R>(s.at(m - 1 - s_lo, 0), ONE, nullptr, ZERO, D(Dm, m), nkr, 0, 0);
+        if (m + 1 <= s_hi) j.seg[j.nseg++] = seg<R>(s.at(m + 1 - s_lo, 0), ONE, nullptr, ZERO, D(Dp, m), nkr, 0, 1);
+        j.out[0] = outp<R>(loc.at(m + nkO, 0), ZERO, ZERO, s.at(m - s_lo, 0), I1);
+        j.out[1] = outp<R>(loc.at(m + nkO, 1), MONE, ONE);
+        j.out[2] = outp<R>(loc.at(m + nkO, 2), I1, I1);
+        j.nout = 3;
+        j.K = K;
+        j.Ncol = nkr_loc;
+        if (poiss) {
+            const int64_t fc = loc.nkx * loc.ncol * loc.nmod, fm = loc.nkx * loc.ncol;
+            for (int c = 0; c < 3; c++) {
+                j.out[c].poiss = 1;
+                j.out[c].poiss_n = poiss->gn + (m + nkO) * fm + c * fc;
+                j.out[c].poiss_np1 = poiss->gp + (m + nkO) * fm + c * fc;
+                j.out[c].poiss_w2 = poiss->w2 + (m + nkO) * fm;
+            }
+        }
+        jobs.push_back(j);
+    }
+}
+
+// div into output modes olo..ohi: out = i kx v_x[m] (m on the grid)
+// - D- (v_y - i v_z)[m-1] + D+ (v_y + i v_z)[m+1]
+template <typename R>
+void build_div_env_jobs(std::vector<FbJobT<R>>& jobs, FbViewW<R> out, int olo, int ohi, FbView<R> v, const R* Dp,
+                        const R* Dm, int nkr, int nkr_loc, int nkO) {
+    auto D = [&](const R* B, int m) { return B + (int64_t)nkr * nkr_loc * (m + nkO + 1); };
+    for (int m = olo; m <= ohi; m++) {
+        FbJobT<R> j = job0<R>();
+        if (m > -nkO) j.seg[j.nseg++] = seg<R>(v.at(m - 1 + nkO, 2), I1, v.at(m - 1 + nkO, 1), MONE, D(Dm, m), nkr, 0, 0);
+        if (m < nkO) j.seg[j.nseg++] = seg<R>(v.at(m + 1 + nkO, 2), I1, v.at(m + 1 + nkO, 1), ONE, D(Dp, m), nkr, 0, 0);
+        if (m >= -nkO && m <= nkO) j.out[0] = outp<R>(out.at(m - olo, 0), ONE, ZERO, v.at(m + nkO, 0), I1);
+        else j.out[0] = outp<R>(out.at(m - olo, 0), ONE, ZERO);
+        j.nout = 1;
+        j.K = nkr;
+        j.Ncol = nkr_loc;
+        jobs.push_back(j);
+    }
+}
+
+template <typename R>
+int32_t fb_op_env_t(int op, cplx_t<R>* out, const cplx_t<R>* in, const R* Dp, const R* Dm, const double* kx_dev,
+                    int nkx, int nkr, int nkO, int nkr_loc, const R* scale_tab, const PoissArgs<R>* poiss,
+                    double dt_inv) {
+    typedef cplx_t<R> C;
+    std::vector<FbJobT<R>> jobs;
+    FbGemmArgs g;
+    memset(&g, 0, sizeof g);
+    g.nx = nkx;
+    g.lda = nkx;
+    g.ldo = nkx;
+    g.kx = kx_dev;
+    g.dt_inv = dt_inv;
+    const int nm = 2 * nkO + 1;
+    switch (op) {
+        case FB_ROT:
+            build_rot_env_jobs(jobs, FbViewW<R>{out, nkx, nkr_loc, nm}, FbView<R>{in, nkx, nkr, nm}, Dp, Dm, nkr,
+                               nkr_loc, nkO, scale_tab);
+            return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
+        case FB_GRAD:
+            build_grad_env_jobs(jobs, FbViewW<R>{out, nkx, nkr_loc, nm}, FbView<R>{in, nkx, nkr, nm}, -nkO, nkO, Dp, Dm,
+                                nkr, nkr_loc, nkr, nkO);
+            return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
+        case FB_DIV:
+            build_div_env_jobs(jobs, FbViewW<R>{out, nkx, nkr_loc, nm}, -nkO, nkO, FbView<R>{in, nkx, nkr, nm}, Dp, Dm,
+                               nkr, nkr_loc, nkO);
+            return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
+        case FB_GRADDIV: {
+            if (nkr_loc != nkr) return fail(ST_EUNSUP, "fb_graddiv_env needs nkr_loc == nkr (got %d, %d)", nkr_loc, nkr);
+            C* s = static_cast<C*>(scratch(3, (size_t)nkx * nkr * (nm + 2) * sizeof(C)));
+            if (!s) return ST_ENOMEM;
+            build_div_env_jobs(jobs, FbViewW<R>{s, nkx, nkr, nm + 2}, -nkO - 1, nkO + 1, FbView<R>{in, nkx, nkr, nm},
+                               Dp, Dm, nkr, nkr, nkO);
+            CH_TRY(launch_fb_gemm<R>(jobs, g, KF_FBOPS));
+            jobs.clear();
+            build_grad_env_jobs(jobs, FbViewW<R>{out, nkx, nkr, nm}, FbView<R>{s, nkx, nkr, nm + 2}, -nkO - 1, nkO + 1,
+                                Dp, Dm, nkr, nkr, nkr, nkO, poiss);
+            return launch_fb_gemm<R>(jobs, g, KF_FBOPS);
+        }
+        default:
+            return fail(ST_EINVAL, "unknown FB operator %d", op);
+    }
+}
+
 #define CH_FB_INSTANTIATE(R)                                                                                          \
     template int32_t fb_forward_t<R>(cplx_t<R>*, cplx_t<R>*, double, const double*, const R*, int, int, int, int, int, \
                                      const R*, int, const double*);                                                   \
@@ -1972,7 +2106,9 @@ int32_t fb_graddiv_poisson_t(cplx_t<R>* j, const cplx_t<R>* gn, const cplx_t<R>*
     template int32_t fb_op_t<R>(int, cplx_t<R>*, const cplx_t<R>*, const R*, const R*, const double*, int, int, int,  \
                                 int, const R*);                                                                       \
     template int32_t fb_graddiv_poisson_t<R>(cplx_t<R>*, const cplx_t<R>*, const cplx_t<R>*, double, const R*,         \
-                                             const R*, const R*, const double*, int, int, int);
+                                             const R*, const R*, const double*, int, int, int);                       \
+    template int32_t fb_op_env_t<R>(int, cplx_t<R>*, const cplx_t<R>*, const R*, const R*, const double*, int, int,   \
+                                    int, int, const R*, const PoissArgs<R>*, double);
 CH_FB_INSTANTIATE(double)
 CH_FB_INSTANTIATE(float)
 #undef CH_FB_INSTANTIATE
@@ -2115,6 +2251,52 @@ int32_t chimera_fb_graddiv(double* vec_fb, const double* DpS2S, const double* Dm
     return fb_op_entry(FB_GRADDIV, vec_fb, vec_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 3, 3);
 }
 
+static int32_t fb_op_env_entry(int op, double* out, const double* in, const double* DpS2S, const double* DmS2S,
+                               const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc, int in_comp,
+                               int out_comp) {
+    if (nkx < 1 || nkr < 1 || nkO < 0 || nkr_loc < 1) return fail(ST_EINVAL, "FB envelope operator: bad sizes");
+    if (nkr_loc > nkr) return fail(ST_EINVAL, "FB envelope operator: nkr_loc=%d exceeds nkr=%d", nkr_loc, nkr);
+    const int nm = 2 * nkO + 1;
+    Stage st;
+    const size_t nin = (size_t)nkx * nkr * nm * in_comp;
+    const size_t nout = (size_t)nkx * nkr_loc * nm * out_comp;
+    double2* o;
+    const double2* i;
+    if (op == FB_GRADDIV) {
+        o = reinterpret_cast<double2*>(st.inout(out, 2 * nin));
+        i = o;
+    } else {
+        o = reinterpret_cast<double2*>(st.out(out, 2 * nout));
+        i = reinterpret_cast<const double2*>(st.in(in, 2 * nin));
+    }
+    const size_t nd = (size_t)nkr * nkr_loc * (nm + 2);
+    const double* Dp = st.in(DpS2S, nd);
+    const double* Dm = st.in(DmS2S, nd);
+    const double* k = st.in(kx, nkx);
+    CH_TRY(st.status());
+    CH_TRY(fb_op_env_t<double>(op, o, i, Dp, Dm, k, nkx, nkr, nkO, nkr_loc, nullptr, nullptr, 0.0));
+    return st.finish();
+}
+
+int32_t chimera_fb_rot_env(double* vec_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                           const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc) {
+    return fb_op_env_entry(FB_ROT, vec_fb_loc, vec_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 3, 3);
+}
+
+int32_t chimera_fb_grad_env(double* vec_fb_loc, const double* scl_fb, const double* DpS2S, const double* DmS2S,
+                            const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc) {
+    return fb_op_env_entry(FB_GRAD, vec_fb_loc, scl_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 1, 3);
+}
+
+int32_t chimera_fb_div_env(double* scl_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                           const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc) {
+    return fb_op_env_entry(FB_DIV, scl_fb_loc, vec_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 3, 1);
+}
+
+int32_t chimera_fb_graddiv_env(double* vec_fb, const double* DpS2S, const double* DmS2S, const double* kx,
+                               int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc) {
+    return fb_op_env_entry(FB_GRADDIV, vec_fb, vec_fb, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc, 3, 3);
+}
 
 /* ---------------------------------------------------------------- resident tier (device pointers) */
 

@@ -46,6 +46,32 @@ __global__ void k_psatd_wo(cplx_t<R>* __restrict__ EG, const cplx_t<R>* __restri
     }
 }
 
+// maxwell_solvers.f90:62-96 with the complex tables of the envelope solver
+// (solvers.py:244-267): EG' = C1.(E, G, J), G' = C2.(E, G, J) per element.
+template <typename R>
+__global__ void k_psatd_wo_cplx(cplx_t<R>* __restrict__ EG, const cplx_t<R>* __restrict__ J,
+                                const cplx_t<R>* __restrict__ C1, const cplx_t<R>* __restrict__ C2, int64_t n) {
+    typedef cplx_t<R> C_;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const C_ a1 = C1[e], a2 = C1[e + n], a3 = C1[e + 2 * n];
+    const C_ b1 = C2[e], b2 = C2[e + n], b3 = C2[e + 2 * n];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        const C_ E = EG[e + l * n], G = EG[e + (3 + l) * n], j = J[e + l * n];
+        EG[e + l * n] = cadd(cadd(cmul(a1, E), cmul(a2, G)), cmul(a3, j));
+        EG[e + (3 + l) * n] = cadd(cadd(cmul(b1, E), cmul(b2, G)), cmul(b3, j));
+    }
+}
+
+template <typename R>
+int32_t psatd_wo_cplx(cplx_t<R>* EG, const cplx_t<R>* J, const cplx_t<R>* C1, const cplx_t<R>* C2, int64_t n) {
+    KTimer kt(KF_PSATD);
+    hipLaunchKernelGGL((k_psatd_wo_cplx<R>), blocks_for(n, 256), 256, 0, stream(), EG, J, C1, C2, n);
+    CH_LAUNCH_CHECK("k_psatd_wo_cplx");
+    return ST_OK;
+}
+
 // maxwell_solvers.f90:98-129 (complex tables)
 template <typename R>
 __global__ void k_init_push(cplx_t<R>* __restrict__ EG, const cplx_t<R>* __restrict__ J, const cplx_t<R>* __restrict__ Gn,
@@ -249,6 +275,20 @@ int32_t chimera_maxwell_push_wo_spchrg(double* EG_fb, const double* j_fb, const 
     const double* b = st.in(C2, (size_t)n * 3);
     CH_TRY(st.status());
     CH_TRY(psatd_wo<double>(EG, J, a, b, n));
+    return st.finish();
+}
+
+int32_t chimera_maxwell_push_wo_spchrg_env(double* EG_fb, const double* j_fb, const double* C1, const double* C2,
+                                           int32_t nkx, int32_t nkr, int32_t nkO) {
+    SIZES_OK(nkx, nkr, nkO);
+    const int64_t n = nel(nkx, nkr, nkO);
+    Stage st;
+    double2* EG = reinterpret_cast<double2*>(st.inout(EG_fb, 2 * (size_t)n * 6));
+    const double2* J = reinterpret_cast<const double2*>(st.in(j_fb, 2 * (size_t)n * 3));
+    const double2* a = reinterpret_cast<const double2*>(st.in(C1, 2 * (size_t)n * 3));
+    const double2* b = reinterpret_cast<const double2*>(st.in(C2, 2 * (size_t)n * 3));
+    CH_TRY(st.status());
+    CH_TRY(psatd_wo_cplx<double>(EG, J, a, b, n));
     return st.finish();
 }
 

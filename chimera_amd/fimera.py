@@ -27,6 +27,8 @@ __all__ = [
     "fb_vec_in", "fb_scl_in", "fb_eb_out", "fb_filtr", "fb_rot", "fb_grad", "fb_div", "fb_graddiv",
     "maxwell_push_with_spchrg", "maxwell_push_wo_spchrg", "maxwell_init_push", "poiss_corr", "omp_mult_vec",
     "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic", "undul_analytic_taper", "undul_mapped", "undul_mapped_tap", "planewave", "gaussbeam", "poiss_corr_stat", "field_drift",
+    "dep_curr_env", "dep_dens_env", "dep_curr_env_chnk", "dep_dens_env_chnk", "proj_fld_env", "eb_correction_env",
+    "fb_rot_env", "fb_grad_env", "fb_div_env", "fb_graddiv_env",
 ]
 
 
@@ -228,6 +230,98 @@ def fb_graddiv(vec_fb, DpS2S, DmS2S, kx):
     return _fbop("chimera_fb_graddiv", vec_fb, None, DpS2S, DmS2S, kx, out_is_inp=True)
 
 
+# ------------------------------------------------------------------ grid_deps_env.f90, fb_math_env.f90
+# envelope (KxShift) grids: the mode axis runs over -nkO..nkO (length 2 nkO + 1)
+
+def _nko_env(n):
+    if n % 2 != 1:
+        raise ValueError(f"envelope mode axis of length {n}: the modes are -nkO..nkO (odd length)")
+    return (n - 1) // 2
+
+
+def dep_curr_env(coord, momenta, wghts, curr, leftX, Rgrid, dx_inv, dr_inv, kx0):
+    """grid_deps_env.f90:18-95 -> curr"""
+    x, u, w = _arr(coord, np.float64), _arr(momenta, np.float64), _arr(wghts, np.float64)
+    J, R = _arr(curr, np.complex128), _arr(Rgrid, np.float64)
+    _call("chimera_dep_curr_env", _p(x), _p(u), _p(w), _p(J), float(leftX), _p(R), float(dx_inv), float(dr_inv),
+          float(kx0), x.shape[1], J.shape[0] - 1, J.shape[1] - 1, _nko_env(J.shape[2]))
+    return J
+
+
+def dep_dens_env(coord, wghts, dens, leftX, Rgrid, dx_inv, dr_inv, kx0):
+    """grid_deps_env.f90:97-162 -> dens"""
+    x, w = _arr(coord, np.float64), _arr(wghts, np.float64)
+    D, R = _arr(dens, np.complex128), _arr(Rgrid, np.float64)
+    _call("chimera_dep_dens_env", _p(x), _p(w), _p(D), float(leftX), _p(R), float(dx_inv), float(dr_inv), float(kx0),
+          x.shape[1], D.shape[0] - 1, D.shape[1] - 1, _nko_env(D.shape[2]))
+    return D
+
+
+def dep_curr_env_chnk(coord, momenta, wghts, curr, IndInChunk, guards, leftX, Rgrid, dx_inv, dr_inv, kx0):
+    """grid_deps_env_chnk.f90:18-145 -> curr"""
+    x, u, w = _arr(coord, np.float64), _arr(momenta, np.float64), _arr(wghts, np.float64)
+    J, R, I = _arr(curr, np.complex128), _arr(Rgrid, np.float64), _arr(IndInChunk, np.int32)
+    _call("chimera_dep_curr_env_chnk", _p(x), _p(u), _p(w), _p(J), _p(I), int(guards), float(leftX), _p(R),
+          float(dx_inv), float(dr_inv), float(kx0), x.shape[1], J.shape[0] - 1, J.shape[1] - 1, _nko_env(J.shape[2]),
+          I.shape[0] - 1)
+    return J
+
+
+def dep_dens_env_chnk(coord, wghts, dens, IndInChunk, guards, leftX, Rgrid, dx_inv, dr_inv, kx0):
+    """grid_deps_env_chnk.f90:147-261 -> dens"""
+    x, w = _arr(coord, np.float64), _arr(wghts, np.float64)
+    D, R, I = _arr(dens, np.complex128), _arr(Rgrid, np.float64), _arr(IndInChunk, np.int32)
+    _call("chimera_dep_dens_env_chnk", _p(x), _p(w), _p(D), _p(I), int(guards), float(leftX), _p(R), float(dx_inv),
+          float(dr_inv), float(kx0), x.shape[1], D.shape[0] - 1, D.shape[1] - 1, _nko_env(D.shape[2]),
+          I.shape[0] - 1)
+    return D
+
+
+def proj_fld_env(coord, wghts, Fld, Fld_tot, leftX, Rgrid, dx_inv, dr_inv, kx0):
+    """grid_deps_env.f90:164-238 -> Fld_tot"""
+    x, w = _arr(coord, np.float64), _arr(wghts, np.float64)
+    F, Ft, R = _arr(Fld, np.complex128), _arr(Fld_tot, np.float64), _arr(Rgrid, np.float64)
+    _call("chimera_proj_fld_env", _p(x), _p(w), _p(F), _p(Ft), float(leftX), _p(R), float(dx_inv), float(dr_inv),
+          float(kx0), x.shape[1], F.shape[0] - 1, F.shape[1] - 1, _nko_env(F.shape[2]))
+    return Ft
+
+
+def eb_correction_env(eb_spc):
+    """grid_deps_env.f90:240-283 -> eb_spc"""
+    E = _arr(eb_spc, np.complex128)
+    _call("chimera_eb_correction_env", _p(E), E.shape[0] - 1, E.shape[1] - 1, _nko_env(E.shape[2]))
+    return E
+
+
+def _fbop_env(name, out, inp, DpS2S, DmS2S, kx, out_is_inp=False):
+    o = _arr(out, np.complex128)
+    i = o if out_is_inp else _arr(inp, np.complex128)
+    Dp, Dm, k = _arr(DpS2S, np.float64), _arr(DmS2S, np.float64), _arr(kx, np.float64)
+    _call(name, _p(o), *(() if out_is_inp else (_p(i),)), _p(Dp), _p(Dm), _p(k), i.shape[0], i.shape[1],
+          _nko_env(i.shape[2]), Dp.shape[1])
+    return o
+
+
+def fb_rot_env(vec_fb_loc, vec_fb, DpS2S, DmS2S, kx):
+    """fb_math_env.f90:106-162 -> vec_fb_loc"""
+    return _fbop_env("chimera_fb_rot_env", vec_fb_loc, vec_fb, DpS2S, DmS2S, kx)
+
+
+def fb_grad_env(vec_fb_loc, scl_fb, DpS2S, DmS2S, kx):
+    """fb_math_env.f90:18-61 -> vec_fb_loc"""
+    return _fbop_env("chimera_fb_grad_env", vec_fb_loc, scl_fb, DpS2S, DmS2S, kx)
+
+
+def fb_div_env(scl_fb_loc, vec_fb, DpS2S, DmS2S, kx):
+    """fb_math_env.f90:63-104 -> scl_fb_loc"""
+    return _fbop_env("chimera_fb_div_env", scl_fb_loc, vec_fb, DpS2S, DmS2S, kx)
+
+
+def fb_graddiv_env(vec_fb, DpS2S, DmS2S, kx):
+    """fb_math_env.f90:164-233 -> vec_fb"""
+    return _fbop_env("chimera_fb_graddiv_env", vec_fb, None, DpS2S, DmS2S, kx, out_is_inp=True)
+
+
 # ------------------------------------------------------------------ maxwell_solvers.f90
 
 def maxwell_push_with_spchrg(EG_fb, j_fb, grad_rho_n_fb, grad_rho_np1_fb, C1, C2):
@@ -240,13 +334,14 @@ def maxwell_push_with_spchrg(EG_fb, j_fb, grad_rho_n_fb, grad_rho_np1_fb, C1, C2
 
 
 def maxwell_push_wo_spchrg(EG_fb, j_fb, C1, C2):
-    """maxwell_solvers.f90:62-96 -> EG_fb (real PSATD tables, as the driver passes them)"""
+    """maxwell_solvers.f90:62-96 -> EG_fb (real PSATD tables of the plain solver, complex ones of the envelope solver)"""
     EG, j = _arr(EG_fb, np.complex128), _arr(j_fb, np.complex128)
     C1, C2 = np.asarray(C1), np.asarray(C2)
     if np.iscomplexobj(C1) or np.iscomplexobj(C2):
-        if np.any(np.imag(C1)) or np.any(np.imag(C2)):
-            raise ValueError("maxwell_push_wo_spchrg: complex (KxShift) PSATD tables are outside the PIC-step path")
-        C1, C2 = C1.real, C2.real
+        # the envelope solver's complex tables (solvers.py:261-267)
+        a, b = _arr(C1, np.complex128), _arr(C2, np.complex128)
+        _call("chimera_maxwell_push_wo_spchrg_env", _p(EG), _p(j), _p(a), _p(b), *EG.shape[:3])
+        return EG
     a, b = _arr(C1, np.float64), _arr(C2, np.float64)
     _call("chimera_maxwell_push_wo_spchrg", _p(EG), _p(j), _p(a), _p(b), *EG.shape[:3])
     return EG

@@ -290,6 +290,52 @@ int32_t chimera_planewave(const double* coord, double* Fld, double t, const doub
  * Replaces GaussBeam, devices.f90:254-297. */
 int32_t chimera_gaussbeam(const double* coord, double* Fld, double time, double a0, const double* params, int32_t np);
 
+/* ---------------------------------------------------------------- envelope (KxShift) solver
+ * The f2py entry points of grid_deps_env.f90, grid_deps_env_chnk.f90 and
+ * fb_math_env.f90.  Envelope grids carry the modes -nkO..nkO (mode axis of
+ * length 2 nkO + 1, Fortran order, e.g. curr(0:nx,0:nr,-nkO:nkO,3)); the
+ * coupling matrices DpS2S/DmS2S (nkr, nkr_loc, -nkO-1:nkO+1).  Arrays host or
+ * device memory, as the drop-in tier above. */
+/* = dep_curr_env (grid_deps_env.f90:18-95): the third current component only,
+ * complex weight w e^{-i kx0 x}, then the ghost fold. */
+int32_t chimera_dep_curr_env(const double* coord, const double* momenta, const double* wghts, double* curr,
+                             double leftX, const double* Rgrid, double dx_inv, double dr_inv, double kx0, int32_t np,
+                             int32_t nx, int32_t nr, int32_t nkO);
+/* = dep_dens_env (grid_deps_env.f90:97-162): the complex weight applied twice. */
+int32_t chimera_dep_dens_env(const double* coord, const double* wghts, double* dens, double leftX, const double* Rgrid,
+                             double dx_inv, double dr_inv, double kx0, int32_t np, int32_t nx, int32_t nr,
+                             int32_t nkO);
+/* = dep_curr_env_chnk / dep_dens_env_chnk (grid_deps_env_chnk.f90:18-261): the
+ * particles IndInChunk(0)..IndInChunk(nchnk)-1 deposited with the unchunked
+ * semantics (the reference's guard merges drop or race on chunk edges). */
+int32_t chimera_dep_curr_env_chnk(const double* coord, const double* momenta, const double* wghts, double* curr,
+                                  const int32_t* IndInChunk, int32_t guards, double leftX, const double* Rgrid,
+                                  double dx_inv, double dr_inv, double kx0, int32_t np, int32_t nx, int32_t nr,
+                                  int32_t nkO, int32_t nchnk);
+int32_t chimera_dep_dens_env_chnk(const double* coord, const double* wghts, double* dens, const int32_t* IndInChunk,
+                                  int32_t guards, double leftX, const double* Rgrid, double dx_inv, double dr_inv,
+                                  double kx0, int32_t np, int32_t nx, int32_t nr, int32_t nkO, int32_t nchnk);
+/* = proj_fld_env (grid_deps_env.f90:164-238); Fld(0:nx,0:nr,-nkO:nkO,6). */
+int32_t chimera_proj_fld_env(const double* coord, const double* wghts, const double* Fld, double* Fld_tot,
+                             double leftX, const double* Rgrid, double dx_inv, double dr_inv, double kx0, int32_t np,
+                             int32_t nx, int32_t nr, int32_t nkO);
+/* = eb_correction_env (grid_deps_env.f90:240-283). */
+int32_t chimera_eb_correction_env(double* eb_spc, int32_t nx, int32_t nr, int32_t nkO);
+/* = fb_rot_env, fb_grad_env, fb_div_env, fb_graddiv_env (fb_math_env.f90:18-233). */
+int32_t chimera_fb_rot_env(double* vec_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                           const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc);
+int32_t chimera_fb_grad_env(double* vec_fb_loc, const double* scl_fb, const double* DpS2S, const double* DmS2S,
+                            const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc);
+int32_t chimera_fb_div_env(double* scl_fb_loc, const double* vec_fb, const double* DpS2S, const double* DmS2S,
+                           const double* kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc);
+int32_t chimera_fb_graddiv_env(double* vec_fb, const double* DpS2S, const double* DmS2S, const double* kx,
+                               int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc);
+/* = maxwell_push_wo_spchrg (maxwell_solvers.f90:62-96) with the complex tables
+ * C1, C2 (nkx,nkr,nkO,3) complex128 of the envelope solver (solvers.py:244-267);
+ * nkO is the mode count here, as in maxwell_solvers.f90. */
+int32_t chimera_maxwell_push_wo_spchrg_env(double* EG_fb, const double* j_fb, const double* C1, const double* C2,
+                                           int32_t nkx, int32_t nkr, int32_t nkO);
+
 /* ================================================================ resident tier
  *
  * Entry points for simulations whose state stays in device memory (the

@@ -1016,3 +1016,403 @@ void ref_undul_analytic(const double *coord, double *Fld, double t, const double
         Fld[6 * ip + 3] = Fld[6 * ip + 3] + ampl * cos(ku * (x - X0)) * sinh(ku * y);
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* grid_deps_env.f90: deposits and gather of the envelope (KxShift) solver   */
+/* ------------------------------------------------------------------------- */
+/* Mode arrays run over -nkO..nkO: mode iO sits at index iO + nkO.  The
+ * particle's weight carries the carrier phase e^{-i kx0 x} (deposits) or its
+ * conjugate (gather). */
+
+/* Cell, weights and phase powers of an envelope particle: the deposits use
+ * (y - i z)/r and 0 on the axis, the gather (y + i z)/r and 1 on the axis
+ * (grid_deps_env.f90:56-60, :131-135, :203-207).  Particles whose cell lies
+ * outside the arrays are skipped, as for ref_geom. */
+static inline int ref_geom_env(pgeom *g, double xp, double yp, double zp, double rp, double leftX,
+                               const double *Rgrid, double dx_inv, double dr_inv, int nx, int nr, int nkO, int gather)
+{
+    const double fx = (xp - leftX) * dx_inv, fr = (rp - Rgrid[0]) * dr_inv;
+    if (!(fx >= 0.0 && fx < (double)nx) || !(fr >= 0.0 && fr < (double)nr)) return 0;
+    g->ix = (int64_t)floor(fx);
+    g->ir = (int64_t)floor(fr);
+    g->S0[1][0] = (xp - leftX) * dx_inv - (double)g->ix;
+    g->S0[0][0] = 1.0 - g->S0[1][0];
+    g->S0[1][1] = (rp - Rgrid[g->ir]) * dr_inv;
+    g->S0[0][1] = 1.0 - g->S0[1][1];
+    cplx ph;
+    if (rp > 0.0) ph = gather ? cmk(yp / rp, zp / rp) : cmk(yp / rp, -zp / rp);
+    else ph = gather ? cmk(1.0, 0.0) : cmk(0.0, 0.0);
+    g->phase[0] = cmk(1.0, 0.0);
+    for (int iO = 1; iO <= nkO; iO++) g->phase[iO] = cmul(g->phase[iO - 1], ph);
+    return 1;
+}
+
+/* Envelope current.  grid_deps_env.f90:18-95: only the third (x... the
+ * reference's l = 3) component is deposited (`do l=3,3`), with the velocity
+ * u/gamma and the complex weight w e^{-i kx0 x}; mode -iO gets the conjugate
+ * phase; then the ghost fold of all three components.
+ * curr(0:nx,0:nr,-nkO:nkO,3). */
+void ref_dep_curr_env(const double *coord, const double *momenta, const double *wghts, double *curr_,
+                      double leftX, const double *Rgrid, double dx_inv, double dr_inv, double kx0,
+                      int32_t np, int32_t nx, int32_t nr, int32_t nkO)
+{
+    cplx *curr = (cplx *)curr_;
+    const int nm = 2 * nkO + 1;
+    const int64_t sr = nx + 1, sm = (int64_t)(nx + 1) * (nr + 1), sl = sm * nm;
+    for (int64_t ip = 0; ip < np; ip++) {
+        const double w = wghts[ip];
+        if (w == 0.0) continue;
+        const double xp = coord[3 * ip], yp = coord[3 * ip + 1], zp = coord[3 * ip + 2];
+        double v[3] = {momenta[3 * ip], momenta[3 * ip + 1], momenta[3 * ip + 2]};
+        const double rp = sqrt(yp * yp + zp * zp);
+        if (rp >= Rgrid[nr] || (fabs(v[0]) + fabs(v[1]) + fabs(v[2])) == 0.0) continue;
+        const double gp = sqrt(1.0 + v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int c = 0; c < 3; c++) v[c] = v[c] / gp;
+        const cplx wp = cmul(cmk(w, 0.0), cmk(cos(xp * kx0), -sin(xp * kx0)));
+        pgeom g;
+        if (!ref_geom_env(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nx, nr, nkO, 0)) continue;
+        cplx cp[2][2];
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) cp[a][b] = cmul(cmk(g.S0[a][0] * g.S0[b][1], 0.0), wp);
+        const int l = 2;
+        for (int iO = 0; iO <= nkO; iO++) {
+            const cplx pv = cmul(g.phase[iO], cmk(v[l], 0.0));
+            const cplx pc = cmul(cconj(g.phase[iO]), cmk(v[l], 0.0));
+            for (int b = 0; b < 2; b++)
+                for (int a = 0; a < 2; a++) {
+                    cplx *dst = curr + (g.ix + a) + (g.ir + b) * sr + (iO + nkO) * sm + l * sl;
+                    *dst = cadd(*dst, cmul(pv, cp[a][b]));
+                }
+            if (iO > 0)
+                for (int b = 0; b < 2; b++)
+                    for (int a = 0; a < 2; a++) {
+                        cplx *dst = curr + (g.ix + a) + (g.ir + b) * sr + (nkO - iO) * sm + l * sl;
+                        *dst = cadd(*dst, cmul(pc, cp[a][b]));
+                    }
+        }
+    }
+    for (int l = 0; l < 3; l++)
+        for (int m = 0; m < nm; m++)
+            for (int64_t ix = 0; ix <= nx; ix++) {
+                cplx *r0 = curr + ix + m * sm + l * sl;
+                r0[sr] = csub(r0[sr], r0[0]);
+                r0[0] = cmk(0.0, 0.0);
+            }
+}
+
+/* Envelope charge.  grid_deps_env.f90:97-162, the complex weight applied
+ * twice (grid_deps_env.f90:145-147).  dens(0:nx,0:nr,-nkO:nkO). */
+void ref_dep_dens_env(const double *coord, const double *wghts, double *dens_, double leftX, const double *Rgrid,
+                      double dx_inv, double dr_inv, double kx0, int32_t np, int32_t nx, int32_t nr, int32_t nkO)
+{
+    cplx *dens = (cplx *)dens_;
+    const int nm = 2 * nkO + 1;
+    const int64_t sr = nx + 1, sm = (int64_t)(nx + 1) * (nr + 1);
+    for (int64_t ip = 0; ip < np; ip++) {
+        const double w = wghts[ip];
+        if (w == 0.0) continue;
+        const double xp = coord[3 * ip], yp = coord[3 * ip + 1], zp = coord[3 * ip + 2];
+        const double rp = sqrt(yp * yp + zp * zp);
+        if (rp >= Rgrid[nr]) continue;
+        const cplx wp = cmul(cmk(w, 0.0), cmk(cos(xp * kx0), -sin(xp * kx0)));
+        pgeom g;
+        if (!ref_geom_env(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nx, nr, nkO, 0)) continue;
+        cplx dp[2][2];
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) {
+                dp[a][b] = cmul(cmk(g.S0[a][0] * g.S0[b][1], 0.0), wp);
+                dp[a][b] = cmul(dp[a][b], wp);
+            }
+        for (int iO = 0; iO <= nkO; iO++) {
+            for (int b = 0; b < 2; b++)
+                for (int a = 0; a < 2; a++) {
+                    cplx *dst = dens + (g.ix + a) + (g.ir + b) * sr + (iO + nkO) * sm;
+                    *dst = cadd(*dst, cmul(g.phase[iO], dp[a][b]));
+                }
+            if (iO > 0)
+                for (int b = 0; b < 2; b++)
+                    for (int a = 0; a < 2; a++) {
+                        cplx *dst = dens + (g.ix + a) + (g.ir + b) * sr + (nkO - iO) * sm;
+                        *dst = cadd(*dst, cmul(cconj(g.phase[iO]), dp[a][b]));
+                    }
+        }
+    }
+    for (int m = 0; m < nm; m++)
+        for (int64_t ix = 0; ix <= nx; ix++) {
+            cplx *r0 = dens + ix + m * sm;
+            r0[sr] = csub(r0[sr], r0[0]);
+            r0[0] = cmk(0.0, 0.0);
+        }
+}
+
+/* Envelope gather.  grid_deps_env.f90:164-238: the interpolation weights
+ * times e^{+i kx0 x} and the phase powers (conjugated for mode -iO); the real
+ * part of each product, summed per mode (mode +iO, then -iO) and added to
+ * Fld_tot.  Fld(0:nx,0:nr,-nkO:nkO,6). */
+void ref_proj_fld_env(const double *coord, const double *wghts, const double *Fld_, double *Fld_tot,
+                      double leftX, const double *Rgrid, double dx_inv, double dr_inv, double kx0,
+                      int32_t np, int32_t nx, int32_t nr, int32_t nkO)
+{
+    const cplx *Fld = (const cplx *)Fld_;
+    const int nm = 2 * nkO + 1;
+    const int64_t sr = nx + 1, sm = (int64_t)(nx + 1) * (nr + 1), sl = sm * nm;
+#pragma omp parallel for schedule(static)
+    for (int64_t ip = 0; ip < np; ip++) {
+        const double wp = wghts[ip];
+        if (wp == 0.0) continue;
+        const double xp = coord[3 * ip], yp = coord[3 * ip + 1], zp = coord[3 * ip + 2];
+        const double rp = sqrt(yp * yp + zp * zp);
+        if (rp >= Rgrid[nr]) continue;
+        pgeom g;
+        if (!ref_geom_env(&g, xp, yp, zp, rp, leftX, Rgrid, dx_inv, dr_inv, nx, nr, nkO, 1)) continue;
+        const cplx carrier = cmk(cos(xp * kx0), sin(xp * kx0));
+        cplx pc[2][2];
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) pc[a][b] = cmul(cmk(g.S0[b][1] * g.S0[a][0], 0.0), carrier);
+        double Fp[6] = {0, 0, 0, 0, 0, 0};
+        for (int iO = 0; iO <= nkO; iO++) {
+            for (int sgn = 0; sgn < (iO > 0 ? 2 : 1); sgn++) {
+                const cplx ph = sgn ? cconj(g.phase[iO]) : g.phase[iO];
+                const int m = sgn ? nkO - iO : nkO + iO;
+                cplx po[2][2];
+                for (int b = 0; b < 2; b++)
+                    for (int a = 0; a < 2; a++) po[a][b] = cmul(pc[a][b], ph);
+                for (int l = 0; l < 6; l++) {
+                    double s = 0.0;
+                    for (int b = 0; b < 2; b++)
+                        for (int a = 0; a < 2; a++) {
+                            const cplx f = Fld[(g.ix + a) + (g.ir + b) * sr + m * sm + l * sl];
+                            s += po[a][b].re * f.re - po[a][b].im * f.im;
+                        }
+                    Fp[l] = Fp[l] + s;
+                }
+            }
+        }
+        for (int l = 0; l < 6; l++) Fld_tot[6 * ip + l] = Fld_tot[6 * ip + l] + Fp[l];
+    }
+}
+
+/* grid_deps_env.f90:240-283: every mode times 1/pi; the ghost row is +row 1
+ * when nkO == 0 and -row 1 for every mode otherwise.  eb(0:nx,0:nr,-nkO:nkO,6). */
+void ref_eb_correction_env(double *eb_, int32_t nx, int32_t nr, int32_t nkO)
+{
+    cplx *eb = (cplx *)eb_;
+    const double pi = 4.0 * atan(1.0);
+    const double pi_inv = 1. / pi;
+    const int nm = 2 * nkO + 1;
+    const int64_t sr = nx + 1, sm = (int64_t)(nx + 1) * (nr + 1), n = sm * nm * 6;
+    for (int64_t j = 0; j < n; j++) eb[j] = crs(pi_inv, eb[j]);
+    for (int64_t c = 0; c < (int64_t)nm * 6; c++)
+        for (int64_t ix = 0; ix <= nx; ix++) {
+            cplx *col = eb + c * sm + ix;
+            col[0] = nkO == 0 ? col[sr] : cneg(col[sr]);
+        }
+}
+
+/* ------------------------------------------------------------------------- */
+/* fb_math_env.f90: FB-space operators of the envelope solver                */
+/* ------------------------------------------------------------------------- */
+/* Fields (nkx, nkr, -nkO:nkO, c); D+-(nkr, nkr_loc, -nkO-1:nkO+1).  Mode m at
+ * field index m + nkO, at D index m + nkO + 1.  The coupling matrices apply
+ * only between modes present on the grid (no reflected mode). */
+#define FE(a, x, k, m, c) (a)[(x) + (int64_t)nkx * ((k) + (int64_t)nkr * ((m) + nkO + (int64_t)nm * (c)))]
+#define FL(a, x, k, m, c) (a)[(x) + (int64_t)nkx * ((k) + (int64_t)nkr_loc * ((m) + nkO + (int64_t)nm * (c)))]
+#define DE(D, k, kl, m) (D)[(k) + (int64_t)nkr * ((kl) + (int64_t)nkr_loc * ((m) + nkO + 1))]
+
+/* Gradient.  fb_math_env.f90:18-61. */
+void ref_fb_grad_env(double *loc_, const double *scl_, const double *DpS2S, const double *DmS2S, const double *kx,
+                     int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc)
+{
+    cplx *loc = (cplx *)loc_;
+    const cplx *s = (const cplx *)scl_;
+    const int nm = 2 * nkO + 1;
+    memset(loc, 0, sizeof(cplx) * (size_t)nkx * nkr_loc * nm * 3);
+    cplx *sl = (cplx *)malloc(sizeof(cplx) * (size_t)nkx);
+    for (int iO = -nkO; iO <= nkO; iO++)
+        for (int kl = 0; kl < nkr_loc; kl++) {
+            for (int64_t x = 0; x < nkx; x++)
+                FL(loc, x, kl, iO, 0) = cadd(FL(loc, x, kl, iO, 0), cmuli(crs(kx[x], FE(s, x, kl, iO, 0))));
+            if (iO > -nkO) {
+                for (int64_t x = 0; x < nkx; x++) sl[x] = cmk(0, 0);
+                for (int k = 0; k < nkr; k++) {
+                    const double d = DE(DmS2S, k, kl, iO);
+                    for (int64_t x = 0; x < nkx; x++) sl[x] = cadd(sl[x], crs(d, FE(s, x, k, iO - 1, 0)));
+                }
+                for (int64_t x = 0; x < nkx; x++) {
+                    FL(loc, x, kl, iO, 1) = csub(FL(loc, x, kl, iO, 1), sl[x]);
+                    FL(loc, x, kl, iO, 2) = cadd(FL(loc, x, kl, iO, 2), cmuli(sl[x]));
+                }
+            }
+            if (iO < nkO) {
+                for (int64_t x = 0; x < nkx; x++) sl[x] = cmk(0, 0);
+                for (int k = 0; k < nkr; k++) {
+                    const double d = DE(DpS2S, k, kl, iO);
+                    for (int64_t x = 0; x < nkx; x++) sl[x] = cadd(sl[x], crs(d, FE(s, x, k, iO + 1, 0)));
+                }
+                for (int64_t x = 0; x < nkx; x++) {
+                    FL(loc, x, kl, iO, 1) = cadd(FL(loc, x, kl, iO, 1), sl[x]);
+                    FL(loc, x, kl, iO, 2) = cadd(FL(loc, x, kl, iO, 2), cmuli(sl[x]));
+                }
+            }
+        }
+    free(sl);
+}
+
+/* Divergence core of fb_div_env (fb_math_env.f90:63-104) for modes
+ * mlo..mhi of the output (nkr_loc rows, mode m at index m - mlo of nmo), the
+ * input modes -nkO..nkO; the i kx term only where the output mode exists in
+ * the input.  Shared with the first half of fb_graddiv_env. */
+static void div_env_core(cplx *out, int mlo, int mhi, const cplx *v, const double *DpS2S, const double *DmS2S,
+                         const double *kx, int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc)
+{
+    const int nm = 2 * nkO + 1, nmo = mhi - mlo + 1;
+    cplx *sl = (cplx *)malloc(sizeof(cplx) * (size_t)nkx);
+#define FO(x, kl, m) out[(x) + (int64_t)nkx * ((kl) + (int64_t)nkr_loc * ((m) - mlo))]
+    for (int iO = mlo; iO <= mhi; iO++)
+        for (int kl = 0; kl < nkr_loc; kl++) {
+            if (iO >= -nkO && iO <= nkO)
+                for (int64_t x = 0; x < nkx; x++)
+                    FO(x, kl, iO) = cadd(FO(x, kl, iO), cmuli(crs(kx[x], FE(v, x, kl, iO, 0))));
+            for (int64_t x = 0; x < nkx; x++) sl[x] = cmk(0, 0);
+            if (iO > -nkO)
+                for (int k = 0; k < nkr; k++) {
+                    const double d = DE(DmS2S, k, kl, iO);
+                    for (int64_t x = 0; x < nkx; x++)
+                        sl[x] = csub(sl[x], crs(d, csub(FE(v, x, k, iO - 1, 1), cmuli(FE(v, x, k, iO - 1, 2)))));
+                }
+            if (iO < nkO)
+                for (int k = 0; k < nkr; k++) {
+                    const double d = DE(DpS2S, k, kl, iO);
+                    for (int64_t x = 0; x < nkx; x++)
+                        sl[x] = cadd(sl[x], crs(d, cadd(FE(v, x, k, iO + 1, 1), cmuli(FE(v, x, k, iO + 1, 2)))));
+                }
+            for (int64_t x = 0; x < nkx; x++) FO(x, kl, iO) = cadd(FO(x, kl, iO), sl[x]);
+        }
+#undef FO
+    free(sl);
+    (void)nmo;
+    (void)nm;
+}
+
+/* Divergence.  fb_math_env.f90:63-104. */
+void ref_fb_div_env(double *scl_loc_, const double *vec_, const double *DpS2S, const double *DmS2S, const double *kx,
+                    int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc)
+{
+    cplx *out = (cplx *)scl_loc_;
+    memset(out, 0, sizeof(cplx) * (size_t)nkx * nkr_loc * (2 * nkO + 1));
+    div_env_core(out, -nkO, nkO, (const cplx *)vec_, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc);
+}
+
+/* Curl.  fb_math_env.f90:106-162.  In the m-1 branch the reference forms the
+ * x-component term -D-.(i v_y + v_z) and then discards it (sliceKR is reset
+ * before it is used, fb_math_env.f90:141-146); here it is not formed. */
+void ref_fb_rot_env(double *loc_, const double *vec_, const double *DpS2S, const double *DmS2S, const double *kx,
+                    int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc)
+{
+    cplx *loc = (cplx *)loc_;
+    const cplx *v = (const cplx *)vec_;
+    const int nm = 2 * nkO + 1;
+    memset(loc, 0, sizeof(cplx) * (size_t)nkx * nkr_loc * nm * 3);
+    cplx *sl = (cplx *)malloc(sizeof(cplx) * (size_t)nkx);
+    for (int iO = -nkO; iO <= nkO; iO++)
+        for (int kl = 0; kl < nkr_loc; kl++) {
+            for (int64_t x = 0; x < nkx; x++) {
+                FL(loc, x, kl, iO, 1) = csub(FL(loc, x, kl, iO, 1), cmuli(crs(kx[x], FE(v, x, kl, iO, 2))));
+                FL(loc, x, kl, iO, 2) = cadd(FL(loc, x, kl, iO, 2), cmuli(crs(kx[x], FE(v, x, kl, iO, 1))));
+            }
+            if (iO < nkO) {
+                for (int64_t x = 0; x < nkx; x++) sl[x] = cmk(0, 0);
+                for (int k = 0; k < nkr; k++) {
+                    const double d = DE(DpS2S, k, kl, iO);
+                    for (int64_t x = 0; x < nkx; x++)
+                        sl[x] = csub(sl[x], crs(d, csub(cmuli(FE(v, x, k, iO + 1, 1)), FE(v, x, k, iO + 1, 2))));
+                }
+                for (int64_t x = 0; x < nkx; x++) FL(loc, x, kl, iO, 0) = cadd(FL(loc, x, kl, iO, 0), sl[x]);
+                for (int64_t x = 0; x < nkx; x++) sl[x] = cmk(0, 0);
+                for (int k = 0; k < nkr; k++) {
+                    const double d = DE(DpS2S, k, kl, iO);
+                    for (int64_t x = 0; x < nkx; x++) sl[x] = cadd(sl[x], crs(d, FE(v, x, k, iO + 1, 0)));
+                }
+                for (int64_t x = 0; x < nkx; x++) {
+                    FL(loc, x, kl, iO, 1) = cadd(FL(loc, x, kl, iO, 1), cmuli(sl[x]));
+                    FL(loc, x, kl, iO, 2) = csub(FL(loc, x, kl, iO, 2), sl[x]);
+                }
+            }
+            if (iO > -nkO) {
+                for (int64_t x = 0; x < nkx; x++) sl[x] = cmk(0, 0);
+                for (int k = 0; k < nkr; k++) {
+                    const double d = DE(DmS2S, k, kl, iO);
+                    for (int64_t x = 0; x < nkx; x++) sl[x] = cadd(sl[x], crs(d, FE(v, x, k, iO - 1, 0)));
+                }
+                for (int64_t x = 0; x < nkx; x++) {
+                    FL(loc, x, kl, iO, 1) = cadd(FL(loc, x, kl, iO, 1), cmuli(sl[x]));
+                    FL(loc, x, kl, iO, 2) = cadd(FL(loc, x, kl, iO, 2), sl[x]);
+                }
+            }
+        }
+    free(sl);
+}
+
+/* Gradient of the divergence, in place.  fb_math_env.f90:164-233: the
+ * divergence on modes -nkO-1..nkO+1 (the outer two from the coupling terms
+ * alone), then its gradient on modes -nkO..nkO, each mode taking D- of mode
+ * m-1 and D+ of mode m+1 unconditionally. */
+void ref_fb_graddiv_env(double *vec_, const double *DpS2S, const double *DmS2S, const double *kx,
+                        int32_t nkx, int32_t nkr, int32_t nkO, int32_t nkr_loc)
+{
+    cplx *v = (cplx *)vec_;
+    const int nm = 2 * nkO + 1, nmx = nm + 2;
+    cplx *s = (cplx *)calloc((size_t)nkx * nkr_loc * nmx, sizeof(cplx));
+    div_env_core(s, -nkO - 1, nkO + 1, v, DpS2S, DmS2S, kx, nkx, nkr, nkO, nkr_loc);
+#define FS(x, k, m) s[(x) + (int64_t)nkx * ((k) + (int64_t)nkr_loc * ((m) + nkO + 1))]
+    memset(v, 0, sizeof(cplx) * (size_t)nkx * nkr * nm * 3);
+    cplx *sl = (cplx *)malloc(sizeof(cplx) * (size_t)nkx);
+    for (int iO = -nkO; iO <= nkO; iO++)
+        for (int kl = 0; kl < nkr_loc; kl++) {
+            for (int64_t x = 0; x < nkx; x++) FE(v, x, kl, iO, 0) = cadd(FE(v, x, kl, iO, 0), cmuli(crs(kx[x], FS(x, kl, iO))));
+            for (int64_t x = 0; x < nkx; x++) sl[x] = cmk(0, 0);
+            for (int k = 0; k < nkr; k++) {
+                const double d = DE(DmS2S, k, kl, iO);
+                for (int64_t x = 0; x < nkx; x++) sl[x] = cadd(sl[x], crs(d, FS(x, k, iO - 1)));
+            }
+            for (int64_t x = 0; x < nkx; x++) {
+                FE(v, x, kl, iO, 1) = csub(FE(v, x, kl, iO, 1), sl[x]);
+                FE(v, x, kl, iO, 2) = cadd(FE(v, x, kl, iO, 2), cmuli(sl[x]));
+            }
+            for (int64_t x = 0; x < nkx; x++) sl[x] = cmk(0, 0);
+            for (int k = 0; k < nkr; k++) {
+                const double d = DE(DpS2S, k, kl, iO);
+                for (int64_t x = 0; x < nkx; x++) sl[x] = cadd(sl[x], crs(d, FS(x, k, iO + 1)));
+            }
+            for (int64_t x = 0; x < nkx; x++) {
+                FE(v, x, kl, iO, 1) = cadd(FE(v, x, kl, iO, 1), sl[x]);
+                FE(v, x, kl, iO, 2) = cadd(FE(v, x, kl, iO, 2), cmuli(sl[x]));
+            }
+        }
+#undef FS
+    free(sl);
+    free(s);
+}
+#undef FE
+#undef FL
+#undef DE
+
+/* maxwell_push_wo_spchrg with complex coefficient tables, as the envelope
+ * solver passes them (maxwell_solvers.f90:62-96; solvers.py:244-267).
+ * C1, C2 complex (nkx,nkr,nkO,3); nkO = the number of modes. */
+void ref_maxwell_push_wo_spchrg_cplx(double *EG_, const double *j_, const double *C1_, const double *C2_,
+                                     int32_t nkx, int32_t nkr, int32_t nkO)
+{
+    cplx *EG = (cplx *)EG_;
+    const cplx *J = (const cplx *)j_, *C1 = (const cplx *)C1_, *C2 = (const cplx *)C2_;
+    const int64_t n = (int64_t)nkx * nkr * nkO;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < n; e++)
+        for (int l = 0; l < 3; l++) {
+            const cplx E = EG[e + l * n], G = EG[e + (3 + l) * n], j = J[e + l * n];
+            const cplx Enew = cadd(cadd(cmul(C1[e], E), cmul(C1[e + n], G)), cmul(C1[e + 2 * n], j));
+            const cplx Gnew = cadd(cadd(cmul(C2[e], E), cmul(C2[e + n], G)), cmul(C2[e + 2 * n], j));
+            EG[e + (3 + l) * n] = Gnew;
+            EG[e + l * n] = Enew;
+        }
+}

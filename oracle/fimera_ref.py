@@ -9,7 +9,8 @@ Two back ends sit behind the same Python signatures:
                               cpu_ref.c (every hot-path routine, FFTs included);
 * ``Fimera("reference")``  -> oracle/_ref/libchimera_ref.so, the reference's own
                               FFTW-free Fortran (particle_tools.f90, grid_deps.f90,
-                              maxwell_solvers.f90, fb_math.f90, devices.f90)
+                              grid_deps_env.f90, maxwell_solvers.f90, fb_math.f90,
+                              fb_math_env.f90, devices.f90)
                               compiled by oracle/Makefile; fb_io.f90 routines are
                               absent from it (FFTW3 is not available).
 
@@ -251,6 +252,56 @@ class Fimera:
         self._call("fb_graddiv", v, Dp, Dm, kx, v.shape[0], v.shape[1], v.shape[2] - 1, Dp.shape[1])
         return v
 
+    # ---------------------------------------------------------------- grid_deps_env.f90
+    # envelope arrays: mode axis -nkO..nkO (length 2 nkO + 1)
+    def dep_curr_env(self, coord, momenta, wghts, curr, leftX, Rgrid, dx_inv, dr_inv, kx0):
+        coord, momenta, wghts = _f(coord, np.float64), _f(momenta, np.float64), _f(wghts, np.float64)
+        curr, Rgrid = _f(curr, np.complex128), _f(Rgrid, np.float64)
+        self._call("dep_curr_env", coord, momenta, wghts, curr, float(leftX), Rgrid, float(dx_inv), float(dr_inv),
+                   float(kx0), coord.shape[1], curr.shape[0] - 1, curr.shape[1] - 1, (curr.shape[2] - 1) // 2)
+        return curr
+
+    def dep_dens_env(self, coord, wghts, dens, leftX, Rgrid, dx_inv, dr_inv, kx0):
+        coord, wghts = _f(coord, np.float64), _f(wghts, np.float64)
+        dens, Rgrid = _f(dens, np.complex128), _f(Rgrid, np.float64)
+        self._call("dep_dens_env", coord, wghts, dens, float(leftX), Rgrid, float(dx_inv), float(dr_inv),
+                   float(kx0), coord.shape[1], dens.shape[0] - 1, dens.shape[1] - 1, (dens.shape[2] - 1) // 2)
+        return dens
+
+    def proj_fld_env(self, coord, wghts, Fld, Fld_tot, leftX, Rgrid, dx_inv, dr_inv, kx0):
+        coord, wghts = _f(coord, np.float64), _f(wghts, np.float64)
+        Fld, Fld_tot, Rgrid = _f(Fld, np.complex128), _f(Fld_tot, np.float64), _f(Rgrid, np.float64)
+        self._call("proj_fld_env", coord, wghts, Fld, Fld_tot, float(leftX), Rgrid, float(dx_inv), float(dr_inv),
+                   float(kx0), coord.shape[1], Fld.shape[0] - 1, Fld.shape[1] - 1, (Fld.shape[2] - 1) // 2)
+        return Fld_tot
+
+    def eb_correction_env(self, eb_spc):
+        eb = _f(eb_spc, np.complex128)
+        self._call("eb_correction_env", eb, eb.shape[0] - 1, eb.shape[1] - 1, (eb.shape[2] - 1) // 2)
+        return eb
+
+    # ---------------------------------------------------------------- fb_math_env.f90
+    def _fbop_env(self, name, out, inp, DpS2S, DmS2S, kx):
+        out, inp = _f(out, np.complex128), _f(inp, np.complex128)
+        Dp, Dm, kx = _f(DpS2S, np.float64), _f(DmS2S, np.float64), _f(kx, np.float64)
+        self._call(name, out, inp, Dp, Dm, kx, inp.shape[0], inp.shape[1], (inp.shape[2] - 1) // 2, out.shape[1])
+        return out
+
+    def fb_grad_env(self, vec_fb_loc, scl_fb, DpS2S, DmS2S, kx):
+        return self._fbop_env("fb_grad_env", vec_fb_loc, scl_fb, DpS2S, DmS2S, kx)
+
+    def fb_div_env(self, scl_fb_loc, vec_fb, DpS2S, DmS2S, kx):
+        return self._fbop_env("fb_div_env", scl_fb_loc, vec_fb, DpS2S, DmS2S, kx)
+
+    def fb_rot_env(self, vec_fb_loc, vec_fb, DpS2S, DmS2S, kx):
+        return self._fbop_env("fb_rot_env", vec_fb_loc, vec_fb, DpS2S, DmS2S, kx)
+
+    def fb_graddiv_env(self, vec_fb, DpS2S, DmS2S, kx):
+        v = _f(vec_fb, np.complex128)
+        Dp, Dm, kx = _f(DpS2S, np.float64), _f(DmS2S, np.float64), _f(kx, np.float64)
+        self._call("fb_graddiv_env", v, Dp, Dm, kx, v.shape[0], v.shape[1], (v.shape[2] - 1) // 2, Dp.shape[1])
+        return v
+
     # ---------------------------------------------------------------- maxwell_solvers.f90
     def maxwell_push_with_spchrg(self, EG_fb, j_fb, grad_rho_n_fb, grad_rho_np1_fb, C1, C2):
         EG = _f(EG_fb, np.complex128)
@@ -262,9 +313,12 @@ class Fimera:
     def maxwell_push_wo_spchrg(self, EG_fb, j_fb, C1, C2):
         EG, j = _f(EG_fb, np.complex128), _f(j_fb, np.complex128)
         # the Fortran declares the tables complex (maxwell_solvers.f90:67-68); f2py casts
-        dt = np.complex128 if self._fortran else np.float64
+        # the real tables of the plain solver, the envelope solver's are complex
+        cplx = np.iscomplexobj(C1) or np.iscomplexobj(C2)
+        dt = np.complex128 if (self._fortran or cplx) else np.float64
         C1, C2 = _f(C1, dt), _f(C2, dt)
-        self._call("maxwell_push_wo_spchrg", EG, j, C1, C2, EG.shape[0], EG.shape[1], EG.shape[2])
+        name = "maxwell_push_wo_spchrg_cplx" if (cplx and not self._fortran) else "maxwell_push_wo_spchrg"
+        self._call(name, EG, j, C1, C2, EG.shape[0], EG.shape[1], EG.shape[2])
         return EG
 
     def maxwell_init_push(self, EG_fb, j_fb, grad_rho_n_fb, C1, C2):

@@ -23,7 +23,9 @@ def test_header_declares_the_reference_entry_points():
             "align_data_scl", "dep_curr", "dep_dens", "dep_curr_chnk", "dep_dens_chnk", "proj_fld", "eb_correction",
             "fb_vec_in", "fb_scl_in", "fb_eb_out", "fb_filtr", "fb_rot", "fb_grad", "fb_div", "fb_graddiv",
             "maxwell_push_with_spchrg", "maxwell_push_wo_spchrg", "maxwell_init_push", "poiss_corr", "omp_mult_vec",
-            "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic"]
+            "omp_mult_scl", "omp_add_vec", "omp_add_scl", "undul_analytic", "dep_curr_env", "dep_dens_env",
+            "dep_curr_env_chnk", "dep_dens_env_chnk", "proj_fld_env", "eb_correction_env", "fb_rot_env", "fb_grad_env",
+            "fb_div_env", "fb_graddiv_env"]
     for name in f2py:
         assert "chimera_" + name in decls, name
     # scalars by value, arrays by pointer: dep_curr has 8 pointers/doubles then 4 int32 dimensions

@@ -18,6 +18,12 @@ def say(msg):
     sys.stdout.flush()
 
 
+def _bits(a):
+    """The array's bit pattern (complex128 -> uint64 pairs, complex64 -> uint32 pairs)."""
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint64 if a.dtype == np.complex128 else np.uint32)
+
+
 def _c8_op(op, out_shape, inp, Dp, Dm, kx, nko):
     from chimera_amd.device import DeviceArray, call
     d_in = DeviceArray.from_numpy(inp, np.complex64)
@@ -52,11 +58,11 @@ def test_autotuned_tiles_same_bits(gpu, tmp_path):
         a64 = gpu.fb_rot(z3.copy(order="F"), v, Dp, Dm, kx)
         n1 = L.chimera_autotune_entries()
         assert n1 > n0
-        assert np.array_equal(a64.view(np.uint64), ref64.view(np.uint64))
+        assert np.array_equal(_bits(a64), _bits(ref64))
         for form in (0, 1):
             L.chimera_set_mixed_gemm(form)
             a8 = _c8_op(0, z3.shape, v, Dp, Dm, kx, nko)
-            assert np.array_equal(a8.view(np.uint32), ref8[form].view(np.uint32)), form
+            assert np.array_equal(_bits(a8), _bits(ref8[form])), form
         n2 = L.chimera_autotune_entries()
         gpu.fb_rot(z3.copy(order="F"), v, Dp, Dm, kx)
         assert L.chimera_autotune_entries() == n2               # tuned once per shape
