@@ -335,7 +335,8 @@ def traffic_for(kernel, workload):
 
 # kernels whose HBM traffic makes up a family's figure (the deposit family also
 # runs a small bucket sort of the slot descriptors, left out)
-TRAFFIC_KERNELS = {"dep_curr": ("k_push_deposit", "k_dep_combine"), "gather_push": ("k_gather_push",)}
+TRAFFIC_KERNELS = {"dep_curr": ("k_push_deposit", "k_dep_combine"), "gather_push": ("k_gather_push",),
+                   "psatd": ("k_psatd_spchrg",)}
 
 
 def traffic_key(workload, precision):
@@ -392,6 +393,74 @@ def summarize_traffic(root, workload, steps, out_dir, precision="fp64"):
     return rec, lines
 
 
+# MI355X: 256 CUs of 4 SIMDs in 8 XCDs; rocprofv3 sums GRBM_GUI_ACTIVE over the XCDs
+N_SIMD, N_XCD = 1024, 8
+
+
+def summarize_mfma(root, tag):
+    """MFMA counters per GEMM kernel instantiation from a rocprofv3 --pmc run
+    (pmc_<tag>/: SQ_INSTS_VALU_MFMA_MOPS_F64/_F32/_BF16 -- units of 512 FLOPs per
+    the counter definition -- SQ_VALU_MFMA_BUSY_CYCLES, SQ_BUSY_CYCLES,
+    GRBM_GUI_ACTIVE), mean per dispatch, and the MFMA busy fraction
+    SQ_VALU_MFMA_BUSY_CYCLES / (SIMDs x GPU-active cycles of one XCD) (the
+    MfmaUtil expression of the counter list).  Returns ({kernel: {...}}, lines)."""
+    import csv
+    import re
+    d = os.path.join(root, f"pmc_{tag}")
+    f = [x for x in os.listdir(d) if x.endswith("counter_collection.csv")][0]
+    per = {}
+    for r in csv.DictReader(open(os.path.join(d, f))):
+        k = r["Kernel_Name"]
+        if "k_fb_gemm" not in k:
+            continue
+        m = re.search(r"(k_fb_gemm\w*<[^>]*>|k_fb_gemm\w*)", k)
+        key = m.group(1) if m else k[:60]
+        per.setdefault(key, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    out, lines = {}, []
+    for k, cs in per.items():
+        rec = {c: float(np.mean(v)) for c, v in cs.items()}
+        rec["dispatches"] = max(len(v) for v in cs.values())
+        act = rec.get("GRBM_GUI_ACTIVE", 0.0) / N_XCD
+        rec["mfma_busy"] = rec.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (N_SIMD * act) if act else None
+        gflop = {t: rec.get(f"SQ_INSTS_VALU_MFMA_MOPS_{t}", 0.0) * 512 / 1e9 for t in ("F64", "F32", "BF16")}
+        rec["issued_gflop"] = gflop
+        out[k] = rec
+        kind = max(gflop, key=gflop.get)
+        busy = "-" if rec["mfma_busy"] is None else f"{rec['mfma_busy']:.3f}"
+        lines.append(f"| `{k}` | {rec['dispatches']} | {kind} | {gflop[kind]:.1f} | {act / 1e6:.3f} | {busy} |")
+    return out, lines
+
+
+def summarize_trace(root, steps, step_kernel="k_psatd_spchrg"):
+    """Kernel time over the last `steps` steps of a rocprofv3 --kernel-trace run
+    (root/*kernel_trace.csv): the window opens at the end of the (steps+1)-th
+    last launch of step_kernel (one launch per step).  Returns markdown lines
+    (kernel, calls, ms per step, mean us per call, share)."""
+    import csv
+    import re
+    f = [x for x in os.listdir(root) if x.endswith("kernel_trace.csv")][0]
+    rows = list(csv.DictReader(open(os.path.join(root, f))))
+    marks = sorted(int(r["End_Timestamp"]) for r in rows if step_kernel in r["Kernel_Name"])
+    if len(marks) < steps + 1:
+        raise SystemExit(f"{root}: {len(marks)} launches of {step_kernel}, need {steps + 1}")
+    t0, t1 = marks[-steps - 1], marks[-1]
+    agg = {}
+    for r in rows:
+        a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        if a < t0 or b > t1:
+            continue
+        name = r["Kernel_Name"].replace("void ", "").replace("chimera::", "")
+        m = re.match(r"([\w:]+(<[^()]*>)?)", name)
+        key = m.group(1) if m else name[:60]
+        n, t = agg.get(key, (0, 0))
+        agg[key] = (n + 1, t + (b - a))
+    tot = sum(t for _, t in agg.values())
+    lines = [f"| `{k}` | {n} | {t / 1e6 / steps:.3f} | {t / n / 1e3:.1f} | {100 * t / tot:.1f} |"
+             for k, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])]
+    lines.append(f"| all kernels | {sum(n for n, _ in agg.values())} | {tot / 1e6 / steps:.3f} | | 100 |")
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,6 +491,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--summarize-traffic", metavar="DIR",
                     help="write profiles/traffic.json from the four rocprofv3 --pmc runs under DIR")
+    ap.add_argument("--summarize-mfma", metavar="DIR", help="print the MFMA counters of DIR/pmc_<precision>_mfma")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print the kernel time of the last --steps steps of the rocprofv3 kernel trace in DIR")
     args = ap.parse_args()
     if args.summarize_traffic:
         prec = "fp64" if args.precision == "both" else args.precision
@@ -429,6 +501,15 @@ def main():
                                        os.path.join(HERE, "profiles"), prec)
         print(json.dumps(rec, indent=1))
         print("\n".join(lines))
+        return
+    if args.summarize_mfma:
+        prec = "fp64" if args.precision == "both" else args.precision
+        rec, lines = summarize_mfma(args.summarize_mfma, f"{prec}_mfma")
+        print(json.dumps(rec, indent=1))
+        print("\n".join(lines))
+        return
+    if args.summarize_trace:
+        print("\n".join(summarize_trace(args.summarize_trace, args.steps)))
         return
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
