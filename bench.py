@@ -308,6 +308,46 @@ def serve_bench(args):
     return out
 
 
+def bunch_stages(nx, nkr, precision, particles=3e5, steps=20, reps=50, seed=5):
+    """One bunch of workloads.beam (space charge, analytic undulator, window
+    moving with the bunch) on an nx x nkr grid, modes 0-1: after three steps,
+    each FB stage of the step is called `reps` times and timed (wall clock
+    around the calls, us per call), then `steps` whole steps."""
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    from chimera_amd._lib import lib
+    L = lib()
+    spec = W.with_precision(W.beam(Nx=nx, Nkr=nkr, gamma=200.0, sig_x=4.0, sig_r=2.0, emittance=0.5,
+                                   n_target=particles, K=2.0, lambda_u=300.0), precision)
+    run = W.build_beam(G, spec, seed)
+    for i in (1, 2, 3):
+        run.make_step(i)
+    L.chimera_synchronize()
+    s = run.Solvers[0]
+    print(f"bunch {nx} x {nkr}, modes 0-1, {run.Particles[0].np} particles, {precision}", flush=True)
+
+    def timed(name, fn):
+        fn()
+        L.chimera_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        L.chimera_synchronize()
+        print(f"{name:24s} {(time.perf_counter() - t0) / reps * 1e6:8.1f} us", flush=True)
+
+    timed("fb_curr_in", s.fb_curr_in)
+    timed("fb_dens_in", s.fb_dens_in)
+    timed("FBGradDens", s.FBGradDens)
+    timed("poiss_corr (3 iter)", s.poiss_corr)
+    timed("G2B_FBRot", s.G2B_FBRot)
+    timed("fb_fld_out", s.fb_fld_out)
+    t0 = time.perf_counter()
+    for i in range(4, 4 + steps):
+        run.make_step(i)
+    L.chimera_synchronize()
+    print(f"{'step':24s} {(time.perf_counter() - t0) / steps * 1e6:8.1f} us", flush=True)
+
+
 def copy_calibration(gib):
     """One device-to-device copy of a known size: the FETCH_SIZE / WRITE_SIZE it
     reads under rocprofv3 --pmc calibrate the counters of the PIC kernels."""
@@ -477,6 +517,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=1, help="1: GEMM tile shapes chosen by timing (first use)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-slab", type=int, default=8, help="the CPU baseline runs 1/N of the workload's x extent")
+    ap.add_argument("--bunch-stages", type=int, nargs=2, metavar=("NX", "NKR"),
+                    help="time the FB stages and the step of one bunch on an NX x NKR grid (--precision fp64|mixed)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB gibibytes device to device once (reference kernel for the HBM counters)")
     ap.add_argument("--no-scan", action="store_true", help="skip the batched bunch scan (scan object)")
@@ -510,6 +552,9 @@ def main():
         return
     if args.summarize_trace:
         print("\n".join(summarize_trace(args.summarize_trace, args.steps)))
+        return
+    if args.bunch_stages:
+        bunch_stages(*args.bunch_stages, "mixed" if args.precision == "both" else args.precision)
         return
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
