@@ -14,6 +14,7 @@ chimera_main.py line by line (cited per method).
 """
 from __future__ import annotations
 
+import contextlib
 import inspect
 
 import numpy as np
@@ -29,6 +30,22 @@ def _ch():
     if ch is None:
         ch = Fimera("port")
     return ch
+
+
+@contextlib.contextmanager
+def rounding_variant():
+    """Runs built and stepped inside this block use the port compiled with FMA
+    contraction: the same operations in the same order, each multiply-add
+    rounded once instead of twice.  Its spread from the plain port is the
+    oracle's own rounding floor (which a shuffled particle order does not
+    show for quantities whose rounding is amplified by cancellation)."""
+    global ch
+    saved = ch
+    ch = Fimera("port_fma")
+    try:
+        yield
+    finally:
+        ch = saved
 
 
 class Solver:

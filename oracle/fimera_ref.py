@@ -31,6 +31,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PORT_LIB = os.path.join(_HERE, "_build", "libcpu_ref.so")
 REF_LIB = os.path.join(_HERE, "_ref", "libchimera_ref.so")
+# the port compiled with FMA contraction: the same arithmetic rounded differently
+PORT_FMA_LIB = os.path.join(_HERE, "_build", "libcpu_ref_fma.so")
 
 _P = ctypes.c_void_p
 
@@ -48,11 +50,11 @@ def _ptr(a):
 
 
 class Fimera:
-    """f2py-shaped wrapper over one oracle back end ("port" or "reference")."""
+    """f2py-shaped wrapper over one oracle back end ("port", "port_fma" or "reference")."""
 
     def __init__(self, backend: str = "port"):
         self.backend = backend
-        path = PORT_LIB if backend == "port" else REF_LIB
+        path = {"port": PORT_LIB, "port_fma": PORT_FMA_LIB, "reference": REF_LIB}[backend]
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library {path} is not built (run `make -C oracle` / `make -C oracle ref`)")
         self._lib = ctypes.CDLL(path)

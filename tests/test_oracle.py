@@ -380,3 +380,26 @@ def test_particles_outside_the_grid_are_skipped(port):
     Fld = C(r, g.nx, g.nkr + 1, 2, 6)
     ft = port.proj_fld(mix(coord, out), mix(w, wo), Fld, np.zeros((6, coord.shape[1] + 4), order="F"), *args)
     assert np.all(ft[:, -4:] == 0.0) and np.any(ft[:, :-4] != 0.0)
+
+
+def test_fma_variant_rounds_differently_only(port):
+    """The port compiled with FMA contraction (the oracle's rounding-floor
+    variant) computes the same arithmetic: it agrees with the plain port to
+    rounding (1e-14 relative on a deposit and a Hankel transform) and is not
+    bit-identical to it (the contraction is on)."""
+    from oracle.fimera_ref import Fimera
+    fma = Fimera("port_fma")
+    g = Grid(nx=48, nkr=12, nko=2)
+    coord, mom, w = particles(g, 3000, seed=77)
+    args = (g.leftX, g.Rgrid, 1 / g.dx, 1 / g.dr)
+    z = np.zeros((g.nx, g.nkr + 1, 3, 3), complex, order="F")
+    a = port.dep_curr(coord, mom, w, z.copy(order="F"), *args)
+    b = fma.dep_curr(coord, mom, w, z.copy(order="F"), *args)
+    assert np.linalg.norm(a - b) <= 1e-14 * np.linalg.norm(a)
+    r = rng(78)
+    vec, In = C(r, g.nx, g.nkr + 1, 3, 3), F(r, g.nkr, g.nkr, 3)
+    zf = np.zeros((g.nx, g.nkr, 3, 3), complex, order="F")
+    a = port.fb_vec_in(zf.copy(order="F"), vec, g.leftX, g.kx, In)
+    b = fma.fb_vec_in(zf.copy(order="F"), vec, g.leftX, g.kx, In)
+    assert np.linalg.norm(a - b) <= 1e-14 * np.linalg.norm(a)
+    assert not np.array_equal(a, b)
