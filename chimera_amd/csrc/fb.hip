@@ -85,10 +85,10 @@ __device__ __forceinline__ void fft_store(const FftArgsT<R>& a, int64_t line, in
     }
 }
 
-template <int EPT, typename R>
-__global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
+template <int EPT, typename R, int TPB = FFT_THREADS>
+__global__ void __launch_bounds__(TPB) k_fft_lines(FftArgsT<R> a) {
     typedef cplx_t<R> C;
-    constexpr int TOT = EPT * FFT_THREADS;
+    constexpr int TOT = EPT * TPB;
     extern __shared__ __align__(16) unsigned char fft_smem[];
     C* const sbuf = reinterpret_cast<C*>(fft_smem);
     const int N = a.N;
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
         C vin[EPT];
 #pragma unroll
         for (int j = 0; j < EPT; j++) {
-            const int i = threadIdx.x + FFT_THREADS * j;
+            const int i = threadIdx.x + TPB * j;
             const int64_t line = line0 + i / N;
             vin[j] = cmake<R>(0, 0);
             if (line < a.nlines) {
@@ -114,14 +114,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
         }
         C* const stw = sbuf + TOT;
         if (a.tw_lds) {
-            for (int i = threadIdx.x; i < N; i += FFT_THREADS) stw[i] = a.tw[i];
+            for (int i = threadIdx.x; i < N; i += TPB) stw[i] = a.tw[i];
             tw = stw;
         }
         if (a.ph_at) {
             // the table of k_phase, formed here: one launch fewer per transform
             C* const p = stw + N;
             const double lx = a.ph_leftX_dev ? *a.ph_leftX_dev : a.ph_leftX;
-            for (int i = threadIdx.x; i < N; i += FFT_THREADS) {
+            for (int i = threadIdx.x; i < N; i += TPB) {
                 double sn, cs;
                 sincos(lx * a.ph_kx[i], &sn, &cs);
                 p[i] = cmake<R>((R)cs, (R)(a.ph_s * sn));
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
         }
 #pragma unroll
         for (int j = 0; j < EPT; j++) {
-            const int i = threadIdx.x + FFT_THREADS * j;
+            const int i = threadIdx.x + TPB * j;
             const int x = i % N;
             C v = vin[j];
             if (line0 + i / N < a.nlines) {
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
         }
     } else {
         // ---- load (with the per-x pre-multiplier)
-        for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+        for (int i = threadIdx.x; i < TOT; i += TPB) {
             const int64_t line = line0 + i / N;
             const int x = i % N;
             C v = cmake<R>(0, 0);
@@ -155,18 +155,18 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_lines(FftArgsT<R> a) {
         }
     }
     __syncthreads();
-    fft_lds<EPT, R>(sbuf, N, a.log2N, a.sign, tw);
+    fft_lds<EPT, R, TPB>(sbuf, N, a.log2N, a.sign, tw);
     if (a.filtr) {
         // fb_io.f90:263: multiply the first nxfilt points, transform back
-        for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+        for (int i = threadIdx.x; i < TOT; i += TPB) {
             const int x = i % N;
             if (x < a.nxfilt) sbuf[i] = crs(a.filtr[x], sbuf[i]);
         }
         __syncthreads();
-        fft_lds<EPT, R>(sbuf, N, a.log2N, -a.sign, tw);
+        fft_lds<EPT, R, TPB>(sbuf, N, a.log2N, -a.sign, tw);
     }
     // ---- store (with post-multiplier, real scale table, mode normalisation)
-    for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+    for (int i = threadIdx.x; i < TOT; i += TPB) {
         const int64_t line = line0 + i / N;
         if (line >= a.nlines) continue;
         const int x = i % N;
@@ -367,6 +367,9 @@ static int32_t launch_fft_bluestein(FftArgsT<R> a) {
     return ST_OK;
 }
 
+// Threads per workgroup for lines of 4096 and 8192 points (EPT points each).
+constexpr int FFT_BIG_TPB = 1024, FFT_BIG_EPT4 = 4096 / FFT_BIG_TPB, FFT_BIG_EPT8 = 8192 / FFT_BIG_TPB;
+
 template <typename R>
 int32_t launch_fft(FftArgsT<R> a) {
     typedef cplx_t<R> C;
@@ -380,8 +383,13 @@ int32_t launch_fft(FftArgsT<R> a) {
     if constexpr (sizeof(R) == 8) a.tw = fft_twiddles(N);
     else a.tw = fft_twiddles_f(N);
     if (!a.tw) return ST_ENOMEM;
-    int ept = N <= 2048 ? 8 : N / FFT_THREADS;
-    const int tot = ept * FFT_THREADS;
+    // lines of up to 2048 points: 256 threads of 8 points, several lines per
+    // workgroup; longer lines fill a workgroup's LDS alone (64 and 128 KiB in
+    // complex128) and run with FFT_BIG_TPB threads, so that the CU holds 16
+    // waves to hide the latency of the passes' LDS traffic instead of four
+    const int tpb = N <= 2048 ? FFT_THREADS : FFT_BIG_TPB;
+    const int ept = N <= 2048 ? 8 : N / tpb;
+    const int tot = ept * tpb;
     const int lpb = tot / N;
     const unsigned grid = (unsigned)((a.nlines + lpb - 1) / lpb);
     // LDS of an EPT = 8 workgroup kept within 48 KiB (three per CU): the
@@ -401,25 +409,23 @@ int32_t launch_fft(FftArgsT<R> a) {
         // workgroup; EPT = 8 at most 64 KiB (2048 complex128 points and their twiddles)
         CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<8, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(64 * 1024)));
-        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<16, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(16 * FFT_THREADS * sizeof(C))));
-        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<32, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(32 * FFT_THREADS * sizeof(C))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<FFT_BIG_EPT4, R, FFT_BIG_TPB>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(FFT_BIG_EPT4 * FFT_BIG_TPB * sizeof(C))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<FFT_BIG_EPT8, R, FFT_BIG_TPB>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(FFT_BIG_EPT8 * FFT_BIG_TPB * sizeof(C))));
         attr_done = true;
     }
     KTimer kt(KF_FFT);
-    switch (ept) {
-        case 8:
-            hipLaunchKernelGGL((k_fft_lines<8, R>), grid, FFT_THREADS, shmem, stream(), a);
-            break;
-        case 16:
-            hipLaunchKernelGGL((k_fft_lines<16, R>), grid, FFT_THREADS, shmem, stream(), a);
-            break;
-        case 32:
-            hipLaunchKernelGGL((k_fft_lines<32, R>), grid, FFT_THREADS, shmem, stream(), a);
-            break;
-        default:
-            return fail(ST_EUNSUP, "FFT length %d", N);
+    if (tpb == FFT_THREADS && ept == 8) {
+        hipLaunchKernelGGL((k_fft_lines<8, R>), grid, FFT_THREADS, shmem, stream(), a);
+    } else if (tpb == FFT_BIG_TPB && ept == FFT_BIG_EPT4) {
+        hipLaunchKernelGGL((k_fft_lines<FFT_BIG_EPT4, R, FFT_BIG_TPB>), grid, FFT_BIG_TPB, shmem, stream(), a);
+    } else if (tpb == FFT_BIG_TPB && ept == FFT_BIG_EPT8) {
+        hipLaunchKernelGGL((k_fft_lines<FFT_BIG_EPT8, R, FFT_BIG_TPB>), grid, FFT_BIG_TPB, shmem, stream(), a);
+    } else {
+        return fail(ST_EUNSUP, "FFT length %d", N);
     }
     CH_LAUNCH_CHECK("k_fft_lines");
     return ST_OK;
@@ -507,12 +513,20 @@ __device__ __forceinline__ float2 seg_finish(const FbSegT<float>& s, float2 v0, 
     return v;
 }
 
+// D stage in LDS: column-major, column n at n*GB_LDK, GB_LDK = GB_K + 1 doubles
+// (odd), so that the 16 columns a quarter-wave reads for one k, and the two k
+// of a half-wave, fall on distinct banks, and the 16 k of one column that a
+// quarter-wave stores are contiguous.  (Row-major rows padded to 144 doubles
+// put the stores of 16 consecutive k on two bank pairs: an 8-way conflict on
+// every stage.)
+constexpr int GB_LDK = GB_K + 1;
+
 // One K-stage (GB_K = 16) of a wave's (16*NI) x (16*WN) tile: NI x WN MFMA
 // tiles (NI = 4 for 64-complex x tiles, 2 for 32-complex ones).
 template <int WN, int NI>
 __device__ __forceinline__ void mfma_stage(dbl4 (&acc)[NI][WN], const double* __restrict__ As,
                                            const double* __restrict__ Ds, int lane, int wr, int wc) {
-    constexpr int LDA = 32 * NI + GB_PAD, LDD = 32 * WN + GB_PAD;
+    constexpr int LDA = 32 * NI + GB_PAD;
 #pragma unroll
     for (int kq = 0; kq < GB_K / 4; kq++) {
         const int kk = kq * 4 + (lane >> 4);
@@ -520,7 +534,7 @@ __device__ __forceinline__ void mfma_stage(dbl4 (&acc)[NI][WN], const double* __
 #pragma unroll
         for (int i = 0; i < NI; i++) a[i] = As[kk * LDA + wr * 16 * NI + i * 16 + (lane & 15)];
 #pragma unroll
-        for (int j = 0; j < WN; j++) b[j] = Ds[kk * LDD + wc * 16 * WN + j * 16 + (lane & 15)];
+        for (int j = 0; j < WN; j++) b[j] = Ds[(wc * 16 * WN + j * 16 + (lane & 15)) * GB_LDK + kk];
 #pragma unroll
         for (int i = 0; i < NI; i++)
 #pragma unroll
@@ -530,7 +544,7 @@ __device__ __forceinline__ void mfma_stage(dbl4 (&acc)[NI][WN], const double* __
 
 // LDS bytes of one k_fb_gemm<NACC, WN, GX> block: A and D stages, double-buffered.
 constexpr size_t gb_lds_bytes(int WN, int GX) {
-    return 2 * (size_t)GB_K * ((2 * GX + GB_PAD) + (32 * WN + GB_PAD)) * sizeof(double);
+    return 2 * ((size_t)GB_K * (2 * GX + GB_PAD) + (size_t)32 * WN * GB_LDK) * sizeof(double);
 }
 
 // Fast-path loads of the K-stages.  With the thread layout of the fetch
@@ -721,8 +735,8 @@ __device__ __forceinline__ void gemm_epilogue_serial(const FbJobT<R>& J, const F
 // buffer, with one barrier per stage.
 template <int NACC, int WN, int GX>
 __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restrict__ jobs, FbGemmArgs g) {
-    constexpr int NI = GX / 16, BN = 32 * WN, LDA = 2 * GX + GB_PAD, LDD = BN + GB_PAD;
-    constexpr int A_STAGE = GB_K * LDA, D_STAGE = GB_K * LDD;
+    constexpr int NI = GX / 16, BN = 32 * WN, LDA = 2 * GX + GB_PAD;
+    constexpr int A_STAGE = GB_K * LDA, D_STAGE = BN * GB_LDK;
     const FbJob& J = jobs[blockIdx.z];
     const int Ncol = J.Ncol, K = J.K, nseg = J.nseg;
     const int n0 = blockIdx.y * BN;
@@ -732,7 +746,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
 
     extern __shared__ double gb_smem[];
     double* const As = gb_smem;                  // [2][GB_K][LDA]
-    double* const Ds = gb_smem + 2 * A_STAGE;    // [2][GB_K][LDD]
+    double* const Ds = gb_smem + 2 * A_STAGE;    // [2][BN][GB_LDK]
 
     dbl4 acc0[NI][WN], acc1[NI][NACC == 2 ? WN : 1];
 #pragma unroll
@@ -820,7 +834,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
 #pragma unroll
         for (int i = 0; i < ND; i++) {
             const int e = tid + GB_THREADS * i;
-            D[(e % GB_K) * LDD + e / GB_K] = rd[i];
+            D[(e / GB_K) * GB_LDK + e % GB_K] = rd[i];
         }
     };
     auto stash = [&](int buf, int t) __attribute__((always_inline)) {
@@ -1350,19 +1364,31 @@ static int32_t launch_gemm_variant_f32(const FbJobT<float>* dj, size_t njobs, in
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int GBB_K = 32, GBB_RS = GBB_K + 8;          // K per stage; LDS row stride in BF16
 
-__device__ __forceinline__ uint32_t bf16_rne_bits(float v) {
-    const uint32_t u = __float_as_uint(v);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
-}
-// v = h + m + l exactly (finite v of normal magnitude); pieces as BF16 bit patterns
+// v = h + m + l exactly (finite v of normal magnitude); pieces as BF16 bit
+// patterns.  Each piece is the round-to-nearest-even BF16 of what is left
+// (gfx950's v_cvt_pk_bf16_f32); the remainders are exact in FP32, since a
+// piece keeps the leading 8 bits of its remainder.
 __device__ __forceinline__ void split3_bf16(float v, uint16_t& h, uint16_t& m, uint16_t& l) {
-    const uint32_t hb = bf16_rne_bits(v);
-    const float r1 = v - __uint_as_float(hb);
-    const uint32_t mb = bf16_rne_bits(r1);
-    const float r2 = r1 - __uint_as_float(mb);
-    h = (uint16_t)(hb >> 16);
-    m = (uint16_t)(mb >> 16);
-    l = (uint16_t)(bf16_rne_bits(r2) >> 16);
+    const __bf16 bh = (__bf16)v;
+    const float r1 = v - (float)bh;
+    const __bf16 bm = (__bf16)r1;
+    const float r2 = r1 - (float)bm;
+    h = __builtin_bit_cast(uint16_t, bh);
+    m = __builtin_bit_cast(uint16_t, bm);
+    l = __builtin_bit_cast(uint16_t, (__bf16)r2);
+}
+
+// n consecutive BF16 bit patterns (n = 2, 4 or 8; dst aligned to 2n bytes)
+// as one LDS store
+template <int n>
+__device__ __forceinline__ void lds_store_bf16(uint16_t* dst, const uint16_t (&v)[n]) {
+    static_assert(n == 2 || n == 4 || n == 8, "2, 4 or 8 pieces");
+    uint32_t w[n / 2];
+#pragma unroll
+    for (int i = 0; i < n / 2; i++) w[i] = (uint32_t)v[2 * i] | ((uint32_t)v[2 * i + 1] << 16);
+    if constexpr (n == 8) *reinterpret_cast<uint4*>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+    else if constexpr (n == 4) *reinterpret_cast<uint2*>(dst) = make_uint2(w[0], w[1]);
+    else *reinterpret_cast<uint32_t*>(dst) = w[0];
 }
 
 constexpr size_t gb_lds_bytes_bf16(int WN, int GX) {
@@ -1464,32 +1490,28 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_bf16(const FbJobT<flo
             split3_bf16(v.x, hr[q], mr[q], lr[q]);
             split3_bf16(v.y, hi[q], mi[q], li[q]);
         }
+        // (rows 80 bytes apart and kb a multiple of KA: every store is aligned to its size)
         uint16_t* const a_re = As + (2 * xl) * GBB_RS + kb;
         uint16_t* const a_im = a_re + GBB_RS;
         constexpr int PA = ROWS * GBB_RS;
-#pragma unroll
-        for (int q = 0; q < KA; q++) {
-            a_re[q] = hr[q];
-            a_re[PA + q] = mr[q];
-            a_re[2 * PA + q] = lr[q];
-            a_im[q] = hi[q];
-            a_im[PA + q] = mi[q];
-            a_im[2 * PA + q] = li[q];
-        }
+        lds_store_bf16<KA>(a_re, hr);
+        lds_store_bf16<KA>(a_re + PA, mr);
+        lds_store_bf16<KA>(a_re + 2 * PA, lr);
+        lds_store_bf16<KA>(a_im, hi);
+        lds_store_bf16<KA>(a_im + PA, mi);
+        lds_store_bf16<KA>(a_im + 2 * PA, li);
         constexpr int PD = BN * GBB_RS;
 #pragma unroll
         for (int c = 0; c < DC; c++) {
             const int nl = dn + (GB_THREADS / 4) * c;
             if (nl >= BN) continue;
             uint16_t* const d = Ds + nl * GBB_RS + dk;
+            uint16_t h[8], m[8], l[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                uint16_t h, m, l;
-                split3_bf16(rd[c][q], h, m, l);
-                d[q] = h;
-                d[PD + q] = m;
-                d[2 * PD + q] = l;
-            }
+            for (int q = 0; q < 8; q++) split3_bf16(rd[c][q], h[q], m[q], l[q]);
+            lds_store_bf16<8>(d, h);
+            lds_store_bf16<8>(d + PD, m);
+            lds_store_bf16<8>(d + 2 * PD, l);
         }
     };
     auto stash = [&](int t) __attribute__((always_inline)) {

@@ -1,6 +1,6 @@
 // fft.hpp -- batched complex FFT along x (R = double: complex128, R = float:
-// complex64), one block of 256 threads per TOT = 256*EPT points of LDS (one
-// line of length N = TOT, or TOT/N lines).
+// complex64), one block of TPB threads (256 by default) per TOT = TPB*EPT
+// points of LDS (one line of length N = TOT, or TOT/N lines).
 //
 // Stockham autosort formulation: every pass reads R points at stride N/R,
 // twiddles them, does a radix-R DFT and writes them back in natural order, so
@@ -24,7 +24,7 @@ __device__ __forceinline__ C tw_get(const C* __restrict__ tw, int idx, int sign)
     return sign < 0 ? w : cconj(w);
 }
 
-template <int EPT, typename R>
+template <int EPT, typename R, int TPB = FFT_THREADS>
 __device__ __forceinline__ void fft_pass_r4(cplx_t<R>* buf, int N, int Ns, int sign, const cplx_t<R>* __restrict__ tw) {
     typedef cplx_t<R> C;
     constexpr int NB = EPT / 4;
@@ -33,7 +33,7 @@ __device__ __forceinline__ void fft_pass_r4(cplx_t<R>* buf, int N, int Ns, int s
     C v[NB][4];
 #pragma unroll
     for (int q = 0; q < NB; q++) {
-        const int b = tid + FFT_THREADS * q;
+        const int b = tid + TPB * q;
         const int line = b / q4, j = b - line * q4;
         const C* src = buf + line * N + j;
 #pragma unroll
@@ -42,7 +42,7 @@ __device__ __forceinline__ void fft_pass_r4(cplx_t<R>* buf, int N, int Ns, int s
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NB; q++) {
-        const int b = tid + FFT_THREADS * q;
+        const int b = tid + TPB * q;
         const int line = b / q4, j = b - line * q4;
         const int k = j % Ns;
         if (Ns > 1) {
@@ -63,7 +63,7 @@ __device__ __forceinline__ void fft_pass_r4(cplx_t<R>* buf, int N, int Ns, int s
     __syncthreads();
 }
 
-template <int EPT, typename R>
+template <int EPT, typename R, int TPB = FFT_THREADS>
 __device__ __forceinline__ void fft_pass_r2(cplx_t<R>* buf, int N, int Ns, int sign, const cplx_t<R>* __restrict__ tw) {
     typedef cplx_t<R> C;
     constexpr int NB = EPT / 2;
@@ -72,7 +72,7 @@ __device__ __forceinline__ void fft_pass_r2(cplx_t<R>* buf, int N, int Ns, int s
     C v[NB][2];
 #pragma unroll
     for (int q = 0; q < NB; q++) {
-        const int b = tid + FFT_THREADS * q;
+        const int b = tid + TPB * q;
         const int line = b / q2, j = b - line * q2;
         v[q][0] = buf[line * N + j];
         v[q][1] = buf[line * N + j + q2];
@@ -80,7 +80,7 @@ __device__ __forceinline__ void fft_pass_r2(cplx_t<R>* buf, int N, int Ns, int s
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NB; q++) {
-        const int b = tid + FFT_THREADS * q;
+        const int b = tid + TPB * q;
         const int line = b / q2, j = b - line * q2;
         const int k = j % Ns;
         if (Ns > 1) v[q][1] = cmul(v[q][1], tw_get(tw, k * (N / (Ns * 2)), sign));
@@ -92,15 +92,15 @@ __device__ __forceinline__ void fft_pass_r2(cplx_t<R>* buf, int N, int Ns, int s
 }
 
 // In-place transform of the TOT/N lines held in buf (N a power of two).
-template <int EPT, typename R>
+template <int EPT, typename R, int TPB = FFT_THREADS>
 __device__ __forceinline__ void fft_lds(cplx_t<R>* buf, int N, int log2N, int sign, const cplx_t<R>* __restrict__ tw) {
     int Ns = 1;
     if (log2N & 1) {
-        fft_pass_r2<EPT, R>(buf, N, Ns, sign, tw);
+        fft_pass_r2<EPT, R, TPB>(buf, N, Ns, sign, tw);
         Ns = 2;
     }
     while (Ns < N) {
-        fft_pass_r4<EPT, R>(buf, N, Ns, sign, tw);
+        fft_pass_r4<EPT, R, TPB>(buf, N, Ns, sign, tw);
         Ns *= 4;
     }
 }
