@@ -766,19 +766,27 @@ static int32_t gather_push_impl(const double* coord, R* momenta, const R* wghts,
     return ST_OK;
 }
 
+// Stable removal of the particles outside lims.  With destination arrays
+// (dst, leading dimension ld_dst) the kept particles are scattered there and
+// the sources are left as they were; without, they are scattered to scratch
+// memory and copied back in place.
 template <typename R>
 static int32_t compact_impl(double* coord, double* coord_half, R* momenta, R* wghts, int64_t ld, int32_t np,
-                            const double* lims, int32_t test_half, int32_t* np_out) {
+                            const double* lims, int32_t test_half, int32_t* np_out, double* d_coord = nullptr,
+                            double* d_half = nullptr, R* d_mom = nullptr, R* d_w = nullptr, int64_t ld_dst = 0) {
     CH_TRY(check_resident(np, ld, 0));
     if (lims == nullptr || np_out == nullptr) return fail(ST_EINVAL, "soa_compact: null lims/np_out");
     if (np == 0) {
         *np_out = 0;
         return ST_OK;
     }
+    const bool into = d_coord != nullptr;
+    if (into && (!d_half || !d_mom || !d_w || ld_dst < np))
+        return fail(ST_EINVAL, "soa_compact_into: four destination arrays of leading dimension >= np");
     int32_t* flag = static_cast<int32_t*>(scratch(4, (size_t)np * sizeof(int32_t)));
     int64_t* pos = static_cast<int64_t*>(scratch(5, ((size_t)np + 8) * sizeof(int64_t)));
-    double* tmp = static_cast<double*>(scratch(7, (size_t)np * 3 * sizeof(double)));
-    if (!flag || !pos || !tmp) return ST_ENOMEM;
+    double* tmp = into ? nullptr : static_cast<double*>(scratch(7, (size_t)np * 3 * sizeof(double)));
+    if (!flag || !pos || (!into && !tmp)) return ST_ENOMEM;
     int64_t* total = pos + np;
     const double* test = test_half ? coord_half : coord;
     KTimer kt(KF_SORT);
@@ -786,9 +794,16 @@ static int32_t compact_impl(double* coord, double* coord_half, R* momenta, R* wg
                        lims[3], flag, np);
     CH_LAUNCH_CHECK("k_keep_flags");
     CH_TRY(exclusive_scan(flag, pos, np, total, 6));
-    // each array: scatter the kept particles to the work buffer, copy back
-    auto one = [&](auto* arr, int nc) -> int32_t {
+    // each array: scatter the kept particles to the destination, or to the
+    // work buffer and back
+    auto one = [&](auto* arr, decltype(arr) dst, int nc) -> int32_t {
         typedef typename std::remove_pointer<decltype(arr)>::type T;
+        if (into) {
+            hipLaunchKernelGGL((k_compact_scatter<T>), blocks_for(np, 256), 256, 0, stream(), flag, pos, arr, dst, ld,
+                               ld_dst, nc, np);
+            CH_LAUNCH_CHECK("k_compact_scatter");
+            return ST_OK;
+        }
         T* t = reinterpret_cast<T*>(tmp);
         hipLaunchKernelGGL((k_compact_scatter<T>), blocks_for(np, 256), 256, 0, stream(), flag, pos, arr, t, ld,
                            (int64_t)np, nc, np);
@@ -797,10 +812,10 @@ static int32_t compact_impl(double* coord, double* coord_half, R* momenta, R* wg
                                 hipMemcpyDeviceToDevice, stream()));
         return ST_OK;
     };
-    CH_TRY(one(coord, 3));
-    CH_TRY(one(coord_half, 3));
-    CH_TRY(one(momenta, 3));
-    CH_TRY(one(wghts, 1));
+    CH_TRY(one(coord, d_coord, 3));
+    CH_TRY(one(coord_half, d_half, 3));
+    CH_TRY(one(momenta, d_mom, 3));
+    CH_TRY(one(wghts, d_w, 1));
     int64_t h = 0;
     CH_HIP(hipMemcpyAsync(&h, total, sizeof h, hipMemcpyDeviceToHost, stream()));
     CH_HIP(hipStreamSynchronize(stream()));
@@ -1007,6 +1022,26 @@ int32_t chimera_soa_compact(double* coord, double* coord_half, double* momenta, 
 int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* momenta, float* wghts, int64_t ld,
                                   int32_t np, const double* lims, int32_t test_half, int32_t* np_out) {
     return compact_impl<float>(coord, coord_half, momenta, wghts, ld, np, lims, test_half, np_out);
+}
+
+int32_t chimera_soa_compact_into(const double* coord, const double* coord_half, const double* momenta,
+                                 const double* wghts, int64_t ld, int32_t np, double* d_coord, double* d_half,
+                                 double* d_mom, double* d_w, int64_t ld_dst, const double* lims, int32_t test_half,
+                                 int32_t* np_out) {
+    if (!d_coord) return fail(ST_EINVAL, "soa_compact_into: null destination");
+    return compact_impl<double>(const_cast<double*>(coord), const_cast<double*>(coord_half),
+                                const_cast<double*>(momenta), const_cast<double*>(wghts), ld, np, lims, test_half,
+                                np_out, d_coord, d_half, d_mom, d_w, ld_dst);
+}
+
+int32_t chimera_soa_compact_into_mixed(const double* coord, const double* coord_half, const float* momenta,
+                                       const float* wghts, int64_t ld, int32_t np, double* d_coord, double* d_half,
+                                       float* d_mom, float* d_w, int64_t ld_dst, const double* lims,
+                                       int32_t test_half, int32_t* np_out) {
+    if (!d_coord) return fail(ST_EINVAL, "soa_compact_into: null destination");
+    return compact_impl<float>(const_cast<double*>(coord), const_cast<double*>(coord_half),
+                               const_cast<float*>(momenta), const_cast<float*>(wghts), ld, np, lims, test_half,
+                               np_out, d_coord, d_half, d_mom, d_w, ld_dst);
 }
 
 

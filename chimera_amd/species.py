@@ -278,8 +278,18 @@ class Specie:
         half = ("Xchunked" in A and "NoSorting" not in wind["Features"] and position == "cntr")
         lims = np.ascontiguousarray(SimDom, dtype=np.float64)
         nout = ctypes.c_int32(0)
-        call("chimera_soa_compact_mixed" if self.mixed else "chimera_soa_compact", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"),
-             self.ptr("weights"), self.ld, self.np, lims.ctypes.data, 1 if half else 0, ctypes.addressof(nout))
+        # the survivors go to a spare set of arrays, which then becomes the
+        # species' arrays: no copy back (the spares stay allocated)
+        spare = getattr(self, "_spare", None)        # shared with sort_cells
+        if spare is None or spare["weights"].shape[0] != self.cap:
+            spare = {k: DeviceArray((self.cap, 3), np.float64) for k in ("coords", "coords_halfstep")}
+            spare["momenta"] = DeviceArray((self.cap, 3), self.rt)
+            spare["weights"] = DeviceArray((self.cap,), self.rt)
+        call("chimera_soa_compact_into_mixed" if self.mixed else "chimera_soa_compact_into", self.ptr("coords"),
+             self.ptr("coords_halfstep"), self.ptr("momenta"), self.ptr("weights"), self.ld, self.np,
+             spare["coords"].ptr, spare["coords_halfstep"].ptr, spare["momenta"].ptr, spare["weights"].ptr, self.cap,
+             lims.ctypes.data, 1 if half else 0, ctypes.addressof(nout))
+        self._spare, self.Data = self.Data, spare
         self.np = int(nout.value)
 
     def sort_cells(self, solver, SimDom=None, position="stag"):

@@ -438,6 +438,13 @@ int32_t chimera_soa_gather_push_dev(const double* coord, double* momenta, const 
  * receives the new count.  = sortpartsout + align_data_* (species.py:385-398). */
 int32_t chimera_soa_compact(double* coord, double* coord_half, double* momenta, double* wghts, int64_t ld, int32_t np,
                             const double* lims, int32_t test_half, int32_t* np_out);
+/* The same selection written to a second set of arrays (leading dimension
+ * ld_dst >= np) and the sources left as they were: a caller that keeps spare
+ * arrays swaps them in instead of copying the survivors back. */
+int32_t chimera_soa_compact_into(const double* coord, const double* coord_half, const double* momenta,
+                                 const double* wghts, int64_t ld, int32_t np, double* d_coord, double* d_half,
+                                 double* d_mom, double* d_w, int64_t ld_dst, const double* lims, int32_t test_half,
+                                 int32_t* np_out);
 
 /* Stable counting sort of the particles by cell, absorbing those outside lims
  * (HOST double[4], tested on coord or, with test_half, coord_half; NULL keeps
@@ -583,6 +590,10 @@ int32_t chimera_soa_gather_push_dev_mixed(const double* coord, float* momenta, c
                                           double t, const double* frame);
 int32_t chimera_soa_compact_mixed(double* coord, double* coord_half, float* momenta, float* wghts, int64_t ld,
                                   int32_t np, const double* lims, int32_t test_half, int32_t* np_out);
+int32_t chimera_soa_compact_into_mixed(const double* coord, const double* coord_half, const float* momenta,
+                                       const float* wghts, int64_t ld, int32_t np, double* d_coord, double* d_half,
+                                       float* d_mom, float* d_w, int64_t ld_dst, const double* lims,
+                                       int32_t test_half, int32_t* np_out);
 int32_t chimera_beam_moments_mixed(const double* coord_half, const float* momenta, const float* wghts, int64_t ld,
                                    int32_t np, double* sums);
 int32_t chimera_momentum_mean_mixed(const float* momenta, const float* wghts, int32_t np, double* px0);
