@@ -12,7 +12,8 @@ JSON line on stdout; progress goes to stderr.
 value = pushed-particle steps per second over the timed steps (electrons; the
 still ions are not pushed) of the FP64 run.  With --precision both (default)
 the same workload then runs in mixed precision (float32 momenta and weights,
-complex64 fields, FP32 MFMA transforms, FP64 positions) and its figures go in
+complex64 fields, split-BF16 MFMA transforms with FP32 accumulation, FP64
+positions) and its figures go in
 the "mixed" object of the same line.  Per-kernel-family times come from HIP
 events that libchimera_hip records on its own stream around every launch.
 
@@ -46,6 +47,8 @@ METRIC = "particle-steps/s + FB-transform ms/step, FP64 vs mixed; p50/p99 scan l
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6      # dense MFMA
 FP32_PEAK_TFS = 157.3
+BF16_PEAK_TFS = 2500.0    # dense MFMA
+BF16_PRODUCTS = 6         # split-BF16: six BF16 products per algorithmic product
 FAMILIES = ["push_coords", "dep_curr", "dep_dens", "gather_push", "dht", "fft", "fb_ops", "psatd", "sort", "other"]
 
 
@@ -118,6 +121,7 @@ def run_gpu(args, precision):
         if n:
             log(f"  {name:12s} {ms / args.steps:9.3f} ms/step  ({n} launches)")
     return dict(run=run, wall=wall, psteps=psteps, fams=fams, Nx=Nx, Nkr=Nkr, M=M, P=e.np, workload=args.workload,
+                mixed_form=int(L.chimera_get_mixed_gemm()),
                 Gr=Nx * A["Nr"] * M, Gk=Nx * Nkr * M, precision=precision, first=first, last=first + args.steps - 1,
                 run_steps=spec["steps"])
 
@@ -142,14 +146,20 @@ def family_roofline(res, K, name):
     if name in ("dht", "fb_ops"):
         Nx, Nkr, M = res["Nx"], res["Nkr"], res["M"]
         flops = 40.0 * M * Nx * Nkr ** 2 if name == "dht" else 4.0 * (18 * M - 3) * Nx * Nkr ** 2
-        peak = FP64_PEAK_TFS if res["precision"] == "fp64" else FP32_PEAK_TFS
+        form = res.get("mixed_form")
+        peak = FP64_PEAK_TFS if res["precision"] == "fp64" else BF16_PEAK_TFS if form == 0 else FP32_PEAK_TFS
         per_launch = flops / (n / K)
         avg_s = ms / n / 1e3
         achieved = per_launch / avg_s / 1e12
-        return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4),
-                "algorithmic_flops_per_launch": int(per_launch), "launches_per_step": n / K,
-                "share_of_step": round(ms / sum(v[0] for v in res["fams"].values()), 3)}
+        out = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+               "frac": round(achieved / peak, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4),
+               "algorithmic_flops_per_launch": int(per_launch), "launches_per_step": n / K,
+               "share_of_step": round(ms / sum(v[0] for v in res["fams"].values()), 3)}
+        if res["precision"] != "fp64" and form == 0:
+            # split-BF16 issues six BF16 products per algorithmic product
+            out["issued_bf16_tflops"] = round(BF16_PRODUCTS * achieved, 1)
+            out["issued_frac"] = round(BF16_PRODUCTS * achieved / BF16_PEAK_TFS, 4)
+        return out
     alg = algorithmic(name, res["P"], res["Gr"], res["Gk"], res["Nx"], res["Nkr"], res["M"], K, res["precision"])
     if alg is None:
         return None
@@ -636,7 +646,7 @@ def mixed_object(summaries):
     mx = summaries["mixed"]
     o = {k: mx[k] for k in ("value", "ms_per_step", "fb_transform_ms_per_step", "spectral_operator_ms_per_step",
                             "kernel_ms_per_step", "roofline", "families", "timed_steps")}
-    o["dtype"] = "f32 momenta/weights, c64 fields, FP32 MFMA; f64 positions and tables"
+    o["dtype"] = "f32 momenta/weights, c64 fields, split-BF16 MFMA (f32 accumulation); f64 positions and tables"
     o["speedup_vs_fp64"] = round(mx["value"] / summaries["fp64"]["value"], 3)
     return o
 

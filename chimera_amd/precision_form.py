@@ -4,8 +4,9 @@
                                          [--markdown FILE]
 
 Runs the workload twice from the same seed, first in FP64 and then in mixed
-precision (float32 momenta and weights, complex64 grids, FP32 MFMA transforms;
-FP64 positions, tables and reductions), and at every checkpoint compares:
+precision (float32 momenta and weights, complex64 grids, split-BF16 MFMA
+transforms with FP32 accumulation; FP64 positions, tables and reductions), and
+at every checkpoint compares:
 
 * each component of the gathered field EB (Ex, Ey, Ez, Bx, By, Bz), relative L2
   over the whole grid and per azimuthal mode, and against the norm of all six

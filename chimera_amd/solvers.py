@@ -11,7 +11,9 @@ raises NotImplementedError.
 Extra key 'Precision': 'fp64' (default, the reference's arithmetic) or
 'mixed': every field and FB-space array complex64, the Hankel, D+- and PSATD
 tables built in FP64 and rounded to float32 (kx stays FP64), transforms and
-operators on FP32 MFMA with FP32 accumulation.
+operators as split-BF16 MFMA products with FP32 accumulation (six products of
+three-piece BF16 splits: FP32 accuracy), or FP32 MFMA when the library is set
+to that form.
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
-"""Mixed precision (float32 momenta and weights, complex64 fields, FP32 MFMA
-transforms, FP64 positions and tables) against the FP64 GPU path on the same
-seeded inputs.
+"""Mixed precision (float32 momenta and weights, complex64 fields, split-BF16
+MFMA transforms with FP32 accumulation, FP64 positions and tables) against the
+FP64 GPU path on the same seeded inputs.
 
 The errors are reported per field component and per particle moment.  Bar:
 relative L2 <= 1e-5 for E, B (each component of the gathered field EB), the
