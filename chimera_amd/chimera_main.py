@@ -125,7 +125,7 @@ class ChimeraRun:
         self.gather_push(istep)
 
     def _fusable(self, istep):
-        if len(self.Solvers) != 1:
+        if len(self.Solvers) != 1 or self.Solvers[0].env:
             return False
         for sp in self.Particles:
             if "Xchunked" in sp.Args and "NoSorting" not in sp.Args["Features"] and \
@@ -196,9 +196,18 @@ class ChimeraRun:
         for species in self.Particles:
             if species.np == 0 or "Still" in species.Args["Features"]:
                 continue
+            if solver.env:                                                 # chimera_main.py:158-170
+                call("chimera_soa_dep_curr_env", species.ptr("coords_halfstep"), species.ptr("momenta"),
+                     species.ptr("weights"), species.ld, species.np, J.ptr, *solver.geom_args_env(),
+                     solver.frame_ptr(0))
+                continue
             call(self._pfn("chimera_soa_push_deposit"), species.ptr("coords"), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
                  J.ptr, None, None)
+
+    def _dep_dens_env(self, solver, species, component, out):
+        call("chimera_soa_dep_dens_env", species.ptr(component), species.ptr("weights"), species.ld, species.np,
+             out.ptr, *solver.geom_args_env(), None)
 
     def dep_dens(self, solver, component="coords"):
         R = solver.Data["Rho"]
@@ -210,6 +219,9 @@ class ChimeraRun:
         for species in self.Particles:
             if species.np == 0 or "Still" in species.Args["Features"]:
                 continue
+            if solver.env:                                                 # chimera_main.py:195-206
+                self._dep_dens_env(solver, species, component, R)
+                continue
             call(self._pfn("chimera_soa_push_deposit"), species.ptr(component), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
                  None, R.ptr, None)
@@ -220,6 +232,9 @@ class ChimeraRun:
         g = solver.geom_args()
         for species in self.Particles:
             if species.np == 0 or "Still" not in species.Args["Features"]:
+                continue
+            if solver.env:                                                 # chimera_main.py:225-236
+                self._dep_dens_env(solver, species, "coords", B)
                 continue
             call(self._pfn("chimera_soa_push_deposit"), species.ptr("coords"), species.ptr("coords_halfstep"),
                  species.ptr("momenta"), species.ptr("weights"), species.ld, species.np, 0.0, g[0], g[0], *g[1:],
@@ -265,6 +280,12 @@ class ChimeraRun:
             else:
                 EB, g = 0, (0.0, 0, 1.0, 1.0, 0.0, -1.0, 1, 1, 0)   # rmax < 0: nothing gathered
             recs, t = devs if devs is not None else (None, 0.0)
+            if self.Solvers and self.Solvers[0].env and EB:                # proj_fld_env, chimera_main.py:142-146
+                call("chimera_soa_gather_push_env", species.ptr("coords"), species.ptr("momenta"),
+                     species.ptr("weights"), species.ld, species.np, EB, *self.Solvers[0].geom_args_env(), push_dt,
+                     ctypes.addressof(recs) if recs is not None else None, len(recs) if recs is not None else 0, t,
+                     frame)
+                continue
             call(self._pfn("chimera_soa_gather_push_dev"), species.ptr("coords"), species.ptr("momenta"),
                  species.ptr("weights"), species.ld, species.np, EB, *g, push_dt,
                  ctypes.addressof(recs) if recs is not None else None, len(recs) if recs is not None else 0, t, frame)
@@ -353,6 +374,8 @@ class ChimeraRun:
                 out.append("the still background is re-deposited after each move")
         if self.Solvers and "StaticKick" in self.Solvers[0].Args["Features"]:
             out.append("StaticKick reads the mean momentum back to the host on every step")
+        if self.Solvers and self.Solvers[0].env:
+            out.append("the envelope (KxShift) solver runs the unfused particle calls")
         for sp in self.Particles:
             for dev in getattr(sp, "Devices", ()):
                 fn = dev[0]

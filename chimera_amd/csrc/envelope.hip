@@ -62,6 +62,7 @@ __global__ void k_dep_curr_env(const double* __restrict__ coord, PStride sx, con
                                int ip0, int ip1) {
     const int ip = ip0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= ip1) return;
+    g = frame_geom(g);
     const double wr = w[ip];
     if (wr == 0.0) return;
     double v[3] = {u[su.at(ip, 0)], u[su.at(ip, 1)], u[su.at(ip, 2)]};
@@ -93,6 +94,7 @@ __global__ void k_dep_dens_env(const double* __restrict__ coord, PStride sx, con
                                double2* __restrict__ dens, GridGeom g, double kx0, int ip0, int ip1) {
     const int ip = ip0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= ip1) return;
+    g = frame_geom(g);
     const double wr = (double)w[ip];
     if (wr == 0.0) return;
     const double xp = coord[sx.at(ip, 0)];
@@ -237,6 +239,61 @@ static int32_t dep_dens_env_impl(const double* coord, const double* wghts, doubl
     return st.finish();
 }
 
+// ---------------------------------------------------------------- resident tier
+//
+// The same particle loops on structure-of-arrays particles in device memory
+// (component c of particle i at c*ld + i), asynchronous on the library
+// stream, deposits WITHOUT the ghost fold (the forward transform applies it,
+// as in the plain resident path), and the gather fused with the external
+// devices and the Boris push (chimera_main.py:130-151, :91-92): the field on
+// the particle starts at zero, gets the envelope gather, the devices' fields
+// in the species' order, then pushes the momenta.
+
+template <typename R>
+__global__ void k_gather_push_env(const double* __restrict__ coord, PStride sx, R* __restrict__ u, PStride su,
+                                  const R* __restrict__ w, const double2* __restrict__ Fld, GridGeom g, double kx0,
+                                  double push_dt, ExtDevices devs, int np) {
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    g = frame_geom(g);
+    const double xp = coord[sx.at(ip, 0)], yp = coord[sx.at(ip, 1)], zp = coord[sx.at(ip, 2)];
+    double Fp[6] = {0, 0, 0, 0, 0, 0};
+    PartGeom pg;
+    if ((double)w[ip] != 0.0 && env_geom(pg, xp, yp, zp, g, true)) {
+        // grid_deps_env.f90:180-236, in the order of k_proj_fld_env
+        const double2 carrier = cmk(cos(xp * kx0), sin(xp * kx0));
+        const int nm = 2 * g.nkO + 1;
+        const int64_t sr = g.nx + 1, sm = sr * (g.nr + 1), sl = sm * nm;
+        double2 pc[2][2];
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) pc[a][b] = cmul(cmk(pg.S[1][b] * pg.S[0][a], 0.0), carrier);
+        const double2* base = Fld + pg.ix + pg.ir * sr;
+        for (int iO = 0; iO <= g.nkO; iO++)
+            for (int sgn = 0; sgn < (iO > 0 ? 2 : 1); sgn++) {
+                const double2 ph = sgn ? cconj(pg.ph[iO]) : pg.ph[iO];
+                const int m = sgn ? g.nkO - iO : g.nkO + iO;
+                double2 po[2][2];
+                for (int b = 0; b < 2; b++)
+                    for (int a = 0; a < 2; a++) po[a][b] = cmul(pc[a][b], ph);
+                for (int l = 0; l < 6; l++) {
+                    double s = 0.0;
+                    for (int b = 0; b < 2; b++)
+                        for (int a = 0; a < 2; a++) {
+                            const double2 f = base[a + b * sr + m * sm + l * sl];
+                            s += po[a][b].x * f.x - po[a][b].y * f.y;
+                        }
+                    Fp[l] = Fp[l] + s;
+                }
+            }
+    }
+    R f[6];
+    for (int l = 0; l < 6; l++) f[l] = (R)Fp[l];
+    if (devs.n) ext_fields<R>(devs, xp, yp, zp, f);
+    R m3[3] = {u[su.at(ip, 0)], u[su.at(ip, 1)], u[su.at(ip, 2)]};
+    boris<R>(m3, f, (R)(0.5 * push_dt));
+    for (int c = 0; c < 3; c++) u[su.at(ip, c)] = m3[c];
+}
+
 // the particle range [lo, hi) of the chunks (IndInChunk(0), IndInChunk(nchnk))
 static int32_t chunk_range(const int32_t* IndInChunk, int32_t nchnk, int32_t np, int32_t* lo, int32_t* hi) {
     if (nchnk < 1 || IndInChunk == nullptr) return fail(ST_EINVAL, "chunked deposit: nchnk < 1");
@@ -307,6 +364,67 @@ int32_t chimera_proj_fld_env(const double* coord, const double* wghts, const dou
                        ENV_AOS6, g, kx0, np);
     CH_LAUNCH_CHECK("k_proj_fld_env");
     return st.finish();
+}
+
+int32_t chimera_soa_dep_curr_env(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
+                                 int32_t np, double* curr, double leftX, const double* Rgrid, double dx_inv,
+                                 double dr_inv, double r0, double rmax, double kx0, int32_t nx, int32_t nr,
+                                 int32_t nkO, const double* frame) {
+    CH_TRY(env_check(np, nx, nr, nkO, "soa_dep_curr_env"));
+    if (ld < np) return fail(ST_EINVAL, "soa_dep_curr_env: ld=%lld < np=%d", (long long)ld, np);
+    if (np == 0) return ST_OK;
+    if (!is_device_ptr(coord_half) || !is_device_ptr(momenta) || !is_device_ptr(wghts) || !is_device_ptr(curr) ||
+        !is_device_ptr(Rgrid))
+        return fail(ST_EINVAL, "soa_dep_curr_env: resident entry points take device pointers");
+    GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    g.leftX_dev = frame;
+    const PStride s = {1, ld};
+    KTimer kt(KF_DEP_CURR);
+    hipLaunchKernelGGL(k_dep_curr_env, blocks_for(np, 256), 256, 0, stream(), coord_half, s, momenta, s, wghts,
+                       reinterpret_cast<double2*>(curr), g, kx0, 0, np);
+    CH_LAUNCH_CHECK("k_dep_curr_env");
+    return ST_OK;
+}
+
+int32_t chimera_soa_dep_dens_env(const double* coord, const double* wghts, int64_t ld, int32_t np, double* dens,
+                                 double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0,
+                                 double rmax, double kx0, int32_t nx, int32_t nr, int32_t nkO, const double* frame) {
+    CH_TRY(env_check(np, nx, nr, nkO, "soa_dep_dens_env"));
+    if (ld < np) return fail(ST_EINVAL, "soa_dep_dens_env: ld=%lld < np=%d", (long long)ld, np);
+    if (np == 0) return ST_OK;
+    if (!is_device_ptr(coord) || !is_device_ptr(wghts) || !is_device_ptr(dens) || !is_device_ptr(Rgrid))
+        return fail(ST_EINVAL, "soa_dep_dens_env: resident entry points take device pointers");
+    GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    g.leftX_dev = frame;
+    const PStride s = {1, ld};
+    KTimer kt(KF_DEP_DENS);
+    hipLaunchKernelGGL(k_dep_dens_env<double>, blocks_for(np, 256), 256, 0, stream(), coord, s, wghts,
+                       reinterpret_cast<double2*>(dens), g, kx0, 0, np);
+    CH_LAUNCH_CHECK("k_dep_dens_env");
+    return ST_OK;
+}
+
+int32_t chimera_soa_gather_push_env(const double* coord, double* momenta, const double* wghts, int64_t ld, int32_t np,
+                                    const double* Fld, double leftX, const double* Rgrid, double dx_inv,
+                                    double dr_inv, double r0, double rmax, double kx0, int32_t nx, int32_t nr,
+                                    int32_t nkO, double push_dt, const chimera_device_t* devices, int32_t ndev,
+                                    double t, const double* frame) {
+    CH_TRY(env_check(np, nx, nr, nkO, "soa_gather_push_env"));
+    if (ld < np) return fail(ST_EINVAL, "soa_gather_push_env: ld=%lld < np=%d", (long long)ld, np);
+    if (np == 0) return ST_OK;
+    if (!is_device_ptr(coord) || !is_device_ptr(momenta) || !is_device_ptr(wghts) || !is_device_ptr(Fld) ||
+        !is_device_ptr(Rgrid))
+        return fail(ST_EINVAL, "soa_gather_push_env: resident entry points take device pointers");
+    ExtDevices devs;
+    CH_TRY(ext_devices(devs, devices, ndev, t));
+    GridGeom g = resident_geom(leftX, Rgrid, dx_inv, dr_inv, r0, rmax, nx, nr, nkO);
+    g.leftX_dev = frame;
+    const PStride s = {1, ld};
+    KTimer kt(KF_GATHER_PUSH);
+    hipLaunchKernelGGL(k_gather_push_env<double>, blocks_for(np, 256), 256, 0, stream(), coord, s, momenta, s, wghts,
+                       reinterpret_cast<const double2*>(Fld), g, kx0, push_dt, devs, np);
+    CH_LAUNCH_CHECK("k_gather_push_env");
+    return ST_OK;
 }
 
 int32_t chimera_eb_correction_env(double* eb_spc, int32_t nx, int32_t nr, int32_t nkO) {

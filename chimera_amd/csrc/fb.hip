@@ -76,12 +76,17 @@ __device__ __forceinline__ void fft_store(const FftArgsT<R>& a, int64_t line, in
     if (a.ghost) {
         m = (int)(q % a.nmodes);
         if (a.ghost == 2) v = crs((R)(m == 0 ? 0.5 / M_PI : 1.0 / M_PI), v);
+        else if (a.ghost >= 3) v = crs((R)(1.0 / M_PI), v);
     }
     C* o = a.out + r * a.out_r + q * a.out_q + x;
     *o = v;
     if (a.ghost && r == 0) {
-        // ghost radial row below row 1: zero (fb_io.f90:199) or +-row 1 (grid_deps.f90:248-265)
-        o[-a.out_r] = a.ghost == 1 ? cmake<R>(0, 0) : (m == 0 ? v : cneg(v));
+        // ghost radial row below row 1: zero (fb_io.f90:199), +-row 1 by mode
+        // (grid_deps.f90:248-265), or +row 1 (3) / -row 1 (4) for every mode of
+        // the envelope grid (grid_deps_env.f90:240-283: + when nkO = 0)
+        o[-a.out_r] = a.ghost == 1 ? cmake<R>(0, 0)
+                    : a.ghost == 2 ? (m == 0 ? v : cneg(v))
+                    : a.ghost == 3 ? v : cneg(v);
     }
 }
 
@@ -857,7 +862,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm(const FbJob* __restri
     }
     __syncthreads();
     if constexpr (NACC == 2) {
-        pipe(acc0, 0, stages_per_seg);
+        pipe(acc0, 0, nseg > 0 ? stages_per_seg : 0);   // a job without segments stages nothing
 #pragma unroll
         for (int i = 0; i < NI; i++)
 #pragma unroll
@@ -1312,7 +1317,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_f32(const FbJobT<floa
     }
     __syncthreads();
     if constexpr (NACC == 2) {
-        pipe(acc0, 0, stages_per_seg);
+        pipe(acc0, 0, nseg > 0 ? stages_per_seg : 0);   // a job without segments stages nothing
 #pragma unroll
         for (int i = 0; i < NI; i++)
 #pragma unroll
@@ -1562,7 +1567,7 @@ __global__ void __launch_bounds__(GB_THREADS, 2) k_fb_gemm_bf16(const FbJobT<flo
     }
     __syncthreads();
     if constexpr (NACC == 2) {
-        pipe(acc0, 0, stages_per_seg);
+        pipe(acc0, 0, nseg > 0 ? stages_per_seg : 0);   // a job without segments stages nothing
 #pragma unroll
         for (int i = 0; i < NI; i++)
 #pragma unroll
@@ -2337,7 +2342,7 @@ int32_t chimera_fb_forward(double* out, double* in, double leftX, const double* 
 int32_t chimera_fb_inverse(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
                            const double* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost,
                            const double* frame) {
-    if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 2)
+    if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 4)
         return fail(ST_EINVAL, "fb_inverse: bad sizes or ghost mode");
     CH_TRY(fft_length_check(nkx));
     if (!is_device_ptr(eb_spc) || !is_device_ptr(e_fb) || !is_device_ptr(b_fb) || !is_device_ptr(kx) ||
@@ -2404,7 +2409,7 @@ int32_t chimera_fb_forward_c8(float* out, float* in, double leftX, const double*
 int32_t chimera_fb_inverse_c8(float* eb_spc, const float* e_fb, const float* b_fb, double leftX, const double* kx,
                               const float* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost,
                               const double* frame) {
-    if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 2)
+    if (nkx < 1 || nr < 1 || nmodes < 1 || nkr < 1 || ghost < 1 || ghost > 4)
         return fail(ST_EINVAL, "fb_inverse_c8: bad sizes or ghost mode");
     CH_TRY(fft_length_check(nkx));
     if (!is_device_ptr(eb_spc) || !is_device_ptr(e_fb) || !is_device_ptr(b_fb) || !is_device_ptr(kx) ||

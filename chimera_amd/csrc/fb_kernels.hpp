@@ -24,7 +24,8 @@ struct FftArgsT {
     const R* scale;        // real factor scale[x + N*r] after the transform (nullable)
     const R* filtr;        // absorbing profile: mask first nxfilt points, transform back (nullable)
     int nxfilt;
-    int ghost;             // 0: none; 1: zero the row below r = 0; 2: eb_correction (mode scale, +-row 1)
+    int ghost;             // 0: none; 1: zero the row below r = 0; 2: eb_correction (mode scale, +-row 1);
+                           // 3, 4: eb_correction_env (1/pi, +row 1 / -row 1)
     int nmodes;            // mode = q % nmodes (ghost != 0)
     int sign;              // -1 FFTW_FORWARD, +1 FFTW_BACKWARD
     int N, log2N;

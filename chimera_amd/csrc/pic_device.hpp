@@ -2,7 +2,10 @@
 // particle kernels and the fused kernels of resident simulations.
 #pragma once
 
+#include <cstring>
+
 #include "common.hpp"
+#include "../../include/chimera_hip.h"
 
 namespace chimera {
 
@@ -259,6 +262,32 @@ __device__ __forceinline__ void ext_fields(const ExtDevices& D, double x, double
 // Gaussian beam's scalar amplitude is passed as a0.  EINVAL on a short array.
 int32_t make_ext_device(ExtDevice& d, int kind, const double* params, int nparams, const double* map, int nmap,
                         double a0);
+
+// Geometry of a resident launch: Rgrid in device memory, its first and last
+// node given on the host.
+inline GridGeom resident_geom(double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
+                              int nx, int nr, int nkO) {
+    GridGeom g = make_geom(leftX, Rgrid, dx_inv, dr_inv, nx, nr, nkO);
+    g.r0 = r0;
+    g.rmax = rmax;
+    return g;
+}
+
+// The device list of a resident gather-push from the C-ABI records (host
+// memory; a mapped undulator's map in device memory), at time t.
+inline int32_t ext_devices(ExtDevices& devs, const chimera_device_t* d, int32_t ndev, double t) {
+    memset(&devs, 0, sizeof devs);
+    if (ndev < 0 || ndev > MAX_EXT_DEVICES || (ndev > 0 && d == nullptr))
+        return fail(ST_EINVAL, "gather_push: %d devices (at most %d per species)", ndev, MAX_EXT_DEVICES);
+    for (int k = 0; k < ndev; k++) {
+        if ((d[k].kind == DEV_MAPPED || d[k].kind == DEV_MAPPED_TAP) && !is_device_ptr(d[k].map))
+            return fail(ST_EINVAL, "gather_push: the map of a mapped undulator is device memory");
+        CH_TRY(make_ext_device(devs.d[k], d[k].kind, d[k].params, 8, d[k].map, d[k].nmap, d[k].a0));
+    }
+    devs.n = ndev;
+    devs.t = t;
+    return ST_OK;
+}
 
 // Launchers of the particle kernels (particles.hip).
 int32_t launch_push_coords(double* x, PStride sx, const double* u, PStride su, double* xc, PStride sc, double dt,

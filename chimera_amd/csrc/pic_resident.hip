@@ -625,14 +625,6 @@ __global__ void k_compact_scatter(const int32_t* __restrict__ flag, const int64_
         default: LAUNCH(8); break;                   \
     }
 
-static GridGeom resident_geom(double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0, double rmax,
-                              int nx, int nr, int nkO) {
-    GridGeom g = make_geom(leftX, Rgrid, dx_inv, dr_inv, nx, nr, nkO);
-    g.r0 = r0;
-    g.rmax = rmax;
-    return g;
-}
-
 static int32_t check_resident(int np, int64_t ld, int nkO) {
     if (np < 0 || ld < np) return fail(ST_EINVAL, "resident particles: np=%d ld=%lld", np, (long long)ld);
     if (nkO < 0 || nkO >= MAX_MODES)
@@ -732,19 +724,6 @@ static int32_t undul_devices(ExtDevices& devs, const double* undul_params) {
     return make_ext_device(devs.d[0], DEV_UNDUL, undul_params, 4, nullptr, 0, 0.0);
 }
 
-static int32_t ext_devices(ExtDevices& devs, const chimera_device_t* d, int32_t ndev, double t) {
-    memset(&devs, 0, sizeof devs);
-    if (ndev < 0 || ndev > MAX_EXT_DEVICES || (ndev > 0 && d == nullptr))
-        return fail(ST_EINVAL, "gather_push: %d devices (at most %d per species)", ndev, MAX_EXT_DEVICES);
-    for (int k = 0; k < ndev; k++) {
-        if ((d[k].kind == DEV_MAPPED || d[k].kind == DEV_MAPPED_TAP) && !is_device_ptr(d[k].map))
-            return fail(ST_EINVAL, "gather_push: the map of a mapped undulator is device memory");
-        CH_TRY(make_ext_device(devs.d[k], d[k].kind, d[k].params, 8, d[k].map, d[k].nmap, d[k].a0));
-    }
-    devs.n = ndev;
-    devs.t = t;
-    return ST_OK;
-}
 
 template <typename R>
 static int32_t gather_push_impl(const double* coord, R* momenta, const R* wghts, int64_t ld, int32_t np,

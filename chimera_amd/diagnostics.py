@@ -198,7 +198,8 @@ class Diagnostics:
 
     def _vec_out(self, sol):
         """fb_vec_out (fb_io.f90:100-140) of the E part of EG_fb with the full
-        radial grid on the device: the first three components of the returned
+        radial grid on the device, on the transforms' wave numbers (FBoutFull,
+        solvers.py:181: kx_env for the envelope solver): the first three components of the returned
         (Nx, Nr_full + 1, M, 6) device array, ghost row zero."""
         A = sol.Args
         Nx, Nkr, M = sol.n_fb
@@ -209,7 +210,7 @@ class Diagnostics:
             vec = sol.dev["PwrVec"] = DeviceArray((Nx, nr + 1, M, 6), sol.ct)
         EG = sol.Data["EG_fb"]
         # E in both halves of the transform's input: the first three output components are fb_vec_out
-        call(sol.fn("chimera_fb_inverse"), vec.ptr, EG.ptr, EG.ptr, float(A["leftX"]), sol.dev["kx"].ptr,
+        call(sol.fn("chimera_fb_inverse"), vec.ptr, EG.ptr, EG.ptr, float(A["leftX"]), sol.dev["kx_base"].ptr,
              out_full.ptr, Nx, nr, M, Nkr, 1, None)
         return vec, nr
 

@@ -2,13 +2,15 @@
 PSATD solver.  Everything here runs once per Solver; the results are uploaded
 to the device and stay there.
 
-Restates (non-KxShift branch):
-  * get_spectral_operators   solvers.py:717-759
+Restates:
+  * get_spectral_operators   solvers.py:717-759 (both branches)
   * grid set-up              solvers.py:60-110
   * DepFact, PoissFact       solvers.py:106, :169-176
   * PSATD coefficient tables solvers.py:227-279
   * static-solution tables   solvers.py:350-359
   * absorbing profile        solvers.py:612-617
+  * envelope (KxShift) tables: complex PSATD coefficients solvers.py:244-267,
+    the anti-echo and band-pass filter of the deposition solvers.py:660-715
 """
 from __future__ import annotations
 
@@ -101,6 +103,98 @@ def spectral_operators(Nx, Nkr, Nko, lengthR, kx, RgridFull, ext=0):
         DpS2S[:, :, jm] = In[:, :, jm].dot(0.5 * krp * jn(jm, R * krp))
         DmS2S[:, :, jm] = In[:, :, jm].dot(0.5 * krm * jn(jm, R * krm))
     return dict(In=In, Out=Out, DpS2S=DpS2S, DmS2S=DmS2S, w=w, kr=kr, kr_g=kr_g, kx_g=kx_g)
+
+
+@_memo
+def spectral_operators_env(Nx, Nkr, Nko, lengthR, kx, RgridFull, ext=0):
+    """The KxShift branch of get_spectral_operators (solvers.py:717-759): modes
+    Mmin..Mmax = -Nko-ext..Nko+ext, the Bessel roots of |m| (scipy's jn_zeros
+    of a negative order), arrays indexed by the mode with NumPy's wrap-around
+    for negative m, and for Nko > 0 reordered to Mmin..Mmax along the last
+    axis as the reference does (for Nko = 0 the order stays 0..Mmax, Mmin..-1:
+    the envelope operators never read the coupling matrices of a single mode)."""
+    Mmin, Mmax, Mtot = -Nko - ext, Nko + ext, 2 * Nko + 1 + 2 * ext
+    Mmin_ext, Mmax_ext, Mtot_ext = Mmin - 1, Mmax + 1, Mtot + 2
+    kr = np.zeros((Nkr, Mtot_ext))
+    kr_g = np.zeros((Nx, Nkr, Mtot))
+    w = np.zeros((Nx, Nkr, Mtot))
+    Out = np.zeros((Nkr, Nkr, Mtot))
+    In = np.zeros((Nkr, Nkr, Mtot))
+    DpS2S = np.zeros((Nkr, Nkr, Mtot))
+    DmS2S = np.zeros((Nkr, Nkr, Mtot))
+    for jm in range(Mmin_ext, Mmax_ext + 1):
+        kr[:, jm] = jn_zeros(jm, Nkr) / lengthR
+    R = RgridFull[1:, None]
+    kx_g = None
+    for jm in range(Mmin, Mmax + 1):
+        kr_g[:, :, jm], kx_g = np.meshgrid(kr[:, jm], kx)
+        w[:, :, jm] = np.sqrt(kx_g ** 2 + kr_g[:, :, jm] ** 2)
+        Out[:, :, jm] = jn(jm, R * kr[:, jm][None, :])
+        In[:, :, jm] = np.linalg.inv(Out[:, :, jm])
+        krp = kr[:, jm + 1][None, :]
+        krm = kr[:, abs(jm - 1)][None, :]
+        DpS2S[:, :, jm] = In[:, :, jm].dot(0.5 * krp * jn(jm, R * krp))
+        DmS2S[:, :, jm] = In[:, :, jm].dot(0.5 * krm * jn(jm, R * krm))
+    if Nko > 0:
+        cat = lambda a: np.concatenate((a[..., Mmin:], a[..., :Mmax + 1]), axis=-1)   # noqa: E731
+        Out, In, DpS2S, DmS2S, w, kr_g, kr = (cat(a) for a in (Out, In, DpS2S, DmS2S, w, kr_g, kr))
+    return dict(In=In, Out=Out, DpS2S=DpS2S, DmS2S=DmS2S, w=w, kr=kr, kr_g=kr_g, kx_g=kx_g,
+                idxM=np.arange(Mmin, Mmax + 1))
+
+
+@_memo
+def psatd_tables_env(w, kx_g, dt, beta=1.0):
+    """Complex PSATD_E / PSATD_G of the envelope solver without space charge
+    (solvers.py:236-267): the kx of the coefficients scaled by the frame
+    velocity beta (CoPropagative)."""
+    k = beta * kx_g[:, :, None]
+    E = np.zeros(w.shape + (3,), dtype=complex, order="F")
+    G = np.zeros_like(E)
+    c, s = np.cos(dt * w), np.sin(dt * w)
+    E[..., 0] = c
+    E[..., 1] = s / w
+    G[..., 0] = -w * s
+    G[..., 1] = c
+    with np.errstate(divide="ignore", invalid="ignore"):     # kx = 0 on the grid: 0/0 (NaN), as in the reference
+        E[..., 2] = np.exp(-0.5j * k * dt) * 1j * k / (w ** 2 - k ** 2) * (1.0 - np.exp(1j * k * dt) * (w / (1j * k) * s + c))
+        G[..., 2] = np.exp(-0.5j * k * dt) * w ** 2 / (w ** 2 - k ** 2) * (1.0 + np.exp(1j * k * dt) * (1j * k / w * s - c))
+    return E, G
+
+
+def antialias(kx_env, kx, kx0, Nx, features=()):
+    """Deposition filter of the envelope solver (solvers.py:660-715): a band-pass
+    on |kx_env| times, unless 'NoAntiEcho', a notch at each grid echo of the
+    carrier inside the band (strength 'AntiEchoStrength', default 2; a list
+    gives one per echo; <= 0 skips it).  Shape (Nx, 1, 1).  (The reference's
+    echo count uses np.int, which NumPy 2 removed; int() truncates the same
+    positive value.)"""
+    cutafter = 0.85
+    x = np.abs(kx_env) / np.abs(kx_env.max())
+    filt_bandpass = ((x < cutafter) + (x >= cutafter) * np.cos(np.pi / 2 * (x - cutafter) / (1 - cutafter)) ** 2)
+    filt_bandpass = filt_bandpass[:, None, None]
+    filt_antialias = np.ones_like(filt_bandpass)
+    if "NoAntiEcho" not in features:
+        ae = features["AntiEchoStrength"] if "AntiEchoStrength" in features else 2
+        num_echoes = int(np.abs(kx).max() / np.abs(kx_env).max()) + 1
+        cell_echos = np.abs(kx_env).max() / kx0 * np.arange(num_echoes) - 1.0
+        full_band = np.array([kx.min() / kx0, kx.max() / kx0]) - 1.0
+        echo_ind = 0
+        for cellecho in cell_echos:
+            if full_band[0] < cellecho < full_band[1]:
+                ae_loc = ae[echo_ind] if isinstance(ae, (list, tuple)) else ae
+                echo_ind += 1
+                if ae_loc <= 0:
+                    continue
+                xe = kx / kx0 - 1
+                with np.errstate(divide="ignore", invalid="ignore"):     # the echo at -1 has zero width
+                    notch = 1 - np.exp(-(xe - cellecho) ** 2 / (ae_loc * (cellecho + 1.0) / Nx) ** 2)
+                filt_antialias = filt_antialias * notch[:, None, None]
+    return filt_antialias * filt_bandpass
+
+
+def dep_fact_env(Nx, dr, kr_g, filt_x):
+    """(2 pi)^2/Nx filt_x cos^2(kr dr/2) (solvers.py:171-176, KxShift)."""
+    return np.asfortranarray((2 * np.pi) ** 2 / Nx * filt_x * np.cos(0.5 * dr * kr_g) ** 2)
 
 
 @_memo

@@ -3,10 +3,13 @@ device memory.  Same constructor dictionary, attribute names and method names as
 the reference's Solver (solvers.py:26-225); the field arrays in ``Data`` are
 DeviceArrays of the reference's shapes, and every method runs libchimera_hip.
 
-Supported: the plain (non-KxShift) solver with or without SpaceCharge and
+Supported: the plain solver with or without SpaceCharge and
 StillAsBackground, Rcut, NoPoissonCorrection, StaticKick, Xchunked grids,
-CoPropagative.  The envelope (KxShift) path is not part of the PIC step and
-raises NotImplementedError.
+CoPropagative; and the envelope (KxShift) solver of the FEL runs in FP64
+without space charge: modes -Nko..Nko, the carrier e^{-i kx0 x} taken out of
+the deposits and put back in the gather, the transforms on kx_env = kx - kx0,
+the operators and the complex PSATD tables on the full kx, the anti-echo
+deposition filter (solvers.py:76-83, :164-176, :236-267, :660-759).
 
 Extra key 'Precision': 'fp64' (default, the reference's arithmetic) or
 'mixed': every field and FB-space array complex64, the Hankel, D+- and PSATD
@@ -18,6 +21,7 @@ to that form.
 from __future__ import annotations
 
 import numpy as np
+from scipy.special import j1
 
 from . import fimera as F
 from . import operators as ops
@@ -34,12 +38,19 @@ class Solver:
             A["Features"] = ()
         if "TimeActive" not in A:
             A["TimeInterval"] = (0.0, np.inf)        # solvers.py:48-49 (key kept as the reference sets it)
-        if "KxShift" in A:
-            raise NotImplementedError("the envelope (KxShift) solver is outside the PIC-step path")
         A.setdefault("Precision", "fp64")
         if A["Precision"] not in ("fp64", "mixed"):
             raise ValueError(f"Precision {A['Precision']!r}: 'fp64' or 'mixed'")
         self.mixed = A["Precision"] == "mixed"
+        self.env = "KxShift" in A
+        if self.env:
+            if self.mixed:
+                raise NotImplementedError("the envelope (KxShift) solver runs in FP64 only")
+            for feat in ("SpaceCharge", "StaticKick", "StillAsBackground"):
+                # the reference builds no consistent envelope tables for these
+                # (solvers.py:244-249 stores the complex coefficients into real ones)
+                if feat in A["Features"]:
+                    raise NotImplementedError(f"the envelope (KxShift) solver with {feat}")
         self.rt = np.float32 if self.mixed else np.float64          # real tables
         self.ct = np.complex64 if self.mixed else np.complex128     # fields
 
@@ -58,19 +69,20 @@ class Solver:
         if lib().chimera_fft_length_check(int(A["Nx"])) != 0:
             raise ValueError(lib().chimera_last_error().decode())
         A["leftX"] = A["Xgrid"][0]
-        A["kx0"] = 0.0
+        A["kx0"] = 2 * np.pi * A["KxShift"] if self.env else 0.0
         A["kx_env"] = 2 * np.pi * np.fft.fftfreq(A["Nx"], A["dx"])
         A["kx"] = A["kx0"] + A["kx_env"]
         A["dkx"] = (A["kx"][1] - A["kx"][0]) / (2.0 * np.pi)
         A["Nkr"], A["RgridFull"], A["lengthR"] = ops.r_grid(A["lengthR"], A["dr"])
         A["Nr"] = A["Nkr"] + 1
 
-        ext1 = ops.spectral_operators(A["Nx"], A["Nkr"], A["Nko"], A["lengthR"], A["kx"], A["RgridFull"], ext=1)
-        ext0 = ops.spectral_operators(A["Nx"], A["Nkr"], A["Nko"], A["lengthR"], A["kx"], A["RgridFull"], ext=0)
+        so = ops.spectral_operators_env if self.env else ops.spectral_operators
+        ext1 = so(A["Nx"], A["Nkr"], A["Nko"], A["lengthR"], A["kx"], A["RgridFull"], ext=1)
+        ext0 = so(A["Nx"], A["Nkr"], A["Nko"], A["lengthR"], A["kx"], A["RgridFull"], ext=0)
         A["DpS2S"], A["DmS2S"] = ext1["DpS2S"], ext1["DmS2S"]
         A["InFull"], A["OutFull"] = ext0["In"], ext0["Out"]
         A["w"], A["kr"], A["kr_g"], A["kx_g"] = ext0["w"], ext0["kr"], ext0["kr_g"], ext0["kx_g"]
-        A["idxM"] = np.arange(0, A["Nko"] + 1)
+        A["idxM"] = ext0["idxM"] if self.env else np.arange(0, A["Nko"] + 1)
         A["Mtot"] = A["InFull"].shape[-1]
         A["PoissFact"] = 1.0 / A["w"] ** 2
         VGrid = 2 * np.pi * A["dx"] * A["dr"] * A["RgridFull"]
@@ -98,16 +110,25 @@ class Solver:
 
         A["FBDiff"] = [A[kw] for kw in ("DpS2S", "DmS2S", "kx")]
         A["DepProj"] = [A[kw] for kw in ("Rgrid", "dx_inv", "dr_inv")]
-        A["DepFact"] = ops.dep_fact(A["Nx"], A["dx"], A["dr"], A["kx_g"], A["kr_g"])
-        A["FBCurrIn"] = (A["kx"], A["InCurr"])
-        A["FBIn"] = (A["kx"], A["In"])
-        A["FBout"] = (A["kx"], A["Out"])
+        if self.env:                                               # solvers.py:164-167
+            filt_x = ops.antialias(A["kx_env"], A["kx"], A["kx0"], A["Nx"], A["Features"])
+            A["DepFact"] = ops.dep_fact_env(A["Nx"], A["dr"], A["kr_g"], filt_x)
+            A["DepProj"] += [A["kx0"]]
+            kx_base = A["kx_env"]
+        else:
+            A["DepFact"] = ops.dep_fact(A["Nx"], A["dx"], A["dr"], A["kx_g"], A["kr_g"])
+            kx_base = A["kx"]
+        A["FBCurrIn"] = (kx_base, A["InCurr"])
+        A["FBIn"] = (kx_base, A["In"])
+        A["FBout"] = (kx_base, A["Out"])
 
-        # ---- device copies of the operators
+        # ---- device copies of the operators; kx_base is the wave number of
+        # the x transforms (kx_env for the envelope), kx that of the operators
         self.dev = {kw: DeviceArray.from_numpy(A[kw], self.rt)
                     for kw in ("DpS2S", "DmS2S", "In", "InCurr", "Out", "DepFact", "PoissFact")}
         for kw in ("kx", "Rgrid"):
             self.dev[kw] = DeviceArray.from_numpy(A[kw], np.float64)
+        self.dev["kx_base"] = DeviceArray.from_numpy(kx_base, np.float64) if self.env else self.dev["kx"]
 
         # ---- device state (solvers.py:183-212)
         Nx, Nr, Nkr, M = A["Nx"], A["Nr"], A["Nkr"], A["Mtot"]
@@ -145,12 +166,22 @@ class Solver:
         A = self.Args
         return A["Nx"], A["Nkr"], A["Mtot"]
 
+    @property
+    def nko_env(self):
+        """Highest envelope mode: the mode axis holds -nko_env..nko_env."""
+        return (self.Args["Mtot"] - 1) // 2
+
     def geom_args(self):
         """(leftX, Rgrid, dx_inv, dr_inv, Rgrid(0), Rgrid(nr), nx, nr, nkO) for the resident particle calls."""
         A = self.Args
         R = A["Rgrid"]
         return (float(A["leftX"]), self.dev["Rgrid"].ptr, float(A["dx_inv"]), float(A["dr_inv"]), float(R[0]),
                 float(R[-1]), A["Nx"] - 1, A["Nr"] - 1, A["Mtot"] - 1)
+
+    def geom_args_env(self):
+        """(leftX, Rgrid, dx_inv, dr_inv, Rgrid(0), Rgrid(nr), kx0, nx, nr, nkO) for the resident envelope calls."""
+        g = self.geom_args()
+        return g[:6] + (float(self.Args["kx0"]), g[6], g[7], self.nko_env)
 
     def frame_ptr(self, stage):
         """Device address of the window edge after the stage-1 (0) or the
@@ -161,6 +192,11 @@ class Solver:
     # ------------------------------------------------------------------ PSATD (solvers.py:227-299)
     def PSATD_coeffs(self, beta=1.0):
         A = self.Args
+        if self.env:
+            E, G = ops.psatd_tables_env(A["w"], A["kx_g"], A["TimeStep"], beta)
+            self.Data["PSATD_E"] = DeviceArray.from_numpy(E, np.complex128)
+            self.Data["PSATD_G"] = DeviceArray.from_numpy(G, np.complex128)
+            return
         E, G = ops.psatd_tables(A["w"], A["TimeStep"], "SpaceCharge" in A["Features"])
         self.Data["PSATD_E"] = DeviceArray.from_numpy(E, self.rt)
         self.Data["PSATD_G"] = DeviceArray.from_numpy(G, self.rt)
@@ -168,7 +204,10 @@ class Solver:
     def maxwell_solver(self):
         D = self.Data
         Nx, Nkr, M = self.n_fb
-        if "SpaceCharge" in self.Args["Features"]:
+        if self.env:
+            call("chimera_maxwell_push_wo_spchrg_env", D["EG_fb"].ptr, D["J_fb"].ptr, D["PSATD_E"].ptr, D["PSATD_G"].ptr,
+                 Nx, Nkr, M)
+        elif "SpaceCharge" in self.Args["Features"]:
             call(self.fn("chimera_maxwell_push_with_spchrg"), D["EG_fb"].ptr, D["J_fb"].ptr, D["gradRho_fb_prv"].ptr,
                  D["gradRho_fb_nxt"].ptr, D["PSATD_E"].ptr, D["PSATD_G"].ptr, Nx, Nkr, M)
         else:
@@ -227,25 +266,36 @@ class Solver:
         """J (unfolded deposit) -> J_fb, with the ghost fold and DepFact fused."""
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
-        call(self.fn("chimera_fb_forward"), D["J_fb"].ptr, D["J"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
+        call(self.fn("chimera_fb_forward"), D["J_fb"].ptr, D["J"].ptr, float(A["leftX"]), self.dev["kx_base"].ptr,
              self.dev["InCurr"].ptr, self.dev["DepFact"].ptr, Nx, A["Nr"] - 1, M, Nkr, 3, 1, self.frame_ptr(0))
 
     def fb_dens_in(self):
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
-        call(self.fn("chimera_fb_forward"), D["Rho_fb"].ptr, D["Rho"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
+        call(self.fn("chimera_fb_forward"), D["Rho_fb"].ptr, D["Rho"].ptr, float(A["leftX"]), self.dev["kx_base"].ptr,
              self.dev["InCurr"].ptr, self.dev["DepFact"].ptr, Nx, A["Nr"] - 1, M, Nkr, 1, 1, self.frame_ptr(1))
 
     def fb_fld_out(self):
-        """EG_fb, B_fb -> EB with eb_correction fused."""
+        """EG_fb, B_fb -> EB with eb_correction (eb_correction_env for the
+        envelope: ghost row +row 1 for a single mode, -row 1 otherwise) fused."""
         A, D = self.Args, self.Data
         Nx, Nkr, M = self.n_fb
-        call(self.fn("chimera_fb_inverse"), D["EB"].ptr, D["EG_fb"].ptr, D["B_fb"].ptr, float(A["leftX"]), self.dev["kx"].ptr,
-             self.dev["Out"].ptr, Nx, A["Nr"] - 1, M, Nkr, 2, self.frame_ptr(1))
+        ghost = (3 if A["Nko"] == 0 else 4) if self.env else 2
+        call(self.fn("chimera_fb_inverse"), D["EB"].ptr, D["EG_fb"].ptr, D["B_fb"].ptr, float(A["leftX"]),
+             self.dev["kx_base"].ptr, self.dev["Out"].ptr, Nx, A["Nr"] - 1, M, Nkr, ghost, self.frame_ptr(1))
 
     # ------------------------------------------------------------------ FB operators (solvers.py:467-553)
+    _ENV_OPS = {1: "chimera_fb_grad_env", 2: "chimera_fb_div_env"}
+
     def _op(self, op, out, inp):
         Nx, Nkr, M = self.n_fb
+        if self.env:
+            Dp, Dm, kx = self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr, self.dev["kx"].ptr
+            if op == 3:
+                call("chimera_fb_graddiv_env", out.ptr, Dp, Dm, kx, Nx, Nkr, self.nko_env, Nkr)
+            else:
+                call(self._ENV_OPS[op], out.ptr, inp.ptr, Dp, Dm, kx, Nx, Nkr, self.nko_env, Nkr)
+            return
         call(self.fn("chimera_fb_op"), op, out.ptr, inp.ptr, self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr, self.dev["kx"].ptr,
              Nx, Nkr, M - 1)
 
@@ -266,19 +316,35 @@ class Solver:
         D = self.Data
         Nx, Nkr, M = self.n_fb
         G = D["EG_fb"].offset_ptr(3 * Nx * Nkr * M)
+        if self.env:                                               # solvers.py:536-553 (fb_rot_env)
+            call("chimera_fb_rot_env", D["B_fb"].ptr, G, self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr,
+                 self.dev["kx"].ptr, Nx, Nkr, self.nko_env, Nkr)
+            call("chimera_omp_mult_vec", D["B_fb"].ptr, self.dev["PoissFact"].ptr, Nx, Nkr, M)
+            return
         call(self.fn("chimera_fb_rot_scaled"), D["B_fb"].ptr, G, self.dev["DpS2S"].ptr, self.dev["DmS2S"].ptr,
              self.dev["kx"].ptr, self.dev["PoissFact"].ptr, Nx, Nkr, M - 1)
 
     # ------------------------------------------------------------------ laser and absorbing layer
     def add_gauss_beam(self, S):
-        """solvers.py:555-610 (non-KxShift branch): the profile is built on the
-        host, every transform and operator runs on the device."""
+        """solvers.py:555-610: the profile is built on the host, every
+        transform and operator runs on the device.  The envelope branch writes
+        the Gaussian pulse straight into FB space, on mode m = 0, around the
+        carrier k0."""
         A, D = self.Args, self.Data
         k0 = 2 * np.pi * S["k0"]
         a0 = 2 * np.pi * S["a0"]
         X_focus = S["x0"] - S["x_foc"]
         kx_g, w = A["kx_g"], A["w"][:, :, :, None]
         Nx, Nkr, M = self.n_fb
+        if self.env:
+            nk = A["Nko"]                                          # mode m = 0 on the -Nko..Nko axis
+            kr0 = A["kr_g"][:, :, nk]
+            e_s0 = a0 * 0.5 * np.pi ** 0.5 * S["Lx"] * S["LR"] ** 2 * A["dkx"] / A["lengthR"] ** 2
+            vec = np.zeros((Nx, Nkr, M, 3), dtype=complex, order="F")
+            vec[:, :, nk, 2] = e_s0 / j1(A["lengthR"] * kr0) ** 2 * np.exp(-1j * kx_g * S["x0"]) \
+                * np.exp(-0.25 * (kx_g - k0) ** 2 * S["Lx"] ** 2 - 0.25 * kr0 ** 2 * S["LR"] ** 2)
+            self._add_beam(vec, -1.0j * w, w, kx_g, X_focus)
+            return
         Xgrid, Rgrid = A["Xgrid"], A["Rgrid"]
         scl_spc = np.zeros((Nx, A["Nr"], M), dtype=complex, order="F")
         scl_spc[:, :, 0] = a0 * np.cos(k0 * (Xgrid[:, None] - S["x0"])) * np.exp(
@@ -289,6 +355,12 @@ class Solver:
         vec = np.zeros((Nx, Nkr, M, 3), dtype=complex, order="F")
         vec[:, :, :, 2] = scl_fb / A["Nx"]
         DT = -1.0j * w * np.sign(kx_g[:, :, None, None] + (kx_g[:, :, None, None] == 0))
+        self._add_beam(vec, DT, w, kx_g, X_focus)
+
+    def _add_beam(self, vec, DT, w, kx_g, X_focus):
+        """Divergence cleaning, free propagation from the focus over X_focus and
+        accumulation into EG_fb (solvers.py:594-610)."""
+        D = self.Data
         EE = self.div_clean_host(vec)
         GG = DT * EE
         vf = np.cos(w * X_focus) * EE + np.sin(w * X_focus) / w * GG
@@ -305,7 +377,8 @@ class Solver:
     def div_clean_host(self, vec):
         """solvers.py:644-650 on a host array (laser set-up)."""
         A = self.Args
-        v = F.fb_graddiv(np.array(vec, order="F"), *A["FBDiff"])
+        gd = F.fb_graddiv_env if self.env else F.fb_graddiv
+        v = gd(np.array(vec, order="F"), *A["FBDiff"])
         v = F.omp_mult_vec(v, A["PoissFact"])
         return F.omp_add_vec(np.array(vec, order="F"), v)
 

@@ -71,8 +71,8 @@ class Specie:
             packR = np.asfortranarray((packR.ravel() - 0.5) / fc[1])
             packO = np.asfortranarray(np.exp(2.0j * np.pi * (packO.ravel() - 1) / fc[2]))
             A["Pax"] = (packX, packR, packO)
-        elif "RandCell" in A:
-            raise NotImplementedError("RandCell generation is outside the PIC-step path (FixedCell is supported)")
+        elif "RandCell" in A:                                                # species.py:108-109
+            A["Num_p"] = int(A["RandCell"])
         else:
             A["Num_p"] = 0
         A["wght0"] = (A["Charge"] * A["Density"] * A["dr"] * A["dx"] * 2 * np.pi / A["Num_p"]) if A["Num_p"] else 0.0
@@ -167,11 +167,14 @@ class Specie:
             Rgrid = Rgrid[irb:ire]
         elif Xsteps is not None:
             Xgrid = Xgrid[-2 * Xsteps:-Xsteps]
-        if "FixedCell" not in A:
+        if A["Num_p"] == 0:
             return [np.zeros((3, 0), order="F"), np.zeros((3, 0), order="F"), np.zeros(0)]
         coords = np.zeros((4, Xgrid.shape[0] * Rgrid.shape[0] * A["Num_p"]), order="F")
-        RandPackO = self.rng.rand(Xgrid.shape[0], Rgrid.shape[0]).astype("d", order="F")
-        coords, n = F.genparts(coords, Xgrid, Rgrid, RandPackO, *A["Pax"])
+        if "FixedCell" in A:
+            RandPackO = self.rng.rand(Xgrid.shape[0], Rgrid.shape[0]).astype("d", order="F")
+            coords, n = F.genparts(coords, Xgrid, Rgrid, RandPackO, *A["Pax"])
+        else:
+            coords, n = self.gen_randcell(coords, Xgrid, Rgrid)
         coords = coords[:, :n]
         if ProfileFunc is None:
             coords[-1] *= A["wght0"]
@@ -193,6 +196,30 @@ class Specie:
         momenta = np.vstack([mm[c] + ms[c] * rand_mom[c] for c in range(3)]).astype("d", order="F")
         weights = coords[-1].astype("d", order="F")
         return [np.asfortranarray(coords[0:3]), momenta, weights]
+
+    def gen_randcell(self, coords, Xgrid, Rgrid):
+        """species.py:429-471: Num_p particles in each cell between neighbouring
+        nodes, stratified per axis (the k-th of Num_p sub-intervals, jittered
+        by up to half its width) and paired across axes by three shuffles.
+        A set-up step on the host: the legacy generator's shuffle draws a
+        data-dependent number of values, so the cells are drawn one by one,
+        in the reference's order, to keep its random stream."""
+        npc, dx, dr = self.Args["Num_p"], self.Args["dx"], self.Args["dr"]
+        k = np.arange(npc, dtype=np.float64)
+        ip = 0
+        for ix in range(Xgrid.shape[0] - 1):
+            for ir in range(Rgrid.shape[0] - 1):
+                u = 2 * (self.rng.rand(3, npc) - 0.5)
+                x = Xgrid[ix] + dx * (k + 0.5 * u[0]) / npc
+                r = Rgrid[ir] + dr * (k + 0.5 * u[1]) / npc
+                o = 2 * np.pi * (k + 0.5 * u[2]) / npc
+                self.rng.shuffle(x)
+                self.rng.shuffle(r)
+                self.rng.shuffle(o)
+                sl = slice(ip, ip + npc)
+                coords[0, sl], coords[1, sl], coords[2, sl], coords[3, sl] = x, r * np.cos(o), r * np.sin(o), r
+                ip += npc
+        return coords, ip
 
     def add_particles(self, coords, momenta, weights):
         """Append host particles (species.py:217-244); the half-step copy starts at coords."""

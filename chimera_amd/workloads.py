@@ -16,6 +16,11 @@ instantiates them with any module exposing Solver / Specie / ChimeraRun.
   beam       one relativistic bunch with space charge in an undulator, in a
              window moving with it; beam_scan draws the 64 bunches of a scan
              (Nx 256-2048 x Nkr 32-256, modes m=0,1, 1e4-2e6 particles, 300 steps)
+  fel_testrun  the reference's envelope CI scenario (doc/tests/fel-testrun.py):
+             KxShift at the FEL resonance, Nx=120 on 4 x-chunks x Nr=120 cut
+             at Rcut (85 nodes), mode m=0 of the envelope, RandCell 50 beam
+             (doubled by denoise) in an analytic undulator, seed laser, window
+             moving at the beam velocity, 300 steps
 
 Plasma density n0 = 0.005 n_c (as doc/tests/lpa-testrun.py:27) and thermal
 spread 1e-3 m_e c.  Random draws come from NumPy's legacy generator, seeded
@@ -164,6 +169,50 @@ def beam_scan(n=64, seed=2026, steps=300):
                         n_target=float(np.exp(rng.uniform(np.log(1e4), np.log(2e6)))), K=float(rng.uniform(1, 3)),
                         lambda_u=float(rng.uniform(100, 1000)), steps=steps))
     return out
+
+
+def fel_testrun(Nx=120, Nr=120, steps=None, periods=10):
+    """doc/tests/fel-testrun.py:11-63 as a configuration: units of the
+    undulator period (lam0 = 2.8 cm), K0 = 1.95, a 200 MeV beam of 20 pC in a
+    80 um x 80 um box, the envelope solver shifted to the resonant wave number
+    2 gamma_z^2 and co-propagating with the beam, no Poisson correction, dt =
+    1/30.  steps defaults to the reference's periods / dt = 300."""
+    K0, lam0 = 1.95, 2.8
+    g0 = 200 / 0.511
+    lbx = lbr = 80e-4 / lam0
+    dens = 20e-12 / 1.6022e-19 / (np.pi * 80e-4 ** 3) / (1.1e21 / 2.8e4 ** 2)
+    gg = g0 / (1.0 + K0 ** 2 / 2) ** 0.5
+    k_res = 2 * gg ** 2
+    vb = (1.0 - gg ** -2) ** 0.5
+    Lgx, Rg, Rg_cut = 200e-4 / lam0, 1000e-4 / lam0, 700e-4 / lam0
+    dt = 1.0 / 30
+    solver_in = {"Grid": (-0.5 * Lgx, 0.5 * Lgx, Rg, Lgx / Nx, Rg / Nr), "TimeStep": dt, "MaxAzimuthMode": 0,
+                 "KxShift": k_res, "Rcut": Rg_cut, "CoPropagative": vb, "Xchunked": (4, 6),
+                 "Features": {"NoPoissonCorrection": True}}
+    seed_in = {"a0": 0.15, "k0": k_res, "x0": -30e-4 / lam0, "x_foc": 70.0 / lam0, "Lx": 15e-4 / lam0,
+               "LR": 180e-4 / lam0}
+    beam_in = {"Grid": (-0.5 * Lgx, 0.5 * Lgx, Rg, Lgx / Nx, lbr / Nr), "TimeStep": dt, "Charge": -1.0, "Mass": 1.0,
+               "Density": dens, "RandCell": 50, "MomentaMeans": (g0, 0.0, 0.0),
+               "MomentaSpreads": (1e-4 * g0, 2e-5 * g0, 2e-5 * g0), "Xchunked": (4, 6),
+               "Devices": (("undul_analytic", np.array([K0, 1.0, 1.0, float(periods)])),)}
+    frame = {"TimeStep": dt, "Steps": 1, "Velocity": vb, "Features": ("Staged", "NoSorting")}
+    return {"kind": "fel", "solver": solver_in, "laser": seed_in, "beam": beam_in, "frames": (frame,),
+            "domain": (-0.5 * lbx, 0.5 * lbx, 0.0, lbr), "k_res": k_res,
+            "steps": int(periods / dt) if steps is None else steps}
+
+
+def build_fel(mod, spec, seed=12345, own_rng=False):
+    """Instantiate fel_testrun with mod.Solver / mod.Specie / mod.ChimeraRun in
+    the reference script's order: seed laser, beam generation, denoise at the
+    resonant wave number, run."""
+    spec = copy.deepcopy(spec)
+    solver = mod.Solver(spec["solver"])
+    solver.add_gauss_beam(spec["laser"])
+    bunch = mod.Specie(spec["beam"])
+    _seed((bunch,), seed, own_rng)
+    bunch.add_particles(*bunch.gen_parts(Domain=spec["domain"]))
+    bunch.denoise((spec["k_res"],))
+    return mod.ChimeraRun({"Solvers": (solver,), "Particles": (bunch,), "MovingFrames": spec["frames"]})
 
 
 def _seed(species, seed, own_rng):

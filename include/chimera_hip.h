@@ -432,6 +432,25 @@ int32_t chimera_soa_gather_push_dev(const double* coord, double* momenta, const 
                                     double dr_inv, double r0, double rmax, int32_t nx, int32_t nr, int32_t nkO,
                                     double push_dt, const chimera_device_t* devices, int32_t ndev, double t,
                                     const double* frame);
+/* Resident tier of the envelope path: structure-of-arrays particles (component c
+ * of particle i at c*ld + i), every array in device memory, asynchronous.
+ * = dep_curr_env / dep_dens_env WITHOUT the ghost fold (chimera_fb_forward
+ * with fold=1 applies it), and proj_fld_env fused with the species' devices
+ * and push_velocs (chimera_main.py:130-151, :91-92; devices as for
+ * chimera_soa_gather_push_dev).  Rgrid(0) = r0 and Rgrid(nr) = rmax come from
+ * the host; frame as for the other resident calls. */
+int32_t chimera_soa_dep_curr_env(const double* coord_half, const double* momenta, const double* wghts, int64_t ld,
+                                 int32_t np, double* curr, double leftX, const double* Rgrid, double dx_inv,
+                                 double dr_inv, double r0, double rmax, double kx0, int32_t nx, int32_t nr,
+                                 int32_t nkO, const double* frame);
+int32_t chimera_soa_dep_dens_env(const double* coord, const double* wghts, int64_t ld, int32_t np, double* dens,
+                                 double leftX, const double* Rgrid, double dx_inv, double dr_inv, double r0,
+                                 double rmax, double kx0, int32_t nx, int32_t nr, int32_t nkO, const double* frame);
+int32_t chimera_soa_gather_push_env(const double* coord, double* momenta, const double* wghts, int64_t ld, int32_t np,
+                                    const double* Fld, double leftX, const double* Rgrid, double dx_inv,
+                                    double dr_inv, double r0, double rmax, double kx0, int32_t nx, int32_t nr,
+                                    int32_t nkO, double push_dt, const chimera_device_t* devices, int32_t ndev,
+                                    double t, const double* frame);
 
 /* Stable removal of the particles outside lims (HOST double[4]: x range,
  * r^2 range), tested on coord (test_half=0) or coord_half (1); *np_out (HOST)
@@ -472,7 +491,9 @@ int32_t chimera_fb_forward(double* out, double* in, double leftX, const double* 
                            int32_t fold, const double* frame);
 
 /* Inverse FB transform into eb_spc(nkx,0:nr,nmodes,6); ghost=1: ghost row zero
- * (fb_eb_out alone), ghost=2: eb_correction applied as well.
+ * (fb_eb_out alone), ghost=2: eb_correction applied as well, ghost=3 or 4:
+ * eb_correction_env (grid_deps_env.f90:240-283: every mode times 1/pi, the
+ * ghost row +row 1 (3, the reference's choice for nkO = 0) or -row 1 (4)).
  * = fb_eb_out + eb_correction (fb_io.f90:182-228, grid_deps.f90:219-266). */
 int32_t chimera_fb_inverse(double* eb_spc, const double* e_fb, const double* b_fb, double leftX, const double* kx,
                            const double* Out, int32_t nkx, int32_t nr, int32_t nmodes, int32_t nkr, int32_t ghost,

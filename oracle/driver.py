@@ -5,12 +5,15 @@ calls its f2py module.
 TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py as the checker; the product never imports it.
 
-Covered: FixedCell species (Still or pushed, analytic-undulator device),
+Covered: FixedCell and RandCell species (Still or pushed, external-field devices),
 SpaceCharge / StillAsBackground / NoPoissonCorrection / StaticKick solvers, Rcut, moving
 frames with AbsorbLayer / AddPlasma / IonsOnTop / Staged, and the
 'PreDepositBackground' run feature (background deposited before the initial
-half step).  Arrays and call order follow solvers.py, species.py and
-chimera_main.py line by line (cited per method).
+half step), and the envelope (KxShift) solver without space charge: modes
+-Nko..Nko, the env deposits and gather, complex PSATD tables, the
+anti-echo deposition filter and the spectral laser set-up.  Arrays and call
+order follow solvers.py, species.py and chimera_main.py line by line (cited
+per method).
 """
 from __future__ import annotations
 
@@ -18,7 +21,7 @@ import contextlib
 import inspect
 
 import numpy as np
-from scipy.special import jn, jn_zeros
+from scipy.special import j1, jn, jn_zeros
 
 from .fimera_ref import Fimera
 
@@ -49,7 +52,7 @@ def rounding_variant():
 
 
 class Solver:
-    """solvers.py:26-225 (non-KxShift)."""
+    """solvers.py:26-225."""
 
     def __init__(self, solver_in):
         c = _ch()
@@ -67,12 +70,16 @@ class Solver:
             A["Nx"] = int(2 * np.round(0.5 / A["dx"] * (A["rightX"] - A["leftX"])))
         A["Xgrid"] = A["rightX"] - A["dx"] * np.arange(A["Nx"])[::-1]
         A["leftX"] = A["Xgrid"][0]
-        A["kx"] = 2 * np.pi * np.fft.fftfreq(A["Nx"], A["dx"])
+        self.env = "KxShift" in A                                         # solvers.py:76-83
+        A["kx0"] = 2 * np.pi * A["KxShift"] if self.env else 0.0
+        A["kx_env"] = 2 * np.pi * np.fft.fftfreq(A["Nx"], A["dx"])
+        A["kx"] = A["kx0"] + A["kx_env"]
+        A["dkx"] = (A["kx"][1] - A["kx"][0]) / (2.0 * np.pi)
         A["Nkr"] = int(np.round(A["lengthR"] / A["dr"]))
         A["RgridFull"] = A["dr"] * (np.arange(A["Nkr"] + 1) - 0.5)
         A["lengthR"] = A["RgridFull"][-1] + A["dr"]
-        _, _, A["DpS2S"], A["DmS2S"], _, _, _, _ = self.get_spectral_operators(ext=1)
-        In, Out, _, _, A["w"], A["kr"], A["kr_g"], A["kx_g"] = self.get_spectral_operators(ext=0)
+        _, _, A["DpS2S"], A["DmS2S"], _, _, _, _, _ = self.get_spectral_operators(ext=1)
+        In, Out, _, _, A["w"], A["kr"], A["kr_g"], A["kx_g"], A["idxM"] = self.get_spectral_operators(ext=0)
         A["Mtot"] = In.shape[-1]
         A["PoissFact"] = 1.0 / A["w"] ** 2
         VG = 2 * np.pi * A["dx"] * A["dr"] * A["RgridFull"]
@@ -88,9 +95,15 @@ class Solver:
             A[kw] = np.asfortranarray(np.swapaxes(A[kw], 0, 1))
         A["FBDiff"] = [A["DpS2S"], A["DmS2S"], A["kx"]]
         A["DepProj"] = [A["Rgrid"], A["dx_inv"], A["dr_inv"]]
-        filt_x = np.cos(0.5 * A["dx"] * A["kx_g"])[:, :, None] ** 2
+        if self.env:                                                      # solvers.py:164-176
+            filt_x = self.antialias()
+            A["DepProj"] += [A["kx0"]]
+            kx_base = A["kx_env"]
+        else:
+            filt_x = np.cos(0.5 * A["dx"] * A["kx_g"])[:, :, None] ** 2
+            kx_base = A["kx"]
         A["DepFact"] = np.asfortranarray((2 * np.pi) ** 2 / A["Nx"] * filt_x * np.cos(0.5 * A["dr"] * A["kr_g"]) ** 2)
-        A["FBCurrIn"], A["FBIn"], A["FBout"] = (A["kx"], A["InCurr"]), (A["kx"], A["In"]), (A["kx"], A["Out"])
+        A["FBCurrIn"], A["FBIn"], A["FBout"] = (kx_base, A["InCurr"]), (kx_base, A["In"]), (kx_base, A["Out"])
         sp, fb = (A["Nx"], A["Nr"], A["Mtot"]), (A["Nx"], A["Nkr"], A["Mtot"])
         z = lambda s: np.zeros(s, dtype=complex, order="F")
         D = self.Data = {"EB": z(sp + (6,)), "J": z(sp + (3,)), "scl_spc": z(sp), "EG_fb": z(fb + (6,)),
@@ -103,10 +116,37 @@ class Solver:
         self.PSATD_coeffs()
         self.c = c
 
+    def antialias(self):                                                  # solvers.py:660-715
+        A = self.Args
+        kx_env, kx, kx0, Nx = A["kx_env"], A["kx"], A["kx0"], A["Nx"]
+        ca = 0.85
+        x = np.abs(kx_env) / np.abs(kx_env.max())
+        band = ((x < ca) + (x >= ca) * np.cos(np.pi / 2 * (x - ca) / (1 - ca)) ** 2)[:, None, None]
+        anti = np.ones_like(band)
+        feats = A["Features"]
+        if "NoAntiEcho" not in feats:
+            ae = feats["AntiEchoStrength"] if "AntiEchoStrength" in feats else 2
+            n_echo = int(np.abs(kx).max() / np.abs(kx_env).max()) + 1     # np.int in the reference
+            echoes = np.abs(kx_env).max() / kx0 * np.arange(n_echo) - 1.0
+            band_lims = np.array([kx.min() / kx0, kx.max() / kx0]) - 1.0
+            k = 0
+            for ce in echoes:
+                if band_lims[0] < ce < band_lims[1]:
+                    a = ae[k] if isinstance(ae, (list, tuple)) else ae
+                    k += 1
+                    if a <= 0:
+                        continue
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        f = 1 - np.exp(-(kx / kx0 - 1 - ce) ** 2 / (a * (ce + 1.0) / Nx) ** 2)
+                    anti = anti * f[:, None, None]
+        return anti * band
+
     def get_spectral_operators(self, ext=0):
-        """solvers.py:717-759 (non-KxShift)."""
+        """solvers.py:717-759."""
         A = self.Args
         Nx, Nkr, Nko, lengthR, kx, Rg = A["Nx"], A["Nkr"], A["Nko"], A["lengthR"], A["kx"], A["RgridFull"]
+        if self.env:
+            return self._spectral_operators_env(ext)
         Mmax, Mtot = Nko + ext, Nko + 1 + ext
         kr = np.zeros((Nkr, Mtot + 1))
         kr_g, w = np.zeros((Nx, Nkr, Mtot)), np.zeros((Nx, Nkr, Mtot))
@@ -122,11 +162,52 @@ class Solver:
             Dp[:, :, jm] = In[:, :, jm].dot(0.5 * kr[:, jm + 1][None, :] * jn(jm, Rg[1:, None] * kr[:, jm + 1][None, :]))
             km = kr[:, abs(jm - 1)][None, :]
             Dm[:, :, jm] = In[:, :, jm].dot(0.5 * km * jn(jm, Rg[1:, None] * km))
-        return In, Out, Dp, Dm, w, kr, kr_g, kx_g
+        return In, Out, Dp, Dm, w, kr, kr_g, kx_g, np.arange(0, Mmax + 1)
+
+    def _spectral_operators_env(self, ext):
+        """The KxShift branch of solvers.py:717-759: modes -Nko-ext..Nko+ext,
+        negative m stored at NumPy's wrapped index, reordered when Nko > 0."""
+        A = self.Args
+        Nx, Nkr, Nko, lengthR, kx, Rg = A["Nx"], A["Nkr"], A["Nko"], A["lengthR"], A["kx"], A["RgridFull"]
+        m0, m1, mt = -Nko - ext, Nko + ext, 2 * Nko + 1 + 2 * ext
+        kr = np.zeros((Nkr, mt + 2))
+        kr_g, w = np.zeros((Nx, Nkr, mt)), np.zeros((Nx, Nkr, mt))
+        Out, In = np.zeros((Nkr, Nkr, mt)), np.zeros((Nkr, Nkr, mt))
+        Dp, Dm = np.zeros((Nkr, Nkr, mt)), np.zeros((Nkr, Nkr, mt))
+        for jm in range(m0 - 1, m1 + 2):
+            kr[:, jm] = jn_zeros(jm, Nkr) / lengthR
+        kx_g = None
+        for jm in range(m0, m1 + 1):
+            kr_g[:, :, jm], kx_g = np.meshgrid(kr[:, jm], kx)
+            w[:, :, jm] = np.sqrt(kx_g ** 2 + kr_g[:, :, jm] ** 2)
+            Out[:, :, jm] = jn(jm, Rg[1:, None] * kr[:, jm][None, :])
+            In[:, :, jm] = np.linalg.inv(Out[:, :, jm])
+            kp = kr[:, jm + 1][None, :]
+            Dp[:, :, jm] = In[:, :, jm].dot(0.5 * kp * jn(jm, Rg[1:, None] * kp))
+            km = kr[:, abs(jm - 1)][None, :]
+            Dm[:, :, jm] = In[:, :, jm].dot(0.5 * km * jn(jm, Rg[1:, None] * km))
+        if Nko > 0:
+            Out, In, Dp, Dm, w, kr_g, kr = (np.concatenate((a[..., m0:], a[..., :m1 + 1]), axis=-1)
+                                            for a in (Out, In, Dp, Dm, w, kr_g, kr))
+        return In, Out, Dp, Dm, w, kr, kr_g, kx_g, np.arange(m0, m1 + 1)
 
     def PSATD_coeffs(self):
-        """solvers.py:227-279 (beta only enters the KxShift tables)."""
+        """solvers.py:227-279."""
         A, w, dt = self.Args, self.Args["w"], self.Args["TimeStep"]
+        if self.env:
+            if "SpaceCharge" in A["Features"]:
+                raise NotImplementedError("envelope solver with space charge")
+            k = A.get("CoPropagative", 1.0) * A["kx_g"][:, :, None]
+            E, G = np.zeros(w.shape + (3,), complex, order="F"), np.zeros(w.shape + (3,), complex, order="F")
+            E[..., 0], E[..., 1] = np.cos(dt * w), np.sin(dt * w) / w
+            G[..., 0], G[..., 1] = -w * np.sin(dt * w), np.cos(dt * w)
+            with np.errstate(divide="ignore", invalid="ignore"):           # kx = 0 on the grid: 0/0, as in the reference
+                E[..., 2] = np.exp(-0.5j * k * dt) * 1j * k / (w ** 2 - k ** 2) * (
+                    1.0 - np.exp(1j * k * dt) * (w / (1j * k) * np.sin(dt * w) + np.cos(dt * w)))
+                G[..., 2] = np.exp(-0.5j * k * dt) * w ** 2 / (w ** 2 - k ** 2) * (
+                    1.0 + np.exp(1j * k * dt) * (1j * k / w * np.sin(dt * w) - np.cos(dt * w)))
+            self.Data["PSATD_E"], self.Data["PSATD_G"] = E, G
+            return
         n = 5 if "SpaceCharge" in A["Features"] else 3
         E, G = np.zeros(w.shape + (n,), order="F"), np.zeros(w.shape + (n,), order="F")
         E[..., 0], E[..., 1] = np.cos(dt * w), np.sin(dt * w) / w
@@ -200,24 +281,33 @@ class Solver:
     def fb_fld_out(self):                                                 # solvers.py:450-465
         A, D = self.Args, self.Data
         D["EB"] = self.c.fb_eb_out(D["EB"], D["EG_fb"], D["B_fb"], A["leftX"], *A["FBout"])
-        D["EB"] = self.c.eb_correction(D["EB"])
+        D["EB"] = self.c.eb_correction_env(D["EB"]) if self.env else self.c.eb_correction(D["EB"])
 
     def FBGradDiv(self):
-        self.Data["vec_fb"] = self.c.fb_graddiv(self.Data["vec_fb"], *self.Args["FBDiff"])
+        fn = self.c.fb_graddiv_env if self.env else self.c.fb_graddiv
+        self.Data["vec_fb"] = fn(self.Data["vec_fb"], *self.Args["FBDiff"])
 
     def FBGradDens(self):
-        self.Data["gradRho_fb_nxt"] = self.c.fb_grad(self.Data["gradRho_fb_nxt"], self.Data["Rho_fb"],
-                                                     *self.Args["FBDiff"])
+        fn = self.c.fb_grad_env if self.env else self.c.fb_grad
+        self.Data["gradRho_fb_nxt"] = fn(self.Data["gradRho_fb_nxt"], self.Data["Rho_fb"], *self.Args["FBDiff"])
 
     def G2B_FBRot(self):                                                  # solvers.py:536-553
         D = self.Data
-        D["B_fb"] = self.c.fb_rot(D["B_fb"], np.asfortranarray(D["EG_fb"][:, :, :, 3:]), *self.Args["FBDiff"])
+        fn = self.c.fb_rot_env if self.env else self.c.fb_rot
+        D["B_fb"] = fn(D["B_fb"], np.asfortranarray(D["EG_fb"][:, :, :, 3:]), *self.Args["FBDiff"])
         D["B_fb"] = self.c.omp_mult_vec(D["B_fb"], self.Args["PoissFact"])
 
     def add_gauss_beam(self, S):                                          # solvers.py:555-610
         A, D = self.Args, self.Data
         k0, a0, X_focus = 2 * np.pi * S["k0"], 2 * np.pi * S["a0"], S["x0"] - S["x_foc"]
         kx_g, w = A["kx_g"], A["w"][:, :, :, None]
+        if self.env:
+            nk, kr0 = A["Nko"], A["kr_g"][:, :, A["Nko"]]
+            e_s0 = a0 * 0.5 * np.pi ** 0.5 * S["Lx"] * S["LR"] ** 2 * A["dkx"] / A["lengthR"] ** 2
+            D["vec_fb"][:, :, nk, 2] = e_s0 / j1(A["lengthR"] * kr0) ** 2 * np.exp(-1j * kx_g * S["x0"]) \
+                * np.exp(-0.25 * (kx_g - k0) ** 2 * S["Lx"] ** 2 - 0.25 * kr0 ** 2 * S["LR"] ** 2)
+            self._propagate_beam(-1.0j * w, w, kx_g, X_focus)
+            return
         Xg, Rg = A["Xgrid"], A["Rgrid"]
         D["scl_spc"][:, :, 0] = a0 * np.cos(k0 * (Xg[:, None] - S["x0"])) * np.exp(
             -(Xg[:, None] - S["x0"]) ** 2 / S["Lx"] ** 2 - Rg[None, :] ** 2 / S["LR"] ** 2) \
@@ -226,6 +316,10 @@ class Solver:
         D["scl_fb"] = self.c.fb_scl_in(D["scl_fb"], D["scl_spc"], A["leftX"], *A["FBIn"])
         D["vec_fb"][:, :, :, 2] = D["scl_fb"] / A["Nx"]
         DT = -1.0j * w * np.sign(kx_g[:, :, None, None] + (kx_g[:, :, None, None] == 0))
+        self._propagate_beam(DT, w, kx_g, X_focus)
+
+    def _propagate_beam(self, DT, w, kx_g, X_focus):                     # solvers.py:594-610
+        D = self.Data
         EE = self.div_clean(D["vec_fb"].copy(order="F"))
         GG = DT * EE
         D["vec_fb"][:] = np.cos(w * X_focus) * EE + np.sin(w * X_focus) / w * GG
@@ -258,7 +352,7 @@ class Solver:
 
 
 class Specie:
-    """species.py:24-398 (FixedCell)."""
+    """species.py:24-398 (FixedCell, RandCell)."""
 
     def __init__(self, specie_in):
         A = self.Args = specie_in
@@ -278,12 +372,15 @@ class Specie:
         A["Nr"] = int(np.round(A["lengthR"] / A["dr"]))
         A["Rgrid"] = A["dr"] * (np.arange(A["Nr"]) - 0.5)
         A["lowerR"], A["upperR"] = (A["Rgrid"] * (A["Rgrid"] >= 0)).min(), A["Rgrid"].max()
-        fc = A["FixedCell"]
-        A["Num_p"] = int(np.prod(fc))
-        pX, pR, pO = np.mgrid[1:fc[0]:fc[0] * 1j, 1:fc[1]:fc[1] * 1j, 1:fc[2]:fc[2] * 1j]
-        A["Pax"] = (np.asfortranarray((pX.ravel() - 0.5) / fc[0]), np.asfortranarray((pR.ravel() - 0.5) / fc[1]),
-                    np.asfortranarray(np.exp(2.0j * np.pi * (pO.ravel() - 1) / fc[2])))
-        A["wght0"] = A["Charge"] * A["Density"] * A["dr"] * A["dx"] * 2 * np.pi / A["Num_p"]
+        if "FixedCell" in A:                                              # species.py:95-111
+            fc = A["FixedCell"]
+            A["Num_p"] = int(np.prod(fc))
+            pX, pR, pO = np.mgrid[1:fc[0]:fc[0] * 1j, 1:fc[1]:fc[1] * 1j, 1:fc[2]:fc[2] * 1j]
+            A["Pax"] = (np.asfortranarray((pX.ravel() - 0.5) / fc[0]), np.asfortranarray((pR.ravel() - 0.5) / fc[1]),
+                        np.asfortranarray(np.exp(2.0j * np.pi * (pO.ravel() - 1) / fc[2])))
+        else:
+            A["Num_p"] = int(A.get("RandCell", 0))
+        A["wght0"] = A["Charge"] * A["Density"] * A["dr"] * A["dx"] * 2 * np.pi / A["Num_p"] if A["Num_p"] else 0.0
         self.Data = {"EB": np.zeros((6, 0), order="F"), "coords": np.zeros((3, 0), order="F"),
                      "weights": np.zeros((0,), order="F"), "coords_halfstep": np.zeros((3, 0), order="F"),
                      "momenta": np.zeros((3, 0), order="F")}
@@ -302,8 +399,11 @@ class Specie:
         elif Xsteps is not None:
             Xgrid = Xgrid[-2 * Xsteps:-Xsteps]
         coords = np.zeros((4, Xgrid.shape[0] * Rgrid.shape[0] * A["Num_p"]), order="F")
-        RandPackO = np.random.rand(Xgrid.shape[0], Rgrid.shape[0]).astype("d", order="F")
-        coords, n = _ch().genparts(coords, Xgrid, Rgrid, RandPackO, *A["Pax"])
+        if "FixedCell" in A:
+            RandPackO = np.random.rand(Xgrid.shape[0], Rgrid.shape[0]).astype("d", order="F")
+            coords, n = _ch().genparts(coords, Xgrid, Rgrid, RandPackO, *A["Pax"])
+        else:
+            coords, n = self.gen_randcell(coords, Xgrid, Rgrid)
         coords = coords[:, :n]
         if ProfileFunc is None:
             coords[-1] *= A["wght0"]
@@ -311,10 +411,36 @@ class Specie:
             na = len(inspect.signature(ProfileFunc).parameters)
             arg = {1: (coords[0],), 2: (coords[0], np.sqrt((coords[1:3] ** 2).sum(0))), 3: tuple(coords[0:3])}[na]
             coords[-1] *= A["wght0"] * ProfileFunc(*arg)
-        rm = np.random.randn(3, n)
+        rm = 2 * np.random.rand(3, n) - 1.0 if "FlatSpectrum" in A["Features"] else np.random.randn(3, n)
         mm, ms = A["MomentaMeans"], A["MomentaSpreads"]
         momenta = np.vstack([mm[c] + ms[c] * rm[c] for c in range(3)]).astype("d", order="F")
         return [np.asfortranarray(coords[0:3]), momenta, coords[-1].astype("d", order="F")]
+
+    def gen_randcell(self, coords, Xgrid, Rgrid):                         # species.py:429-471
+        """Num_p particles per cell (one cell per pair of neighbouring grid
+        nodes), each axis stratified: the k-th particle sits in the k-th
+        sub-interval, jittered by up to half of one, and the three axes are
+        paired by three independent shuffles.  Draws from NumPy's global
+        stream in the order the reference does: rand(3, Num_p), then the x, r
+        and angle shuffles, cell by cell with r running fastest."""
+        npc, dx, dr = self.Args["Num_p"], self.Args["dx"], self.Args["dr"]
+        k = np.arange(npc)
+        ip = 0
+        for ix in range(Xgrid.shape[0] - 1):
+            for ir in range(Rgrid.shape[0] - 1):
+                jit = 2 * (np.random.rand(3, npc) - 0.5)
+                xc = Xgrid[ix] + dx * (k + 0.5 * jit[0]) / npc
+                rc = Rgrid[ir] + dr * (k + 0.5 * jit[1]) / npc
+                oc = 2 * np.pi * (k + 0.5 * jit[2]) / npc
+                np.random.shuffle(xc)
+                np.random.shuffle(rc)
+                np.random.shuffle(oc)
+                coords[0, ip:ip + npc] = xc
+                coords[1, ip:ip + npc] = rc * np.cos(oc)
+                coords[2, ip:ip + npc] = rc * np.sin(oc)
+                coords[3, ip:ip + npc] = rc
+                ip += npc
+        return coords, ip
 
     def beam_focus(self, x_foc):                                          # species.py:473-486
         D = self.Data
@@ -463,8 +589,9 @@ class ChimeraRun:
             for sp in self.Particles:
                 if sp.Data["coords"].shape[1] == 0 or "Still" in sp.Args["Features"]:
                     continue
-                D["J"] = _ch().dep_curr(sp.Data["coords_halfstep"], sp.Data["momenta"], sp.Data["weights"], D["J"],
-                                        solver.Args["leftX"], *solver.Args["DepProj"])
+                fn = _ch().dep_curr_env if solver.env else _ch().dep_curr    # chimera_main.py:153-181
+                D["J"] = fn(sp.Data["coords_halfstep"], sp.Data["momenta"], sp.Data["weights"], D["J"],
+                            solver.Args["leftX"], *solver.Args["DepProj"])
             solver.fb_curr_in()
 
     def update_fields(self):                                              # chimera_main.py:116-128
@@ -499,8 +626,8 @@ class ChimeraRun:
         for sp in self.Particles:
             if sp.Data["coords"].shape[1] == 0 or "Still" in sp.Args["Features"]:
                 continue
-            D["Rho"] = _ch().dep_dens(sp.Data[component], sp.Data["weights"], D["Rho"], solver.Args["leftX"],
-                                      *solver.Args["DepProj"])
+            fn = _ch().dep_dens_env if solver.env else _ch().dep_dens
+            D["Rho"] = fn(sp.Data[component], sp.Data["weights"], D["Rho"], solver.Args["leftX"], *solver.Args["DepProj"])
 
     def dep_bg(self, solver):                                             # chimera_main.py:220-248
         D = solver.Data
@@ -508,8 +635,9 @@ class ChimeraRun:
         for sp in self.Particles:
             if sp.Data["coords"].shape[1] == 0 or "Still" not in sp.Args["Features"]:
                 continue
-            D["BckGrndRho"] = _ch().dep_dens(sp.Data["coords"], sp.Data["weights"], D["BckGrndRho"],
-                                             solver.Args["leftX"], *solver.Args["DepProj"])
+            fn = _ch().dep_dens_env if solver.env else _ch().dep_dens
+            D["BckGrndRho"] = fn(sp.Data["coords"], sp.Data["weights"], D["BckGrndRho"], solver.Args["leftX"],
+                                 *solver.Args["DepProj"])
 
     def project_fields(self):                                             # chimera_main.py:130-151
         for sp in self.Particles:
@@ -520,8 +648,9 @@ class ChimeraRun:
             for sp in self.Particles:
                 if sp.Data["coords"].shape[1] == 0 or "Still" in sp.Args["Features"]:
                     continue
-                sp.Data["EB"] = _ch().proj_fld(sp.Data["coords"], sp.Data["weights"], solver.Data["EB"], sp.Data["EB"],
-                                               solver.Args["leftX"], *solver.Args["DepProj"])
+                fn = _ch().proj_fld_env if solver.env else _ch().proj_fld    # chimera_main.py:139-151
+                sp.Data["EB"] = fn(sp.Data["coords"], sp.Data["weights"], solver.Data["EB"], sp.Data["EB"],
+                                   solver.Args["leftX"], *solver.Args["DepProj"])
 
     def frame_act(self, istep, act="stage1"):                             # chimera_main.py:250-304
         for wind in self.MovingFrames:

@@ -10,6 +10,7 @@ order), <= 1e-14 elementwise (FMA contraction only), <= 1e-12 for the
 operators (the D+- products).  Particles on the axis and with zero weight or
 zero momentum are among the inputs.
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -144,3 +145,60 @@ def test_env_psatd_complex_tables(gpu, port):
     b = port.maxwell_push_wo_spchrg(EG.copy(order="F"), j, C1, C2)
     say(f"  {rel_max(a, b):.2e}")
     assert rel_max(a, b) < TOL_ELEM
+
+
+@pytest.mark.parametrize("nko", [0, 1])
+def test_env_resident_entries(gpu, port, nko):
+    """The resident envelope entries (particles structure-of-arrays in device
+    memory) against the oracle: the deposits are left unfolded (the forward
+    transform folds them), so the ghost fold is applied here on the host; the
+    gather runs with an analytic undulator and the momentum push fused, the
+    oracle as proj_fld_env, undul_analytic, push_velocs."""
+    from chimera_amd.device import DeviceArray, call
+    from chimera_amd.species import DeviceRecord
+    say(f"case: soa_dep_curr_env / soa_dep_dens_env / soa_gather_push_env nko={nko}")
+    g = Grid(nx=64, nkr=16, nko=nko)
+    coord, mom, w = particles(g, 5000, seed=590 + nko)
+    n, nm = coord.shape[1], 2 * nko + 1
+    X = DeviceArray.from_numpy(np.asfortranarray(coord.T))
+    U = DeviceArray.from_numpy(np.asfortranarray(mom.T))
+    Wd = DeviceArray.from_numpy(w)
+    R = DeviceArray.from_numpy(g.Rgrid)
+    geom = (g.leftX, R.ptr, 1 / g.dx, 1 / g.dr, float(g.Rgrid[0]), float(g.Rgrid[-1]), KX0, g.nx - 1, g.nkr, nko)
+    args = (g.leftX, g.Rgrid, 1 / g.dx, 1 / g.dr, KX0)
+
+    def fold(a):
+        a[:, 1] -= a[:, 0]
+        a[:, 0] = 0.0
+        return a
+
+    J = DeviceArray((g.nx, g.nkr + 1, nm, 3), np.complex128)
+    call("chimera_soa_dep_curr_env", X.ptr, U.ptr, Wd.ptr, n, n, J.ptr, *geom, None)
+    a = fold(J.numpy())
+    b = port.dep_curr_env(coord, mom, w, np.zeros((g.nx, g.nkr + 1, nm, 3), complex, order="F"), *args)
+    say(f"  curr {rel_l2(a, b):.2e}")
+    assert rel_l2(a, b) < TOL_SUM
+    Dn = DeviceArray((g.nx, g.nkr + 1, nm), np.complex128)
+    call("chimera_soa_dep_dens_env", X.ptr, Wd.ptr, n, n, Dn.ptr, *geom, None)
+    a = fold(Dn.numpy())
+    b = port.dep_dens_env(coord, w, np.zeros((g.nx, g.nkr + 1, nm), complex, order="F"), *args)
+    say(f"  dens {rel_l2(a, b):.2e}")
+    assert rel_l2(a, b) < TOL_SUM
+
+    EB = C(rng(592), g.nx, g.nkr + 1, nm, 6)
+    EBd = DeviceArray.from_numpy(EB)
+    und = np.array([1.3, 2.0, g.leftX - 1.0, 40.0])
+    recs = (DeviceRecord * 1)()
+    recs[0].kind = 0
+    for i, v in enumerate(und):
+        recs[0].params[i] = v
+    push_dt, t = 0.37, 1.5
+    call("chimera_soa_gather_push_env", X.ptr, U.ptr, Wd.ptr, n, n, EBd.ptr, *geom, push_dt, ctypes.addressof(recs), 1,
+         t, None)
+    a = U.numpy().T
+    ft = port.proj_fld_env(coord, w, EB, np.zeros((6, n), order="F"), *args)
+    ft = port.undul_analytic(coord, ft, t, und)
+    b = port.push_velocs(mom.copy(order="F"), ft, push_dt)
+    say(f"  gather+push {rel_max(a, b):.2e}")
+    assert rel_max(a, b) < TOL_SUM
+    assert rel_l2(a, mom) > 1e-3                      # the push acted

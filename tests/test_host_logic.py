@@ -22,6 +22,7 @@ class _Comp:
         self.Args["leftX"], self.Args["rightX"] = self.Args["Xgrid"][0], self.Args["Xgrid"][-1]
         self.Data = {"gradRho_fb_prv": "prv", "gradRho_fb_nxt": "nxt"}
         self.frame = None
+        self.env = False
 
 
 def _run(frames, features=("SpaceCharge",), xgrid=None):

@@ -49,7 +49,7 @@ def solver(nko=2, grid=(-6.4, 0.0, 9.6, 0.1, 0.3), features=("SpaceCharge",), dt
 def test_bessel_roots_and_radial_grid():
     s = solver()
     A = s.Args
-    In, Out, Dp, Dm, w, kr, kr_g, kx_g = s.get_spectral_operators(ext=1)
+    In, Out, Dp, Dm, w, kr, kr_g, kx_g, idxM = s.get_spectral_operators(ext=1)
     Nkr, dr = A["Nkr"], A["dr"]
     assert A["lengthR"] == pytest.approx(dr * (Nkr + 0.5), rel=1e-15)
     assert np.allclose(A["RgridFull"], dr * (np.arange(Nkr + 1) - 0.5), rtol=0, atol=1e-15)
@@ -78,7 +78,7 @@ def test_hankel_pair_and_single_mode():
 
 def test_dpm_against_bessel_recurrences():
     s = solver(nko=3)
-    In, Out, Dp, Dm, w, kr, kr_g, kx_g = s.get_spectral_operators(ext=1)
+    In, Out, Dp, Dm, w, kr, kr_g, kx_g, idxM = s.get_spectral_operators(ext=1)
     r = s.Args["RgridFull"][1:, None]
     rng = np.random.default_rng(1)
     n = s.Args["Nkr"]

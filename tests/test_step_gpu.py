@@ -23,6 +23,12 @@ def say(msg):
 
 
 def moments(u, w):
+    """Total charge, weighted mean and RMS of each momentum component.  The
+    arrays are made C-contiguous first: NumPy sums a Fortran-ordered (3, n)
+    array along axis 1 in another order (not pairwise), and for a beam whose
+    mean momentum is 1e4 times its spread that alone moves the mean by 3e-10
+    of the spread."""
+    u, w = np.ascontiguousarray(u), np.ascontiguousarray(w)
     W = w.sum()
     mean = (u * w).sum(1) / W
     rms = np.sqrt(((u - mean[:, None]) ** 2 * w).sum(1) / W)
@@ -45,7 +51,7 @@ def snapshot_gpu(run):
     s, e = run.Solvers[0], run.Particles[0]
     snap = {k: s.Data[k].numpy() for k in ("EG_fb", "J_fb", "EB")}
     snap.update(np=e.np, momenta=e.host("momenta"), coords=e.host("coords"), weights=e.host("weights"),
-                np_ions=run.Particles[1].np)
+                np_ions=run.Particles[1].np if len(run.Particles) > 1 else 0)
     return snap
 
 
@@ -53,7 +59,8 @@ def snapshot_cpu(run):
     s, e = run.Solvers[0], run.Particles[0]
     snap = {k: s.Data[k].copy() for k in ("EG_fb", "J_fb", "EB")}
     snap.update(np=e.Data["coords"].shape[1], momenta=e.Data["momenta"].copy(), coords=e.Data["coords"].copy(),
-                weights=e.Data["weights"].copy(), np_ions=run.Particles[1].Data["coords"].shape[1])
+                weights=e.Data["weights"].copy(),
+                np_ions=run.Particles[1].Data["coords"].shape[1] if len(run.Particles) > 1 else 0)
     return snap
 
 
