@@ -87,7 +87,8 @@ def run_gpu(args, precision):
     spec = W.with_precision(W.WORKLOADS[args.workload](), precision)
     log(f"building {args.workload} ({precision})")
     t0 = time.time()
-    run = W.build(G, spec, seed=args.seed)
+    opts = {"SortEvery": args.sort_every} if args.sort_every > 0 else {}
+    run = W.build(G, spec, seed=args.seed, **opts)
     L = lib()
     L.chimera_synchronize()
     s, e = run.Solvers[0], run.Particles[0]
@@ -525,6 +526,8 @@ def main():
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", type=int, default=1, help="1: GEMM tile shapes chosen by timing (first use)")
+    ap.add_argument("--sort-every", type=int, default=20, help="sort the electrons by cell every N steps (on the "
+                    "window moves of the LPA workloads, where the sort also does the absorption); 0: never")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-slab", type=int, default=8, help="the CPU baseline runs 1/N of the workload's x extent")
     ap.add_argument("--bunch-stages", type=int, nargs=2, metavar=("NX", "NKR"),
@@ -594,7 +597,7 @@ def main():
         "data": "synthetic: seeded FixedCell 2x2x4 plasma (n=0.005 n_c) + Gaussian laser, random draws from NumPy",
         "config": {"workload": args.workload, "Nx": m["Nx"], "Nkr": m["Nkr"], "modes": m["M"],
                    "electrons": m["P"], "precision": main_prec, "moving_window": True,
-                   "timed_steps": m["timed_steps"]},
+                   "timed_steps": m["timed_steps"], "sort_every": args.sort_every},
         "fb_transform_ms_per_step": m["fb_transform_ms_per_step"],
         "spectral_operator_ms_per_step": m["spectral_operator_ms_per_step"],
         "kernel_ms_per_step": m["kernel_ms_per_step"],

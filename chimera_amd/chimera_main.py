@@ -39,8 +39,8 @@ class ChimeraRun:
         if len(precs) > 1:
             raise ValueError(f"solvers and species must share one Precision, got {sorted(precs)}")
         self.mixed = precs == {"mixed"}
-        if self.mixed and self.SortEvery:
-            raise NotImplementedError("SortEvery is implemented for FP64 runs")
+        self._sort_step = False     # this step sorts the pushed species by cell (SortEvery)
+        self._sorted = set()        # ids of the species already sorted in this step
         self.init_Moving_Frames()
         self.init_damp_profile()
         if "PreDepositBackground" in self.Features:
@@ -87,10 +87,12 @@ class ChimeraRun:
     def make_step(self, istep):
         if self.Solvers and self.Solvers[0].frame is not None:
             call("chimera_frame_step", self.Solvers[0].frame.ptr, *self._frame_shifts(istep))
+        self._sort_step = bool(self.SortEvery and np.mod(istep, self.SortEvery) == 0 and self.Solvers)
+        self._sorted.clear()
         self.frame_act(istep)                                              # chimera_main.py:82-92
-        if self.SortEvery and np.mod(istep, self.SortEvery) == 0 and self.Solvers:
+        if self._sort_step:
             for species in self.Particles:
-                if "Still" not in species.Args["Features"]:
+                if "Still" not in species.Args["Features"] and id(species) not in self._sorted:
                     species.sort_cells(self.Solvers[0])
         if self._fusable(istep):
             # push_coords + dep_curr + dep_dens in one pass per species; the
@@ -299,10 +301,18 @@ class ChimeraRun:
                 solver.damp_field()
 
     def damp_plasma(self, wind):
+        """The absorbing layer on the particles (species.py:351-398).  On a
+        step that sorts by cell, a pushed species is sorted here instead, with
+        the layer's bounds: the sort drops the same particles, so the one
+        pass replaces the compaction."""
         if "AbsorbLayer" not in wind:
             return
         for species in self.Particles:
-            species.chunk_and_damp(wind=wind)
+            if self._sort_step and "Still" not in species.Args["Features"]:
+                species.sort_cells(self.Solvers[0], SimDom=species.damp_lims(wind))
+                self._sorted.add(id(species))
+            else:
+                species.chunk_and_damp(wind=wind)
 
     def add_plasma(self, wind):
         if "AddPlasma" not in wind:

@@ -598,7 +598,8 @@ __global__ void k_keep_flags(const double* __restrict__ coord, int64_t ld, doubl
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= np) return;
     const double x = coord[ip], y = coord[ld + ip], z = coord[2 * ld + ip];
-    const double r2 = y * y + z * z;
+    // rounded as written (no contraction), as in the selection of k_cell_keys
+    const double r2 = __dadd_rn(__dmul_rn(y, y), __dmul_rn(z, z));
     flag[ip] = (x >= l0 && x <= l1 && r2 >= l2 && r2 <= l3) ? 1 : 0;
 }
 
@@ -694,10 +695,11 @@ __global__ void k_cell_keys(const double* __restrict__ x, const double* __restri
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(kept, (int32_t)__popcll(m));
 }
 
+template <typename R>
 __global__ void k_gather_particles(const int32_t* __restrict__ idx, int n, const double* __restrict__ x,
-                                   const double* __restrict__ xh, const double* __restrict__ u,
-                                   const double* __restrict__ w, int64_t ld, double* __restrict__ x2,
-                                   double* __restrict__ xh2, double* __restrict__ u2, double* __restrict__ w2,
+                                   const double* __restrict__ xh, const R* __restrict__ u,
+                                   const R* __restrict__ w, int64_t ld, double* __restrict__ x2,
+                                   double* __restrict__ xh2, R* __restrict__ u2, R* __restrict__ w2,
                                    int64_t ld2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -714,6 +716,45 @@ __global__ void k_gather_particles(const int32_t* __restrict__ idx, int n, const
 }  // namespace chimera
 
 using namespace chimera;
+
+template <typename R>
+static int32_t sort_cells_impl(const double* coord, const double* coord_half, const R* momenta, const R* wghts,
+                               int64_t ld, int32_t np, double* coord2, double* coord_half2, R* momenta2, R* wghts2,
+                               int64_t ld2, double leftX, double dx_inv, double dr_inv, double r0, int32_t nx,
+                               int32_t nr, const double* lims, int32_t test_half, int32_t* np_out, int32_t* perm) {
+    CH_TRY(check_resident(np, ld, 0));
+    if (np_out == nullptr || ld2 < np || nx < 1 || nr < 1) return fail(ST_EINVAL, "soa_sort_cells: bad arguments");
+    if ((int64_t)nx * nr + 2 > (int64_t)UINT32_MAX) return fail(ST_EINVAL, "soa_sort_cells: too many cells");
+    if (np == 0) {
+        *np_out = 0;
+        return ST_OK;
+    }
+    KTimer kt(KF_SORT);
+    uint32_t* keys = static_cast<uint32_t*>(scratch(8, (size_t)np * sizeof(uint32_t)));
+    int32_t* idx = perm ? perm : static_cast<int32_t*>(scratch(9, (size_t)np * sizeof(int32_t)));
+    uint32_t* ktmp = static_cast<uint32_t*>(scratch(10, (size_t)np * sizeof(uint32_t)));
+    int32_t* itmp = static_cast<int32_t*>(scratch(11, (size_t)np * sizeof(int32_t)));
+    int32_t* kept = static_cast<int32_t*>(scratch(15, 256));
+    if (!keys || !idx || !ktmp || !itmp || !kept) return ST_ENOMEM;
+    CH_HIP(hipMemsetAsync(kept, 0, sizeof(int32_t), stream()));
+    double L[4] = {0, 0, 0, 0};
+    if (lims) memcpy(L, lims, sizeof L);
+    hipLaunchKernelGGL(k_cell_keys, blocks_for(np, 256), 256, 0, stream(), coord, coord_half, ld, np, leftX, dx_inv,
+                       dr_inv, r0, nx, nr, lims ? 1 : 0, L[0], L[1], L[2], L[3], test_half, keys, kept);
+    CH_LAUNCH_CHECK("k_cell_keys");
+    const uint64_t maxkey = (uint64_t)nx * nr + 1;
+    int bits = 1;
+    while ((1ull << bits) <= maxkey) bits++;
+    CH_TRY(radix_sort_keys(keys, idx, ktmp, itmp, np, bits, 12, 13));
+    int32_t hk = 0;
+    CH_HIP(hipMemcpyAsync(&hk, kept, sizeof hk, hipMemcpyDeviceToHost, stream()));
+    CH_HIP(hipStreamSynchronize(stream()));
+    hipLaunchKernelGGL(k_gather_particles<R>, blocks_for(hk, 256), 256, 0, stream(), idx, hk, coord, coord_half, momenta,
+                       wghts, ld, coord2, coord_half2, momenta2, wghts2, ld2);
+    CH_LAUNCH_CHECK("k_gather_particles");
+    *np_out = hk;
+    return ST_OK;
+}
 
 // The device list of a resident gather-push: none, or the one analytic
 // undulator of undul_params (host, 4 doubles), or the described devices.
@@ -1049,38 +1090,17 @@ int32_t chimera_soa_sort_cells(const double* coord, const double* coord_half, co
                                double* momenta2, double* wghts2, int64_t ld2, double leftX, double dx_inv,
                                double dr_inv, double r0, int32_t nx, int32_t nr, const double* lims, int32_t test_half,
                                int32_t* np_out, int32_t* perm) {
-    CH_TRY(check_resident(np, ld, 0));
-    if (np_out == nullptr || ld2 < np || nx < 1 || nr < 1) return fail(ST_EINVAL, "soa_sort_cells: bad arguments");
-    if ((int64_t)nx * nr + 2 > (int64_t)UINT32_MAX) return fail(ST_EINVAL, "soa_sort_cells: too many cells");
-    if (np == 0) {
-        *np_out = 0;
-        return ST_OK;
-    }
-    KTimer kt(KF_SORT);
-    uint32_t* keys = static_cast<uint32_t*>(scratch(8, (size_t)np * sizeof(uint32_t)));
-    int32_t* idx = perm ? perm : static_cast<int32_t*>(scratch(9, (size_t)np * sizeof(int32_t)));
-    uint32_t* ktmp = static_cast<uint32_t*>(scratch(10, (size_t)np * sizeof(uint32_t)));
-    int32_t* itmp = static_cast<int32_t*>(scratch(11, (size_t)np * sizeof(int32_t)));
-    int32_t* kept = static_cast<int32_t*>(scratch(15, 256));
-    if (!keys || !idx || !ktmp || !itmp || !kept) return ST_ENOMEM;
-    CH_HIP(hipMemsetAsync(kept, 0, sizeof(int32_t), stream()));
-    double L[4] = {0, 0, 0, 0};
-    if (lims) memcpy(L, lims, sizeof L);
-    hipLaunchKernelGGL(k_cell_keys, blocks_for(np, 256), 256, 0, stream(), coord, coord_half, ld, np, leftX, dx_inv,
-                       dr_inv, r0, nx, nr, lims ? 1 : 0, L[0], L[1], L[2], L[3], test_half, keys, kept);
-    CH_LAUNCH_CHECK("k_cell_keys");
-    const uint64_t maxkey = (uint64_t)nx * nr + 1;
-    int bits = 1;
-    while ((1ull << bits) <= maxkey) bits++;
-    CH_TRY(radix_sort_keys(keys, idx, ktmp, itmp, np, bits, 12, 13));
-    int32_t hk = 0;
-    CH_HIP(hipMemcpyAsync(&hk, kept, sizeof hk, hipMemcpyDeviceToHost, stream()));
-    CH_HIP(hipStreamSynchronize(stream()));
-    hipLaunchKernelGGL(k_gather_particles, blocks_for(hk, 256), 256, 0, stream(), idx, hk, coord, coord_half, momenta,
-                       wghts, ld, coord2, coord_half2, momenta2, wghts2, ld2);
-    CH_LAUNCH_CHECK("k_gather_particles");
-    *np_out = hk;
-    return ST_OK;
+    return sort_cells_impl<double>(coord, coord_half, momenta, wghts, ld, np, coord2, coord_half2, momenta2, wghts2,
+                                   ld2, leftX, dx_inv, dr_inv, r0, nx, nr, lims, test_half, np_out, perm);
+}
+
+int32_t chimera_soa_sort_cells_mixed(const double* coord, const double* coord_half, const float* momenta,
+                                     const float* wghts, int64_t ld, int32_t np, double* coord2, double* coord_half2,
+                                     float* momenta2, float* wghts2, int64_t ld2, double leftX, double dx_inv,
+                                     double dr_inv, double r0, int32_t nx, int32_t nr, const double* lims,
+                                     int32_t test_half, int32_t* np_out, int32_t* perm) {
+    return sort_cells_impl<float>(coord, coord_half, momenta, wghts, ld, np, coord2, coord_half2, momenta2, wghts2,
+                                  ld2, leftX, dx_inv, dr_inv, r0, nx, nr, lims, test_half, np_out, perm);
 }
 
 }  // extern "C"

@@ -301,7 +301,7 @@ class Specie:
         if wind is None:
             wind = {"Features": (), "AbsorbLayer": 0.0}
         if SimDom is None:
-            SimDom = np.array([A["leftX"] + wind["AbsorbLayer"] * A["dx"], A["rightX"], 0.0, A["upperR"] ** 2])
+            SimDom = self.damp_lims(wind)
         half = ("Xchunked" in A and "NoSorting" not in wind["Features"] and position == "cntr")
         lims = np.ascontiguousarray(SimDom, dtype=np.float64)
         nout = ctypes.c_int32(0)
@@ -319,6 +319,12 @@ class Specie:
         self._spare, self.Data = self.Data, spare
         self.np = int(nout.value)
 
+    def damp_lims(self, wind):
+        """The bounds kept by the absorbing layer of `wind` (species.py:360-362):
+        x from the layer's inner edge to the right edge, r^2 up to upperR^2."""
+        A = self.Args
+        return np.array([A["leftX"] + wind["AbsorbLayer"] * A["dx"], A["rightX"], 0.0, A["upperR"] ** 2])
+
     def sort_cells(self, solver, SimDom=None, position="stag"):
         """Stable counting sort of the particles by cell of `solver`'s grid,
         dropping those outside SimDom when it is given (libchimera_hip
@@ -326,16 +332,15 @@ class Specie:
         that deposition and gather touch neighbouring nodes."""
         if self.np == 0:
             return
-        if self.mixed:
-            raise NotImplementedError("sorting by cell is implemented for FP64 species")
-        spare = getattr(self, "_spare", None)
+        spare = getattr(self, "_spare", None)        # shared with chunk_and_damp
         if spare is None or spare["weights"].shape[0] != self.cap:
-            spare = {k: DeviceArray((self.cap, 3), np.float64) for k in ("coords", "coords_halfstep", "momenta")}
-            spare["weights"] = DeviceArray((self.cap,), np.float64)
+            spare = {k: DeviceArray((self.cap, 3), np.float64) for k in ("coords", "coords_halfstep")}
+            spare["momenta"] = DeviceArray((self.cap, 3), self.rt)
+            spare["weights"] = DeviceArray((self.cap,), self.rt)
         g = solver.geom_args()
         lims = np.ascontiguousarray(SimDom, dtype=np.float64) if SimDom is not None else None
         nout = ctypes.c_int32(0)
-        call("chimera_soa_sort_cells", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"),
+        call("chimera_soa_sort_cells_mixed" if self.mixed else "chimera_soa_sort_cells", self.ptr("coords"), self.ptr("coords_halfstep"), self.ptr("momenta"),
              self.ptr("weights"), self.ld, self.np, spare["coords"].ptr, spare["coords_halfstep"].ptr,
              spare["momenta"].ptr, spare["weights"].ptr, self.cap, g[0], g[2], g[3], g[4], g[6], g[7],
              lims.ctypes.data if lims is not None else None, 1 if position == "cntr" else 0, ctypes.addressof(nout),

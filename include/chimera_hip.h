@@ -479,6 +479,12 @@ int32_t chimera_soa_sort_cells(const double* coord, const double* coord_half, co
                                double* momenta2, double* wghts2, int64_t ld2, double leftX, double dx_inv,
                                double dr_inv, double r0, int32_t nx, int32_t nr, const double* lims, int32_t test_half,
                                int32_t* np_out, int32_t* perm);
+/* The same with float32 momenta and weights (the species of a mixed-precision run). */
+int32_t chimera_soa_sort_cells_mixed(const double* coord, const double* coord_half, const float* momenta,
+                                     const float* wghts, int64_t ld, int32_t np, double* coord2, double* coord_half2,
+                                     float* momenta2, float* wghts2, int64_t ld2, double leftX, double dx_inv,
+                                     double dr_inv, double r0, int32_t nx, int32_t nr, const double* lims,
+                                     int32_t test_half, int32_t* np_out, int32_t* perm);
 
 /* Forward FB transform of ncomp components, fb_vec_in/fb_scl_in with nkO =
  * nmodes, times scale(nkx,nkr,nmodes) (DepFact) when not NULL.  With fold=1
