@@ -8,7 +8,9 @@ per species, and push_coords precedes the deposits exactly as in the reference.
 SimComps may carry ``'SortEvery': n``: every n steps the pushed species are
 sorted by cell on the device (a performance option; the reference sorts only
 by x-chunk, chimera_main.py:306-318, and the physics does not depend on the
-particle order beyond the order of floating-point sums).
+particle order beyond the order of floating-point sums).  When such a step
+moves a window with an absorbing layer, the sort takes the layer's bounds and
+drops the absorbed particles itself, in place of the compaction.
 
 SimComps may carry ``'Features': ('PreDepositBackground',)``: the still
 background is then deposited before the initial half step, so that the static

@@ -660,39 +660,51 @@ static int32_t dens_on_grid_impl(const double* coord, const W* wghts, int64_t ld
 // dropped).  The cell comes from the same expressions as grid_deps.f90:46-47,
 // evaluated with explicitly rounded operations so that the integer key of a
 // particle does not depend on contraction choices of the compiler.
-__global__ void k_cell_keys(const double* __restrict__ x, const double* __restrict__ xh, int64_t ld, int np,
-                            double leftX, double dx_inv, double dr_inv, double r0, int nx, int nr, int use_lims,
-                            double l0, double l1, double l2, double l3, int test_half, uint32_t* __restrict__ keys,
-                            int32_t* __restrict__ kept) {
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = ip < np;
-    uint32_t key = 0;
-    if (valid) {
-        const double px = x[ip], py = x[ld + ip], pz = x[2 * ld + ip];
-        const uint32_t ncell = (uint32_t)nx * (uint32_t)nr;
-        bool drop = false;
-        if (use_lims) {
-            const double* t = test_half ? xh : x;
-            const double tx = t[ip], ty = t[ld + ip], tz = t[2 * ld + ip];
-            const double r2 = __dadd_rn(__dmul_rn(ty, ty), __dmul_rn(tz, tz));
-            drop = !(tx >= l0 && tx <= l1 && r2 >= l2 && r2 <= l3);
+constexpr int KEYS_T = 256, KEYS_MAX_BLOCKS = 2048;
+__global__ void __launch_bounds__(KEYS_T) k_cell_keys(const double* __restrict__ x, const double* __restrict__ xh,
+                                                       int64_t ld, int np, double leftX, double dx_inv, double dr_inv,
+                                                       double r0, int nx, int nr, int use_lims, double l0, double l1,
+                                                       double l2, double l3, int test_half,
+                                                       uint32_t* __restrict__ keys, int32_t* __restrict__ kept) {
+    __shared__ int32_t wkept[KEYS_T / 64];
+    const uint32_t ncell = (uint32_t)nx * (uint32_t)nr;
+    int32_t nkept = 0;    // this wave's kept particles (the same in every lane)
+    // grid-stride: a bounded number of blocks, one counter update per block
+    for (int64_t ip = (int64_t)blockIdx.x * KEYS_T + threadIdx.x; ip - threadIdx.x < np;
+         ip += (int64_t)gridDim.x * KEYS_T) {
+        const bool valid = ip < np;
+        uint32_t key = 0;
+        if (valid) {
+            const double px = x[ip], py = x[ld + ip], pz = x[2 * ld + ip];
+            bool drop = false;
+            if (use_lims) {
+                const double* t = test_half ? xh : x;
+                const double tx = t[ip], ty = t[ld + ip], tz = t[2 * ld + ip];
+                const double r2 = __dadd_rn(__dmul_rn(ty, ty), __dmul_rn(tz, tz));
+                drop = !(tx >= l0 && tx <= l1 && r2 >= l2 && r2 <= l3);
+            }
+            if (drop) {
+                key = ncell + 1;
+            } else {
+                const double rp = sqrt(__dadd_rn(__dmul_rn(py, py), __dmul_rn(pz, pz)));
+                const double fx = __dmul_rn(__dsub_rn(px, leftX), dx_inv);
+                const double fr = __dmul_rn(__dsub_rn(rp, r0), dr_inv);
+                if (fx >= 0.0 && fx < (double)nx && fr >= 0.0 && fr < (double)nr)
+                    key = (uint32_t)floor(fr) * (uint32_t)nx + (uint32_t)floor(fx);
+                else
+                    key = ncell;
+            }
+            keys[ip] = key;
         }
-        if (drop) {
-            key = ncell + 1;
-        } else {
-            const double rp = sqrt(__dadd_rn(__dmul_rn(py, py), __dmul_rn(pz, pz)));
-            const double fx = __dmul_rn(__dsub_rn(px, leftX), dx_inv);
-            const double fr = __dmul_rn(__dsub_rn(rp, r0), dr_inv);
-            if (fx >= 0.0 && fx < (double)nx && fr >= 0.0 && fr < (double)nr)
-                key = (uint32_t)floor(fr) * (uint32_t)nx + (uint32_t)floor(fx);
-            else
-                key = ncell;
-        }
-        keys[ip] = key;
+        nkept += (int32_t)__popcll(__ballot(valid && key <= ncell));
     }
-    // kept-particle count, one atomic per wave
-    const uint64_t m = __ballot(valid && key <= (uint32_t)nx * (uint32_t)nr);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(kept, (int32_t)__popcll(m));
+    if ((threadIdx.x & 63) == 0) wkept[threadIdx.x >> 6] = nkept;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t t = 0;
+        for (int w = 0; w < KEYS_T / 64; w++) t += wkept[w];
+        if (t) atomicAdd(kept, t);
+    }
 }
 
 template <typename R>
@@ -739,7 +751,9 @@ static int32_t sort_cells_impl(const double* coord, const double* coord_half, co
     CH_HIP(hipMemsetAsync(kept, 0, sizeof(int32_t), stream()));
     double L[4] = {0, 0, 0, 0};
     if (lims) memcpy(L, lims, sizeof L);
-    hipLaunchKernelGGL(k_cell_keys, blocks_for(np, 256), 256, 0, stream(), coord, coord_half, ld, np, leftX, dx_inv,
+    const int64_t kb = (np + KEYS_T - 1) / KEYS_T;
+    hipLaunchKernelGGL(k_cell_keys, (unsigned)(kb < KEYS_MAX_BLOCKS ? kb : KEYS_MAX_BLOCKS), KEYS_T, 0, stream(), coord,
+                       coord_half, ld, np, leftX, dx_inv,
                        dr_inv, r0, nx, nr, lims ? 1 : 0, L[0], L[1], L[2], L[3], test_half, keys, kept);
     CH_LAUNCH_CHECK("k_cell_keys");
     const uint64_t maxkey = (uint64_t)nx * nr + 1;
