@@ -411,6 +411,8 @@ def omp_add_scl(scl_fb, A):
 def undul_analytic(coord, Fld, t, params):
     """devices.f90:162-203 -> Fld"""
     x, F, p = _arr(coord, np.float64), _arr(Fld, np.float64), _arr(params, np.float64)
+    if p.size < 4:
+        raise ValueError("undul_analytic: params = (a0, lambda, X0, Lx)")
     _call("chimera_undul_analytic", _p(x), _p(F), float(t), _p(p), x.shape[1])
     return F
 
