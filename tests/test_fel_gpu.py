@@ -15,7 +15,8 @@ rounding of one position moves J_fb by ~2e-11 of itself at step 1 and by
 ~2e-8 after 300 steps within the oracle itself; the fields, dominated by the
 seed, stay near 1e-10.  The energy (nrg_out) and power (pwr_out) lines the
 reference script records are checked against the oracle's formulas on the
-GPU's own fields at 1e-12.
+GPU's own fields at 1e-12.  A shorter run with the envelope modes -1, 0, 1
+covers the reordered tables and the couplings between modes.
 """
 import sys
 
@@ -101,3 +102,43 @@ def test_fel_testrun_against_oracle(port):
         c.make_step(i)
         if i in checks:
             _compare(snaps[i], snapshot_cpu(c), fl[i], 1e-12 if i == 1 else 1e-9, f"  fel step {i}")
+
+
+@pytest.mark.timeout(600)
+def test_envelope_modes_minus1_to_1_against_oracle():
+    """The same scenario with MaxAzimuthMode = 1 (envelope modes -1, 0, 1: the
+    tables reordered to -Nko..Nko, the deposits' conjugate modes, the operators'
+    couplings between neighbouring modes) and RandCell 8, 30 steps."""
+    import chimera_amd as G
+    from chimera_amd import workloads as W
+    from oracle import driver as O
+    spec = W.fel_testrun(steps=30)
+    spec["solver"]["MaxAzimuthMode"] = 1
+    spec["beam"]["RandCell"] = 8
+    checks = (1, 30)
+    say("case: fel_testrun with modes -1..1, RandCell 8, half step and steps 1, 30")
+    g = W.build_fel(G, spec, 7)
+    assert g.Solvers[0].Args["Mtot"] == 3 and list(g.Solvers[0].Args["idxM"]) == [-1, 0, 1]
+    snaps = {0: snapshot_gpu(g)}
+    for i in range(1, 31):
+        g.make_step(i)
+        if i in checks:
+            snaps[i] = snapshot_gpu(g)
+    del g
+    with O.rounding_variant():
+        cf = W.build_fel(O, spec, 7)
+        fl = {0: snapshot_cpu(cf)}
+        for i in range(1, 31):
+            cf.make_step(i)
+            if i in checks:
+                fl[i] = snapshot_cpu(cf)
+    del cf
+    c = W.build_fel(O, spec, 7)
+    _compare(snaps[0], snapshot_cpu(c), fl[0], 1e-12, "  env m=-1..1 half step")
+    for i in range(1, 31):
+        c.make_step(i)
+        if i in checks:
+            s = snapshot_cpu(c)
+            for m in range(3):       # every mode carries field: the beam drives m = +-1
+                assert np.abs(s["EG_fb"][:, :, m]).max() > 0, m
+            _compare(snaps[i], s, fl[i], 1e-12 if i == 1 else 1e-9, f"  env m=-1..1 step {i}")
