@@ -38,7 +38,8 @@ namespace chimera {
 // finds the bounding box of their cells, and copies the field nodes of that
 // box (all six components and modes) into LDS with coalesced loads; each
 // particle then reads its 4 x 6 x NMOD values from LDS instead of from L2 (at
-// 16 particles per cell the block's 256 particles share 17 x 3 nodes).  A box
+// 16 particles per cell the block's 256 particles share 17 x 3 nodes).  The
+// momenta load with the positions, in flight while the tile loads.  A box
 // too large for GP_LDS_BYTES reads the field from global memory.  The values
 // and the order of the sums are the same either way.
 constexpr int GP_T = 256, GP_LDS_BYTES = 24 * 1024;
@@ -74,10 +75,15 @@ __global__ void __launch_bounds__(GP_T) k_gather_push(const double* __restrict__
     const bool live = ip < np;
     const GridGeom g = frame_geom(g_in);
     double xp = 0, yp = 0, zp = 0;
+    R m[3] = {0, 0, 0};
     if (live) {
         xp = coord[sx.at(ip, 0)];
         yp = coord[sx.at(ip, 1)];
         zp = coord[sx.at(ip, 2)];
+        // the momenta load now, so that it is in flight while the field tile loads
+        m[0] = u[su.at(ip, 0)];
+        m[1] = u[su.at(ip, 1)];
+        m[2] = u[su.at(ip, 2)];
     }
     PartGeom pg;
     const bool has = live && w[ip] != 0 && part_geom(pg, xp, yp, zp, g, +1.0);
@@ -144,7 +150,6 @@ __global__ void __launch_bounds__(GP_T) k_gather_push(const double* __restrict__
         }
     }
     if (devs.n) ext_fields<R>(devs, xp, yp, zp, f);
-    R m[3] = {u[su.at(ip, 0)], u[su.at(ip, 1)], u[su.at(ip, 2)]};
     boris<R>(m, f, (R)(0.5 * push_dt));
     for (int c = 0; c < 3; c++) u[su.at(ip, c)] = m[c];
 }
@@ -161,14 +166,16 @@ __global__ void __launch_bounds__(GP_T) k_gather_push(const double* __restrict__
 // and hands the tile on:
 //  * a block whose tile spans at most DEP_SLOT_R node rows and DEP_SLOT_W node
 //    columns (the common case for particles in the reference's generation
-//    order or sorted by cell: 256 particles cover about 16 cells of one or two
+//    order or sorted by cell: 512 particles cover about 32 cells of one or two
 //    cell rows) stores its tile with plain stores in a slot of its own and
 //    records (first node row, first node column, width, rows) in a
 //    descriptor; k_dep_combine then adds the slots into the grid, so no
 //    global atomics;
 //  * any other block adds its tile to the grid with one global atomic per
 //    node and component, and a box too large for the LDS budget makes the
-//    block deposit straight to the grid with global atomics.
+//    block deposit straight to the grid with global atomics, one per node
+//    value and cell from the same per-cell register sums (a block that spans
+//    the end of one cell row and the start of the next is such a block).
 // Deposits are unfolded; the forward FB transform applies the ghost fold.
 
 
@@ -201,13 +208,17 @@ __device__ __forceinline__ void st4(double* p, dv4 v) { *reinterpret_cast<dv4*>(
 
 // A thread owns DEP_PPT consecutive particles, handled in groups of DEP_G (one
 // 32-byte vector load per component and group).
-constexpr int DEP_THREADS = 64, DEP_PPT = 4, DEP_G = 4;
+constexpr int DEP_THREADS = 64, DEP_PPT = 8, DEP_G = 4;
 // LDS tile of a block: one component (J_x, J_y, J_z or rho) at a time, all
-// modes, complex FP64: 16 KiB hold 1024 / nmod nodes -- the 17 x 3 nodes of 256
-// cell-ordered particles at 16 per cell, or the 65 x 3 at 4 per cell up to
-// nmod = 4 -- and eight blocks per CU use 128 of the 160 KiB of LDS.
+// modes, complex FP64: 16 KiB hold 1024 / nmod nodes -- the 33 x 3 nodes of 512
+// cell-ordered particles at 16 per cell up to nmod = 4 -- and eight blocks per
+// CU use 128 of the 160 KiB of LDS.  Eight particles per thread (512 per
+// block) with 64-column slots: at lpa_cfg2 on cell-sorted particles the
+// deposit takes 6.9 ms per FP64 step against 7.5 ms with four per thread and
+// 32-column slots (with those slots, eight per thread had overflowed them into
+// global atomics and measured slower).
 constexpr int DEP_LDS_BYTES = 16 * 1024;
-constexpr int DEP_SLOT_W = 32, DEP_SLOT_R = 4;   // node columns and rows of one slot
+constexpr int DEP_SLOT_W = 64, DEP_SLOT_R = 4;   // node columns and rows of one slot
 
 __device__ __forceinline__ void lds_add(double* p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -250,7 +261,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     // ---- phase 1: load the thread's DEP_PPT consecutive particles, DEP_G at a
     // time (32-byte vector loads when the arrays allow: with DEP_PPT = DEP_G a
     // wave then reads 2 KB contiguous per component), push, store, bounding
-    // boxes.  (Eight particles per thread measured slower in FP64.)
+    // boxes.
     const int64_t ib0 = p0 + (int64_t)tid * DEP_PPT;
     // group h of the thread's particles: component c of array p at particle ib0 + h*DEP_G + j
     auto load_parts = [&](const auto* p, double (&v)[3][DEP_G], int ncomp, int h) __attribute__((always_inline)) {
@@ -370,8 +381,21 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
         }
         double2 acc[NMOD][4];   // per-cell sums in FP64 in both precisions
         int cx = -1, cr = 0;
+        // a cell's sums go to the tile, or, for a block without one (its box
+        // too large), to the grid with one atomic per node value and cell
         auto flush = [&]() __attribute__((always_inline)) {
             if (cx < 0) return;
+            if (!tiled) {
+#pragma unroll
+                for (int m = 0; m < NMOD; m++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        R* dst = reinterpret_cast<R*>(grid_l + (cx + (q & 1)) + (int64_t)(cr + (q >> 1)) * sr + m * sm);
+                        unsafeAtomicAdd(dst, (R)acc[m][q].x);
+                        unsafeAtomicAdd(dst + 1, (R)acc[m][q].y);
+                    }
+                return;
+            }
             const int tx = cx - bx0, tr = cr - br0;
 #pragma unroll
             for (int m = 0; m < NMOD; m++)
@@ -405,7 +429,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
             }
             // in-tile test on the cell found here (phase 1 may round differently at a cell edge)
             const int64_t tx = pg.ix - bx0, tr = pg.ir - br0;
-            if (tiled && tx >= 0 && tx + 1 < TX && tr >= 0 && tr + 1 < TR) {
+            if (!tiled || (tx >= 0 && tx + 1 < TX && tr >= 0 && tr + 1 < TR)) {
                 if (pg.ix != cx || pg.ir != cr) {
                     flush();
 #pragma unroll

@@ -31,8 +31,26 @@ from tests.test_step_gpu import (compare_snaps, moments, moments_err, noise_floo
 pytestmark = pytest.mark.gpu
 
 
+_CAPTURE = None
+
+
+@pytest.fixture(autouse=True)
+def _progress_uncaptured(capsys):
+    """These runs take minutes: their progress lines go out as they happen,
+    past pytest's output capture, so that a quiet stretch is never mistaken
+    for a hang."""
+    global _CAPTURE
+    _CAPTURE = capsys
+    yield
+    _CAPTURE = None
+
+
 def say(msg):
-    print(msg, flush=True)
+    if _CAPTURE is not None:
+        with _CAPTURE.disabled():
+            print(msg, flush=True)
+    else:
+        print(msg, flush=True)
     sys.stdout.flush()
 
 
@@ -106,11 +124,13 @@ def test_lpa_cfg2_against_oracle():
     say(f"  GPU run done ({np_gpu} electrons); oracle runs")
     with O.rounding_variant():
         cf = W.build(O, spec, 43)
+        say("  FMA-contracted oracle run: built, half step done")
         cf.make_step(1)
     fma1 = snapshot_cpu(cf)
     del cf
     say("  FMA-contracted oracle run: step 1 done")
     c = W.build(O, spec, 43)
+    say("  oracle run: built, half step done")
     compare_snaps(s0, snapshot_cpu(c), 1e-12, 1e-12, "  cfg2 half step")
     del s0
     c.make_step(1)

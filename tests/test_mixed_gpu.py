@@ -8,7 +8,14 @@ FB-space E/G and J after 100 steps of lpa_cfg0, and <= 1e-5 for the particle
 moments (total charge, mean momentum relative to the RMS spread, RMS momentum).
 After the first step a component whose field is still ~1e-3 of the others
 (E_y of the linearly polarised laser) is compared against the norm of all six
-components instead of its own.
+components instead of its own, and J_fb is held to max(1e-4, 4 x its input
+rounding floor): the FP64 run again with the particles' momenta and weights
+rounded to float32 before the step, against the FP64 run.  The Poisson
+correction adds (grad rho_n+1 - grad rho_n)/dt, and rho of the neutral plasma
+is the small sum of the electron and ion densities, so J_fb after one step
+carries the float32 rounding of the particle data amplified, in any mixed
+form.  (The spread between the two mixed forms, split-BF16 and FP32 MFMA, is
+printed beside it: it is small, the GEMM form is not what J_fb is sensitive to.)
 """
 import sys
 
@@ -58,22 +65,50 @@ def test_mixed_against_fp64_cfg0_100_steps():
     import chimera_amd as G
     from chimera_amd import workloads as W
     say("case: lpa_cfg0 mixed vs fp64, 100 steps")
+    from chimera_amd._lib import lib
     spec = W.lpa_cfg0()
+    L = lib()
+    old = L.chimera_get_mixed_gemm()
+    L.chimera_set_mixed_gemm(1)                  # the FP32-MFMA form, one step: the spread of the mixed forms
+    try:
+        m32 = W.build(G, W.with_precision(spec, "mixed"), 31)
+        m32.make_step(1)
+        j32 = m32.Solvers[0].Data["J_fb"].numpy().astype(np.complex128)
+        del m32
+    finally:
+        L.chimera_set_mixed_gemm(old)
+    # the FP64 run with its particle data rounded to float32: J_fb's input rounding floor
+    fr = W.build(G, W.with_precision(spec, "fp64"), 31)
+    for sp in fr.Particles:
+        if sp.np:
+            sp.restore_particles(sp.host("coords"), sp.host("coords_halfstep"),
+                                 sp.host("momenta").astype(np.float32).astype(np.float64),
+                                 sp.host("weights").astype(np.float32).astype(np.float64))
+    fr.make_step(1)
+    jfr = fr.Solvers[0].Data["J_fb"].numpy()
+    del fr
     fp = W.build(G, W.with_precision(spec, "fp64"), 31)
     mx = W.build(G, W.with_precision(spec, "mixed"), 31)
     assert mx.Solvers[0].Data["EB"].dtype == np.complex64 and mx.Particles[0].Data["momenta"].dtype == np.float32
+    spread = None
     for i in range(1, 101):
         fp.make_step(i)
         mx.make_step(i)
         if i in (1, 100):
             e = _errors(mx, fp)
             say(f"  step {i}: " + " ".join(f"{k}={v:.2e}" for k, v in e.items()))
+            if i == 1:
+                spread = rel_l2(mx.Solvers[0].Data["J_fb"].numpy().astype(np.complex128), j32)
+                floor = rel_l2(jfr, fp.Solvers[0].Data["J_fb"].numpy())
+                say(f"  J_fb after step 1: spread between the mixed forms {spread:.2e}, "
+                    f"input rounding floor {floor:.2e}")
             assert mx.Particles[0].np == fp.Particles[0].np
             for name in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
                 err = e[name] if i == 100 else e[name + "/all"]
                 assert err <= 1e-5, (i, name, err)
             for key in ("EG_fb", "J_fb", "charge", "mean/rms", "rms"):
-                assert e[key] <= (1e-5 if i == 100 else 1e-4), (i, key, e[key])
+                bar = 1e-5 if i == 100 else (max(1e-4, 4 * floor) if key == "J_fb" else 1e-4)
+                assert e[key] <= bar, (i, key, e[key], bar)
 
 
 def test_mixed_moving_window():
