@@ -156,15 +156,15 @@ __global__ void __launch_bounds__(GP_T) k_gather_push(const double* __restrict__
 
 // ------------------------------------------------------------------ fused push + deposition
 //
-// One block takes DEP_PPT consecutive particles per thread (thread t owns
-// particles p0 + t*DEP_PPT + j, so the lanes of a wave sit DEP_PPT particles
+// One block takes PPT consecutive particles per thread (thread t owns
+// particles p0 + t*PPT + j, so the lanes of a wave sit PPT particles
 // apart and, with particles in cell order, mostly in different cells).
 // Phase 1 optionally pushes positions (particle_tools.f90:58-82) and finds the
 // block's cell bounding box for J (at the half-step positions) and rho (at the
 // new positions).  Then, one component (J_x, J_y, J_z, rho) at a time, the
 // block accumulates into an LDS tile covering the box with LDS FP64 atomics
 // and hands the tile on:
-//  * a block whose tile spans at most DEP_SLOT_R node rows and DEP_SLOT_W node
+//  * a block whose tile spans at most DEP_SLOT_R node rows and 8*PPT node
 //    columns (the common case for particles in the reference's generation
 //    order or sorted by cell: 512 particles cover about 32 cells of one or two
 //    cell rows) stores its tile with plain stores in a slot of its own and
@@ -193,8 +193,8 @@ struct DepArgsT {
     GridGeom gJ, gR;    // geometry for J (leftX at the current deposit) and rho
     C* J;               // (nx+1, nr+1, M, 3) or null
     C* R;               // (nx+1, nr+1, M) or null
-    C* slotJ;           // [block][3*M][DEP_SLOT_R][DEP_SLOT_W] (with J)
-    C* slotR;           // [block][M][DEP_SLOT_R][DEP_SLOT_W] (with R)
+    C* slotJ;           // [block][3*M][DEP_SLOT_R][8*PPT] (with J)
+    C* slotR;           // [block][M][DEP_SLOT_R][8*PPT] (with R)
     int4* desc;         // [2][blocks]: (first node row or -1, first node column, width, rows) of J, then of R
     int nblocks;
     int vec4;           // particle arrays allow 32-byte vector access (ld % 4 == 0, 32-byte aligned bases)
@@ -206,9 +206,9 @@ __device__ __forceinline__ dv4 ld4(const double* p) { return *reinterpret_cast<c
 __device__ __forceinline__ fv4 ld4(const float* p) { return *reinterpret_cast<const fv4*>(p); }
 __device__ __forceinline__ void st4(double* p, dv4 v) { *reinterpret_cast<dv4*>(p) = v; }
 
-// A thread owns DEP_PPT consecutive particles, handled in groups of DEP_G (one
+// A thread owns PPT consecutive particles, handled in groups of DEP_G (one
 // 32-byte vector load per component and group).
-constexpr int DEP_THREADS = 64, DEP_PPT = 8, DEP_G = 4;
+constexpr int DEP_THREADS = 64, DEP_G = 4;
 // LDS tile of a block: one component (J_x, J_y, J_z or rho) at a time, all
 // modes, complex FP64: 16 KiB hold 1024 / nmod nodes -- the 33 x 3 nodes of 512
 // cell-ordered particles at 16 per cell up to nmod = 4 -- and eight blocks per
@@ -218,7 +218,10 @@ constexpr int DEP_THREADS = 64, DEP_PPT = 8, DEP_G = 4;
 // 32-column slots (with those slots, eight per thread had overflowed them into
 // global atomics and measured slower).
 constexpr int DEP_LDS_BYTES = 16 * 1024;
-constexpr int DEP_SLOT_W = 64, DEP_SLOT_R = 4;   // node columns and rows of one slot
+constexpr int DEP_SLOT_R = 4;   // node rows of one slot; its columns: 8 per particle of a thread
+// Particles per thread (PPT) by launch: 8 (512 per block, 64-column slots) when
+// the particles average at least 8 per cell, else 4 (256 per block, 32-column
+// slots): a sparse bunch's 512 particles span more columns than a slot holds.
 
 __device__ __forceinline__ void lds_add(double* p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -243,10 +246,11 @@ __device__ __forceinline__ bool cell_of(double xp, double yp, double zp, const G
 // corner contribution is formed and summed per cell in FP64, and the LDS tile
 // is FP64 in both precisions (FP64 LDS atomics measured far cheaper here than
 // FP32 ones under this access pattern), rounded to R when it leaves the block.
-template <int NMOD, typename R>
+template <int NMOD, typename R, int PPT>
 __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
+    constexpr int SW = 8 * PPT;   // slot columns
     typedef cplx_t<R> C;
-    constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
+    constexpr int DEP_CHUNK = DEP_THREADS * PPT;
     __shared__ double tile[DEP_LDS_BYTES / sizeof(double)];
     __shared__ int red[8][DEP_THREADS / 64];
     __shared__ int box[8];
@@ -258,11 +262,11 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     const int64_t ld = a.ld;
     const bool doJ = a.J != nullptr, doR = a.R != nullptr;
 
-    // ---- phase 1: load the thread's DEP_PPT consecutive particles, DEP_G at a
-    // time (32-byte vector loads when the arrays allow: with DEP_PPT = DEP_G a
+    // ---- phase 1: load the thread's PPT consecutive particles, DEP_G at a
+    // time (32-byte vector loads when the arrays allow: with PPT = DEP_G a
     // wave then reads 2 KB contiguous per component), push, store, bounding
     // boxes.
-    const int64_t ib0 = p0 + (int64_t)tid * DEP_PPT;
+    const int64_t ib0 = p0 + (int64_t)tid * PPT;
     // group h of the thread's particles: component c of array p at particle ib0 + h*DEP_G + j
     auto load_parts = [&](const auto* p, double (&v)[3][DEP_G], int ncomp, int h) __attribute__((always_inline)) {
         const int64_t ib = ib0 + h * DEP_G;
@@ -284,7 +288,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
         }
     };
     int bx[8] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
-    for (int h = 0; h < DEP_PPT / DEP_G; h++) {
+    for (int h = 0; h < PPT / DEP_G; h++) {
     const int64_t ib = ib0 + h * DEP_G;
     const bool full = a.vec4 && ib + DEP_G <= a.np;
     double X[3][DEP_G], XH[3][DEP_G], U[3][DEP_G], W[3][DEP_G];   // W: row 0 only
@@ -353,12 +357,12 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
     const int64_t nJ = (int64_t)TXJ * TRJ * nmod * 2, nR = (int64_t)TXR * TRR * nmod * 2;
     const bool tileJ = doJ && nJ > 0 && nJ * (int64_t)sizeof(double) <= DEP_LDS_BYTES;
     const bool tileR = doR && nR > 0 && nR * (int64_t)sizeof(double) <= DEP_LDS_BYTES;
-    const bool slotJ = tileJ && TRJ <= DEP_SLOT_R && TXJ <= DEP_SLOT_W;
-    const bool slotR = tileR && TRR <= DEP_SLOT_R && TXR <= DEP_SLOT_W;
+    const bool slotJ = tileJ && TRJ <= DEP_SLOT_R && TXJ <= SW;
+    const bool slotR = tileR && TRR <= DEP_SLOT_R && TXR <= SW;
 
     // ---- phases 2 and 3, one pass per component (J_x, J_y, J_z, rho): zero
     // the tile, deposit into it, then write it to the block's slot (a tile no
-    // taller than DEP_SLOT_R rows and no wider than DEP_SLOT_W columns) or add
+    // taller than DEP_SLOT_R rows and no wider than SW columns) or add
     // it to the grid with global atomics.  A thread's consecutive particles
     // usually share a cell (the lattice of a cell is ordered with the
     // azimuthal index innermost), so each thread sums its per-corner
@@ -406,7 +410,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
                     lds_add(tile_l + t + 1, (double)acc[m][q].y);
                 }
         };
-        for (int h = 0; h < DEP_PPT / DEP_G; h++) {
+        for (int h = 0; h < PPT / DEP_G; h++) {
         double P[3][DEP_G], V[3][DEP_G], Wt[3][DEP_G];   // Wt: row 0 only
         load_parts(isJ ? a.xh : a.x, P, 3, h);
         if (isJ) load_parts(a.u, V, 3, h);
@@ -466,14 +470,14 @@ __global__ void __launch_bounds__(DEP_THREADS) k_push_deposit(DepArgsT<R> a) {
         const bool slot = isJ ? slotJ : slotR;
         const int64_t n = nt / 2;
         if (slot) {
-            // tile index (m*TR + rr)*TX + xx -> slot index ((l*nmod + m)*DEP_SLOT_R + rr)*DEP_SLOT_W + xx
+            // tile index (m*TR + rr)*TX + xx -> slot index ((l*nmod + m)*DEP_SLOT_R + rr)*SW + xx
             // (rho: l = 0 of its own slot array)
-            C* sl_base = isJ ? a.slotJ + (int64_t)blk * (3 * nmod * DEP_SLOT_R * DEP_SLOT_W) +
-                                   (int64_t)l * nmod * DEP_SLOT_R * DEP_SLOT_W
-                             : a.slotR + (int64_t)blk * (nmod * DEP_SLOT_R * DEP_SLOT_W);
+            C* sl_base = isJ ? a.slotJ + (int64_t)blk * (3 * nmod * DEP_SLOT_R * SW) +
+                                   (int64_t)l * nmod * DEP_SLOT_R * SW
+                             : a.slotR + (int64_t)blk * (nmod * DEP_SLOT_R * SW);
             for (int64_t i = tid; i < n; i += DEP_THREADS) {
                 const int64_t rest = i / TX;
-                sl_base[((rest / TR) * DEP_SLOT_R + rest % TR) * DEP_SLOT_W + i % TX] =
+                sl_base[((rest / TR) * DEP_SLOT_R + rest % TR) * SW + i % TX] =
                     cmake<R>((R)tile[2 * i], (R)tile[2 * i + 1]);
             }
         } else {
@@ -532,7 +536,7 @@ constexpr int CMB_T = 256, CMB_LIST = 1024;
 // bounds is null (small launches, where the sort's dozen launches cost more
 // than the scan), from all nb descriptors in block order: the same slots in
 // the same order, so the two forms give the same sums.
-template <int NMOD, typename RT>
+template <int NMOD, typename RT, int SW>
 __global__ void __launch_bounds__(CMB_T) k_dep_combine(cplx_t<RT>* __restrict__ J, cplx_t<RT>* __restrict__ R,
                                                         const cplx_t<RT>* __restrict__ slotJ,
                                                         const cplx_t<RT>* __restrict__ slotR,
@@ -546,7 +550,7 @@ __global__ void __launch_bounds__(CMB_T) k_dep_combine(cplx_t<RT>* __restrict__ 
     if (grid == nullptr) return;
     const C* slot = which ? slotR : slotJ;
     const int ncomp = which ? NMOD : NC;
-    const int64_t sstride = (int64_t)ncomp * DEP_SLOT_R * DEP_SLOT_W;
+    const int64_t sstride = (int64_t)ncomp * DEP_SLOT_R * SW;
     const int r = blockIdx.y, xs = blockIdx.x * CMB_T, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int x = xs + tid;
     __shared__ int4 list[CMB_LIST];   // (4*block + layer, first column, width, 0)
@@ -562,10 +566,10 @@ __global__ void __launch_bounds__(CMB_T) k_dep_combine(cplx_t<RT>* __restrict__ 
             if (x >= ent.y && x < ent.y + ent.z) {
                 hit = true;
                 const int b = ent.x >> 2, layer = ent.x & 3;
-                const C* sp = slot + (int64_t)b * sstride + layer * DEP_SLOT_W + (x - ent.y);
+                const C* sp = slot + (int64_t)b * sstride + layer * SW + (x - ent.y);
 #pragma unroll
                 for (int c = 0; c < NC; c++)
-                    if (c < ncomp) acc[c] = cadd(acc[c], sp[c * DEP_SLOT_R * DEP_SLOT_W]);
+                    if (c < ncomp) acc[c] = cadd(acc[c], sp[c * DEP_SLOT_R * SW]);
             }
         }
     };
@@ -908,11 +912,14 @@ static int32_t push_deposit_impl(double* coord, double* coord_half, const R* mom
         auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 31) == 0; };
         a.vec4 = (ld % 4 == 0) && al(coord) && al(coord_half) && al(momenta) && al(wghts);
     }
-    const int nb = (int)blocks_for(np, DEP_THREADS * DEP_PPT);
+    // 512-particle blocks where the particles average 8 or more per cell
+    const bool big = (int64_t)np >= (int64_t)8 * nx * nr;
+    const int ppt = big ? 8 : 4, sw = 8 * ppt;
+    const int nb = (int)blocks_for(np, DEP_THREADS * ppt);
     const int nmod = nkO + 1, nrb = nr + 1;
     a.nblocks = nb;
-    a.slotJ = a.J ? static_cast<C*>(scratch(16, (size_t)nb * 3 * nmod * DEP_SLOT_R * DEP_SLOT_W * sizeof(C))) : nullptr;
-    a.slotR = a.R ? static_cast<C*>(scratch(17, (size_t)nb * nmod * DEP_SLOT_R * DEP_SLOT_W * sizeof(C))) : nullptr;
+    a.slotJ = a.J ? static_cast<C*>(scratch(16, (size_t)nb * 3 * nmod * DEP_SLOT_R * sw * sizeof(C))) : nullptr;
+    a.slotR = a.R ? static_cast<C*>(scratch(17, (size_t)nb * nmod * DEP_SLOT_R * sw * sizeof(C))) : nullptr;
     a.desc = static_cast<int4*>(scratch(18, (size_t)2 * nb * sizeof(int4)));
     uint32_t* keys = static_cast<uint32_t*>(scratch(19, (size_t)2 * nb * 4 * sizeof(uint32_t)));
     int32_t* bounds = static_cast<int32_t*>(scratch(23, (size_t)2 * 2 * nrb * sizeof(int32_t)));
@@ -930,11 +937,23 @@ static int32_t push_deposit_impl(double* coord, double* coord_half, const R* mom
     // less time than the sort)
     const bool scan_all = nb <= 512 || (int64_t)nb * grid.x * grid.y * grid.z <= ((int64_t)1 << 23);
 #define CMB_LAUNCH(N, IDX, BOUNDS)                                                                                \
-    hipLaunchKernelGGL((k_dep_combine<N, R>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR, a.desc, IDX, \
-                       BOUNDS, nb, nrb, sr, sm, nx + 1)
+    do {                                                                                                          \
+        if (big)                                                                                                  \
+            hipLaunchKernelGGL((k_dep_combine<N, R, 64>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR,   \
+                               a.desc, IDX, BOUNDS, nb, nrb, sr, sm, nx + 1);                                     \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_dep_combine<N, R, 32>), grid, CMB_T, 0, stream(), a.J, a.R, a.slotJ, a.slotR,   \
+                               a.desc, IDX, BOUNDS, nb, nrb, sr, sm, nx + 1);                                     \
+    } while (0)
     if (scan_all) {
         // every descriptor of a deposited grid is written by its block
-#define DEP_LAUNCH(N) hipLaunchKernelGGL((k_push_deposit<N, R>), nb, DEP_THREADS, 0, stream(), a)
+#define DEP_LAUNCH(N)                                                                                             \
+    do {                                                                                                          \
+        if (big)                                                                                                  \
+            hipLaunchKernelGGL((k_push_deposit<N, R, 8>), nb, DEP_THREADS, 0, stream(), a);                       \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_push_deposit<N, R, 4>), nb, DEP_THREADS, 0, stream(), a);                       \
+    } while (0)
         CH_MODE_SWITCH(nkO, DEP_LAUNCH)
         CH_LAUNCH_CHECK("k_push_deposit");
 #define CMB_SCAN(N) CMB_LAUNCH(N, nullptr, nullptr)
