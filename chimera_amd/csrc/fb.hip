@@ -372,8 +372,8 @@ static int32_t launch_fft_bluestein(FftArgsT<R> a) {
     return ST_OK;
 }
 
-// Threads per workgroup for lines of 4096 and 8192 points (EPT points each).
-constexpr int FFT_BIG_TPB = 1024, FFT_BIG_EPT4 = 4096 / FFT_BIG_TPB, FFT_BIG_EPT8 = 8192 / FFT_BIG_TPB;
+// Threads per workgroup for lines of 4096 and 8192 points, 8 points each.
+constexpr int FFT_4K_TPB = 4096 / 8, FFT_8K_TPB = 8192 / 8;
 
 template <typename R>
 int32_t launch_fft(FftArgsT<R> a) {
@@ -388,12 +388,12 @@ int32_t launch_fft(FftArgsT<R> a) {
     if constexpr (sizeof(R) == 8) a.tw = fft_twiddles(N);
     else a.tw = fft_twiddles_f(N);
     if (!a.tw) return ST_ENOMEM;
-    // lines of up to 2048 points: 256 threads of 8 points, several lines per
-    // workgroup; longer lines fill a workgroup's LDS alone (64 and 128 KiB in
-    // complex128) and run with FFT_BIG_TPB threads, so that the CU holds 16
-    // waves to hide the latency of the passes' LDS traffic instead of four
-    const int tpb = N <= 2048 ? FFT_THREADS : FFT_BIG_TPB;
-    const int ept = N <= 2048 ? 8 : N / tpb;
+    // every line length runs 8 points a thread (radix-8 passes): lines of up
+    // to 2048 points with 256 threads, several lines per workgroup; longer
+    // lines fill a workgroup's LDS alone (64 and 128 KiB in complex128) with
+    // N / 8 threads, 8 and 16 waves, to hide the passes' LDS latency
+    const int tpb = N <= 2048 ? FFT_THREADS : N / 8;
+    const int ept = 8;
     const int tot = ept * tpb;
     const int lpb = tot / N;
     const unsigned grid = (unsigned)((a.nlines + lpb - 1) / lpb);
@@ -414,21 +414,19 @@ int32_t launch_fft(FftArgsT<R> a) {
         // workgroup; EPT = 8 at most 64 KiB (2048 complex128 points and their twiddles)
         CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<8, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(64 * 1024)));
-        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<FFT_BIG_EPT4, R, FFT_BIG_TPB>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(FFT_BIG_EPT4 * FFT_BIG_TPB * sizeof(C))));
-        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<FFT_BIG_EPT8, R, FFT_BIG_TPB>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(FFT_BIG_EPT8 * FFT_BIG_TPB * sizeof(C))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<8, R, FFT_4K_TPB>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(4096 * sizeof(C))));
+        CH_HIP(hipFuncSetAttribute((const void*)k_fft_lines<8, R, FFT_8K_TPB>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(8192 * sizeof(C))));
         attr_done = true;
     }
     KTimer kt(KF_FFT);
-    if (tpb == FFT_THREADS && ept == 8) {
+    if (tpb == FFT_THREADS) {
         hipLaunchKernelGGL((k_fft_lines<8, R>), grid, FFT_THREADS, shmem, stream(), a);
-    } else if (tpb == FFT_BIG_TPB && ept == FFT_BIG_EPT4) {
-        hipLaunchKernelGGL((k_fft_lines<FFT_BIG_EPT4, R, FFT_BIG_TPB>), grid, FFT_BIG_TPB, shmem, stream(), a);
-    } else if (tpb == FFT_BIG_TPB && ept == FFT_BIG_EPT8) {
-        hipLaunchKernelGGL((k_fft_lines<FFT_BIG_EPT8, R, FFT_BIG_TPB>), grid, FFT_BIG_TPB, shmem, stream(), a);
+    } else if (tpb == FFT_4K_TPB) {
+        hipLaunchKernelGGL((k_fft_lines<8, R, FFT_4K_TPB>), grid, FFT_4K_TPB, shmem, stream(), a);
+    } else if (tpb == FFT_8K_TPB) {
+        hipLaunchKernelGGL((k_fft_lines<8, R, FFT_8K_TPB>), grid, FFT_8K_TPB, shmem, stream(), a);
     } else {
         return fail(ST_EUNSUP, "FFT length %d", N);
     }
